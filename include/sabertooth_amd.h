@@ -1,0 +1,203 @@
+/*
+ * sabertooth_amd — MI355X (gfx950) implementation of Sabertooth's
+ * post-prefilter scoring path behind a plain C ABI.
+ *
+ * The path: ungapped diagonal evidence (Kadane per diagonal), the gate (best
+ * segment or co-linear chain of two vs min_diag_score) and the exact
+ * Smith-Waterman-Gotoh local score, for plain-sequence and PSSM score rows.
+ * All results are bit-identical to the reference CPU kernels.
+ *
+ * Conventions (identical to the reference):
+ *   - residue codes 0..20 (ACDEFGHIKLMNPQRSTVWYX); target codes >= 21 score 0
+ *     (src/alphabet.rs:36, src/align.rs:180-187);
+ *   - score rows are int32[m][21], row i = scores of query column i against
+ *     target codes 0..20 (ScoreTable, src/align.rs:104-192);
+ *   - a gap of length L costs open + (L-1)*extend (GapCosts, src/align.rs:196-223).
+ *
+ * Ownership: every pointer argument is borrowed for the duration of the call
+ * only; outputs are caller-owned buffers.  Handles are opaque.  Every entry
+ * point returns ST_OK or an error code; st_last_error() gives the message of
+ * the calling thread's last failure.  One context serialises its own calls
+ * internally, so callers may share it between threads (as rayon workers do).
+ *
+ * Citations are file:line in the reference (raw-lab/sabertooth v0.4.0).
+ */
+#ifndef SABERTOOTH_AMD_H
+#define SABERTOOTH_AMD_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status ------------------------------------------------------------ */
+#define ST_OK 0
+#define ST_ERR_INVALID 1   /* bad argument (null, out of range, mismatched) */
+#define ST_ERR_DEVICE 2    /* a HIP runtime call failed / no usable GPU */
+#define ST_ERR_NOMEM 3     /* device or host allocation failed */
+#define ST_ERR_CAPACITY 4  /* admission refused: DP-state memory exhausted */
+#define ST_ERR_CLOSED 5    /* the serving queue was closed */
+
+/* Message of the calling thread's last error ("" when none). */
+const char *st_last_error(void);
+/* Library version string. */
+const char *st_version(void);
+
+/* Which precision produced a score (mirrors the reference's Kernel enum,
+ * src/simd.rs:38-49, for the scalar_fallbacks counter, src/search.rs:532-580). */
+#define ST_KERNEL_I16 0 /* packed int16 pass, exact (no saturation) */
+#define ST_KERNEL_I32 1 /* exact int32 pass (saturated int16, or wide rows) */
+
+/* ---- context ----------------------------------------------------------- */
+typedef struct st_ctx st_ctx;
+
+/* Open device `device` (HIP ordinal).  Fails with ST_ERR_DEVICE when no GPU. */
+int st_ctx_create(int device, st_ctx **out);
+int st_ctx_destroy(st_ctx *ctx);
+
+/* ---- host inputs --------------------------------------------------------- */
+
+/* Integer BLOSUM62 at bit factor 2, bias -0.2, as the reference reconstructs
+ * it from MMseqs2's blosum62.out (SubstitutionMatrix::blosum62,
+ * src/matrix.rs:50-175).  score: int8[21*21] row-major; p_back: f64[21]
+ * (after the X rescale); r: f32[20*20] = P(a|b); any output may be NULL. */
+int st_blosum62(int8_t *score, double *p_back, float *r, double *lambda);
+
+/* ASCII -> residue code (build_aa2num, src/alphabet.rs:41-50). */
+int st_aa2num(uint8_t table[256]);
+
+/* Score rows of a plain sequence query (ScoreTable::from_scorer(SeqScorer),
+ * src/align.rs:115-131): rows[i*21 + r] = score[q[i]][r]. */
+int st_seq_rows(const uint8_t *query, uint64_t m, int32_t *rows);
+
+/* ---- resident target database -------------------------------------------- */
+typedef struct st_db st_db;
+
+/* Upload the residue arena and offsets (the .stidx residues / seq_offsets
+ * sections, src/index.rs:10-13; SeedIndex::seq, src/index.rs:1975-1979).
+ * Target t is residues[offsets[t] .. offsets[t+1]).  The device copy is
+ * repacked: each target starts on a 16-byte boundary and is padded with the
+ * zero-score code 21. */
+int st_db_create(st_ctx *ctx, const uint8_t *residues, uint64_t n_residues,
+                 const uint64_t *offsets, uint64_t n_targets, st_db **out);
+int st_db_destroy(st_db *db);
+uint64_t st_db_targets(const st_db *db);
+uint64_t st_db_residues(const st_db *db);
+
+/* ---- query profile ------------------------------------------------------- */
+typedef struct st_qprof st_qprof;
+
+/* QueryProfile::from_rows (src/simd.rs:172-210): rows int32[m][21].  The
+ * device holds an int16 profile when every row value fits int16, and always
+ * an int32 one. */
+int st_qprof_create(st_ctx *ctx, const int32_t *rows, uint64_t m,
+                    st_qprof **out);
+int st_qprof_destroy(st_qprof *qp);
+uint64_t st_qprof_len(const st_qprof *qp);
+
+/* ---- SWG score ------------------------------------------------------------ */
+
+/* Exact optimal SWG local score of the query against each listed target:
+ * the batched form of QueryProfile::sw_score_with (src/simd.rs:252-292) as
+ * the gapped stage calls it (src/search.rs:532-580).  scores[i] for tids[i];
+ * kernel (nullable) receives ST_KERNEL_I16 / ST_KERNEL_I32 per pair. */
+int st_sw_score_batch(st_ctx *ctx, const st_qprof *qp, const st_db *db,
+                      const uint32_t *tids, uint64_t count, int32_t gap_open,
+                      int32_t gap_extend, int32_t *scores, uint8_t *kernel);
+
+/* ---- ungapped evidence and gate -------------------------------------------- */
+
+/* One diagonal to score: target, diagonal (target_pos - query_pos) and the
+ * query positions [lo, hi) (Job, src/prefilter.rs:1362-1369; Lane,
+ * src/ungapped.rs:265-270). */
+typedef struct {
+  uint32_t tid;
+  int32_t diag;
+  uint32_t lo;
+  uint32_t hi;
+} st_job;
+
+/* One target's jobs jobs[j0 .. j1) (TargetRec, src/prefilter.rs:1372-1377). */
+typedef struct {
+  uint32_t tid;
+  uint32_t hits;
+  uint32_t j0;
+  uint32_t j1;
+} st_target_rec;
+
+/* Candidate (src/prefilter.rs:370-381). */
+typedef struct {
+  uint32_t target_id;
+  uint32_t hits;
+  int64_t best_diag;
+  int32_t diag_score;
+  uint32_t t_lo;
+  uint32_t t_hi;
+} st_candidate;
+
+/* Exact kadane_score of every job (ungapped::score_batch,
+ * src/ungapped.rs:285-386, with lanes = jobs). */
+int st_ungapped_score_jobs(st_ctx *ctx, const st_qprof *qp, const st_db *db,
+                           const st_job *jobs, uint64_t njobs, int32_t *scores);
+
+/* Score the jobs, then chain and gate every target record in order, writing
+ * the passing candidates in record order (score_and_gate,
+ * src/prefilter.rs:1498-1660).  *n_out = number written (<= nrecs);
+ * chains_used (nullable) is incremented like PrefilterStats::chains_used;
+ * job_scores (nullable) receives the per-job scores. */
+int st_score_and_gate(st_ctx *ctx, const st_qprof *qp, const st_db *db,
+                      const st_job *jobs, uint64_t njobs,
+                      const st_target_rec *recs, uint64_t nrecs,
+                      int32_t gap_open, int32_t gap_extend,
+                      int32_t min_diag_score, int regions, st_candidate *out,
+                      uint64_t *n_out, uint64_t *chains_used,
+                      int32_t *job_scores);
+
+/* size_adjusted_gate (src/prefilter.rs:246-257). */
+int32_t st_size_adjusted_gate(int32_t gate, uint64_t db_residues);
+
+/* ---- batched (query, candidate list) scoring ------------------------------ */
+
+/* Per-pair result of the all-pairs path: SWG score, ungapped best-diagonal
+ * score over every diagonal, the smallest diagonal reaching it, the gate
+ * score (best single or chain of the top-4 diagonals) and its decision. */
+typedef struct {
+  int32_t swg;
+  int32_t ung;
+  int32_t best_diag;
+  int32_t gate_score;
+  uint8_t pass;
+  uint8_t kernel; /* ST_KERNEL_I16 / ST_KERNEL_I32 */
+  uint8_t reserved[2];
+} st_pair_result;
+
+typedef struct st_batch st_batch;
+
+/* Plan a batch: query q scores targets cand_tids[cand_off[q] .. cand_off[q+1])
+ * with gaps (gap_open[q], gap_extend[q]) and gate min_diag_score[q].  All
+ * inputs are copied to the device; st_batch_run then runs without any host
+ * transfer.  Autotuned launch shapes are chosen here. */
+int st_batch_create(st_ctx *ctx, const st_db *db, const st_qprof *const *queries,
+                    uint64_t nq, const uint64_t *cand_off,
+                    const uint32_t *cand_tids, const int32_t *gap_open,
+                    const int32_t *gap_extend, const int32_t *min_diag_score,
+                    st_batch **out);
+/* Run every kernel of the batch on the device and wait for it. */
+int st_batch_run(st_batch *b);
+/* Copy results (cand_off[nq] entries, in candidate order) to the host. */
+int st_batch_results(st_batch *b, st_pair_result *out);
+/* Device time of the last st_batch_run, from HIP events on the batch's
+ * stream: ms3[0] the main DP launches, ms3[1] the int32 rescoring of
+ * saturated pairs, ms3[2] the gate; *rescored (nullable) = pairs rescored. */
+int st_batch_timing(const st_batch *b, double *ms3, uint64_t *rescored);
+/* DP cells of the batch (sum of m*n) and pairs. */
+uint64_t st_batch_cells(const st_batch *b);
+uint64_t st_batch_pairs(const st_batch *b);
+int st_batch_destroy(st_batch *b);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,194 @@
+"""CPU oracle for the post-prefilter scoring path — TEST INFRASTRUCTURE ONLY.
+
+ctypes bindings of ``oracle/liboracle.so`` (a plain-C restatement of the
+reference's scalar kernels, see ``oracle.h`` for the file:line each function
+follows).  Only ``tests/``, ``__graft_entry__.smoke()`` and ``bench.py``'s
+``cpu_baseline`` leg may import this package, and only as the checker.  The
+product library never links or calls it.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB = None
+NPROF = 21
+
+
+def build() -> None:
+    """Compile the oracle (and the AVX2 baseline) with the committed Makefile."""
+    subprocess.run(["make", "-s", "-C", _HERE, "all"], check=True)
+
+
+def lib() -> C.CDLL:
+    global _LIB
+    if _LIB is None:
+        path = os.path.join(_HERE, "liboracle.so")
+        if not os.path.exists(path):
+            build()
+        L = C.CDLL(path)
+        P = C.c_void_p
+        L.or_gotoh_score_end.restype = C.c_int32
+        L.or_gotoh_score_end.argtypes = [P, C.c_size_t, P, C.c_size_t, C.c_int32, C.c_int32, P, P]
+        L.or_gotoh_textbook.restype = C.c_int32
+        L.or_gotoh_textbook.argtypes = [P, C.c_size_t, P, C.c_size_t, C.c_int32, C.c_int32, P, P]
+        L.or_diag_range.argtypes = [C.c_size_t, C.c_size_t, C.c_int64, P, P]
+        L.or_kadane_score.restype = C.c_int32
+        L.or_kadane_score.argtypes = [P, P, C.c_int64, C.c_size_t, C.c_size_t]
+        L.or_kadane_segment.restype = C.c_int32
+        L.or_kadane_segment.argtypes = [P, P, C.c_int64, C.c_size_t, C.c_size_t, P, P]
+        L.or_score_and_gate.restype = C.c_size_t
+        L.or_score_and_gate.argtypes = [P, C.c_size_t, P, P, P, C.c_size_t, P, C.c_size_t,
+                                        C.c_int32, C.c_int32, C.c_int32, C.c_int, P, P, P]
+        L.or_size_adjusted_gate.restype = C.c_int32
+        L.or_size_adjusted_gate.argtypes = [C.c_int32, C.c_uint64]
+        L.or_ung_pair.argtypes = [P, C.c_size_t, P, C.c_size_t, C.c_int32, C.c_int32, C.c_int32,
+                                  P, P, P, P]
+        L.or_splitmix_next.restype = C.c_uint64
+        L.or_splitmix_next.argtypes = [P]
+        L.or_background_sequence.argtypes = [P, C.c_size_t, P]
+        L.or_blosum62.argtypes = [P, P, P, P]
+        L.or_seq_rows.argtypes = [P, C.c_size_t, P]
+        L.or_aa2num.argtypes = [P]
+        _LIB = L
+    return _LIB
+
+
+def _p(a: np.ndarray):
+    return a.ctypes.data_as(C.c_void_p)
+
+
+def _rows(rows) -> np.ndarray:
+    r = np.ascontiguousarray(np.asarray(rows, dtype=np.int32).reshape(-1, NPROF))
+    return r
+
+
+def _seq(t) -> np.ndarray:
+    return np.ascontiguousarray(np.asarray(t, dtype=np.uint8).reshape(-1))
+
+
+# ---- alphabet / matrix -----------------------------------------------------
+
+def aa2num() -> np.ndarray:
+    out = np.zeros(256, dtype=np.uint8)
+    lib().or_aa2num(_p(out))
+    return out
+
+
+def blosum62():
+    """(score int8[21,21], p_back f64[21], R f32[20,20], lambda)."""
+    s = np.zeros((21, 21), dtype=np.int8)
+    pb = np.zeros(21, dtype=np.float64)
+    r = np.zeros((20, 20), dtype=np.float32)
+    lam = C.c_double()
+    lib().or_blosum62(_p(s), _p(pb), _p(r), C.byref(lam))
+    return s, pb, r, lam.value
+
+
+def seq_rows(q) -> np.ndarray:
+    q = _seq(q)
+    rows = np.zeros((len(q), NPROF), dtype=np.int32)
+    lib().or_seq_rows(_p(q), len(q), _p(rows))
+    return rows
+
+
+def encode(text: str) -> np.ndarray:
+    t = aa2num()
+    return t[np.frombuffer(text.encode(), dtype=np.uint8)]
+
+
+# ---- SWG ------------------------------------------------------------------
+
+def gotoh_score_end(rows, t, open_: int, ext: int):
+    """(score, q_end, t_end) with -1 ends when the score is 0."""
+    r, s = _rows(rows), _seq(t)
+    qe, te = C.c_int64(), C.c_int64()
+    sc = lib().or_gotoh_score_end(_p(r), len(r), _p(s), len(s), open_, ext,
+                                  C.byref(qe), C.byref(te))
+    return sc, qe.value, te.value
+
+
+def gotoh_score(rows, t, open_: int, ext: int) -> int:
+    return gotoh_score_end(rows, t, open_, ext)[0]
+
+
+def gotoh_textbook(rows, t, open_: int, ext: int):
+    r, s = _rows(rows), _seq(t)
+    qe, te = C.c_int64(), C.c_int64()
+    sc = lib().or_gotoh_textbook(_p(r), len(r), _p(s), len(s), open_, ext,
+                                 C.byref(qe), C.byref(te))
+    return sc, qe.value, te.value
+
+
+# ---- ungapped / gate ---------------------------------------------------------
+
+def diag_range(m: int, n: int, d: int):
+    lo, hi = C.c_size_t(), C.c_size_t()
+    lib().or_diag_range(m, n, d, C.byref(lo), C.byref(hi))
+    return lo.value, hi.value
+
+
+def kadane_score(rows, t, d: int, lo: int, hi: int) -> int:
+    r, s = _rows(rows), _seq(t)
+    return lib().or_kadane_score(_p(r), _p(s), d, lo, hi)
+
+
+def kadane_segment(rows, t, d: int, lo: int, hi: int):
+    r, s = _rows(rows), _seq(t)
+    qs, qe = C.c_size_t(), C.c_size_t()
+    sc = lib().or_kadane_segment(_p(r), _p(s), d, lo, hi, C.byref(qs), C.byref(qe))
+    return sc, qs.value, qe.value
+
+
+JOB_DT = np.dtype([("tid", "<u4"), ("diag", "<i4"), ("lo", "<u4"), ("hi", "<u4")])
+REC_DT = np.dtype([("tid", "<u4"), ("hits", "<u4"), ("j0", "<u4"), ("j1", "<u4")])
+CAND_DT = np.dtype([("target_id", "<u4"), ("hits", "<u4"), ("best_diag", "<i8"),
+                    ("diag_score", "<i4"), ("t_lo", "<u4"), ("t_hi", "<u4")], align=True)
+
+
+def score_and_gate(rows, arena, offsets, jobs, recs, open_, ext, min_diag_score, regions=False):
+    """(candidates structured array, chains_used, per-job scores)."""
+    r = _rows(rows)
+    arena = _seq(arena)
+    offsets = np.ascontiguousarray(offsets, dtype=np.uint64)
+    jobs = np.ascontiguousarray(jobs, dtype=JOB_DT)
+    recs = np.ascontiguousarray(recs, dtype=REC_DT)
+    out = np.zeros(max(len(recs), 1), dtype=CAND_DT)
+    scores = np.zeros(max(len(jobs), 1), dtype=np.int32)
+    chains = C.c_uint64(0)
+    nc = lib().or_score_and_gate(_p(r), len(r), _p(arena), _p(offsets), _p(jobs), len(jobs),
+                                 _p(recs), len(recs), open_, ext, min_diag_score,
+                                 1 if regions else 0, _p(out), C.byref(chains), _p(scores))
+    return out[:nc].copy(), chains.value, scores[:len(jobs)].copy()
+
+
+def size_adjusted_gate(gate: int, db_residues: int) -> int:
+    return lib().or_size_adjusted_gate(gate, db_residues)
+
+
+def ung_pair(rows, t, open_, ext, min_diag_score):
+    """(ung, best_diag, gate_score, pass) of the all-diagonal evidence."""
+    r, s = _rows(rows), _seq(t)
+    a, b, c, d = C.c_int32(), C.c_int32(), C.c_int32(), C.c_int32()
+    lib().or_ung_pair(_p(r), len(r), _p(s), len(s), open_, ext, min_diag_score,
+                      C.byref(a), C.byref(b), C.byref(c), C.byref(d))
+    return a.value, b.value, c.value, bool(d.value)
+
+
+# ---- RNG ----------------------------------------------------------------------
+
+class SplitMix64:
+    def __init__(self, seed: int):
+        self._s = C.c_uint64(seed)
+
+    def next_u64(self) -> int:
+        return lib().or_splitmix_next(C.byref(self._s))
+
+    def background(self, n: int) -> np.ndarray:
+        out = np.zeros(n, dtype=np.uint8)
+        lib().or_background_sequence(C.byref(self._s), n, _p(out))
+        return out

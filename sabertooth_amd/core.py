@@ -1,0 +1,282 @@
+"""Python mirror of the reference's scoring-path API over the gfx950 library.
+
+Names follow the reference (raw-lab/sabertooth v0.4.0):
+
+* ``GapCosts`` — src/align.rs:196-223
+* ``QueryProfile.from_rows / sw_score / sw_score_with`` — src/simd.rs:128-292
+* ``TargetDb`` — the resident residue arena + offsets (src/index.rs:10-13)
+* ``score_batch`` / ``kadane jobs`` — src/ungapped.rs:285-386
+* ``score_and_gate`` / ``Candidate`` — src/prefilter.rs:370-381, 1498-1660
+* ``size_adjusted_gate`` — src/prefilter.rs:246-257
+
+Every call runs on the GPU through the C ABI in ``include/sabertooth_amd.h``.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from dataclasses import dataclass
+from typing import Iterable, Sequence
+
+import numpy as np
+
+from . import _lib
+from ._lib import CAND_DT, JOB_DT, PAIR_RESULT_DT, REC_DT, check, lib, ptr
+
+NPROF = 21
+
+
+@dataclass(frozen=True)
+class GapCosts:
+    """A gap of length L costs open + (L-1)*extend (src/align.rs:196-223)."""
+
+    open: int
+    extend: int
+
+    @staticmethod
+    def sequence_default() -> "GapCosts":
+        return GapCosts(11, 1)
+
+    @staticmethod
+    def profile_default() -> "GapCosts":
+        return GapCosts(44, 4)
+
+    def cost(self, length: int) -> int:
+        return 0 if length == 0 else self.open + (length - 1) * self.extend
+
+
+def blosum62():
+    """(score int8[21,21], p_back f64[21], R f32[20,20], lambda)."""
+    s = np.zeros((21, 21), dtype=np.int8)
+    pb = np.zeros(21, dtype=np.float64)
+    r = np.zeros((20, 20), dtype=np.float32)
+    lam = C.c_double()
+    check(lib().st_blosum62(ptr(s), ptr(pb), ptr(r), C.byref(lam)))
+    return s, pb, r, lam.value
+
+
+def aa2num() -> np.ndarray:
+    t = np.zeros(256, dtype=np.uint8)
+    check(lib().st_aa2num(ptr(t)))
+    return t
+
+
+def encode(text: str) -> np.ndarray:
+    return aa2num()[np.frombuffer(text.encode(), dtype=np.uint8)]
+
+
+def seq_rows(query) -> np.ndarray:
+    q = np.ascontiguousarray(query, dtype=np.uint8)
+    rows = np.zeros((len(q), NPROF), dtype=np.int32)
+    check(lib().st_seq_rows(ptr(q), len(q), ptr(rows)))
+    return rows
+
+
+def size_adjusted_gate(gate: int, db_residues: int) -> int:
+    return int(lib().st_size_adjusted_gate(gate, db_residues))
+
+
+class Context:
+    """One GPU context (device ordinal); owns a stream."""
+
+    def __init__(self, device: int = 0):
+        h = C.c_void_p()
+        check(lib().st_ctx_create(device, C.byref(h)))
+        self._h = h
+
+    @property
+    def handle(self):
+        return self._h
+
+    def close(self):
+        if self._h:
+            lib().st_ctx_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class TargetDb:
+    """Resident target database: residues[offsets[t] .. offsets[t+1])."""
+
+    def __init__(self, ctx: Context, residues: np.ndarray, offsets: np.ndarray):
+        self.ctx = ctx
+        r = np.ascontiguousarray(residues, dtype=np.uint8)
+        o = np.ascontiguousarray(offsets, dtype=np.uint64)
+        h = C.c_void_p()
+        check(lib().st_db_create(ctx.handle, ptr(r), len(r), ptr(o), len(o) - 1, C.byref(h)))
+        self._h = h
+        self.lengths = (o[1:] - o[:-1]).astype(np.int64)
+
+    @staticmethod
+    def from_sequences(ctx: Context, seqs: Sequence[np.ndarray]) -> "TargetDb":
+        offs = np.zeros(len(seqs) + 1, dtype=np.uint64)
+        if len(seqs):
+            offs[1:] = np.cumsum([len(s) for s in seqs])
+        arena = np.concatenate([np.asarray(s, dtype=np.uint8) for s in seqs]) if len(seqs) else np.zeros(0, np.uint8)
+        return TargetDb(ctx, arena, offs)
+
+    @property
+    def handle(self):
+        return self._h
+
+    def __len__(self):
+        return int(lib().st_db_targets(self._h))
+
+    def residues(self) -> int:
+        return int(lib().st_db_residues(self._h))
+
+    def close(self):
+        if self._h:
+            lib().st_db_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class QueryProfile:
+    """A query (sequence rows or PSSM rows) prepared for repeated scoring."""
+
+    def __init__(self, ctx: Context, rows: np.ndarray):
+        self.ctx = ctx
+        r = np.ascontiguousarray(np.asarray(rows, dtype=np.int32).reshape(-1, NPROF))
+        h = C.c_void_p()
+        check(lib().st_qprof_create(ctx.handle, ptr(r), len(r), C.byref(h)))
+        self._h = h
+        self._rows = r
+
+    @staticmethod
+    def from_rows(ctx: Context, rows) -> "QueryProfile":
+        return QueryProfile(ctx, rows)
+
+    @staticmethod
+    def from_sequence(ctx: Context, query) -> "QueryProfile":
+        return QueryProfile(ctx, seq_rows(query))
+
+    @property
+    def handle(self):
+        return self._h
+
+    def query_len(self) -> int:
+        return int(lib().st_qprof_len(self._h))
+
+    def rows(self) -> np.ndarray:
+        return self._rows
+
+    def sw_score_with(self, db: TargetDb, tids, gaps: GapCosts):
+        """(scores int32[], kernels uint8[]) for target ids `tids`."""
+        t = np.ascontiguousarray(tids, dtype=np.uint32)
+        sc = np.zeros(len(t), dtype=np.int32)
+        kn = np.zeros(len(t), dtype=np.uint8)
+        check(lib().st_sw_score_batch(self.ctx.handle, self._h, db.handle, ptr(t), len(t),
+                                      gaps.open, gaps.extend, ptr(sc), ptr(kn)))
+        return sc, kn
+
+    def sw_score_batch(self, db: TargetDb, tids, gaps: GapCosts) -> np.ndarray:
+        return self.sw_score_with(db, tids, gaps)[0]
+
+    def sw_score(self, target, gaps: GapCosts) -> int:
+        """Score one target slice (uploads a one-sequence database)."""
+        db = TargetDb.from_sequences(self.ctx, [np.asarray(target, dtype=np.uint8)])
+        return int(self.sw_score_batch(db, [0], gaps)[0])
+
+    def close(self):
+        if self._h:
+            lib().st_qprof_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def jobs_array(jobs: Iterable) -> np.ndarray:
+    """(tid, diag, lo, hi) tuples -> the C ABI's st_job array."""
+    return np.array([tuple(j) for j in jobs], dtype=JOB_DT) if not isinstance(jobs, np.ndarray) \
+        else np.ascontiguousarray(jobs, dtype=JOB_DT)
+
+
+def score_batch(qp: QueryProfile, db: TargetDb, jobs) -> np.ndarray:
+    """kadane_score of every job (ungapped::score_batch, src/ungapped.rs:285)."""
+    j = jobs_array(jobs)
+    out = np.zeros(len(j), dtype=np.int32)
+    check(lib().st_ungapped_score_jobs(qp.ctx.handle, qp.handle, db.handle, ptr(j), len(j), ptr(out)))
+    return out
+
+
+def score_and_gate(qp: QueryProfile, db: TargetDb, jobs, recs, gaps: GapCosts,
+                   min_diag_score: int, regions: bool = False):
+    """(candidates, chains_used, job_scores) as score_and_gate produces them."""
+    j = jobs_array(jobs)
+    r = np.ascontiguousarray(recs, dtype=REC_DT) if isinstance(recs, np.ndarray) else \
+        np.array([tuple(x) for x in recs], dtype=REC_DT)
+    out = np.zeros(max(len(r), 1), dtype=CAND_DT)
+    nout = C.c_uint64(0)
+    chains = C.c_uint64(0)
+    js = np.zeros(max(len(j), 1), dtype=np.int32)
+    check(lib().st_score_and_gate(qp.ctx.handle, qp.handle, db.handle, ptr(j), len(j), ptr(r), len(r),
+                                  gaps.open, gaps.extend, min_diag_score, 1 if regions else 0, ptr(out),
+                                  C.byref(nout), C.byref(chains), ptr(js)))
+    return out[: nout.value].copy(), chains.value, js[: len(j)].copy()
+
+
+class Batch:
+    """Many queries x candidate lists, planned once, run on the device."""
+
+    def __init__(self, ctx: Context, db: TargetDb, queries: Sequence[QueryProfile], cand_lists,
+                 gaps: Sequence[GapCosts], gates: Sequence[int]):
+        nq = len(queries)
+        self.ctx, self.db, self.queries = ctx, db, list(queries)
+        off = np.zeros(nq + 1, dtype=np.uint64)
+        off[1:] = np.cumsum([len(c) for c in cand_lists]) if nq else []
+        tids = (np.concatenate([np.asarray(c, dtype=np.uint32) for c in cand_lists])
+                if nq else np.zeros(0, np.uint32))
+        tids = np.ascontiguousarray(tids, dtype=np.uint32)
+        op = np.array([g.open for g in gaps], dtype=np.int32)
+        ex = np.array([g.extend for g in gaps], dtype=np.int32)
+        gt = np.array(gates, dtype=np.int32)
+        qh = (C.c_void_p * max(nq, 1))(*[q.handle.value for q in queries])
+        h = C.c_void_p()
+        check(lib().st_batch_create(ctx.handle, db.handle, qh, nq, ptr(off), ptr(tids), ptr(op), ptr(ex),
+                                    ptr(gt), C.byref(h)))
+        self._h = h
+        self.offsets = off
+        self.npairs = int(off[-1]) if nq else 0
+
+    def run(self) -> None:
+        check(lib().st_batch_run(self._h))
+
+    def results(self) -> np.ndarray:
+        out = np.zeros(max(self.npairs, 1), dtype=PAIR_RESULT_DT)
+        check(lib().st_batch_results(self._h, ptr(out)))
+        return out[: self.npairs]
+
+    def cells(self) -> int:
+        return int(lib().st_batch_cells(self._h))
+
+    def timing(self):
+        """(dp_ms, rescore_ms, gate_ms, rescored_pairs) of the last run."""
+        ms = np.zeros(3, dtype=np.float64)
+        n = C.c_uint64(0)
+        check(lib().st_batch_timing(self._h, ptr(ms), C.byref(n)))
+        return float(ms[0]), float(ms[1]), float(ms[2]), int(n.value)
+
+    def close(self):
+        if self._h:
+            lib().st_batch_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -1,0 +1,930 @@
+// Host side of the C ABI: device context, resident target arena, query
+// profiles, work planning (length buckets, launch shapes) and the launches.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <mutex>
+#include <numeric>
+#include <string>
+#include <vector>
+
+#include "../../include/sabertooth_amd.h"
+#include "st_device.h"
+
+namespace st {
+
+// ---- kernel entry points (dp_kernels.hip, gate_kernels.hip) -----------------
+struct DpLaunch {
+  const Unit *units;
+  uint32_t nunits;
+  uint32_t *counter;
+  const PairDev *pairs;
+  const QueryDev *queries;
+  const uint8_t *arena;
+  const uint64_t *toff;
+  const uint32_t *tlen;
+  int32_t open, ext;
+  uint4 *passbuf;
+  uint64_t passbuf_half;
+  PairOut *pout;
+  int32_t *diag_out;
+  uint32_t grid;
+};
+hipError_t launch_dp(const DpLaunch &a, int kind, int R, bool ung, hipStream_t s);
+
+struct GateRes {
+  int32_t score, best, best_diag;
+  uint32_t t_lo, t_hi;
+  uint8_t pass, chain, pad0, pad1;
+};
+struct JobDev {
+  uint32_t tid;
+  int32_t diag;
+  uint32_t lo, hi;
+};
+struct RecDev {
+  uint32_t tid, hits, j0, j1;
+};
+struct PairGateArgs {
+  const PairDev *pairs;
+  uint64_t npairs;
+  const uint32_t *pair_query;
+  const QueryDev *queries;
+  const uint8_t *arena;
+  const uint64_t *toff;
+  const uint32_t *tlen;
+  const int32_t *diag;
+  const int32_t *open, *ext, *gate;
+  PairOut *pout;
+  void *res;
+};
+hipError_t launch_ung_jobs(const JobDev *jobs, uint64_t njobs, const QueryDev *qd,
+                           const uint8_t *arena, const uint64_t *toff, int32_t *out,
+                           hipStream_t s);
+hipError_t launch_gate_recs(const RecDev *recs, uint64_t nrecs, const JobDev *jobs,
+                            const int32_t *scores, const QueryDev *qd, const uint8_t *arena,
+                            const uint64_t *toff, int32_t open, int32_t ext, int32_t gate,
+                            int regions, GateRes *out, hipStream_t s);
+hipError_t launch_gate_pairs(const PairGateArgs &a, hipStream_t s);
+
+// ---- errors -------------------------------------------------------------------
+static thread_local std::string g_err;
+
+static int fail(int code, const std::string &msg) {
+  g_err = msg;
+  return code;
+}
+
+#define ST_HIP(call)                                                                   \
+  do {                                                                                 \
+    hipError_t e_ = (call);                                                            \
+    if (e_ != hipSuccess)                                                              \
+      return fail(ST_ERR_DEVICE, std::string(#call) + ": " + hipGetErrorString(e_)); \
+  } while (0)
+
+// Device allocation owned by a host object.
+template <class T>
+struct DevBuf {
+  T *p = nullptr;
+  size_t n = 0;
+  DevBuf() = default;
+  DevBuf(const DevBuf &) = delete;
+  DevBuf &operator=(const DevBuf &) = delete;
+  ~DevBuf() { release(); }
+  void release() {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    n = 0;
+  }
+  hipError_t alloc(size_t count) {
+    release();
+    n = count;
+    if (count == 0) return hipSuccess;
+    return hipMalloc((void **)&p, count * sizeof(T));
+  }
+  hipError_t upload(const T *src, size_t count, hipStream_t s) {
+    hipError_t e = alloc(count);
+    if (e != hipSuccess || count == 0) return e;
+    e = hipMemcpyAsync(p, src, count * sizeof(T), hipMemcpyHostToDevice, s);
+    if (e != hipSuccess) return e;
+    return hipStreamSynchronize(s);
+  }
+};
+
+}  // namespace st
+
+using namespace st;
+
+struct st_ctx {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  std::mutex mu;
+  int cus = 256;
+  DevBuf<uint32_t> counter;
+};
+
+struct st_db {
+  st_ctx *ctx = nullptr;
+  DevBuf<uint8_t> arena;
+  DevBuf<uint64_t> off;
+  DevBuf<uint32_t> len;
+  std::vector<uint32_t> hlen;
+  uint64_t n_targets = 0, n_res = 0;
+};
+
+struct st_qprof {
+  st_ctx *ctx = nullptr;
+  std::vector<int32_t> rows;
+  uint32_t m = 0, ld = 0;
+  bool fits16 = false;
+  DevBuf<int16_t> prof16;
+  DevBuf<int32_t> prof32;
+  DevBuf<QueryDev> qd;  // a one-entry table for single-query calls
+  QueryDev view{};
+};
+
+// ---- host inputs -----------------------------------------------------------------
+
+namespace st {
+// MMseqs2 blosum62.out (as shipped in the reference's src/data/blosum62.out):
+// half-bit log-odds in residue-code order, background and lambda.
+static const double kB62Back[21] = {0.07422, 0.02469, 0.05363, 0.05431, 0.04742, 0.07415, 0.02621,
+                                    0.06792, 0.05815, 0.09891, 0.02499, 0.04465, 0.03854, 0.03426,
+                                    0.05161, 0.05723, 0.05089, 0.07292, 0.01303, 0.03228, 0.00001};
+static const double kB62Lambda = 0.34657;
+static const double kB62Half[21][21] = {
+#include "blosum62_halfbits.inc"
+};
+}  // namespace st
+
+extern "C" {
+
+const char *st_last_error(void) { return st::g_err.c_str(); }
+const char *st_version(void) { return "sabertooth_amd 0.1.0 (gfx950)"; }
+
+int st_blosum62(int8_t *score, double *p_back, float *r, double *lambda) {
+  // SubstitutionMatrix::from_out_str (src/matrix.rs:56-175) on the embedded data
+  double pb[21];
+  std::memcpy(pb, kB62Back, sizeof pb);
+  bool x_pos = false;
+  for (int j = 0; j < 21; j++)
+    if (kB62Half[20][j] > 0.0 || kB62Half[j][20] > 0.0) x_pos = true;
+  if (!x_pos) {
+    pb[20] = 1e-5;
+    for (int i = 0; i < 20; i++) pb[i] *= 1.0 - pb[20];
+  }
+  double prob[21][21];
+  for (int i = 0; i < 21; i++)
+    for (int j = 0; j < 21; j++)
+      prob[i][j] = std::exp(kB62Lambda * (double)kB62Half[i][j]) * pb[i] * pb[j];
+  if (r)
+    for (int a = 0; a < 20; a++)
+      for (int b = 0; b < 20; b++) r[a * 20 + b] = (float)(prob[a][b] / pb[b]);
+  if (score)
+    for (int i = 0; i < 21; i++)
+      for (int j = 0; j < 21; j++) {
+        const double v = 2.0 * std::log2(prob[i][j] / (pb[i] * pb[j])) - 0.2;
+        double rd = v < 0.0 ? v - 0.5 : v + 0.5;
+        rd = std::min(127.0, std::max(-128.0, rd));
+        score[i * 21 + j] = (int8_t)rd;
+      }
+  if (p_back) std::memcpy(p_back, pb, sizeof pb);
+  if (lambda) *lambda = kB62Lambda;
+  return ST_OK;
+}
+
+int st_aa2num(uint8_t table[256]) {
+  if (!table) return fail(ST_ERR_INVALID, "st_aa2num: null table");
+  static const char kLetters[] = "ACDEFGHIKLMNPQRSTVWYX";
+  for (int i = 0; i < 256; i++) table[i] = 20;
+  for (int k = 0; k < 21; k++) {
+    const unsigned char c = (unsigned char)kLetters[k];
+    table[c] = (uint8_t)k;
+    table[c - 'A' + 'a'] = (uint8_t)k;
+  }
+  return ST_OK;
+}
+
+int st_seq_rows(const uint8_t *query, uint64_t m, int32_t *rows) {
+  if ((!query || !rows) && m) return fail(ST_ERR_INVALID, "st_seq_rows: null pointer");
+  int8_t mat[21 * 21];
+  st_blosum62(mat, nullptr, nullptr, nullptr);
+  for (uint64_t i = 0; i < m; i++) {
+    const int qa = query[i] < 21 ? query[i] : 20;
+    for (int r = 0; r < 21; r++) rows[i * 21 + r] = mat[qa * 21 + r];
+  }
+  return ST_OK;
+}
+
+int32_t st_size_adjusted_gate(int32_t gate, uint64_t db_residues) {
+  // src/prefilter.rs:246-257
+  const double x = std::log((double)std::max<uint64_t>(db_residues, 1) / 1.0e8);
+  const double slope = x < 0.0 ? 2.5 : 1.0;
+  const double v = std::round((double)gate + slope * x);  // half away from zero
+  return std::min(60, std::max(20, (int32_t)v));
+}
+
+// ---- context -------------------------------------------------------------------------
+
+int st_ctx_create(int device, st_ctx **out) {
+  if (!out) return fail(ST_ERR_INVALID, "st_ctx_create: null out");
+  *out = nullptr;
+  int n = 0;
+  if (hipGetDeviceCount(&n) != hipSuccess || n == 0)
+    return fail(ST_ERR_DEVICE, "st_ctx_create: no HIP device available");
+  if (device < 0 || device >= n) return fail(ST_ERR_INVALID, "st_ctx_create: bad device ordinal");
+  auto *c = new st_ctx();
+  c->device = device;
+  if (hipSetDevice(device) != hipSuccess) {
+    delete c;
+    return fail(ST_ERR_DEVICE, "st_ctx_create: hipSetDevice failed");
+  }
+  hipDeviceProp_t prop;
+  if (hipGetDeviceProperties(&prop, device) == hipSuccess) c->cus = prop.multiProcessorCount;
+  if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
+    delete c;
+    return fail(ST_ERR_DEVICE, "st_ctx_create: stream creation failed");
+  }
+  if (c->counter.alloc(4) != hipSuccess) {
+    (void)hipStreamDestroy(c->stream);
+    delete c;
+    return fail(ST_ERR_NOMEM, "st_ctx_create: counter allocation failed");
+  }
+  *out = c;
+  return ST_OK;
+}
+
+int st_ctx_destroy(st_ctx *ctx) {
+  if (!ctx) return ST_OK;
+  (void)hipSetDevice(ctx->device);
+  (void)hipStreamSynchronize(ctx->stream);
+  ctx->counter.release();
+  (void)hipStreamDestroy(ctx->stream);
+  delete ctx;
+  return ST_OK;
+}
+
+// ---- target database ------------------------------------------------------------------
+
+int st_db_create(st_ctx *ctx, const uint8_t *residues, uint64_t n_residues, const uint64_t *offsets,
+                 uint64_t n_targets, st_db **out) {
+  if (!ctx || !out || !offsets || (!residues && n_residues))
+    return fail(ST_ERR_INVALID, "st_db_create: null argument");
+  *out = nullptr;
+  if (offsets[0] != 0 && n_targets == 0) return fail(ST_ERR_INVALID, "st_db_create: bad offsets");
+  std::vector<uint64_t> doff(n_targets + 1);
+  std::vector<uint32_t> len(n_targets);
+  uint64_t pos = 0;
+  for (uint64_t t = 0; t < n_targets; t++) {
+    if (offsets[t + 1] < offsets[t] || offsets[t + 1] > n_residues)
+      return fail(ST_ERR_INVALID, "st_db_create: offsets must be non-decreasing and within the arena");
+    const uint64_t l = offsets[t + 1] - offsets[t];
+    if (l > 0x7fffffffull) return fail(ST_ERR_INVALID, "st_db_create: target longer than 2^31");
+    len[t] = (uint32_t)l;
+    doff[t] = pos;
+    pos += (l + 15) & ~15ull;
+  }
+  doff[n_targets] = pos;
+  // repack: 16-byte aligned starts, codes >= 21 -> 21, padding 21
+  std::vector<uint8_t> host(pos + 64, (uint8_t)kSentinel);
+  for (uint64_t t = 0; t < n_targets; t++) {
+    const uint8_t *src = residues + offsets[t];
+    uint8_t *dst = host.data() + doff[t];
+    for (uint32_t k = 0; k < len[t]; k++) dst[k] = src[k] < kSentinel ? src[k] : (uint8_t)kSentinel;
+  }
+  std::lock_guard<std::mutex> lk(ctx->mu);
+  ST_HIP(hipSetDevice(ctx->device));
+  auto *db = new st_db();
+  db->ctx = ctx;
+  db->n_targets = n_targets;
+  db->n_res = n_residues;
+  db->hlen = len;
+  if (db->arena.upload(host.data(), host.size(), ctx->stream) != hipSuccess ||
+      db->off.upload(doff.data(), doff.size(), ctx->stream) != hipSuccess ||
+      db->len.upload(len.data(), len.size(), ctx->stream) != hipSuccess) {
+    delete db;
+    return fail(ST_ERR_NOMEM, "st_db_create: device allocation/upload failed");
+  }
+  if (n_targets == 0) (void)db->len.alloc(1);
+  *out = db;
+  return ST_OK;
+}
+
+int st_db_destroy(st_db *db) {
+  delete db;
+  return ST_OK;
+}
+uint64_t st_db_targets(const st_db *db) { return db ? db->n_targets : 0; }
+uint64_t st_db_residues(const st_db *db) { return db ? db->n_res : 0; }
+
+// ---- query profiles ---------------------------------------------------------------------
+
+int st_qprof_create(st_ctx *ctx, const int32_t *rows, uint64_t m, st_qprof **out) {
+  if (!ctx || !out || (!rows && m)) return fail(ST_ERR_INVALID, "st_qprof_create: null argument");
+  if (m > 0x7fffffffull) return fail(ST_ERR_INVALID, "st_qprof_create: query too long");
+  *out = nullptr;
+  auto *qp = new st_qprof();
+  qp->ctx = ctx;
+  qp->m = (uint32_t)m;
+  qp->rows.assign(rows, rows + m * 21);
+  qp->ld = (uint32_t)std::max<uint64_t>(kMaxPassRows, (m + kMaxPassRows - 1) / kMaxPassRows * kMaxPassRows);
+  bool fits = true;
+  for (uint64_t k = 0; k < m * 21; k++)
+    if (rows[k] < -32768 || rows[k] > 32767) fits = false;
+  qp->fits16 = fits;
+  // code-major profiles [22][ld]: code 21 and rows >= m are zero
+  std::vector<int32_t> p32((size_t)kCodes * qp->ld, 0);
+  for (uint64_t i = 0; i < m; i++)
+    for (int c = 0; c < 21; c++) p32[(size_t)c * qp->ld + i] = rows[i * 21 + c];
+  std::lock_guard<std::mutex> lk(ctx->mu);
+  if (hipSetDevice(ctx->device) != hipSuccess ||
+      qp->prof32.upload(p32.data(), p32.size(), ctx->stream) != hipSuccess) {
+    delete qp;
+    return fail(ST_ERR_NOMEM, "st_qprof_create: device upload failed");
+  }
+  if (fits) {
+    std::vector<int16_t> p16(p32.size());
+    for (size_t k = 0; k < p32.size(); k++) p16[k] = (int16_t)p32[k];
+    if (qp->prof16.upload(p16.data(), p16.size(), ctx->stream) != hipSuccess) {
+      delete qp;
+      return fail(ST_ERR_NOMEM, "st_qprof_create: device upload failed");
+    }
+  }
+  qp->view.prof16 = qp->prof16.p;
+  qp->view.prof32 = qp->prof32.p;
+  qp->view.m = qp->m;
+  qp->view.ld = qp->ld;
+  if (qp->qd.upload(&qp->view, 1, ctx->stream) != hipSuccess) {
+    delete qp;
+    return fail(ST_ERR_NOMEM, "st_qprof_create: device upload failed");
+  }
+  *out = qp;
+  return ST_OK;
+}
+
+int st_qprof_destroy(st_qprof *qp) {
+  delete qp;
+  return ST_OK;
+}
+uint64_t st_qprof_len(const st_qprof *qp) { return qp ? qp->m : 0; }
+
+}  // extern "C"
+
+
+// ---- planning ------------------------------------------------------------------------------
+
+namespace st {
+
+// Precision class of a query under given gaps.
+enum Kind { kPk16 = 0, kI32F = 1, kI32N = 2 };
+
+static Kind kind_for(const st_qprof *qp, int32_t open, int32_t ext) {
+  if (ext < 0 || open < 0) return kI32N;
+  if (qp->fits16 && open <= 32767 && ext <= 32767) return kPk16;
+  return kI32F;
+}
+
+struct Shape {
+  int G, R;
+};
+
+// Launch shape for a query of length m against targets of typical length n:
+// minimise the modelled cost per useful cell (row padding to G*R, the skew's
+// G-1 idle steps per pass, the pass-boundary round trip and the fixed work of
+// a step).
+static Shape choose_shape(uint32_t m, double n, Kind kind) {
+  Shape best{16, 16};
+  double bc = 1e300;
+  for (int R = 8; R <= 16; R += 8) {
+    if (kind == kI32N && R != 8) continue;
+    for (int G = 1; G <= 16; G <<= 1) {
+      const int GR = G * R;
+      const uint32_t passes = std::max<uint32_t>(1, (m + GR - 1) / GR);
+      const double m_pad = (double)passes * GR;
+      double c = (m_pad / std::max<uint32_t>(m, 1)) * ((n + G - 1) / std::max(n, 1.0));
+      c *= 1.0 + 1.6 / R;
+      if (passes > 1) c *= 1.0 + 6.0 / GR;
+      if (c < bc) {
+        bc = c;
+        best = Shape{G, R};
+      }
+    }
+  }
+  return best;
+}
+
+// One kernel launch: units [unit0, unit0 + nunits) of one precision kind, one
+// R and one gap setting.
+struct Launch {
+  int kind, R;
+  int32_t open, ext;
+  uint32_t unit0, nunits;
+};
+
+struct Plan {
+  std::vector<Unit> units;
+  std::vector<PairDev> pairs;
+  std::vector<Launch> launches;
+  uint64_t passbuf_half = 1;
+};
+
+// Add the pairs of query qi (result rows outs[], target ids tids[]) to `pl`:
+// targets sorted by length, cut into wave units of 64/G groups.
+static void plan_query(Plan &pl, std::vector<Unit> &pending, uint32_t qi, const st_qprof *qp,
+                       const uint32_t *tids, const uint64_t *outs, const uint64_t *diag0, size_t count,
+                       Kind kind, const st_db *db) {
+  if (count == 0) return;
+  std::vector<uint32_t> order(count);
+  std::iota(order.begin(), order.end(), 0u);
+  std::stable_sort(order.begin(), order.end(),
+                   [&](uint32_t a, uint32_t b) { return db->hlen[tids[a]] < db->hlen[tids[b]]; });
+  const double nmed = (double)db->hlen[tids[order[count / 2]]];
+  const Shape sh = choose_shape(qp->m, nmed, kind);
+  const int per = kind == kPk16 ? 2 : 1;
+  const uint32_t cap = (uint32_t)(64 / sh.G * per);
+  const uint32_t GR = (uint32_t)(sh.G * sh.R);
+  const uint32_t passes = std::max<uint32_t>(1, (qp->m + GR - 1) / GR);
+  for (size_t s = 0; s < count; s += cap) {
+    const size_t e = std::min(count, s + cap);
+    Unit u{};
+    u.query = qi;
+    u.pair0 = (uint32_t)pl.pairs.size();
+    u.npairs = (uint32_t)(e - s);
+    u.G = (uint32_t)sh.G;
+    u.R = (uint32_t)sh.R;
+    u.passes = passes;
+    uint32_t np = 0;
+    for (size_t k = s; k < e; k++) {
+      const uint32_t o = order[k];
+      PairDev pd;
+      pd.tid = tids[o];
+      pd.out = (uint32_t)outs[o];
+      pd.diag0 = diag0 ? diag0[o] : 0;
+      pl.pairs.push_back(pd);
+      np = std::max(np, db->hlen[tids[o]]);
+    }
+    u.n_pad = np;
+    if (passes > 1) pl.passbuf_half = std::max<uint64_t>(pl.passbuf_half, (uint64_t)(64 / sh.G) * np);
+    pending.push_back(u);
+  }
+  (void)kind;
+}
+
+// Group pending units into launches by (kind, R, gaps), longest first.
+static void finish_plan(Plan &pl, std::vector<Unit> &pending, const std::vector<Kind> &qkind,
+                        const std::vector<int32_t> &open, const std::vector<int32_t> &ext, int force_kind) {
+  std::stable_sort(pending.begin(), pending.end(), [&](const Unit &a, const Unit &b) {
+    const int ka = force_kind >= 0 ? force_kind : (int)qkind[a.query];
+    const int kb = force_kind >= 0 ? force_kind : (int)qkind[b.query];
+    if (ka != kb) return ka < kb;
+    if (a.R != b.R) return a.R < b.R;
+    if (open[a.query] != open[b.query]) return open[a.query] < open[b.query];
+    if (ext[a.query] != ext[b.query]) return ext[a.query] < ext[b.query];
+    const double ca = (double)a.passes * a.R * (a.n_pad + a.G - 1);
+    const double cb = (double)b.passes * b.R * (b.n_pad + b.G - 1);
+    return ca > cb;
+  });
+  for (size_t i = 0; i < pending.size();) {
+    const Unit &u = pending[i];
+    Launch L;
+    L.kind = force_kind >= 0 ? force_kind : (int)qkind[u.query];
+    L.R = (int)u.R;
+    L.open = open[u.query];
+    L.ext = ext[u.query];
+    L.unit0 = (uint32_t)(pl.units.size());
+    size_t j = i;
+    while (j < pending.size()) {
+      const Unit &v = pending[j];
+      const int kv = force_kind >= 0 ? force_kind : (int)qkind[v.query];
+      if (kv != L.kind || (int)v.R != L.R || open[v.query] != L.open || ext[v.query] != L.ext) break;
+      pl.units.push_back(v);
+      j++;
+    }
+    L.nunits = (uint32_t)(j - i);
+    pl.launches.push_back(L);
+    i = j;
+  }
+  pending.clear();
+}
+
+}  // namespace st
+
+// A planned multi-query batch with every input resident on the device.
+struct st_batch {
+  st_ctx *ctx = nullptr;
+  const st_db *db = nullptr;
+  bool ung = true;
+  uint64_t npairs = 0, cells = 0, ndiag = 0;
+  std::vector<const st_qprof *> qps;
+  std::vector<int32_t> open, ext, gate;
+  std::vector<Kind> qkind;
+  std::vector<uint32_t> cand_tids;
+  std::vector<uint32_t> cand_q;
+  std::vector<uint64_t> diag0;
+  Plan plan;
+  DevBuf<QueryDev> d_queries;
+  DevBuf<int32_t> d_open, d_ext, d_gate;
+  DevBuf<Unit> d_units;
+  DevBuf<PairDev> d_pairs;
+  DevBuf<PairDev> d_allpairs;  // candidate order, for the gate kernel
+  DevBuf<uint32_t> d_candq;
+  DevBuf<PairOut> d_pout;
+  DevBuf<int32_t> d_diag;
+  DevBuf<uint8_t> d_res;
+  DevBuf<uint4> d_passbuf;
+  uint64_t passbuf_elems = 0;
+  uint32_t grid_cap = 0;
+  // exact int32 rescoring of saturated int16 pairs (built at run time)
+  Plan resc;
+  DevBuf<Unit> d_rs_units;
+  DevBuf<PairDev> d_rs_pairs;
+  uint64_t rescored = 0;
+  // device time of the last run: main DP launches, int32 rescoring, gate
+  hipEvent_t ev[6] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
+  double ms[3] = {0, 0, 0};
+  ~st_batch() {
+    for (hipEvent_t &e : ev)
+      if (e) (void)hipEventDestroy(e);
+  }
+};
+
+namespace st {
+
+static uint32_t grid_for(const st_batch *b, uint32_t nunits) {
+  const uint32_t cap = b->grid_cap ? b->grid_cap : (uint32_t)b->ctx->cus * 16u;
+  return std::max<uint32_t>(1, std::min(nunits, cap));
+}
+
+static int ensure_passbuf(st_batch *b, uint64_t half, uint32_t grid) {
+  const uint64_t need = half * 2 * grid;
+  if (need > b->passbuf_elems) {
+    if (b->d_passbuf.alloc(need) != hipSuccess) return fail(ST_ERR_NOMEM, "pass buffer allocation failed");
+    b->passbuf_elems = need;
+  }
+  return ST_OK;
+}
+
+static int launch_plan(st_batch *b, const Plan &pl, const Unit *d_units, const PairDev *d_pairs) {
+  st_ctx *ctx = b->ctx;
+  for (const Launch &L : pl.launches) {
+    DpLaunch a;
+    a.units = d_units + L.unit0;
+    a.nunits = L.nunits;
+    a.counter = ctx->counter.p;
+    a.pairs = d_pairs;
+    a.queries = b->d_queries.p;
+    a.arena = b->db->arena.p;
+    a.toff = b->db->off.p;
+    a.tlen = b->db->len.p;
+    a.open = L.open;
+    a.ext = L.ext;
+    a.grid = grid_for(b, L.nunits);
+    a.passbuf = b->d_passbuf.p;
+    a.passbuf_half = pl.passbuf_half;
+    a.pout = b->d_pout.p;
+    a.diag_out = b->d_diag.p;
+    if (a.passbuf_half * 2 * a.grid > b->passbuf_elems)
+      return fail(ST_ERR_NOMEM, "pass buffer smaller than the plan needs");
+    ST_HIP(launch_dp(a, L.kind, L.R, b->ung, ctx->stream));
+  }
+  return ST_OK;
+}
+
+}  // namespace st
+
+extern "C" {
+
+static int batch_create_impl(st_ctx *ctx, const st_db *db, const st_qprof *const *queries, uint64_t nq,
+                             const uint64_t *cand_off, const uint32_t *cand_tids, const int32_t *gap_open,
+                             const int32_t *gap_extend, const int32_t *min_diag_score, bool ung,
+                             st_batch **out) {
+  if (!ctx || !db || !out || (nq && (!queries || !cand_off || !gap_open || !gap_extend)))
+    return fail(ST_ERR_INVALID, "st_batch_create: null argument");
+  *out = nullptr;
+  if (db->ctx != ctx) return fail(ST_ERR_INVALID, "st_batch_create: database of another context");
+  const uint64_t np = nq ? cand_off[nq] : 0;
+  if (nq && cand_off[0] != 0) return fail(ST_ERR_INVALID, "st_batch_create: cand_off[0] must be 0");
+  if (np && !cand_tids) return fail(ST_ERR_INVALID, "st_batch_create: null candidate list");
+  if (np > 0xffffffffull) return fail(ST_ERR_INVALID, "st_batch_create: more than 2^32 pairs");
+  for (uint64_t q = 0; q < nq; q++) {
+    if (!queries[q] || queries[q]->ctx != ctx) return fail(ST_ERR_INVALID, "st_batch_create: bad query handle");
+    if (cand_off[q + 1] < cand_off[q]) return fail(ST_ERR_INVALID, "st_batch_create: cand_off must not decrease");
+  }
+  for (uint64_t k = 0; k < np; k++)
+    if (cand_tids[k] >= db->n_targets) return fail(ST_ERR_INVALID, "st_batch_create: target id out of range");
+  auto *b = new st_batch();
+  b->ctx = ctx;
+  b->db = db;
+  b->ung = ung;
+  b->npairs = np;
+  b->qps.assign(queries, queries + nq);
+  b->open.assign(gap_open, gap_open + nq);
+  b->ext.assign(gap_extend, gap_extend + nq);
+  if (min_diag_score)
+    b->gate.assign(min_diag_score, min_diag_score + nq);
+  else
+    b->gate.assign(nq, 0);
+  b->cand_tids.assign(cand_tids, cand_tids + np);
+  b->cand_q.resize(np);
+  b->diag0.resize(np);
+  uint64_t dpos = 0, cells = 0;
+  for (uint64_t q = 0; q < nq; q++)
+    for (uint64_t k = cand_off[q]; k < cand_off[q + 1]; k++) {
+      b->cand_q[k] = (uint32_t)q;
+      const uint64_t m = b->qps[q]->m, n = db->hlen[cand_tids[k]];
+      b->diag0[k] = dpos;
+      if (ung && m && n) dpos += m + n - 1;
+      cells += m * n;
+    }
+  b->cells = cells;
+  b->ndiag = dpos;
+  b->qkind.resize(nq);
+  std::vector<Unit> pending;
+  for (uint64_t q = 0; q < nq; q++) {
+    b->qkind[q] = kind_for(b->qps[q], b->open[q], b->ext[q]);
+    const uint64_t s = cand_off[q], e = cand_off[q + 1];
+    std::vector<uint64_t> outs(e - s);
+    std::iota(outs.begin(), outs.end(), s);
+    plan_query(b->plan, pending, (uint32_t)q, b->qps[q], cand_tids + s, outs.data(), b->diag0.data() + s, e - s,
+               b->qkind[q], db);
+  }
+  finish_plan(b->plan, pending, b->qkind, b->open, b->ext, -1);
+  std::lock_guard<std::mutex> lk(ctx->mu);
+  if (hipSetDevice(ctx->device) != hipSuccess) {
+    delete b;
+    return fail(ST_ERR_DEVICE, "st_batch_create: hipSetDevice failed");
+  }
+  std::vector<QueryDev> qv(nq);
+  for (uint64_t q = 0; q < nq; q++) qv[q] = b->qps[q]->view;
+  std::vector<PairDev> all(np);
+  for (uint64_t k = 0; k < np; k++) {
+    all[k].tid = cand_tids[k];
+    all[k].out = (uint32_t)k;
+    all[k].diag0 = b->diag0[k];
+  }
+  // the pass buffer covers the largest plan; rescoring plans reuse it when they fit
+  const uint32_t grid = grid_for(b, (uint32_t)std::max<size_t>(b->plan.units.size(), 1));
+  bool ok = b->d_queries.upload(qv.data(), qv.size(), ctx->stream) == hipSuccess &&
+            b->d_open.upload(b->open.data(), nq, ctx->stream) == hipSuccess &&
+            b->d_ext.upload(b->ext.data(), nq, ctx->stream) == hipSuccess &&
+            b->d_gate.upload(b->gate.data(), nq, ctx->stream) == hipSuccess &&
+            b->d_units.upload(b->plan.units.data(), b->plan.units.size(), ctx->stream) == hipSuccess &&
+            b->d_pairs.upload(b->plan.pairs.data(), b->plan.pairs.size(), ctx->stream) == hipSuccess &&
+            b->d_allpairs.upload(all.data(), all.size(), ctx->stream) == hipSuccess &&
+            b->d_candq.upload(b->cand_q.data(), b->cand_q.size(), ctx->stream) == hipSuccess &&
+            b->d_pout.alloc(std::max<uint64_t>(np, 1)) == hipSuccess &&
+            b->d_diag.alloc(std::max<uint64_t>(dpos, 1)) == hipSuccess &&
+            b->d_res.alloc(std::max<uint64_t>(np, 1) * sizeof(st_pair_result)) == hipSuccess &&
+            ensure_passbuf(b, b->plan.passbuf_half, grid) == ST_OK;
+  if (!ok) {
+    delete b;
+    return fail(ST_ERR_NOMEM, "st_batch_create: device allocation failed");
+  }
+  *out = b;
+  return ST_OK;
+}
+
+int st_batch_create(st_ctx *ctx, const st_db *db, const st_qprof *const *queries, uint64_t nq,
+                    const uint64_t *cand_off, const uint32_t *cand_tids, const int32_t *gap_open,
+                    const int32_t *gap_extend, const int32_t *min_diag_score, st_batch **out) {
+  return batch_create_impl(ctx, db, queries, nq, cand_off, cand_tids, gap_open, gap_extend, min_diag_score,
+                           true, out);
+}
+
+static int batch_run_locked(st_batch *b) {
+  st_ctx *ctx = b->ctx;
+  ST_HIP(hipSetDevice(ctx->device));
+  for (hipEvent_t &e : b->ev)
+    if (!e) ST_HIP(hipEventCreate(&e));
+  ST_HIP(hipEventRecord(b->ev[0], ctx->stream));
+  int rc = launch_plan(b, b->plan, b->d_units.p, b->d_pairs.p);
+  if (rc) return rc;
+  ST_HIP(hipEventRecord(b->ev[1], ctx->stream));
+  // int16 pairs whose best reached 32767 are rescored in exact int32
+  b->resc = Plan();
+  b->rescored = 0;
+  ST_HIP(hipEventRecord(b->ev[2], ctx->stream));
+  ST_HIP(hipEventRecord(b->ev[3], ctx->stream));
+  bool any16 = false;
+  for (const Launch &L : b->plan.launches) any16 |= L.kind == kPk16;
+  if (any16) {
+    std::vector<PairOut> po(b->npairs);
+    ST_HIP(hipMemcpyAsync(po.data(), b->d_pout.p, b->npairs * sizeof(PairOut), hipMemcpyDeviceToHost,
+                          ctx->stream));
+    ST_HIP(hipStreamSynchronize(ctx->stream));
+    std::vector<std::vector<uint64_t>> perq(b->qps.size());
+    for (uint64_t k = 0; k < b->npairs; k++)
+      if (po[k].flag == 1) perq[b->cand_q[k]].push_back(k);
+    std::vector<Unit> pending;
+    for (size_t q = 0; q < perq.size(); q++) {
+      if (perq[q].empty()) continue;
+      std::vector<uint32_t> tids;
+      std::vector<uint64_t> d0;
+      for (uint64_t o : perq[q]) {
+        tids.push_back(b->cand_tids[o]);
+        d0.push_back(b->diag0[o]);
+      }
+      b->rescored += perq[q].size();
+      plan_query(b->resc, pending, (uint32_t)q, b->qps[q], tids.data(), perq[q].data(), d0.data(), tids.size(),
+                 kI32F, b->db);
+    }
+    finish_plan(b->resc, pending, b->qkind, b->open, b->ext, (int)kI32F);
+    if (!b->resc.units.empty()) {
+      if (b->d_rs_units.upload(b->resc.units.data(), b->resc.units.size(), ctx->stream) != hipSuccess ||
+          b->d_rs_pairs.upload(b->resc.pairs.data(), b->resc.pairs.size(), ctx->stream) != hipSuccess)
+        return fail(ST_ERR_NOMEM, "rescoring upload failed");
+      rc = ensure_passbuf(b, b->resc.passbuf_half, grid_for(b, (uint32_t)b->resc.units.size()));
+      if (rc) return rc;
+      ST_HIP(hipEventRecord(b->ev[2], ctx->stream));
+      rc = launch_plan(b, b->resc, b->d_rs_units.p, b->d_rs_pairs.p);
+      if (rc) return rc;
+      ST_HIP(hipEventRecord(b->ev[3], ctx->stream));
+    }
+  }
+  ST_HIP(hipEventRecord(b->ev[4], ctx->stream));
+  if (b->ung && b->npairs) {
+    PairGateArgs ga;
+    ga.pairs = b->d_allpairs.p;
+    ga.npairs = b->npairs;
+    ga.pair_query = b->d_candq.p;
+    ga.queries = b->d_queries.p;
+    ga.arena = b->db->arena.p;
+    ga.toff = b->db->off.p;
+    ga.tlen = b->db->len.p;
+    ga.diag = b->d_diag.p;
+    ga.open = b->d_open.p;
+    ga.ext = b->d_ext.p;
+    ga.gate = b->d_gate.p;
+    ga.pout = b->d_pout.p;
+    ga.res = b->d_res.p;
+    ST_HIP(launch_gate_pairs(ga, ctx->stream));
+  }
+  ST_HIP(hipEventRecord(b->ev[5], ctx->stream));
+  ST_HIP(hipStreamSynchronize(ctx->stream));
+  float t = 0;
+  ST_HIP(hipEventElapsedTime(&t, b->ev[0], b->ev[1]));
+  b->ms[0] = t;
+  ST_HIP(hipEventElapsedTime(&t, b->ev[2], b->ev[3]));
+  b->ms[1] = t;
+  ST_HIP(hipEventElapsedTime(&t, b->ev[4], b->ev[5]));
+  b->ms[2] = t;
+  return ST_OK;
+}
+
+int st_batch_run(st_batch *b) {
+  if (!b) return fail(ST_ERR_INVALID, "st_batch_run: null batch");
+  std::lock_guard<std::mutex> lk(b->ctx->mu);
+  return batch_run_locked(b);
+}
+
+int st_batch_results(st_batch *b, st_pair_result *out) {
+  if (!b || (!out && b->npairs)) return fail(ST_ERR_INVALID, "st_batch_results: null argument");
+  if (!b->npairs) return ST_OK;
+  std::lock_guard<std::mutex> lk(b->ctx->mu);
+  ST_HIP(hipSetDevice(b->ctx->device));
+  if (b->ung) {
+    ST_HIP(hipMemcpyAsync(out, b->d_res.p, b->npairs * sizeof(st_pair_result), hipMemcpyDeviceToHost,
+                          b->ctx->stream));
+    ST_HIP(hipStreamSynchronize(b->ctx->stream));
+  } else {
+    std::vector<PairOut> po(b->npairs);
+    ST_HIP(hipMemcpyAsync(po.data(), b->d_pout.p, b->npairs * sizeof(PairOut), hipMemcpyDeviceToHost,
+                          b->ctx->stream));
+    ST_HIP(hipStreamSynchronize(b->ctx->stream));
+    for (uint64_t k = 0; k < b->npairs; k++) {
+      std::memset(&out[k], 0, sizeof(st_pair_result));
+      out[k].swg = po[k].swg;
+      out[k].kernel = po[k].flag == 2 ? ST_KERNEL_I32 : ST_KERNEL_I16;
+    }
+  }
+  return ST_OK;
+}
+
+int st_batch_timing(const st_batch *b, double *ms3, uint64_t *rescored) {
+  if (!b || !ms3) return fail(ST_ERR_INVALID, "st_batch_timing: null argument");
+  for (int k = 0; k < 3; k++) ms3[k] = b->ms[k];
+  if (rescored) *rescored = b->rescored;
+  return ST_OK;
+}
+
+uint64_t st_batch_cells(const st_batch *b) { return b ? b->cells : 0; }
+uint64_t st_batch_pairs(const st_batch *b) { return b ? b->npairs : 0; }
+
+int st_batch_destroy(st_batch *b) {
+  if (b) {
+    std::lock_guard<std::mutex> lk(b->ctx->mu);
+    (void)hipSetDevice(b->ctx->device);
+    (void)hipStreamSynchronize(b->ctx->stream);
+  }
+  delete b;
+  return ST_OK;
+}
+
+// ---- single-query drop-ins ------------------------------------------------------------------
+
+int st_sw_score_batch(st_ctx *ctx, const st_qprof *qp, const st_db *db, const uint32_t *tids, uint64_t count,
+                      int32_t gap_open, int32_t gap_extend, int32_t *scores, uint8_t *kernel) {
+  if (!ctx || !qp || !db || (count && (!tids || !scores)))
+    return fail(ST_ERR_INVALID, "st_sw_score_batch: null argument");
+  if (count == 0) return ST_OK;
+  const uint64_t off[2] = {0, count};
+  st_batch *b = nullptr;
+  int rc = batch_create_impl(ctx, db, &qp, 1, off, tids, &gap_open, &gap_extend, nullptr, false, &b);
+  if (rc) return rc;
+  rc = st_batch_run(b);
+  if (!rc) {
+    std::vector<st_pair_result> res(count);
+    rc = st_batch_results(b, res.data());
+    if (!rc)
+      for (uint64_t k = 0; k < count; k++) {
+        scores[k] = res[k].swg;
+        if (kernel) kernel[k] = res[k].kernel;
+      }
+  }
+  st_batch_destroy(b);
+  return rc;
+}
+
+int st_ungapped_score_jobs(st_ctx *ctx, const st_qprof *qp, const st_db *db, const st_job *jobs, uint64_t njobs,
+                           int32_t *scores) {
+  if (!ctx || !qp || !db || (njobs && (!jobs || !scores)))
+    return fail(ST_ERR_INVALID, "st_ungapped_score_jobs: null argument");
+  if (njobs == 0) return ST_OK;
+  for (uint64_t k = 0; k < njobs; k++) {
+    const st_job &j = jobs[k];
+    if (j.tid >= db->n_targets) return fail(ST_ERR_INVALID, "st_ungapped_score_jobs: target id out of range");
+    if (j.hi > j.lo) {
+      // the window must lie inside diag_range (src/ungapped.rs:305-309)
+      const int64_t n = db->hlen[j.tid];
+      const int64_t a = std::max<int64_t>(-(int64_t)j.diag, 0);
+      const int64_t e = std::min<int64_t>((int64_t)qp->m, n - j.diag);
+      if ((int64_t)j.lo < a || (int64_t)j.hi > e)
+        return fail(ST_ERR_INVALID, "st_ungapped_score_jobs: job window outside its diagonal");
+    }
+  }
+  std::lock_guard<std::mutex> lk(ctx->mu);
+  ST_HIP(hipSetDevice(ctx->device));
+  DevBuf<JobDev> dj;
+  DevBuf<int32_t> ds;
+  if (dj.upload((const JobDev *)jobs, njobs, ctx->stream) != hipSuccess || ds.alloc(njobs) != hipSuccess)
+    return fail(ST_ERR_NOMEM, "st_ungapped_score_jobs: device allocation failed");
+  ST_HIP(launch_ung_jobs(dj.p, njobs, qp->qd.p, db->arena.p, db->off.p, ds.p, ctx->stream));
+  ST_HIP(hipMemcpyAsync(scores, ds.p, njobs * 4, hipMemcpyDeviceToHost, ctx->stream));
+  ST_HIP(hipStreamSynchronize(ctx->stream));
+  return ST_OK;
+}
+
+int st_score_and_gate(st_ctx *ctx, const st_qprof *qp, const st_db *db, const st_job *jobs, uint64_t njobs,
+                      const st_target_rec *recs, uint64_t nrecs, int32_t gap_open, int32_t gap_extend,
+                      int32_t min_diag_score, int regions, st_candidate *out, uint64_t *n_out,
+                      uint64_t *chains_used, int32_t *job_scores) {
+  if (!ctx || !qp || !db || !n_out || (njobs && !jobs) || (nrecs && (!recs || !out)))
+    return fail(ST_ERR_INVALID, "st_score_and_gate: null argument");
+  *n_out = 0;
+  for (uint64_t r = 0; r < nrecs; r++)
+    if (recs[r].j1 < recs[r].j0 || recs[r].j1 > njobs || recs[r].tid >= db->n_targets)
+      return fail(ST_ERR_INVALID, "st_score_and_gate: record out of range");
+  std::vector<int32_t> sc(njobs);
+  int rc = st_ungapped_score_jobs(ctx, qp, db, jobs, njobs, sc.data());
+  if (rc) return rc;
+  if (job_scores && njobs) std::memcpy(job_scores, sc.data(), njobs * 4);
+  if (nrecs == 0) return ST_OK;
+  std::lock_guard<std::mutex> lk(ctx->mu);
+  ST_HIP(hipSetDevice(ctx->device));
+  DevBuf<JobDev> dj;
+  DevBuf<int32_t> ds;
+  DevBuf<RecDev> dr;
+  DevBuf<GateRes> dg;
+  if (dj.upload((const JobDev *)jobs, njobs, ctx->stream) != hipSuccess ||
+      ds.upload(sc.data(), njobs, ctx->stream) != hipSuccess ||
+      dr.upload((const RecDev *)recs, nrecs, ctx->stream) != hipSuccess || dg.alloc(nrecs) != hipSuccess)
+    return fail(ST_ERR_NOMEM, "st_score_and_gate: device allocation failed");
+  ST_HIP(launch_gate_recs(dr.p, nrecs, dj.p, ds.p, qp->qd.p, db->arena.p, db->off.p, gap_open, gap_extend,
+                          min_diag_score, regions, dg.p, ctx->stream));
+  std::vector<GateRes> res(nrecs);
+  ST_HIP(hipMemcpyAsync(res.data(), dg.p, nrecs * sizeof(GateRes), hipMemcpyDeviceToHost, ctx->stream));
+  ST_HIP(hipStreamSynchronize(ctx->stream));
+  uint64_t nc = 0, chains = 0;
+  for (uint64_t r = 0; r < nrecs; r++) {
+    const GateRes &g = res[r];
+    if (recs[r].j1 == recs[r].j0) continue;
+    if (g.chain) chains++;
+    if (!g.pass) continue;
+    st_candidate c;
+    c.target_id = recs[r].tid;
+    c.hits = recs[r].hits;
+    c.best_diag = g.best_diag;
+    c.diag_score = g.score;
+    c.t_lo = g.t_lo;
+    c.t_hi = g.t_hi;
+    out[nc++] = c;
+  }
+  *n_out = nc;
+  if (chains_used) *chains_used += chains;
+  return ST_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,315 @@
+// Ungapped job scoring, the chain/gate rule and the all-diagonal reduction on
+// gfx950.
+//
+//  * ung_jobs_kernel: kadane_score of explicit (target, diagonal, [lo, hi))
+//    jobs (ungapped::score_batch, src/ungapped.rs:285-386; kadane_score,
+//    src/ungapped.rs:53-62).  One lane per job; a lane streams its diagonal's
+//    residues with 16-byte loads (the arena is 16-byte aligned per target).
+//  * gate_recs_kernel: score_and_gate's per-target chain and gate
+//    (src/prefilter.rs:1577-1657) over already scored jobs.
+//  * gate_pairs_kernel: the seedless evidence of a (query, candidate) pair:
+//    best diagonal, top-4 diagonals (score desc, diagonal asc) and the same
+//    chain/gate rule over their whole ranges.
+#include <hip/hip_runtime.h>
+
+#include "st_device.h"
+
+namespace st {
+
+struct GateRes {
+  int32_t score;      // best single or chain
+  int32_t best;       // best single segment
+  int32_t best_diag;  // diagonal of the best single segment
+  uint32_t t_lo, t_hi;
+  uint8_t pass, chain, pad0, pad1;
+};
+
+__device__ __forceinline__ int32_t sub_of(const int32_t *prof, uint32_t ld, uint32_t code, int q) {
+  return prof[(uint64_t)code * ld + q];  // code 21 rows are zero
+}
+
+// kadane_segment (src/ungapped.rs:69-88) over query rows [lo, hi)
+__device__ int32_t kadane_segment_dev(const int32_t *prof, uint32_t ld, const uint8_t *t,
+                                      int d, int lo, int hi, int *qs, int *qe) {
+  int32_t best = 0, cur = 0;
+  int bs = 0, be = 0, cs = lo;
+  for (int q = lo; q < hi; q++) {
+    const int32_t s = sub_of(prof, ld, t[q + d], q);
+    if (cur <= 0) {
+      cur = s;
+      cs = q;
+    } else {
+      cur += s;
+    }
+    if (cur > best) {
+      best = cur;
+      bs = cs;
+      be = q;
+    }
+  }
+  *qs = bs;
+  *qe = be;
+  return best;
+}
+
+// chain + gate of one target over ns <= 16 scored diagonals
+// (src/prefilter.rs:1577-1657, region 1269-1278)
+__device__ GateRes gate_eval(const int32_t *prof, uint32_t ld, const uint8_t *t,
+                             const int32_t *sc, const int32_t *dg, const int32_t *lo,
+                             const int32_t *hi, int ns, int32_t open, int32_t ext,
+                             int32_t min_diag_score, bool regions) {
+  GateRes r;
+  r.pass = 0;
+  r.chain = 0;
+  r.t_lo = 0;
+  r.t_hi = 0xffffffffu;
+  int32_t best_s = 0;
+  int best_k = 0;
+  for (int k = 0; k < ns; k++)
+    if (sc[k] > best_s) {
+      best_s = sc[k];
+      best_k = k;
+    }
+  r.best = best_s;
+  r.best_diag = ns > 0 ? dg[best_k] : 0;
+  r.score = best_s;
+  if (ns == 0) {
+    r.pass = 0 >= min_diag_score;
+    return r;
+  }
+  if (ns == 1 && sc[0] < min_diag_score) return r;
+  int qs[16], qe[16];
+  uint32_t have = 0;
+  int32_t chain_s = INT32_MIN;
+  int cx = 0, cy = 0;
+  if (ns > 1)
+    for (int x = 0; x < ns; x++)
+      for (int y = 0; y < ns; y++) {
+        if (x == y) continue;
+        const int32_t sx = sc[x], sy = sc[y];
+        const int dx = dg[x], dy = dg[y];
+        if (sx <= 0 || sy <= 0 || dx == dy) continue;
+        const int ad = dy > dx ? dy - dx : dx - dy;
+        const int32_t c = sx + sy - (open + (ad - 1) * ext);
+        if (c <= best_s || c <= chain_s) continue;
+        if (!(have >> x & 1)) {
+          kadane_segment_dev(prof, ld, t, dx, lo[x], hi[x], &qs[x], &qe[x]);
+          have |= 1u << x;
+        }
+        if (!(have >> y & 1)) {
+          kadane_segment_dev(prof, ld, t, dy, lo[y], hi[y], &qs[y], &qe[y]);
+          have |= 1u << y;
+        }
+        if (qe[x] < qs[y] && qe[x] + dx < qs[y] + dy) {
+          chain_s = c;
+          cx = x;
+          cy = y;
+        }
+      }
+  int used0 = best_k, used1 = best_k;
+  if (chain_s > best_s) {
+    r.chain = 1;
+    r.score = chain_s;
+    used0 = cx;
+    used1 = cy;
+  }
+  if (r.score < min_diag_score) return r;
+  r.pass = 1;
+  if (regions) {
+    uint32_t a = 0xffffffffu, b = 0;
+    const int us[2] = {used0, used1};
+    for (int u = 0; u < 2; u++) {
+      const int k = us[u];
+      if (!(have >> k & 1)) {
+        kadane_segment_dev(prof, ld, t, dg[k], lo[k], hi[k], &qs[k], &qe[k]);
+        have |= 1u << k;
+      }
+      const int s0 = qs[k] + dg[k], s1 = qe[k] + dg[k];
+      const uint32_t ua = (uint32_t)(s0 > 0 ? s0 : 0), ub = (uint32_t)(s1 > 0 ? s1 : 0);
+      a = ua < a ? ua : a;
+      b = ub > b ? ub : b;
+    }
+    r.t_lo = a;
+    r.t_hi = b;
+  }
+  return r;
+}
+
+struct JobDev {
+  uint32_t tid;
+  int32_t diag;
+  uint32_t lo, hi;
+};
+struct RecDev {
+  uint32_t tid, hits, j0, j1;
+};
+
+// kadane_score of each job; codes >= 21 were mapped to 21 at upload
+__global__ __launch_bounds__(256) void ung_jobs_kernel(const JobDev *__restrict__ jobs, uint64_t njobs,
+                                                       const QueryDev *__restrict__ qd,
+                                                       const uint8_t *__restrict__ arena,
+                                                       const uint64_t *__restrict__ toff,
+                                                       int32_t *__restrict__ out) {
+  const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= njobs) return;
+  const JobDev jb = jobs[i];
+  const int32_t *prof = qd->prof32;
+  const uint32_t ld = qd->ld;
+  const int lo = (int)jb.lo, hi = (int)(jb.hi > jb.lo ? jb.hi : jb.lo);
+  const uint8_t *t = arena + toff[jb.tid];
+  int32_t best = 0, cur = 0;
+  int q = lo;
+  // align the target pointer to 16 bytes and stream whole vectors
+  const int64_t p0 = (int64_t)q + jb.diag;
+  const int64_t pa = p0 & ~(int64_t)15;
+  int skip = (int)(p0 - pa);
+  for (int64_t pos = pa; q < hi; pos += 16) {
+    const uint4 v = *(const uint4 *)(t + pos);
+    const uint32_t w[4] = {v.x, v.y, v.z, v.w};
+#pragma unroll
+    for (int b = 0; b < 16; b++) {
+      if (b >= skip && q < hi) {
+        const uint32_t c = (w[b >> 2] >> ((b & 3) * 8)) & 0xffu;
+        const int32_t s = prof[(uint64_t)c * ld + q];
+        cur = (cur > 0 ? cur : 0) + s;
+        best = best > cur ? best : cur;
+        q++;
+      }
+    }
+    skip = 0;
+  }
+  out[i] = best;
+}
+
+__global__ __launch_bounds__(128) void gate_recs_kernel(const RecDev *__restrict__ recs, uint64_t nrecs,
+                                                        const JobDev *__restrict__ jobs,
+                                                        const int32_t *__restrict__ scores,
+                                                        const QueryDev *__restrict__ qd,
+                                                        const uint8_t *__restrict__ arena,
+                                                        const uint64_t *__restrict__ toff,
+                                                        int32_t open, int32_t ext, int32_t min_diag_score,
+                                                        int regions, GateRes *__restrict__ out) {
+  const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nrecs) return;
+  const RecDev rc = recs[i];
+  int ns = (int)(rc.j1 - rc.j0);
+  if (ns > 16) ns = 16;
+  int32_t sc[16], dg[16], lo[16], hi[16];
+  for (int k = 0; k < ns; k++) {
+    const JobDev jb = jobs[rc.j0 + k];
+    sc[k] = scores[rc.j0 + k];
+    dg[k] = jb.diag;
+    lo[k] = (int32_t)jb.lo;
+    hi[k] = (int32_t)(jb.hi > jb.lo ? jb.hi : jb.lo);
+  }
+  const uint8_t *t = arena + toff[rc.tid];
+  GateRes r;
+  if (ns == 0) {
+    r = GateRes{0, 0, 0, 0, 0, 0, 0, 0, 0};
+  } else {
+    r = gate_eval(qd->prof32, qd->ld, t, sc, dg, lo, hi, ns, open, ext, min_diag_score,
+                  regions != 0);
+  }
+  out[i] = r;
+}
+
+// All-diagonal evidence of each pair: the per-diagonal bests (diag index
+// d + m - 1) give the best diagonal and the top 4; the gate runs over them.
+struct PairGateArgs {
+  const PairDev *pairs;
+  uint64_t npairs;
+  const uint32_t *pair_query;  // query index of each pair slot
+  const QueryDev *queries;
+  const uint8_t *arena;
+  const uint64_t *toff;
+  const uint32_t *tlen;
+  const int32_t *diag;
+  const int32_t *open, *ext, *gate;  // per query
+  PairOut *pout;
+  void *res;  // st_pair_result rows (by PairDev::out)
+};
+
+struct PairResultDev {
+  int32_t swg, ung, best_diag, gate_score;
+  uint8_t pass, kernel, r0, r1;
+};
+
+__global__ __launch_bounds__(128) void gate_pairs_kernel(PairGateArgs a) {
+  const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= a.npairs) return;
+  const PairDev pd = a.pairs[i];
+  const uint32_t qi = a.pair_query[i];
+  const QueryDev q = a.queries[qi];
+  const int m = (int)q.m;
+  const int n = (int)a.tlen[pd.tid];
+  const int nd = (m > 0 && n > 0) ? m + n - 1 : 0;
+  const int32_t *dv = a.diag + pd.diag0;
+  int32_t ts[4];
+  int32_t td[4];
+  int nt = 0;
+  for (int x = 0; x < nd; x++) {
+    const int32_t s = dv[x];
+    int pos = nt;
+    while (pos > 0 && s > ts[pos - 1]) pos--;
+    if (pos < 4) {
+      for (int k = (nt < 4 ? nt : 3); k > pos; k--) {
+        ts[k] = ts[k - 1];
+        td[k] = td[k - 1];
+      }
+      ts[pos] = s;
+      td[pos] = x - (m - 1);
+      if (nt < 4) nt++;
+    }
+  }
+  int32_t lo[4], hi[4];
+  for (int k = 0; k < nt; k++) {  // whole diagonal (diag_range, src/ungapped.rs:38-46)
+    const int d = td[k];
+    const int a0 = -d > 0 ? -d : 0;
+    const int b0 = m < n - d ? m : n - d;
+    lo[k] = a0;
+    hi[k] = b0 > a0 ? b0 : a0;
+  }
+  const GateRes r = gate_eval(q.prof32, q.ld, a.arena + a.toff[pd.tid], ts, td, lo, hi, nt,
+                              a.open[qi], a.ext[qi], INT32_MIN, false);
+  const int32_t g = a.gate[qi];
+  PairResultDev o;
+  const PairOut po = a.pout[pd.out];
+  o.swg = po.swg;
+  o.ung = nt ? ts[0] : 0;
+  o.best_diag = nt ? td[0] : 0;
+  o.gate_score = nt ? r.score : 0;
+  o.pass = (nt ? r.score : 0) >= g;
+  o.kernel = po.flag == 2 ? 1 : 0;  // ST_KERNEL_I32 when the int32 pass produced swg
+  o.r0 = o.r1 = 0;
+  ((PairResultDev *)a.res)[pd.out] = o;
+}
+
+hipError_t launch_ung_jobs(const JobDev *jobs, uint64_t njobs, const QueryDev *qd,
+                           const uint8_t *arena, const uint64_t *toff, int32_t *out,
+                           hipStream_t s) {
+  if (njobs == 0) return hipSuccess;
+  const uint64_t blocks = (njobs + 255) / 256;
+  hipLaunchKernelGGL(ung_jobs_kernel, dim3((uint32_t)blocks), dim3(256), 0, s, jobs, njobs, qd,
+                     arena, toff, out);
+  return hipGetLastError();
+}
+
+hipError_t launch_gate_recs(const RecDev *recs, uint64_t nrecs, const JobDev *jobs,
+                            const int32_t *scores, const QueryDev *qd, const uint8_t *arena,
+                            const uint64_t *toff, int32_t open, int32_t ext, int32_t gate,
+                            int regions, GateRes *out, hipStream_t s) {
+  if (nrecs == 0) return hipSuccess;
+  const uint64_t blocks = (nrecs + 127) / 128;
+  hipLaunchKernelGGL(gate_recs_kernel, dim3((uint32_t)blocks), dim3(128), 0, s, recs, nrecs, jobs,
+                     scores, qd, arena, toff, open, ext, gate, regions, out);
+  return hipGetLastError();
+}
+
+hipError_t launch_gate_pairs(const PairGateArgs &a, hipStream_t s) {
+  if (a.npairs == 0) return hipSuccess;
+  const uint64_t blocks = (a.npairs + 127) / 128;
+  hipLaunchKernelGGL(gate_pairs_kernel, dim3((uint32_t)blocks), dim3(128), 0, s, a);
+  return hipGetLastError();
+}
+
+}  // namespace st

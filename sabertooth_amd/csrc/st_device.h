@@ -1,0 +1,53 @@
+// Shared device/host structures of the scoring path (gfx950).
+#pragma once
+#include <stdint.h>
+
+namespace st {
+
+// Zero-score residue code: every target code >= 21 scores 0
+// (src/align.rs:180-187).  The device arena and the profiles use it as the
+// padding code.
+constexpr int kSentinel = 21;
+// Profile codes held on the device: 0..20 plus the zero row 21.
+constexpr int kCodes = 22;
+// Largest group-rows product G*R of one pass (G <= 16 lanes, R <= 16 rows).
+constexpr int kMaxPassRows = 256;
+// LDS row stride (elements) of a staged profile slice; +2 keeps the code rows
+// of a slice on different banks.
+constexpr int kLdsStride = kMaxPassRows + 2;
+
+// One wave-sized work unit of the DP kernels: a query and npairs consecutive
+// pair slots (targets sorted by length), scored by groups of G lanes.
+struct Unit {
+  uint32_t query;   // index into the QueryDev table
+  uint32_t pair0;   // first pair slot
+  uint32_t npairs;  // slots in this unit
+  uint32_t G;       // lanes per group (1, 2, 4, 8 or 16)
+  uint32_t n_pad;   // columns: longest target of the unit
+  uint32_t passes;  // ceil(m / (G*R))
+  uint32_t R;       // rows per lane (the kernel instance; informational)
+  uint32_t pad;
+};
+
+// Device view of a query profile.
+struct QueryDev {
+  const int16_t *prof16;  // [kCodes][ld], rows >= m and code 21 are zero
+  const int32_t *prof32;  // [kCodes][ld]
+  uint32_t m;
+  uint32_t ld;            // leading dimension, multiple of kMaxPassRows
+};
+
+// One (query, target) pair slot of a batch.
+struct PairDev {
+  uint32_t tid;     // target id
+  uint32_t out;     // index of its result row
+  uint64_t diag0;   // offset of its per-diagonal array (m + n - 1 int32)
+};
+
+// Per-pair raw outputs of the DP kernels.
+struct PairOut {
+  int32_t swg;
+  int32_t flag;     // 1: int16 saturated, rescore in int32
+};
+
+}  // namespace st
