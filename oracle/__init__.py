@@ -192,3 +192,40 @@ class SplitMix64:
         out = np.zeros(n, dtype=np.uint8)
         lib().or_background_sequence(C.byref(self._s), n, _p(out))
         return out
+
+
+# ---- AVX2 baseline (bench cpu_baseline leg) -------------------------------------
+
+_AVX = None
+
+
+def avx2_lib() -> C.CDLL:
+    global _AVX
+    if _AVX is None:
+        path = os.path.join(_HERE, "libavx2base.so")
+        if not os.path.exists(path):
+            build()
+        L = C.CDLL(path)
+        P = C.c_void_p
+        L.avx2_score_pairs.restype = C.c_double
+        L.avx2_score_pairs.argtypes = [C.c_uint32, P, P, C.c_uint64, P, P, P, P, P, C.c_int32, C.c_int32,
+                                       C.c_int, P, P]
+        _AVX = L
+    return _AVX
+
+
+def avx2_score_pairs(rows_list, pair_q, pair_t, arena, offsets, open_, ext, threads=1):
+    """(scores, kernels, seconds): the AVX2 u8 -> i16 -> scalar cascade."""
+    rows_c = [np.ascontiguousarray(r, dtype=np.int32).reshape(-1, NPROF) for r in rows_list]
+    ptrs = (C.c_void_p * len(rows_c))(*[r.ctypes.data for r in rows_c])
+    qlen = np.array([len(r) for r in rows_c], dtype=np.uint32)
+    pq = np.ascontiguousarray(pair_q, dtype=np.uint32)
+    pt = np.ascontiguousarray(pair_t, dtype=np.uint32)
+    arena = _seq(arena)
+    offsets = np.ascontiguousarray(offsets, dtype=np.uint64)
+    lens = (offsets[1:] - offsets[:-1]).astype(np.uint32)
+    sc = np.zeros(len(pq), dtype=np.int32)
+    kn = np.zeros(len(pq), dtype=np.uint8)
+    secs = avx2_lib().avx2_score_pairs(len(rows_c), ptrs, _p(qlen), len(pq), _p(pq), _p(pt), _p(arena),
+                                       _p(offsets), _p(lens), open_, ext, threads, _p(sc), _p(kn))
+    return sc, kn, secs

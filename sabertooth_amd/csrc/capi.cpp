@@ -30,7 +30,7 @@ struct DpLaunch {
   uint4 *passbuf;
   uint64_t passbuf_half;
   PairOut *pout;
-  int32_t *diag_out;
+  uint64_t *top4;
   uint32_t grid;
 };
 hipError_t launch_dp(const DpLaunch &a, int kind, int R, bool ung, hipStream_t s);
@@ -56,7 +56,7 @@ struct PairGateArgs {
   const uint8_t *arena;
   const uint64_t *toff;
   const uint32_t *tlen;
-  const int32_t *diag;
+  const uint64_t *top4;
   const int32_t *open, *ext, *gate;
   PairOut *pout;
   void *res;
@@ -434,7 +434,7 @@ struct Plan {
 // Add the pairs of query qi (result rows outs[], target ids tids[]) to `pl`:
 // targets sorted by length, cut into wave units of 64/G groups.
 static void plan_query(Plan &pl, std::vector<Unit> &pending, uint32_t qi, const st_qprof *qp,
-                       const uint32_t *tids, const uint64_t *outs, const uint64_t *diag0, size_t count,
+                       const uint32_t *tids, const uint64_t *outs, size_t count,
                        Kind kind, const st_db *db) {
   if (count == 0) return;
   std::vector<uint32_t> order(count);
@@ -462,7 +462,6 @@ static void plan_query(Plan &pl, std::vector<Unit> &pending, uint32_t qi, const 
       PairDev pd;
       pd.tid = tids[o];
       pd.out = (uint32_t)outs[o];
-      pd.diag0 = diag0 ? diag0[o] : 0;
       pl.pairs.push_back(pd);
       np = std::max(np, db->hlen[tids[o]]);
     }
@@ -517,13 +516,12 @@ struct st_batch {
   st_ctx *ctx = nullptr;
   const st_db *db = nullptr;
   bool ung = true;
-  uint64_t npairs = 0, cells = 0, ndiag = 0;
+  uint64_t npairs = 0, cells = 0;
   std::vector<const st_qprof *> qps;
   std::vector<int32_t> open, ext, gate;
   std::vector<Kind> qkind;
   std::vector<uint32_t> cand_tids;
   std::vector<uint32_t> cand_q;
-  std::vector<uint64_t> diag0;
   Plan plan;
   DevBuf<QueryDev> d_queries;
   DevBuf<int32_t> d_open, d_ext, d_gate;
@@ -532,7 +530,7 @@ struct st_batch {
   DevBuf<PairDev> d_allpairs;  // candidate order, for the gate kernel
   DevBuf<uint32_t> d_candq;
   DevBuf<PairOut> d_pout;
-  DevBuf<int32_t> d_diag;
+  DevBuf<uint64_t> d_top4;
   DevBuf<uint8_t> d_res;
   DevBuf<uint4> d_passbuf;
   uint64_t passbuf_elems = 0;
@@ -585,7 +583,7 @@ static int launch_plan(st_batch *b, const Plan &pl, const Unit *d_units, const P
     a.passbuf = b->d_passbuf.p;
     a.passbuf_half = pl.passbuf_half;
     a.pout = b->d_pout.p;
-    a.diag_out = b->d_diag.p;
+    a.top4 = b->d_top4.p;
     if (a.passbuf_half * 2 * a.grid > b->passbuf_elems)
       return fail(ST_ERR_NOMEM, "pass buffer smaller than the plan needs");
     ST_HIP(launch_dp(a, L.kind, L.R, b->ung, ctx->stream));
@@ -629,27 +627,25 @@ static int batch_create_impl(st_ctx *ctx, const st_db *db, const st_qprof *const
     b->gate.assign(nq, 0);
   b->cand_tids.assign(cand_tids, cand_tids + np);
   b->cand_q.resize(np);
-  b->diag0.resize(np);
-  uint64_t dpos = 0, cells = 0;
+  uint64_t cells = 0;
   for (uint64_t q = 0; q < nq; q++)
     for (uint64_t k = cand_off[q]; k < cand_off[q + 1]; k++) {
       b->cand_q[k] = (uint32_t)q;
-      const uint64_t m = b->qps[q]->m, n = db->hlen[cand_tids[k]];
-      b->diag0[k] = dpos;
-      if (ung && m && n) dpos += m + n - 1;
-      cells += m * n;
+      cells += (uint64_t)b->qps[q]->m * db->hlen[cand_tids[k]];
     }
   b->cells = cells;
-  b->ndiag = dpos;
   b->qkind.resize(nq);
   std::vector<Unit> pending;
   for (uint64_t q = 0; q < nq; q++) {
     b->qkind[q] = kind_for(b->qps[q], b->open[q], b->ext[q]);
     const uint64_t s = cand_off[q], e = cand_off[q + 1];
+    // the packed kernel's diagonal keys hold m + n - 1 <= kMaxDiagsPk16
+    uint32_t nmax = 0;
+    for (uint64_t k = s; k < e; k++) nmax = std::max(nmax, db->hlen[cand_tids[k]]);
+    if (b->qkind[q] == kPk16 && (uint64_t)b->qps[q]->m + nmax - 1 > kMaxDiagsPk16) b->qkind[q] = kI32F;
     std::vector<uint64_t> outs(e - s);
     std::iota(outs.begin(), outs.end(), s);
-    plan_query(b->plan, pending, (uint32_t)q, b->qps[q], cand_tids + s, outs.data(), b->diag0.data() + s, e - s,
-               b->qkind[q], db);
+    plan_query(b->plan, pending, (uint32_t)q, b->qps[q], cand_tids + s, outs.data(), e - s, b->qkind[q], db);
   }
   finish_plan(b->plan, pending, b->qkind, b->open, b->ext, -1);
   std::lock_guard<std::mutex> lk(ctx->mu);
@@ -663,7 +659,6 @@ static int batch_create_impl(st_ctx *ctx, const st_db *db, const st_qprof *const
   for (uint64_t k = 0; k < np; k++) {
     all[k].tid = cand_tids[k];
     all[k].out = (uint32_t)k;
-    all[k].diag0 = b->diag0[k];
   }
   // the pass buffer covers the largest plan; rescoring plans reuse it when they fit
   const uint32_t grid = grid_for(b, (uint32_t)std::max<size_t>(b->plan.units.size(), 1));
@@ -676,7 +671,7 @@ static int batch_create_impl(st_ctx *ctx, const st_db *db, const st_qprof *const
             b->d_allpairs.upload(all.data(), all.size(), ctx->stream) == hipSuccess &&
             b->d_candq.upload(b->cand_q.data(), b->cand_q.size(), ctx->stream) == hipSuccess &&
             b->d_pout.alloc(std::max<uint64_t>(np, 1)) == hipSuccess &&
-            b->d_diag.alloc(std::max<uint64_t>(dpos, 1)) == hipSuccess &&
+            b->d_top4.alloc(ung ? std::max<uint64_t>(np, 1) * 4 : 1) == hipSuccess &&
             b->d_res.alloc(std::max<uint64_t>(np, 1) * sizeof(st_pair_result)) == hipSuccess &&
             ensure_passbuf(b, b->plan.passbuf_half, grid) == ST_OK;
   if (!ok) {
@@ -722,14 +717,9 @@ static int batch_run_locked(st_batch *b) {
     for (size_t q = 0; q < perq.size(); q++) {
       if (perq[q].empty()) continue;
       std::vector<uint32_t> tids;
-      std::vector<uint64_t> d0;
-      for (uint64_t o : perq[q]) {
-        tids.push_back(b->cand_tids[o]);
-        d0.push_back(b->diag0[o]);
-      }
+      for (uint64_t o : perq[q]) tids.push_back(b->cand_tids[o]);
       b->rescored += perq[q].size();
-      plan_query(b->resc, pending, (uint32_t)q, b->qps[q], tids.data(), perq[q].data(), d0.data(), tids.size(),
-                 kI32F, b->db);
+      plan_query(b->resc, pending, (uint32_t)q, b->qps[q], tids.data(), perq[q].data(), tids.size(), kI32F, b->db);
     }
     finish_plan(b->resc, pending, b->qkind, b->open, b->ext, (int)kI32F);
     if (!b->resc.units.empty()) {
@@ -754,7 +744,7 @@ static int batch_run_locked(st_batch *b) {
     ga.arena = b->db->arena.p;
     ga.toff = b->db->off.p;
     ga.tlen = b->db->len.p;
-    ga.diag = b->d_diag.p;
+    ga.top4 = b->d_top4.p;
     ga.open = b->d_open.p;
     ga.ext = b->d_ext.p;
     ga.gate = b->d_gate.p;

@@ -9,6 +9,8 @@
 // (kadane_score, src/ungapped.rs:53-62):
 //   U[i][j] = max(U[i-1][j-1], 0) + s(i, t_j),  B[i][j] = max(B[i-1][j-1], U)
 // so B at a diagonal's last cell is that diagonal's best ungapped segment.
+// Each pair keeps the 4 best diagonals (score desc, diagonal asc) as sorted
+// keys; the gate kernel finishes from those.
 //
 // Mapping.  A work unit is one wave64 and one query.  Lanes form groups of G
 // (1..16, inside a 16-lane DPP row); a group scores two targets at once in
@@ -18,10 +20,18 @@
 // group walks the target columns with lane l skewed l steps behind lane l-1,
 // so the row boundary (H, F, U, B of the lane's last row) moves to the next
 // lane by one DPP row_shr:1 per step.  Between passes the last lane's
-// boundary row goes through a per-block global buffer.  The query profile of
-// the pass (22 codes x G*R rows) is staged in LDS; targets are padded with the
-// zero-score code 21, which never raises any H, E, F or B, so padded columns,
-// padded rows and the skew's idle steps need no masking.
+// boundary row goes through a per-block global buffer (prefetched one column
+// ahead).  The query profile of the pass (22 codes x G*R rows) is staged in
+// LDS; target codes stream into the group's first lane as dwords fetched
+// eight columns ahead and move down the group with the boundary.  Targets are
+// padded with the zero-score code 21, which never raises any H, E, F or B, so
+// padded columns, padded rows and the skew's idle steps need no masking.
+//
+// Diagonal ends: a diagonal ends on the last padded column (right edge; each
+// lane parks its R values in LDS and folds them into its top-4 lists after
+// the pass) or on the last padded row (bottom; the group's last lane parks
+// one value per step in LDS and the whole group folds them every
+// kBottomChunk steps).  The group's lists merge by butterfly at the end.
 //
 // Exactness of the int16 pass: E and F are kept floored at 0 (exact for
 // ext >= 0, since H >= 0 absorbs every negative E/F), which lets the u16
@@ -37,11 +47,23 @@ namespace st {
 
 typedef short s16x2 __attribute__((ext_vector_type(2)));
 typedef unsigned short u16x2 __attribute__((ext_vector_type(2)));
+typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
+typedef const __attribute__((address_space(1))) uint32_t *gptr_u32;
+typedef const __attribute__((address_space(1))) u32x4 *gptr_u32x4;
 
 template <class To, class From>
 __device__ __forceinline__ To bc(From v) {
   return __builtin_bit_cast(To, v);
 }
+
+// global-address-space loads (plain global_load, never flat)
+__device__ __forceinline__ uint32_t ldg_u32(const uint8_t *p) { return *(gptr_u32)(p); }
+__device__ __forceinline__ uint4 ldg_u4(const uint4 *p) {
+  const u32x4 v = *(gptr_u32x4)(p);
+  return make_uint4(v.x, v.y, v.z, v.w);
+}
+
+constexpr uint32_t kSent4 = 0x15151515u;  // four zero-score codes
 
 // ---- arithmetic policies -----------------------------------------------------
 
@@ -51,6 +73,7 @@ struct Pk16 {
   static constexpr bool kFloor = true;
   using T = uint32_t;
   using L = int16_t;
+  using K = uint32_t;  // score << 17 | (0x1FFFF - x)
   __device__ static T splat(int v) { return (uint32_t)(uint16_t)v * 0x10001u; }
   __device__ static T add(T a, T b) {
     return bc<T>(__builtin_elementwise_add_sat(bc<s16x2>(a), bc<s16x2>(b)));
@@ -65,9 +88,12 @@ struct Pk16 {
   __device__ static T score(const L *rowA, const L *rowB, int k) {
     return (uint32_t)(uint16_t)rowA[k] | ((uint32_t)(uint16_t)rowB[k] << 16);
   }
-  __device__ static int lo(T v) { return (int)(int16_t)(v & 0xffffu); }
-  __device__ static int hi(T v) { return (int)(int16_t)(v >> 16); }
-  __device__ static int half(T v, int x) { return x ? hi(v) : lo(v); }
+  __device__ static int half(T v, int x) { return x ? (int)(int16_t)(v >> 16) : (int)(int16_t)(v & 0xffffu); }
+  __device__ static K key(int score, uint32_t x) { return ((uint32_t)score << 17) | (0x1FFFFu - x); }
+  __device__ static uint64_t key64(K k) {
+    return k ? diag_key((int32_t)(k >> 17), 0x1FFFFu - (k & 0x1FFFFu)) : 0ull;
+  }
+  __device__ static K shfl_xor(K k, int off) { return (K)__shfl_xor((int)k, off); }
 };
 
 // One pair per group in int32, E/F floored at 0 (needs ext >= 0, open >= 0).
@@ -76,6 +102,7 @@ struct I32F {
   static constexpr bool kFloor = true;
   using T = int32_t;
   using L = int32_t;
+  using K = uint64_t;
   __device__ static T splat(int v) { return v; }
   __device__ static T add(T a, T b) { return a + b; }
   __device__ static T max(T a, T b) { return a > b ? a : b; }
@@ -84,32 +111,40 @@ struct I32F {
   }
   __device__ static T score(const L *rowA, const L *, int k) { return rowA[k]; }
   __device__ static int half(T v, int) { return v; }
+  __device__ static K key(int score, uint32_t x) { return diag_key(score, x); }
+  __device__ static uint64_t key64(K k) { return k; }
+  __device__ static K shfl_xor(K k, int off) {
+    const uint32_t lo = (uint32_t)__shfl_xor((int)(uint32_t)k, off);
+    const uint32_t hi = (uint32_t)__shfl_xor((int)(uint32_t)(k >> 32), off);
+    return ((uint64_t)hi << 32) | lo;
+  }
 };
 
 // One pair per group in int32 with the reference's unfloored E/F (any gap
 // costs); cells outside the real matrix are masked.
-struct I32N {
-  static constexpr int kPer = 1;
+struct I32N : I32F {
   static constexpr bool kFloor = false;
-  using T = int32_t;
-  using L = int32_t;
-  __device__ static T splat(int v) { return v; }
-  __device__ static T add(T a, T b) { return a + b; }
-  __device__ static T max(T a, T b) { return a > b ? a : b; }
   __device__ static T subf(T a, T c) { return a - c; }
-  __device__ static T score(const L *rowA, const L *, int k) { return rowA[k]; }
-  __device__ static int half(T v, int) { return v; }
 };
 
-template <class P>
-__device__ __forceinline__ typename P::T shr1(typename P::T v) {
-  // lane i <- lane i-1 inside each 16-lane row; row starts keep their value
-  // and are overwritten by the caller (they are always group starts)
-  return (typename P::T)__builtin_amdgcn_update_dpp((int)v, (int)v, 0x111, 0xf, 0xf, false);
+template <class K>
+__device__ __forceinline__ void insert4(K (&k)[4], K v) {
+  // k[0] >= k[1] >= k[2] >= k[3]; keys are distinct
+  k[3] = k[3] > (k[2] < v ? k[2] : v) ? k[3] : (k[2] < v ? k[2] : v);
+  k[2] = k[2] > (k[1] < v ? k[1] : v) ? k[2] : (k[1] < v ? k[1] : v);
+  k[1] = k[1] > (k[0] < v ? k[0] : v) ? k[1] : (k[0] < v ? k[0] : v);
+  k[0] = k[0] > v ? k[0] : v;
 }
 
 __device__ __forceinline__ uint32_t shr1u(uint32_t v) {
+  // lane i <- lane i-1 inside each 16-lane row; row starts keep their value
+  // and are overwritten by the caller (they are always group starts)
   return (uint32_t)__builtin_amdgcn_update_dpp((int)v, (int)v, 0x111, 0xf, 0xf, false);
+}
+
+template <class P>
+__device__ __forceinline__ typename P::T shr1(typename P::T v) {
+  return (typename P::T)shr1u((uint32_t)v);
 }
 
 template <class P>
@@ -129,17 +164,22 @@ __global__ __launch_bounds__(64) void dp_kernel(
     const uint8_t *__restrict__ arena, const uint64_t *__restrict__ toff,
     const uint32_t *__restrict__ tlen, int32_t gap_open, int32_t gap_ext,
     uint4 *__restrict__ passbuf, uint64_t passbuf_half, PairOut *__restrict__ pout,
-    int32_t *__restrict__ diag_out) {
+    uint64_t *__restrict__ top4) {
   using T = typename P::T;
   using L = typename P::L;
+  using K = typename P::K;
+  constexpr int kPer = P::kPer;
   __shared__ L lds[kCodes * kLdsStride];
+  __shared__ T edge[UNG ? 64 * R : 1];
+  __shared__ T bottom[UNG ? 64 * kBottomChunk : 1];
   const int lane = threadIdx.x;
   uint4 *const myb = passbuf + (uint64_t)blockIdx.x * 2 * passbuf_half;
 
   const T vopen = P::splat(gap_open);
   const T vext = P::splat(gap_ext);
   // boundary state of the row above the matrix / column left of it
-  const T e0 = P::kFloor ? T(0) : T(P::max(-gap_open, (INT32_MIN / 4) - gap_ext));
+  const T e0 = P::kFloor ? T(0) : T(max(-gap_open, (INT32_MIN / 4) - gap_ext));
+  const T kInvalid = (T)0xFFFFFFFFu;  // a negative B: not a diagonal end
 
   for (;;) {
     uint32_t got = 0;
@@ -160,21 +200,28 @@ __global__ __launch_bounds__(64) void dp_kernel(
     // this group's pair slots
     int len[2] = {0, 0};
     const uint8_t *tp[2] = {arena, arena};
-    uint64_t d0[2] = {0, 0};
     uint32_t outi[2] = {0, 0};
     bool valid[2] = {false, false};
 #pragma unroll
-    for (int x = 0; x < P::kPer; x++) {
-      const uint32_t slot = (uint32_t)(g * P::kPer + x);
+    for (int x = 0; x < kPer; x++) {
+      const uint32_t slot = (uint32_t)(g * kPer + x);
       if (slot < u.npairs) {
         const PairDev pd = pairs[u.pair0 + slot];
         valid[x] = true;
         len[x] = (int)tlen[pd.tid];
         tp[x] = arena + toff[pd.tid];
-        d0[x] = pd.diag0;
         outi[x] = pd.out;
       }
     }
+    bool has[2];
+#pragma unroll
+    for (int x = 0; x < 2; x++) has[x] = valid[x] && len[x] > 0 && m > 0;
+
+    K top[kPer][4];
+#pragma unroll
+    for (int x = 0; x < kPer; x++)
+#pragma unroll
+      for (int k = 0; k < 4; k++) top[x][k] = 0;
 
     T best = 0;
     for (int p = 0; p < (int)u.passes; p++) {
@@ -211,12 +258,17 @@ __global__ __launch_bounds__(64) void dp_kernel(
       T hdiag = 0, udiag = 0, bdiag = 0;
       T out_h = 0, out_f = e0, out_u = 0, out_b = 0;
       uint32_t code = kSentinel | (kSentinel << 8);
-      // lane 0 of the group prefetches the next column's residues
-      uint32_t nxt = kSentinel | (kSentinel << 8);
+      // the group's first lane streams residues: cw = columns [t & ~3, +4),
+      // nw = the next four; the boundary row of the previous pass one ahead
+      uint32_t cw[2] = {kSent4, kSent4}, nw[2] = {kSent4, kSent4};
+      uint4 pb = make_uint4(0, (uint32_t)e0, 0, 0);
       if (l == 0) {
-        const uint32_t a = len[0] > 0 ? tp[0][0] : kSentinel;
-        const uint32_t b = len[1] > 0 ? tp[1][0] : kSentinel;
-        nxt = a | (b << 8);
+#pragma unroll
+        for (int x = 0; x < kPer; x++) {
+          if (len[x] > 0) cw[x] = ldg_u32(tp[x]);
+          if (len[x] > 4) nw[x] = ldg_u32(tp[x] + 4);
+        }
+        if (p > 0 && n_pad > 0) pb = ldg_u4(rd);
       }
       const int steps = n_pad + G - 1;
       for (int t = 0; t < steps; t++) {
@@ -229,22 +281,27 @@ __global__ __launch_bounds__(64) void dp_kernel(
         uint32_t in_code = shr1u(code);
         if (l == 0) {
           if (p > 0 && t < n_pad) {
-            const uint4 v = rd[t];
-            in_h = (T)v.x;
-            in_f = (T)v.y;
-            in_u = (T)v.z;
-            in_b = (T)v.w;
+            in_h = (T)pb.x;
+            in_f = (T)pb.y;
+            in_u = (T)pb.z;
+            in_b = (T)pb.w;
           } else {
             in_h = 0;
             in_f = e0;
             in_u = 0;
             in_b = 0;
           }
-          in_code = nxt;
-          const int tn = t + 1;
-          const uint32_t a = tn < len[0] ? tp[0][tn] : kSentinel;
-          const uint32_t b = tn < len[1] ? tp[1][tn] : kSentinel;
-          nxt = a | (b << 8);
+          if (p > 0 && t + 1 < n_pad) pb = ldg_u4(rd + t + 1);
+          const int sh = (t & 3) * 8;
+          in_code = ((cw[0] >> sh) & 0xffu) | (((cw[1] >> sh) & 0xffu) << 8);
+          if ((t & 3) == 3) {
+            const int nx = t + 5;
+#pragma unroll
+            for (int x = 0; x < kPer; x++) {
+              cw[x] = nw[x];
+              nw[x] = nx < len[x] ? ldg_u32(tp[x] + nx) : kSent4;
+            }
+          }
         }
         code = in_code;
         const int j = t - l;
@@ -307,59 +364,83 @@ __global__ __launch_bounds__(64) void dp_kernel(
           wr[j] = v;
         }
         if (UNG) {
-          // a diagonal ends on the last padded column ...
+          // right edge: park this lane's rows; folded after the pass
           if (j == n_pad - 1) {
 #pragma unroll
-            for (int k = 0; k < R; k++) {
-              const int d = (n_pad - 1) - (base + l * R + k);
-#pragma unroll
-              for (int x = 0; x < P::kPer; x++)
-                if (valid[x] && len[x] > 0 && d >= -(m - 1) && d <= len[x] - 1)
-                  diag_out[d0[x] + (uint64_t)(d + m - 1)] = P::half(B[k], x);
-            }
+            for (int k = 0; k < R; k++) edge[lane * R + k] = B[k];
           }
-          // ... or on the last padded row
-          if (last && l == G - 1 && j >= 0 && j < n_pad - 1) {
-            const int d = j - (m_pad - 1);
+          if (last) {
+            // bottom row: the group's last lane parks one value per step
+            const int slot = t % kBottomChunk;
+            if (l == G - 1) bottom[g * kBottomChunk + slot] = (j >= 0 && j < n_pad - 1) ? B[R - 1] : kInvalid;
+            if (slot == kBottomChunk - 1 || t == steps - 1) {
+              const int t0 = t - slot;
+              for (int s = l; s <= slot; s += G) {
+                const T v = bottom[g * kBottomChunk + s];
+                const int d = (t0 + s - (G - 1)) - (m_pad - 1);
 #pragma unroll
-            for (int x = 0; x < P::kPer; x++)
-              if (valid[x] && len[x] > 0 && d >= -(m - 1) && d <= len[x] - 1)
-                diag_out[d0[x] + (uint64_t)(d + m - 1)] = P::half(B[R - 1], x);
+                for (int x = 0; x < kPer; x++) {
+                  const int sc = P::half(v, x);
+                  if (has[x] && sc >= 0 && d >= -(m - 1) && d <= len[x] - 1)
+                    insert4<K>(top[x], P::key(sc, (uint32_t)(d + m - 1)));
+                }
+              }
+            }
           }
         }
       }
+      if (UNG) {
+        // fold this pass's right-edge values (each lane its own rows)
+#pragma unroll
+        for (int k = 0; k < R; k++) {
+          const T v = edge[lane * R + k];
+          const int d = (n_pad - 1) - (base + l * R + k);
+#pragma unroll
+          for (int x = 0; x < kPer; x++)
+            if (has[x] && d >= -(m - 1) && d <= len[x] - 1)
+              insert4<K>(top[x], P::key(P::half(v, x), (uint32_t)(d + m - 1)));
+        }
+      }
     }
-    // best over the group's lanes
+    // best and top-4 over the group's lanes
     for (int off = 1; off < G; off <<= 1) {
       const T o = (T)__shfl_xor((int)best, off);
       best = P::max(best, o);
+      if (UNG) {
+#pragma unroll
+        for (int x = 0; x < kPer; x++) {
+          K other[4];
+#pragma unroll
+          for (int k = 0; k < 4; k++) other[k] = P::shfl_xor(top[x][k], off);
+#pragma unroll
+          for (int k = 0; k < 4; k++) insert4<K>(top[x], other[k]);
+        }
+      }
     }
     if (l == 0) {
 #pragma unroll
-      for (int x = 0; x < P::kPer; x++)
+      for (int x = 0; x < kPer; x++)
         if (valid[x]) {
           PairOut po;
           po.swg = P::half(best, x);
           // 0: exact int16; 1: int16 saturated (rescore); 2: exact int32
-          po.flag = P::kPer == 2 ? (po.swg >= 32767 ? 1 : 0) : 2;
+          po.flag = kPer == 2 ? (po.swg >= 32767 ? 1 : 0) : 2;
           pout[outi[x]] = po;
+          if (UNG) {
+#pragma unroll
+            for (int k = 0; k < 4; k++) top4[(uint64_t)outi[x] * 4 + k] = P::key64(top[x][k]);
+          }
         }
     }
   }
 }
 
-// Clear the unit counters before a launch (a kernel, not a memset, so that a
+// Clear the unit counter before a launch (a kernel, not a memset, so that a
 // captured graph replays it correctly).
 __global__ void zero_u32(uint32_t *p, uint32_t n) {
   const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) p[i] = 0;
 }
-
-}  // namespace st
-
-// ---- launch entry points used by the host library ------------------------------
-
-namespace st {
 
 struct DpLaunch {
   const Unit *units;
@@ -374,17 +455,17 @@ struct DpLaunch {
   uint4 *passbuf;
   uint64_t passbuf_half;
   PairOut *pout;
-  int32_t *diag_out;
+  uint64_t *top4;
   uint32_t grid;
 };
 
-// kind: 0 = Pk16, 1 = I32F, 2 = I32N; R: 8 or 16
+// kind: 0 = Pk16, 1 = I32F, 2 = I32N; R: 8 or 16 (I32N: 8)
 hipError_t launch_dp(const DpLaunch &a, int kind, int R, bool ung, hipStream_t s) {
   hipLaunchKernelGGL(zero_u32, dim3(1), dim3(64), 0, s, a.counter, 1u);
 #define ST_DP(PP, RR, UU)                                                                  \
   hipLaunchKernelGGL((dp_kernel<PP, RR, UU>), dim3(a.grid), dim3(64), 0, s, a.units,       \
                      a.nunits, a.counter, a.pairs, a.queries, a.arena, a.toff, a.tlen,      \
-                     a.open, a.ext, a.passbuf, a.passbuf_half, a.pout, a.diag_out)
+                     a.open, a.ext, a.passbuf, a.passbuf_half, a.pout, a.top4)
   if (kind == 0) {
     if (R == 8) {
       if (ung) ST_DP(Pk16, 8, true); else ST_DP(Pk16, 8, false);

@@ -213,8 +213,9 @@ __global__ __launch_bounds__(128) void gate_recs_kernel(const RecDev *__restrict
   out[i] = r;
 }
 
-// All-diagonal evidence of each pair: the per-diagonal bests (diag index
-// d + m - 1) give the best diagonal and the top 4; the gate runs over them.
+// All-diagonal evidence of each pair: the DP kernel leaves the 4 best
+// diagonals as sorted keys (score desc, diagonal asc); the best one is the
+// pair's ungapped score and best diagonal, and the gate runs over all four.
 struct PairGateArgs {
   const PairDev *pairs;
   uint64_t npairs;
@@ -223,7 +224,7 @@ struct PairGateArgs {
   const uint8_t *arena;
   const uint64_t *toff;
   const uint32_t *tlen;
-  const int32_t *diag;
+  const uint64_t *top4;              // 4 keys per result row
   const int32_t *open, *ext, *gate;  // per query
   PairOut *pout;
   void *res;  // st_pair_result rows (by PairDev::out)
@@ -242,32 +243,20 @@ __global__ __launch_bounds__(128) void gate_pairs_kernel(PairGateArgs a) {
   const QueryDev q = a.queries[qi];
   const int m = (int)q.m;
   const int n = (int)a.tlen[pd.tid];
-  const int nd = (m > 0 && n > 0) ? m + n - 1 : 0;
-  const int32_t *dv = a.diag + pd.diag0;
-  int32_t ts[4];
-  int32_t td[4];
+  int32_t ts[4], td[4], lo[4], hi[4];
   int nt = 0;
-  for (int x = 0; x < nd; x++) {
-    const int32_t s = dv[x];
-    int pos = nt;
-    while (pos > 0 && s > ts[pos - 1]) pos--;
-    if (pos < 4) {
-      for (int k = (nt < 4 ? nt : 3); k > pos; k--) {
-        ts[k] = ts[k - 1];
-        td[k] = td[k - 1];
-      }
-      ts[pos] = s;
-      td[pos] = x - (m - 1);
-      if (nt < 4) nt++;
-    }
-  }
-  int32_t lo[4], hi[4];
-  for (int k = 0; k < nt; k++) {  // whole diagonal (diag_range, src/ungapped.rs:38-46)
+  for (int k = 0; k < 4; k++) {
+    const uint64_t key = a.top4[(uint64_t)pd.out * 4 + k];
+    if (key == 0) break;
+    ts[k] = (int32_t)(key >> 32);
+    td[k] = (int32_t)(0xFFFFFFFFu - (uint32_t)key) - (m - 1);
+    // whole diagonal (diag_range, src/ungapped.rs:38-46)
     const int d = td[k];
     const int a0 = -d > 0 ? -d : 0;
     const int b0 = m < n - d ? m : n - d;
     lo[k] = a0;
     hi[k] = b0 > a0 ? b0 : a0;
+    nt++;
   }
   const GateRes r = gate_eval(q.prof32, q.ld, a.arena + a.toff[pd.tid], ts, td, lo, hi, nt,
                               a.open[qi], a.ext[qi], INT32_MIN, false);

@@ -15,6 +15,12 @@ constexpr int kMaxPassRows = 256;
 // LDS row stride (elements) of a staged profile slice; +2 keeps the code rows
 // of a slice on different banks.
 constexpr int kLdsStride = kMaxPassRows + 2;
+// Bottom-row diagonal ends are staged in LDS and folded into the top-4 lists
+// every kBottomChunk steps.
+constexpr int kBottomChunk = 16;
+// Largest m + n - 1 whose diagonal index fits the 17-bit field of the packed
+// int16 kernel's top-4 keys.
+constexpr uint32_t kMaxDiagsPk16 = 131071;
 
 // One wave-sized work unit of the DP kernels: a query and npairs consecutive
 // pair slots (targets sorted by length), scored by groups of G lanes.
@@ -25,7 +31,7 @@ struct Unit {
   uint32_t G;       // lanes per group (1, 2, 4, 8 or 16)
   uint32_t n_pad;   // columns: longest target of the unit
   uint32_t passes;  // ceil(m / (G*R))
-  uint32_t R;       // rows per lane (the kernel instance; informational)
+  uint32_t R;       // rows per lane (the kernel instance)
   uint32_t pad;
 };
 
@@ -39,15 +45,21 @@ struct QueryDev {
 
 // One (query, target) pair slot of a batch.
 struct PairDev {
-  uint32_t tid;     // target id
-  uint32_t out;     // index of its result row
-  uint64_t diag0;   // offset of its per-diagonal array (m + n - 1 int32)
+  uint32_t tid;  // target id
+  uint32_t out;  // index of its result row (and of its 4 top-diagonal keys)
 };
 
 // Per-pair raw outputs of the DP kernels.
 struct PairOut {
   int32_t swg;
-  int32_t flag;     // 1: int16 saturated, rescore in int32
+  int32_t flag;  // 0: exact int16; 1: int16 saturated (rescore); 2: exact int32
 };
+
+// Top-diagonal key: best ungapped score of a diagonal in the high word, and
+// 0xFFFFFFFF - (d + m - 1) in the low word, so a larger key means a higher
+// score, then a smaller diagonal d; 0 = no diagonal.
+__host__ __device__ inline uint64_t diag_key(int32_t score, uint32_t x) {
+  return ((uint64_t)(uint32_t)score << 32) | (uint64_t)(0xFFFFFFFFu - x);
+}
 
 }  // namespace st
