@@ -7,9 +7,13 @@
 //    residues with 16-byte loads (the arena is 16-byte aligned per target).
 //  * gate_recs_kernel: score_and_gate's per-target chain and gate
 //    (src/prefilter.rs:1577-1657) over already scored jobs.
-//  * gate_pairs_kernel: the seedless evidence of a (query, candidate) pair:
-//    best diagonal, top-4 diagonals (score desc, diagonal asc) and the same
-//    chain/gate rule over their whole ranges.
+//  * gate_pairs_kernel: the seedless evidence of a (query, candidate) pair
+//    from its 4 best diagonals (keys left by the DP kernel): best diagonal
+//    and the same chain/gate rule over their whole ranges.
+//
+// kadane_segment scans read 16 residues per load and look scores up either
+// in an LDS copy of the query's int16 profile (when a block's pairs share one
+// query that fits) or in the int32 profile in HBM.
 #include <hip/hip_runtime.h>
 
 #include "st_device.h"
@@ -24,27 +28,57 @@ struct GateRes {
   uint8_t pass, chain, pad0, pad1;
 };
 
-__device__ __forceinline__ int32_t sub_of(const int32_t *prof, uint32_t ld, uint32_t code, int q) {
-  return prof[(uint64_t)code * ld + q];  // code 21 rows are zero
-}
+// score lookups: code 21 rows of both profiles are zero
+struct ProfGlobal {
+  const int32_t *p;
+  uint32_t ld;
+  __device__ int32_t operator()(uint32_t code, int q) const { return p[(uint64_t)code * ld + q]; }
+};
+struct ProfLds {
+  const int16_t *p;
+  int ld;
+  __device__ int32_t operator()(uint32_t code, int q) const { return p[(int)code * ld + q]; }
+};
 
-// kadane_segment (src/ungapped.rs:69-88) over query rows [lo, hi)
-__device__ int32_t kadane_segment_dev(const int32_t *prof, uint32_t ld, const uint8_t *t,
-                                      int d, int lo, int hi, int *qs, int *qe) {
+// kadane_segment (src/ungapped.rs:69-88) over query rows [lo, hi) of
+// diagonal d: the first maximum in query order, restarting when cur <= 0
+template <class Acc>
+__device__ int32_t kadane_segment_dev(const Acc &acc, const uint8_t *t, int d, int lo, int hi, int *qs,
+                                      int *qe) {
   int32_t best = 0, cur = 0;
   int bs = 0, be = 0, cs = lo;
-  for (int q = lo; q < hi; q++) {
-    const int32_t s = sub_of(prof, ld, t[q + d], q);
-    if (cur <= 0) {
-      cur = s;
-      cs = q;
-    } else {
-      cur += s;
-    }
-    if (cur > best) {
-      best = cur;
-      bs = cs;
-      be = q;
+  int q = lo;
+  if (q < hi) {
+    const int64_t p0 = (int64_t)q + d;
+    const int64_t pa = p0 & ~(int64_t)15;
+    int skip = (int)(p0 - pa);
+    for (int64_t pos = pa; q < hi; pos += 16) {
+      const uint4 v = *(const uint4 *)(t + pos);
+      const uint32_t w[4] = {v.x, v.y, v.z, v.w};
+      int32_t s[16];
+#pragma unroll
+      for (int b = 0; b < 16; b++) {
+        const int qq = q + b - skip;
+        s[b] = (b >= skip && qq < hi) ? acc((w[b >> 2] >> ((b & 3) * 8)) & 0xffu, qq) : 0;
+      }
+#pragma unroll
+      for (int b = 0; b < 16; b++) {
+        if (b >= skip && q < hi) {
+          if (cur <= 0) {
+            cur = s[b];
+            cs = q;
+          } else {
+            cur += s[b];
+          }
+          if (cur > best) {
+            best = cur;
+            bs = cs;
+            be = q;
+          }
+          q++;
+        }
+      }
+      skip = 0;
     }
   }
   *qs = bs;
@@ -54,9 +88,9 @@ __device__ int32_t kadane_segment_dev(const int32_t *prof, uint32_t ld, const ui
 
 // chain + gate of one target over ns <= 16 scored diagonals
 // (src/prefilter.rs:1577-1657, region 1269-1278)
-__device__ GateRes gate_eval(const int32_t *prof, uint32_t ld, const uint8_t *t,
-                             const int32_t *sc, const int32_t *dg, const int32_t *lo,
-                             const int32_t *hi, int ns, int32_t open, int32_t ext,
+template <class Acc>
+__device__ GateRes gate_eval(const Acc &acc, const uint8_t *t, const int32_t *sc, const int32_t *dg,
+                             const int32_t *lo, const int32_t *hi, int ns, int32_t open, int32_t ext,
                              int32_t min_diag_score, bool regions) {
   GateRes r;
   r.pass = 0;
@@ -93,11 +127,11 @@ __device__ GateRes gate_eval(const int32_t *prof, uint32_t ld, const uint8_t *t,
         const int32_t c = sx + sy - (open + (ad - 1) * ext);
         if (c <= best_s || c <= chain_s) continue;
         if (!(have >> x & 1)) {
-          kadane_segment_dev(prof, ld, t, dx, lo[x], hi[x], &qs[x], &qe[x]);
+          kadane_segment_dev(acc, t, dx, lo[x], hi[x], &qs[x], &qe[x]);
           have |= 1u << x;
         }
         if (!(have >> y & 1)) {
-          kadane_segment_dev(prof, ld, t, dy, lo[y], hi[y], &qs[y], &qe[y]);
+          kadane_segment_dev(acc, t, dy, lo[y], hi[y], &qs[y], &qe[y]);
           have |= 1u << y;
         }
         if (qe[x] < qs[y] && qe[x] + dx < qs[y] + dy) {
@@ -121,7 +155,7 @@ __device__ GateRes gate_eval(const int32_t *prof, uint32_t ld, const uint8_t *t,
     for (int u = 0; u < 2; u++) {
       const int k = us[u];
       if (!(have >> k & 1)) {
-        kadane_segment_dev(prof, ld, t, dg[k], lo[k], hi[k], &qs[k], &qe[k]);
+        kadane_segment_dev(acc, t, dg[k], lo[k], hi[k], &qs[k], &qe[k]);
         have |= 1u << k;
       }
       const int s0 = qs[k] + dg[k], s1 = qe[k] + dg[k];
@@ -159,24 +193,30 @@ __global__ __launch_bounds__(256) void ung_jobs_kernel(const JobDev *__restrict_
   const uint8_t *t = arena + toff[jb.tid];
   int32_t best = 0, cur = 0;
   int q = lo;
-  // align the target pointer to 16 bytes and stream whole vectors
-  const int64_t p0 = (int64_t)q + jb.diag;
-  const int64_t pa = p0 & ~(int64_t)15;
-  int skip = (int)(p0 - pa);
-  for (int64_t pos = pa; q < hi; pos += 16) {
-    const uint4 v = *(const uint4 *)(t + pos);
-    const uint32_t w[4] = {v.x, v.y, v.z, v.w};
+  if (q < hi) {
+    // align the target pointer to 16 bytes and stream whole vectors
+    const int64_t p0 = (int64_t)q + jb.diag;
+    const int64_t pa = p0 & ~(int64_t)15;
+    int skip = (int)(p0 - pa);
+    for (int64_t pos = pa; q < hi; pos += 16) {
+      const uint4 v = *(const uint4 *)(t + pos);
+      const uint32_t w[4] = {v.x, v.y, v.z, v.w};
+      int32_t s[16];
 #pragma unroll
-    for (int b = 0; b < 16; b++) {
-      if (b >= skip && q < hi) {
-        const uint32_t c = (w[b >> 2] >> ((b & 3) * 8)) & 0xffu;
-        const int32_t s = prof[(uint64_t)c * ld + q];
-        cur = (cur > 0 ? cur : 0) + s;
-        best = best > cur ? best : cur;
-        q++;
+      for (int b = 0; b < 16; b++) {
+        const int qq = q + b - skip;
+        s[b] = (b >= skip && qq < hi) ? prof[(uint64_t)((w[b >> 2] >> ((b & 3) * 8)) & 0xffu) * ld + qq] : 0;
       }
+#pragma unroll
+      for (int b = 0; b < 16; b++) {
+        if (b >= skip && q < hi) {
+          cur = (cur > 0 ? cur : 0) + s[b];
+          best = best > cur ? best : cur;
+          q++;
+        }
+      }
+      skip = 0;
     }
-    skip = 0;
   }
   out[i] = best;
 }
@@ -207,7 +247,7 @@ __global__ __launch_bounds__(128) void gate_recs_kernel(const RecDev *__restrict
   if (ns == 0) {
     r = GateRes{0, 0, 0, 0, 0, 0, 0, 0, 0};
   } else {
-    r = gate_eval(qd->prof32, qd->ld, t, sc, dg, lo, hi, ns, open, ext, min_diag_score,
+    r = gate_eval(ProfGlobal{qd->prof32, qd->ld}, t, sc, dg, lo, hi, ns, open, ext, min_diag_score,
                   regions != 0);
   }
   out[i] = r;
@@ -235,12 +275,30 @@ struct PairResultDev {
   uint8_t pass, kernel, r0, r1;
 };
 
-__global__ __launch_bounds__(128) void gate_pairs_kernel(PairGateArgs a) {
-  const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+// query rows held in LDS by gate_pairs_kernel
+constexpr int kGateLdsRows = 512;
+
+__global__ __launch_bounds__(64) void gate_pairs_kernel(PairGateArgs a) {
+  __shared__ int16_t lp[kCodes * kGateLdsRows];
+  const uint64_t base = (uint64_t)blockIdx.x * 64;
+  const uint64_t i = base + threadIdx.x;
+  const uint64_t last = base + 63 < a.npairs ? base + 63 : a.npairs - 1;
+  const uint32_t q0 = a.pair_query[base];
+  const QueryDev Q = a.queries[q0];
+  // one query for the whole block, short enough, with an int16 profile
+  const bool uni = a.pair_query[last] == q0 && Q.prof16 != nullptr && Q.m <= (uint32_t)kGateLdsRows;
+  if (uni) {
+    const int m = (int)Q.m;
+    for (int idx = threadIdx.x; idx < kCodes * m; idx += 64) {
+      const int c = idx / m, r = idx - c * m;
+      lp[c * m + r] = Q.prof16[(uint64_t)c * Q.ld + r];
+    }
+  }
+  __syncthreads();
   if (i >= a.npairs) return;
   const PairDev pd = a.pairs[i];
   const uint32_t qi = a.pair_query[i];
-  const QueryDev q = a.queries[qi];
+  const QueryDev q = uni ? Q : a.queries[qi];
   const int m = (int)q.m;
   const int n = (int)a.tlen[pd.tid];
   int32_t ts[4], td[4], lo[4], hi[4];
@@ -258,8 +316,10 @@ __global__ __launch_bounds__(128) void gate_pairs_kernel(PairGateArgs a) {
     hi[k] = b0 > a0 ? b0 : a0;
     nt++;
   }
-  const GateRes r = gate_eval(q.prof32, q.ld, a.arena + a.toff[pd.tid], ts, td, lo, hi, nt,
-                              a.open[qi], a.ext[qi], INT32_MIN, false);
+  const uint8_t *t = a.arena + a.toff[pd.tid];
+  const GateRes r = uni ? gate_eval(ProfLds{lp, m}, t, ts, td, lo, hi, nt, a.open[qi], a.ext[qi], INT32_MIN, false)
+                        : gate_eval(ProfGlobal{q.prof32, q.ld}, t, ts, td, lo, hi, nt, a.open[qi], a.ext[qi],
+                                    INT32_MIN, false);
   const int32_t g = a.gate[qi];
   PairResultDev o;
   const PairOut po = a.pout[pd.out];
@@ -273,31 +333,28 @@ __global__ __launch_bounds__(128) void gate_pairs_kernel(PairGateArgs a) {
   ((PairResultDev *)a.res)[pd.out] = o;
 }
 
-hipError_t launch_ung_jobs(const JobDev *jobs, uint64_t njobs, const QueryDev *qd,
-                           const uint8_t *arena, const uint64_t *toff, int32_t *out,
-                           hipStream_t s) {
+hipError_t launch_ung_jobs(const JobDev *jobs, uint64_t njobs, const QueryDev *qd, const uint8_t *arena,
+                           const uint64_t *toff, int32_t *out, hipStream_t s) {
   if (njobs == 0) return hipSuccess;
   const uint64_t blocks = (njobs + 255) / 256;
-  hipLaunchKernelGGL(ung_jobs_kernel, dim3((uint32_t)blocks), dim3(256), 0, s, jobs, njobs, qd,
-                     arena, toff, out);
+  hipLaunchKernelGGL(ung_jobs_kernel, dim3((uint32_t)blocks), dim3(256), 0, s, jobs, njobs, qd, arena, toff, out);
   return hipGetLastError();
 }
 
-hipError_t launch_gate_recs(const RecDev *recs, uint64_t nrecs, const JobDev *jobs,
-                            const int32_t *scores, const QueryDev *qd, const uint8_t *arena,
-                            const uint64_t *toff, int32_t open, int32_t ext, int32_t gate,
-                            int regions, GateRes *out, hipStream_t s) {
+hipError_t launch_gate_recs(const RecDev *recs, uint64_t nrecs, const JobDev *jobs, const int32_t *scores,
+                            const QueryDev *qd, const uint8_t *arena, const uint64_t *toff, int32_t open,
+                            int32_t ext, int32_t gate, int regions, GateRes *out, hipStream_t s) {
   if (nrecs == 0) return hipSuccess;
   const uint64_t blocks = (nrecs + 127) / 128;
-  hipLaunchKernelGGL(gate_recs_kernel, dim3((uint32_t)blocks), dim3(128), 0, s, recs, nrecs, jobs,
-                     scores, qd, arena, toff, open, ext, gate, regions, out);
+  hipLaunchKernelGGL(gate_recs_kernel, dim3((uint32_t)blocks), dim3(128), 0, s, recs, nrecs, jobs, scores, qd,
+                     arena, toff, open, ext, gate, regions, out);
   return hipGetLastError();
 }
 
 hipError_t launch_gate_pairs(const PairGateArgs &a, hipStream_t s) {
   if (a.npairs == 0) return hipSuccess;
-  const uint64_t blocks = (a.npairs + 127) / 128;
-  hipLaunchKernelGGL(gate_pairs_kernel, dim3((uint32_t)blocks), dim3(128), 0, s, a);
+  const uint64_t blocks = (a.npairs + 63) / 64;
+  hipLaunchKernelGGL(gate_pairs_kernel, dim3((uint32_t)blocks), dim3(64), 0, s, a);
   return hipGetLastError();
 }
 
