@@ -4,6 +4,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstdlib>
 #include <cstdio>
 #include <cstring>
 #include <mutex>
@@ -32,6 +33,7 @@ struct DpLaunch {
   PairOut *pout;
   uint64_t *top4;
   uint32_t grid;
+  int lds_stride;
 };
 hipError_t launch_dp(const DpLaunch &a, int kind, int R, bool ung, hipStream_t s);
 
@@ -120,10 +122,11 @@ using namespace st;
 
 struct st_ctx {
   int device = 0;
-  hipStream_t stream = nullptr;
+  hipStream_t stream = nullptr;  // main stream: uploads, gate, results
+  hipStream_t slots[4] = {nullptr, nullptr, nullptr, nullptr};  // concurrent DP launches
   std::mutex mu;
   int cus = 256;
-  DevBuf<uint32_t> counter;
+  DevBuf<uint32_t> counter;  // one unit counter per launch slot
 };
 
 struct st_db {
@@ -244,11 +247,17 @@ int st_ctx_create(int device, st_ctx **out) {
   }
   hipDeviceProp_t prop;
   if (hipGetDeviceProperties(&prop, device) == hipSuccess) c->cus = prop.multiProcessorCount;
-  if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
+  bool streams = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) == hipSuccess;
+  for (hipStream_t &s : c->slots) streams = streams && hipStreamCreateWithFlags(&s, hipStreamNonBlocking) == hipSuccess;
+  if (!streams) {
+    for (hipStream_t &s : c->slots)
+      if (s) (void)hipStreamDestroy(s);
+    if (c->stream) (void)hipStreamDestroy(c->stream);
     delete c;
     return fail(ST_ERR_DEVICE, "st_ctx_create: stream creation failed");
   }
   if (c->counter.alloc(4) != hipSuccess) {
+    for (hipStream_t &s : c->slots) (void)hipStreamDestroy(s);
     (void)hipStreamDestroy(c->stream);
     delete c;
     return fail(ST_ERR_NOMEM, "st_ctx_create: counter allocation failed");
@@ -261,6 +270,10 @@ int st_ctx_destroy(st_ctx *ctx) {
   if (!ctx) return ST_OK;
   (void)hipSetDevice(ctx->device);
   (void)hipStreamSynchronize(ctx->stream);
+  for (hipStream_t &s : ctx->slots) {
+    (void)hipStreamSynchronize(s);
+    (void)hipStreamDestroy(s);
+  }
   ctx->counter.release();
   (void)hipStreamDestroy(ctx->stream);
   delete ctx;
@@ -378,6 +391,8 @@ uint64_t st_qprof_len(const st_qprof *qp) { return qp ? qp->m : 0; }
 
 namespace st {
 
+int dp_occupancy(int kind, int R, bool ung, size_t lds_bytes);
+
 // Precision class of a query under given gaps.
 enum Kind { kPk16 = 0, kI32F = 1, kI32N = 2 };
 
@@ -391,22 +406,46 @@ struct Shape {
   int G, R;
 };
 
-// Launch shape for a query of length m against targets of typical length n:
-// minimise the modelled cost per useful cell (row padding to G*R, the skew's
-// G-1 idle steps per pass, the pass-boundary round trip and the fixed work of
-// a step).
-static Shape choose_shape(uint32_t m, double n, Kind kind) {
+// ST_FORCE_R=8|16 and ST_FORCE_G=1..64 pin the rows per lane and the lanes
+// per group (tuning runs and tests); read at every plan
+static int forced_rows() {
+  const char *e = std::getenv("ST_FORCE_R");
+  const int v = e ? std::atoi(e) : 0;
+  return (v == 8 || v == 16) ? v : 0;
+}
+
+static int forced_groups() {
+  const char *e = std::getenv("ST_FORCE_G");
+  const int v = e ? std::atoi(e) : 0;
+  return (v >= 1 && v <= 64 && (v & (v - 1)) == 0) ? v : 0;
+}
+
+// A unit longer than this many DP steps is penalised: one wave runs a unit
+// alone, so very long units set the tail of a persistent launch.
+constexpr double kUnitSteps = 12000.0;
+
+// Launch shape for a query of length m against targets of median length
+// n_med (longest n_max): minimise the modelled cost per useful cell (row
+// padding to passes x G x R, the skew's G-1 idle steps per pass, the
+// pass-boundary round trip, the fixed work of a step) with a penalty on
+// units whose step count would dominate the launch's tail.
+static Shape choose_shape(uint32_t m, double n_med, double n_max, Kind kind) {
   Shape best{16, 16};
   double bc = 1e300;
   for (int R = 8; R <= 16; R += 8) {
     if (kind == kI32N && R != 8) continue;
-    for (int G = 1; G <= 16; G <<= 1) {
+    if (forced_rows() && R != forced_rows() && kind != kI32N) continue;
+    for (int G = 1; G <= 64; G <<= 1) {
+      if (forced_groups() && G != forced_groups()) continue;
       const int GR = G * R;
       const uint32_t passes = std::max<uint32_t>(1, (m + GR - 1) / GR);
       const double m_pad = (double)passes * GR;
-      double c = (m_pad / std::max<uint32_t>(m, 1)) * ((n + G - 1) / std::max(n, 1.0));
+      double c = (m_pad / std::max<uint32_t>(m, 1)) * ((n_med + G - 1) / std::max(n_med, 1.0));
       c *= 1.0 + 1.6 / R;
       if (passes > 1) c *= 1.0 + 6.0 / GR;
+      if (G > 16) c *= 1.04;  // cross-row lane shuffles
+      const double steps = (double)passes * (n_max + G - 1) * R / 16.0;
+      if (steps > kUnitSteps) c *= steps / kUnitSteps;
       if (c < bc) {
         bc = c;
         best = Shape{G, R};
@@ -416,33 +455,41 @@ static Shape choose_shape(uint32_t m, double n, Kind kind) {
   return best;
 }
 
+// Streams (and unit counters) for concurrent DP launches.
+constexpr int kLaunchSlots = 4;
+
 // One kernel launch: units [unit0, unit0 + nunits) of one precision kind, one
-// R and one gap setting.
+// R, one gap setting and one LDS class; buffer region [buf0, buf0 + grid *
+// (2 half + edge)) of the batch's pass buffer.
 struct Launch {
   int kind, R;
   int32_t open, ext;
   uint32_t unit0, nunits;
+  int gr_max;
+  uint64_t half, buf0;
+  uint32_t grid;
+  int slot;
 };
 
 struct Plan {
   std::vector<Unit> units;
   std::vector<PairDev> pairs;
   std::vector<Launch> launches;
-  uint64_t passbuf_half = 1;
+  uint64_t buf_elems = 0;
 };
 
 // Add the pairs of query qi (result rows outs[], target ids tids[]) to `pl`:
 // targets sorted by length, cut into wave units of 64/G groups.
 static void plan_query(Plan &pl, std::vector<Unit> &pending, uint32_t qi, const st_qprof *qp,
-                       const uint32_t *tids, const uint64_t *outs, size_t count,
-                       Kind kind, const st_db *db) {
+                       const uint32_t *tids, const uint64_t *outs, size_t count, Kind kind, const st_db *db) {
   if (count == 0) return;
   std::vector<uint32_t> order(count);
   std::iota(order.begin(), order.end(), 0u);
   std::stable_sort(order.begin(), order.end(),
                    [&](uint32_t a, uint32_t b) { return db->hlen[tids[a]] < db->hlen[tids[b]]; });
   const double nmed = (double)db->hlen[tids[order[count / 2]]];
-  const Shape sh = choose_shape(qp->m, nmed, kind);
+  const double nmax = (double)db->hlen[tids[order[count - 1]]];
+  const Shape sh = choose_shape(qp->m, nmed, nmax, kind);
   const int per = kind == kPk16 ? 2 : 1;
   const uint32_t cap = (uint32_t)(64 / sh.G * per);
   const uint32_t GR = (uint32_t)(sh.G * sh.R);
@@ -466,43 +513,55 @@ static void plan_query(Plan &pl, std::vector<Unit> &pending, uint32_t qi, const 
       np = std::max(np, db->hlen[tids[o]]);
     }
     u.n_pad = np;
-    if (passes > 1) pl.passbuf_half = std::max<uint64_t>(pl.passbuf_half, (uint64_t)(64 / sh.G) * np);
     pending.push_back(u);
   }
-  (void)kind;
 }
 
-// Group pending units into launches by (kind, R, gaps), longest first.
+static double unit_cost(const Unit &u) { return (double)u.passes * u.R * (u.n_pad + u.G - 1); }
+
+// Group pending units into launches by (kind, R, gaps, LDS class), longest
+// first, and lay out their pass-buffer regions.
 static void finish_plan(Plan &pl, std::vector<Unit> &pending, const std::vector<Kind> &qkind,
-                        const std::vector<int32_t> &open, const std::vector<int32_t> &ext, int force_kind) {
+                        const std::vector<int32_t> &open, const std::vector<int32_t> &ext, int force_kind,
+                        int cus, bool ung) {
+  auto kind_of = [&](const Unit &u) { return force_kind >= 0 ? force_kind : (int)qkind[u.query]; };
+  auto big = [](const Unit &u) { return u.G * u.R > 256 ? 1 : 0; };
   std::stable_sort(pending.begin(), pending.end(), [&](const Unit &a, const Unit &b) {
-    const int ka = force_kind >= 0 ? force_kind : (int)qkind[a.query];
-    const int kb = force_kind >= 0 ? force_kind : (int)qkind[b.query];
-    if (ka != kb) return ka < kb;
+    if (kind_of(a) != kind_of(b)) return kind_of(a) < kind_of(b);
     if (a.R != b.R) return a.R < b.R;
+    if (big(a) != big(b)) return big(a) < big(b);
     if (open[a.query] != open[b.query]) return open[a.query] < open[b.query];
     if (ext[a.query] != ext[b.query]) return ext[a.query] < ext[b.query];
-    const double ca = (double)a.passes * a.R * (a.n_pad + a.G - 1);
-    const double cb = (double)b.passes * b.R * (b.n_pad + b.G - 1);
-    return ca > cb;
+    return unit_cost(a) > unit_cost(b);
   });
   for (size_t i = 0; i < pending.size();) {
     const Unit &u = pending[i];
-    Launch L;
-    L.kind = force_kind >= 0 ? force_kind : (int)qkind[u.query];
+    Launch L{};
+    L.kind = kind_of(u);
     L.R = (int)u.R;
     L.open = open[u.query];
     L.ext = ext[u.query];
-    L.unit0 = (uint32_t)(pl.units.size());
+    L.unit0 = (uint32_t)pl.units.size();
     size_t j = i;
+    L.gr_max = 0;
+    L.half = 0;
     while (j < pending.size()) {
       const Unit &v = pending[j];
-      const int kv = force_kind >= 0 ? force_kind : (int)qkind[v.query];
-      if (kv != L.kind || (int)v.R != L.R || open[v.query] != L.open || ext[v.query] != L.ext) break;
+      if (kind_of(v) != L.kind || (int)v.R != L.R || big(v) != big(u) || open[v.query] != L.open ||
+          ext[v.query] != L.ext)
+        break;
+      L.gr_max = std::max(L.gr_max, (int)(v.G * v.R));
+      if (v.passes > 1) L.half = std::max<uint64_t>(L.half, (uint64_t)(64 / v.G) * v.n_pad);
       pl.units.push_back(v);
       j++;
     }
     L.nunits = (uint32_t)(j - i);
+    const size_t lds = (size_t)kCodes * (L.gr_max + 2) * (L.kind == kPk16 ? 2 : 4);
+    const int occ = std::max(1, dp_occupancy(L.kind, L.R, ung, lds));
+    L.grid = std::max<uint32_t>(1, std::min<uint32_t>(L.nunits, (uint32_t)(occ * cus)));
+    L.slot = (int)(pl.launches.size() % kLaunchSlots);
+    L.buf0 = pl.buf_elems;
+    pl.buf_elems += (uint64_t)L.grid * (2 * L.half + kEdgeQuadsPerBlock);
     pl.launches.push_back(L);
     i = j;
   }
@@ -533,45 +592,42 @@ struct st_batch {
   DevBuf<uint64_t> d_top4;
   DevBuf<uint8_t> d_res;
   DevBuf<uint4> d_passbuf;
-  uint64_t passbuf_elems = 0;
-  uint32_t grid_cap = 0;
   // exact int32 rescoring of saturated int16 pairs (built at run time)
   Plan resc;
   DevBuf<Unit> d_rs_units;
   DevBuf<PairDev> d_rs_pairs;
+  DevBuf<uint4> d_rs_passbuf;
   uint64_t rescored = 0;
   // device time of the last run: main DP launches, int32 rescoring, gate
   hipEvent_t ev[6] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
+  hipEvent_t join[kLaunchSlots] = {nullptr, nullptr, nullptr, nullptr};
   double ms[3] = {0, 0, 0};
   ~st_batch() {
     for (hipEvent_t &e : ev)
+      if (e) (void)hipEventDestroy(e);
+    for (hipEvent_t &e : join)
       if (e) (void)hipEventDestroy(e);
   }
 };
 
 namespace st {
 
-static uint32_t grid_for(const st_batch *b, uint32_t nunits) {
-  const uint32_t cap = b->grid_cap ? b->grid_cap : (uint32_t)b->ctx->cus * 16u;
-  return std::max<uint32_t>(1, std::min(nunits, cap));
-}
-
-static int ensure_passbuf(st_batch *b, uint64_t half, uint32_t grid) {
-  const uint64_t need = half * 2 * grid;
-  if (need > b->passbuf_elems) {
-    if (b->d_passbuf.alloc(need) != hipSuccess) return fail(ST_ERR_NOMEM, "pass buffer allocation failed");
-    b->passbuf_elems = need;
-  }
-  return ST_OK;
-}
-
-static int launch_plan(st_batch *b, const Plan &pl, const Unit *d_units, const PairDev *d_pairs) {
+// Issue a plan's launches on the context's launch streams, after `start` on
+// the main stream, and make the main stream wait for all of them.
+static int launch_plan(st_batch *b, const Plan &pl, const Unit *d_units, const PairDev *d_pairs, uint4 *d_buf,
+                       hipEvent_t start) {
   st_ctx *ctx = b->ctx;
+  bool used[kLaunchSlots] = {false, false, false, false};
   for (const Launch &L : pl.launches) {
+    hipStream_t s = ctx->slots[L.slot];
+    if (!used[L.slot]) {
+      ST_HIP(hipStreamWaitEvent(s, start, 0));
+      used[L.slot] = true;
+    }
     DpLaunch a;
     a.units = d_units + L.unit0;
     a.nunits = L.nunits;
-    a.counter = ctx->counter.p;
+    a.counter = ctx->counter.p + L.slot;
     a.pairs = d_pairs;
     a.queries = b->d_queries.p;
     a.arena = b->db->arena.p;
@@ -579,15 +635,19 @@ static int launch_plan(st_batch *b, const Plan &pl, const Unit *d_units, const P
     a.tlen = b->db->len.p;
     a.open = L.open;
     a.ext = L.ext;
-    a.grid = grid_for(b, L.nunits);
-    a.passbuf = b->d_passbuf.p;
-    a.passbuf_half = pl.passbuf_half;
+    a.grid = L.grid;
+    a.passbuf = d_buf + L.buf0;
+    a.passbuf_half = L.half;
     a.pout = b->d_pout.p;
     a.top4 = b->d_top4.p;
-    if (a.passbuf_half * 2 * a.grid > b->passbuf_elems)
-      return fail(ST_ERR_NOMEM, "pass buffer smaller than the plan needs");
-    ST_HIP(launch_dp(a, L.kind, L.R, b->ung, ctx->stream));
+    a.lds_stride = L.gr_max + 2;
+    ST_HIP(launch_dp(a, L.kind, L.R, b->ung, s));
   }
+  for (int k = 0; k < kLaunchSlots; k++)
+    if (used[k]) {
+      ST_HIP(hipEventRecord(b->join[k], ctx->slots[k]));
+      ST_HIP(hipStreamWaitEvent(ctx->stream, b->join[k], 0));
+    }
   return ST_OK;
 }
 
@@ -635,6 +695,11 @@ static int batch_create_impl(st_ctx *ctx, const st_db *db, const st_qprof *const
     }
   b->cells = cells;
   b->qkind.resize(nq);
+  std::lock_guard<std::mutex> lk(ctx->mu);
+  if (hipSetDevice(ctx->device) != hipSuccess) {
+    delete b;
+    return fail(ST_ERR_DEVICE, "st_batch_create: hipSetDevice failed");
+  }
   std::vector<Unit> pending;
   for (uint64_t q = 0; q < nq; q++) {
     b->qkind[q] = kind_for(b->qps[q], b->open[q], b->ext[q]);
@@ -647,12 +712,7 @@ static int batch_create_impl(st_ctx *ctx, const st_db *db, const st_qprof *const
     std::iota(outs.begin(), outs.end(), s);
     plan_query(b->plan, pending, (uint32_t)q, b->qps[q], cand_tids + s, outs.data(), e - s, b->qkind[q], db);
   }
-  finish_plan(b->plan, pending, b->qkind, b->open, b->ext, -1);
-  std::lock_guard<std::mutex> lk(ctx->mu);
-  if (hipSetDevice(ctx->device) != hipSuccess) {
-    delete b;
-    return fail(ST_ERR_DEVICE, "st_batch_create: hipSetDevice failed");
-  }
+  finish_plan(b->plan, pending, b->qkind, b->open, b->ext, -1, ctx->cus, ung);
   std::vector<QueryDev> qv(nq);
   for (uint64_t q = 0; q < nq; q++) qv[q] = b->qps[q]->view;
   std::vector<PairDev> all(np);
@@ -660,8 +720,6 @@ static int batch_create_impl(st_ctx *ctx, const st_db *db, const st_qprof *const
     all[k].tid = cand_tids[k];
     all[k].out = (uint32_t)k;
   }
-  // the pass buffer covers the largest plan; rescoring plans reuse it when they fit
-  const uint32_t grid = grid_for(b, (uint32_t)std::max<size_t>(b->plan.units.size(), 1));
   bool ok = b->d_queries.upload(qv.data(), qv.size(), ctx->stream) == hipSuccess &&
             b->d_open.upload(b->open.data(), nq, ctx->stream) == hipSuccess &&
             b->d_ext.upload(b->ext.data(), nq, ctx->stream) == hipSuccess &&
@@ -673,7 +731,7 @@ static int batch_create_impl(st_ctx *ctx, const st_db *db, const st_qprof *const
             b->d_pout.alloc(std::max<uint64_t>(np, 1)) == hipSuccess &&
             b->d_top4.alloc(ung ? std::max<uint64_t>(np, 1) * 4 : 1) == hipSuccess &&
             b->d_res.alloc(std::max<uint64_t>(np, 1) * sizeof(st_pair_result)) == hipSuccess &&
-            ensure_passbuf(b, b->plan.passbuf_half, grid) == ST_OK;
+            b->d_passbuf.alloc(std::max<uint64_t>(b->plan.buf_elems, 1)) == hipSuccess;
   if (!ok) {
     delete b;
     return fail(ST_ERR_NOMEM, "st_batch_create: device allocation failed");
@@ -694,8 +752,10 @@ static int batch_run_locked(st_batch *b) {
   ST_HIP(hipSetDevice(ctx->device));
   for (hipEvent_t &e : b->ev)
     if (!e) ST_HIP(hipEventCreate(&e));
+  for (hipEvent_t &e : b->join)
+    if (!e) ST_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
   ST_HIP(hipEventRecord(b->ev[0], ctx->stream));
-  int rc = launch_plan(b, b->plan, b->d_units.p, b->d_pairs.p);
+  int rc = launch_plan(b, b->plan, b->d_units.p, b->d_pairs.p, b->d_passbuf.p, b->ev[0]);
   if (rc) return rc;
   ST_HIP(hipEventRecord(b->ev[1], ctx->stream));
   // int16 pairs whose best reached 32767 are rescored in exact int32
@@ -721,15 +781,14 @@ static int batch_run_locked(st_batch *b) {
       b->rescored += perq[q].size();
       plan_query(b->resc, pending, (uint32_t)q, b->qps[q], tids.data(), perq[q].data(), tids.size(), kI32F, b->db);
     }
-    finish_plan(b->resc, pending, b->qkind, b->open, b->ext, (int)kI32F);
+    finish_plan(b->resc, pending, b->qkind, b->open, b->ext, (int)kI32F, ctx->cus, b->ung);
     if (!b->resc.units.empty()) {
       if (b->d_rs_units.upload(b->resc.units.data(), b->resc.units.size(), ctx->stream) != hipSuccess ||
-          b->d_rs_pairs.upload(b->resc.pairs.data(), b->resc.pairs.size(), ctx->stream) != hipSuccess)
+          b->d_rs_pairs.upload(b->resc.pairs.data(), b->resc.pairs.size(), ctx->stream) != hipSuccess ||
+          b->d_rs_passbuf.alloc(std::max<uint64_t>(b->resc.buf_elems, 1)) != hipSuccess)
         return fail(ST_ERR_NOMEM, "rescoring upload failed");
-      rc = ensure_passbuf(b, b->resc.passbuf_half, grid_for(b, (uint32_t)b->resc.units.size()));
-      if (rc) return rc;
       ST_HIP(hipEventRecord(b->ev[2], ctx->stream));
-      rc = launch_plan(b, b->resc, b->d_rs_units.p, b->d_rs_pairs.p);
+      rc = launch_plan(b, b->resc, b->d_rs_units.p, b->d_rs_pairs.p, b->d_rs_passbuf.p, b->ev[2]);
       if (rc) return rc;
       ST_HIP(hipEventRecord(b->ev[3], ctx->stream));
     }

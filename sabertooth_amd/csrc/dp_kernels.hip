@@ -13,13 +13,14 @@
 // keys; the gate kernel finishes from those.
 //
 // Mapping.  A work unit is one wave64 and one query.  Lanes form groups of G
-// (1..16, inside a 16-lane DPP row); a group scores two targets at once in
+// (1..64 lanes); a group scores two targets at once in
 // the int16 halves of 32-bit registers (Pk16) or one target in int32
 // (I32F/I32N).  Lane l of a group owns R consecutive query rows of the
 // current pass (G*R rows per pass) with H/E/U/B for those rows in VGPRs; the
 // group walks the target columns with lane l skewed l steps behind lane l-1,
 // so the row boundary (H, F, U, B of the lane's last row) moves to the next
-// lane by one DPP row_shr:1 per step.  Between passes the last lane's
+// lane by one DPP row_shr:1 per step (a cross-row lane shuffle for groups of
+// 32 or 64 lanes).  Between passes the last lane's
 // boundary row goes through a per-block global buffer (prefetched one column
 // ahead).  The query profile of the pass (22 codes x G*R rows) is staged in
 // LDS; target codes stream into the group's first lane as dwords fetched
@@ -64,6 +65,11 @@ __device__ __forceinline__ uint4 ldg_u4(const uint4 *p) {
 }
 
 constexpr uint32_t kSent4 = 0x15151515u;  // four zero-score codes
+#ifndef ST_STEP_UNROLL
+#define ST_STEP_UNROLL 1
+#endif
+// DP steps per loop iteration
+constexpr int kStepUnroll = ST_STEP_UNROLL;
 
 // ---- arithmetic policies -----------------------------------------------------
 
@@ -142,9 +148,12 @@ __device__ __forceinline__ uint32_t shr1u(uint32_t v) {
   return (uint32_t)__builtin_amdgcn_update_dpp((int)v, (int)v, 0x111, 0xf, 0xf, false);
 }
 
+// lane i <- lane i-1 across the whole wave (groups of 32 or 64 lanes)
+__device__ __forceinline__ uint32_t shr1w(uint32_t v) { return (uint32_t)__shfl_up((int)v, 1); }
+
 template <class P>
-__device__ __forceinline__ typename P::T shr1(typename P::T v) {
-  return (typename P::T)shr1u((uint32_t)v);
+__device__ __forceinline__ typename P::T shr1(typename P::T v, bool wide) {
+  return (typename P::T)(wide ? shr1w((uint32_t)v) : shr1u((uint32_t)v));
 }
 
 template <class P>
@@ -156,6 +165,243 @@ __device__ __forceinline__ const int32_t *prof_of<I32F>(const QueryDev &q) { ret
 template <>
 __device__ __forceinline__ const int32_t *prof_of<I32N>(const QueryDev &q) { return q.prof32; }
 
+// One step's profile words: Pk16 reads R/2 dwords (row pairs) per target,
+// the int32 kernels R dwords.  Row pointers are dword aligned (even stride,
+// even R).
+template <class P, int R>
+__device__ __forceinline__ void load_raw(const typename P::L *lds, int stride, uint32_t code, int l,
+                                         uint32_t (&raw)[R]) {
+  if (sizeof(typename P::L) == 2) {
+    const uint32_t *ra = (const uint32_t *)(lds + (int)(code & 0xff) * stride + l * R);
+    const uint32_t *rb = (const uint32_t *)(lds + (int)((code >> 8) & 0xff) * stride + l * R);
+#pragma unroll
+    for (int i = 0; i < R / 2; i++) {
+      raw[i] = ra[i];
+      raw[R / 2 + i] = rb[i];
+    }
+  } else {
+    const uint32_t *ra = (const uint32_t *)(lds + (int)(code & 0xff) * stride + l * R);
+#pragma unroll
+    for (int i = 0; i < R; i++) raw[i] = ra[i];
+  }
+}
+
+// raw words -> per-row scores (Pk16: first target low half, second high)
+template <class P, int R>
+__device__ __forceinline__ void unpack_raw(const uint32_t (&raw)[R], typename P::T (&sc)[R]) {
+  if (sizeof(typename P::L) == 2) {
+#pragma unroll
+    for (int i = 0; i < R / 2; i++) {
+      sc[2 * i] = (typename P::T)__builtin_amdgcn_perm(raw[R / 2 + i], raw[i], 0x05040100u);
+      sc[2 * i + 1] = (typename P::T)__builtin_amdgcn_perm(raw[R / 2 + i], raw[i], 0x07060302u);
+    }
+  } else {
+#pragma unroll
+    for (int i = 0; i < R; i++) sc[i] = (typename P::T)raw[i];
+  }
+}
+
+// Per-unit, per-group state shared by the passes.
+template <class P>
+struct GroupCtx {
+  int G, l, g, n_pad, m, m_pad;
+  int len[2];
+  const uint8_t *tp[2];
+  bool has[2];
+};
+
+// One pass of G*R rows over all n_pad columns.  LAST: the pass holding the
+// matrix's last padded row (bottom-row diagonal ends are folded here).
+template <class P, int R, bool UNG, bool LAST>
+__device__ __forceinline__ void run_pass(const GroupCtx<P> &c, const typename P::L *lds, int stride, int base,
+                                         bool first, bool final_pass, const uint4 *rd, uint4 *wr,
+                                         typename P::T e0, typename P::T vopen, typename P::T vext,
+                                         typename P::T &best, typename P::K (&top)[P::kPer][4],
+                                         typename P::T *edge, int lane) {
+  using T = typename P::T;
+  using K = typename P::K;
+  constexpr int kPer = P::kPer;
+  const int l = c.l, G = c.G, n_pad = c.n_pad, m = c.m;
+  const bool wide = G > 16;  // groups wider than a 16-lane DPP row
+  const T kInvalid = (T)0xFFFFFFFFu;  // a negative B: not a diagonal end
+
+  T H[R], E[R], U[R], B[R];
+#pragma unroll
+  for (int k = 0; k < R; k++) {
+    H[k] = 0;
+    E[k] = e0;
+    U[k] = 0;
+    B[k] = 0;
+  }
+  T hdiag = 0, udiag = 0, bdiag = 0;
+  T out_h = 0, out_f = e0, out_u = 0, out_b = 0;
+  // the group's first lane streams residues: cw = columns [4u, 4u + 4),
+  // nw = the next four; the boundary row of the previous pass one ahead
+  uint32_t cw[2] = {kSent4, kSent4}, nw[2] = {kSent4, kSent4};
+  uint4 pb = make_uint4(0, (uint32_t)e0, 0, 0);
+  uint32_t code = kSentinel | (kSentinel << 8);
+  if (l == 0) {
+#pragma unroll
+    for (int x = 0; x < kPer; x++) {
+      if (c.len[x] > 0) cw[x] = ldg_u32(c.tp[x]);
+      if (c.len[x] > 4) nw[x] = ldg_u32(c.tp[x] + 4);
+    }
+    if (!first && n_pad > 0) pb = ldg_u4(rd);
+    code = (cw[0] & 0xffu) | ((cw[1] & 0xffu) << 8);
+  }
+  // scores of the current step are fetched one step ahead
+  uint32_t raw[R];
+  load_raw<P, R>(lds, stride, code, l, raw);
+  // bottom row (LAST): the value the group's last lane produced at step t is
+  // kept by lane t mod G and folded every G steps
+  T pending = kInvalid;
+
+  const int steps = n_pad + G - 1;
+  for (int t0 = 0; t0 < steps; t0 += kStepUnroll) {
+#pragma unroll
+    for (int s = 0; s < kStepUnroll; s++) {
+      const int t = t0 + s;
+      if (t >= steps) break;
+      // next step's code: the upper lane's current one, or the stream
+      uint32_t nc = wide ? shr1w(code) : shr1u(code);
+      if (l == 0) {
+        if ((t & 3) < 3) {
+          const int sh = ((t + 1) & 3) * 8;
+          nc = ((cw[0] >> sh) & 0xffu) | (((cw[1] >> sh) & 0xffu) << 8);
+        } else {
+          nc = (nw[0] & 0xffu) | ((nw[1] & 0xffu) << 8);
+          const int nx = t + 5;
+#pragma unroll
+          for (int x = 0; x < kPer; x++) {
+            cw[x] = nw[x];
+            nw[x] = nx < c.len[x] ? ldg_u32(c.tp[x] + nx) : kSent4;
+          }
+        }
+      }
+      uint32_t nraw[R];
+      load_raw<P, R>(lds, stride, nc, l, nraw);
+
+      // boundary from the upper lane (or the previous pass / the matrix edge)
+      T in_h = shr1<P>(out_h, wide), in_f = shr1<P>(out_f, wide);
+      T in_u = 0, in_b = 0;
+      if (UNG) {
+        in_u = shr1<P>(out_u, wide);
+        in_b = shr1<P>(out_b, wide);
+      }
+      if (l == 0) {
+        if (!first && t < n_pad) {
+          in_h = (T)pb.x;
+          in_f = (T)pb.y;
+          in_u = (T)pb.z;
+          in_b = (T)pb.w;
+        } else {
+          in_h = 0;
+          in_f = e0;
+          in_u = 0;
+          in_b = 0;
+        }
+        if (!first && t + 1 < n_pad) pb = ldg_u4(rd + t + 1);
+      }
+      const int j = t - l;
+      const bool live = P::kFloor || (j >= 0 && j < n_pad);
+      if (live) {
+        T sc[R];
+        unpack_raw<P, R>(raw, sc);
+        // diagonal moves, last row first so row k-1 still holds column j-1
+#pragma unroll
+        for (int k = R - 1; k >= 1; k--) {
+          if (UNG) {
+            U[k] = P::add(P::max(U[k - 1], 0), sc[k]);
+            B[k] = P::max(B[k - 1], U[k]);
+          }
+          H[k] = P::add(H[k - 1], sc[k]);
+        }
+        if (UNG) {
+          U[0] = P::add(P::max(udiag, 0), sc[0]);
+          B[0] = P::max(bdiag, U[0]);
+        }
+        H[0] = P::add(hdiag, sc[0]);
+        // vertical / horizontal gaps, first row first
+        T F = in_f;
+#pragma unroll
+        for (int k = 0; k < R; k++) {
+          T h = P::max(P::max(H[k], E[k]), F);
+          if (!P::kFloor) h = P::max(h, 0);
+          H[k] = h;
+          if (P::kFloor) {
+            best = P::max(best, h);
+          } else if (j < c.len[0] && base + l * R + k < m) {
+            best = P::max(best, h);
+          }
+          const T ho = P::subf(h, vopen);
+          E[k] = P::max(P::subf(E[k], vext), ho);
+          F = P::max(P::subf(F, vext), ho);
+        }
+        out_h = H[R - 1];
+        out_f = F;
+        if (UNG) {
+          out_u = U[R - 1];
+          out_b = B[R - 1];
+        }
+      }
+      hdiag = in_h;
+      if (UNG) {
+        udiag = in_u;
+        bdiag = in_b;
+      }
+      code = nc;
+#pragma unroll
+      for (int k = 0; k < R; k++) raw[k] = nraw[k];
+      // boundary row for the next pass
+      if (!final_pass && l == G - 1 && j >= 0 && j < n_pad) {
+        uint4 v;
+        v.x = (uint32_t)out_h;
+        v.y = (uint32_t)out_f;
+        v.z = (uint32_t)out_u;
+        v.w = (uint32_t)out_b;
+        wr[j] = v;
+      }
+      if (UNG) {
+        // right edge: park this lane's rows; folded after the pass
+        if (j == n_pad - 1) {
+#pragma unroll
+          for (int k = 0; k < R; k++) edge[lane * R + k] = B[k];
+        }
+        if (LAST) {
+          // bottom row: the group's last lane produces one value per step
+          const T mine = (j >= 0 && j < n_pad - 1) ? B[R - 1] : kInvalid;
+          const int slot = t & (G - 1);
+          const T v = G > 1 ? (T)__shfl((int)mine, (lane & ~(G - 1)) + G - 1) : mine;
+          if (slot == l) pending = v;
+          if (slot == G - 1 || t == steps - 1) {
+            // lane l's pending value came from step t - slot + l
+            const int d = (t - slot + l - (G - 1)) - (c.m_pad - 1);
+#pragma unroll
+            for (int x = 0; x < kPer; x++) {
+              const int sv = P::half(pending, x);
+              if (l <= slot && c.has[x] && sv >= 0 && d >= -(m - 1) && d <= c.len[x] - 1)
+                insert4<K>(top[x], P::key(sv, (uint32_t)(d + m - 1)));
+            }
+            pending = kInvalid;
+          }
+        }
+      }
+    }
+  }
+  if (UNG) {
+    // fold this pass's right-edge values (each lane its own rows)
+#pragma unroll
+    for (int k = 0; k < R; k++) {
+      const T v = edge[lane * R + k];
+      const int d = (n_pad - 1) - (base + l * R + k);
+#pragma unroll
+      for (int x = 0; x < kPer; x++)
+        if (c.has[x] && d >= -(m - 1) && d <= c.len[x] - 1)
+          insert4<K>(top[x], P::key(P::half(v, x), (uint32_t)(d + m - 1)));
+    }
+  }
+}
+
 // Persistent DP kernel: every block (one wave) pulls units from *counter.
 template <class P, int R, bool UNG>
 __global__ __launch_bounds__(64) void dp_kernel(
@@ -164,22 +410,23 @@ __global__ __launch_bounds__(64) void dp_kernel(
     const uint8_t *__restrict__ arena, const uint64_t *__restrict__ toff,
     const uint32_t *__restrict__ tlen, int32_t gap_open, int32_t gap_ext,
     uint4 *__restrict__ passbuf, uint64_t passbuf_half, PairOut *__restrict__ pout,
-    uint64_t *__restrict__ top4) {
+    uint64_t *__restrict__ top4, int lds_stride) {
   using T = typename P::T;
   using L = typename P::L;
   using K = typename P::K;
   constexpr int kPer = P::kPer;
-  __shared__ L lds[kCodes * kLdsStride];
-  __shared__ T edge[UNG ? 64 * R : 1];
-  __shared__ T bottom[UNG ? 64 * kBottomChunk : 1];
+  // the pass's profile slice: kCodes rows of lds_stride (>= G*R + 2) entries
+  extern __shared__ __align__(16) unsigned char lds_raw[];
+  L *const lds = (L *)lds_raw;
   const int lane = threadIdx.x;
-  uint4 *const myb = passbuf + (uint64_t)blockIdx.x * 2 * passbuf_half;
+  // per block: two boundary halves, then 64 lanes x 16 right-edge words
+  uint4 *const myb = passbuf + (uint64_t)blockIdx.x * (2 * passbuf_half + kEdgeQuadsPerBlock);
+  T *const edge = (T *)(myb + 2 * passbuf_half);
 
   const T vopen = P::splat(gap_open);
   const T vext = P::splat(gap_ext);
   // boundary state of the row above the matrix / column left of it
   const T e0 = P::kFloor ? T(0) : T(max(-gap_open, (INT32_MIN / 4) - gap_ext));
-  const T kInvalid = (T)0xFFFFFFFFu;  // a negative B: not a diagonal end
 
   for (;;) {
     uint32_t got = 0;
@@ -189,33 +436,33 @@ __global__ __launch_bounds__(64) void dp_kernel(
     const Unit u = units[ui];
     const QueryDev q = queries[u.query];
     const L *prof = prof_of<P>(q);
-    const int G = (int)u.G;
-    const int l = lane & (G - 1);
-    const int g = lane / G;
-    const int GR = G * R;
-    const int n_pad = (int)u.n_pad;
-    const int m = (int)q.m;
-    const int m_pad = (int)u.passes * GR;
+    GroupCtx<P> c;
+    c.G = (int)u.G;
+    c.l = lane & (c.G - 1);
+    c.g = lane / c.G;
+    const int GR = c.G * R;
+    c.n_pad = (int)u.n_pad;
+    c.m = (int)q.m;
+    c.m_pad = (int)u.passes * GR;
 
     // this group's pair slots
-    int len[2] = {0, 0};
-    const uint8_t *tp[2] = {arena, arena};
     uint32_t outi[2] = {0, 0};
     bool valid[2] = {false, false};
+    c.len[0] = c.len[1] = 0;
+    c.tp[0] = c.tp[1] = arena;
 #pragma unroll
     for (int x = 0; x < kPer; x++) {
-      const uint32_t slot = (uint32_t)(g * kPer + x);
+      const uint32_t slot = (uint32_t)(c.g * kPer + x);
       if (slot < u.npairs) {
         const PairDev pd = pairs[u.pair0 + slot];
         valid[x] = true;
-        len[x] = (int)tlen[pd.tid];
-        tp[x] = arena + toff[pd.tid];
+        c.len[x] = (int)tlen[pd.tid];
+        c.tp[x] = arena + toff[pd.tid];
         outi[x] = pd.out;
       }
     }
-    bool has[2];
 #pragma unroll
-    for (int x = 0; x < 2; x++) has[x] = valid[x] && len[x] > 0 && m > 0;
+    for (int x = 0; x < 2; x++) c.has[x] = valid[x] && c.len[x] > 0 && c.m > 0;
 
     K top[kPer][4];
 #pragma unroll
@@ -231,179 +478,29 @@ __global__ __launch_bounds__(64) void dp_kernel(
       if (sizeof(L) == 2) {
         const int dw = GR / 2;
         for (int idx = lane; idx < kCodes * dw; idx += 64) {
-          const int c = idx / dw, r = idx - c * dw;
-          const uint32_t v = *(const uint32_t *)(prof + (uint64_t)c * q.ld + base + 2 * r);
-          *(uint32_t *)(lds + c * kLdsStride + 2 * r) = v;
+          const int cc = idx / dw, r = idx - cc * dw;
+          const uint32_t v = *(const uint32_t *)(prof + (uint64_t)cc * q.ld + base + 2 * r);
+          *(uint32_t *)(lds + cc * lds_stride + 2 * r) = v;
         }
       } else {
         for (int idx = lane; idx < kCodes * GR; idx += 64) {
-          const int c = idx / GR, r = idx - c * GR;
-          lds[c * kLdsStride + r] = prof[(uint64_t)c * q.ld + base + r];
+          const int cc = idx / GR, r = idx - cc * GR;
+          lds[cc * lds_stride + r] = prof[(uint64_t)cc * q.ld + base + r];
         }
       }
       __syncthreads();
-
-      const uint4 *rd = myb + (uint64_t)((p + 1) & 1) * passbuf_half + (uint64_t)g * n_pad;
-      uint4 *wr = myb + (uint64_t)(p & 1) * passbuf_half + (uint64_t)g * n_pad;
-      const bool last = p == (int)u.passes - 1;
-
-      T H[R], E[R], U[R], B[R];
-#pragma unroll
-      for (int k = 0; k < R; k++) {
-        H[k] = 0;
-        E[k] = e0;
-        U[k] = 0;
-        B[k] = 0;
-      }
-      T hdiag = 0, udiag = 0, bdiag = 0;
-      T out_h = 0, out_f = e0, out_u = 0, out_b = 0;
-      uint32_t code = kSentinel | (kSentinel << 8);
-      // the group's first lane streams residues: cw = columns [t & ~3, +4),
-      // nw = the next four; the boundary row of the previous pass one ahead
-      uint32_t cw[2] = {kSent4, kSent4}, nw[2] = {kSent4, kSent4};
-      uint4 pb = make_uint4(0, (uint32_t)e0, 0, 0);
-      if (l == 0) {
-#pragma unroll
-        for (int x = 0; x < kPer; x++) {
-          if (len[x] > 0) cw[x] = ldg_u32(tp[x]);
-          if (len[x] > 4) nw[x] = ldg_u32(tp[x] + 4);
-        }
-        if (p > 0 && n_pad > 0) pb = ldg_u4(rd);
-      }
-      const int steps = n_pad + G - 1;
-      for (int t = 0; t < steps; t++) {
-        T in_h = shr1<P>(out_h), in_f = shr1<P>(out_f);
-        T in_u = 0, in_b = 0;
-        if (UNG) {
-          in_u = shr1<P>(out_u);
-          in_b = shr1<P>(out_b);
-        }
-        uint32_t in_code = shr1u(code);
-        if (l == 0) {
-          if (p > 0 && t < n_pad) {
-            in_h = (T)pb.x;
-            in_f = (T)pb.y;
-            in_u = (T)pb.z;
-            in_b = (T)pb.w;
-          } else {
-            in_h = 0;
-            in_f = e0;
-            in_u = 0;
-            in_b = 0;
-          }
-          if (p > 0 && t + 1 < n_pad) pb = ldg_u4(rd + t + 1);
-          const int sh = (t & 3) * 8;
-          in_code = ((cw[0] >> sh) & 0xffu) | (((cw[1] >> sh) & 0xffu) << 8);
-          if ((t & 3) == 3) {
-            const int nx = t + 5;
-#pragma unroll
-            for (int x = 0; x < kPer; x++) {
-              cw[x] = nw[x];
-              nw[x] = nx < len[x] ? ldg_u32(tp[x] + nx) : kSent4;
-            }
-          }
-        }
-        code = in_code;
-        const int j = t - l;
-        const bool live = P::kFloor || (j >= 0 && j < n_pad);
-        if (live) {
-          const L *rowA = lds + (int)(code & 0xff) * kLdsStride + l * R;
-          const L *rowB = lds + (int)((code >> 8) & 0xff) * kLdsStride + l * R;
-          T sc[R];
-#pragma unroll
-          for (int k = 0; k < R; k++) sc[k] = P::score(rowA, rowB, k);
-          // diagonal moves, last row first so row k-1 still holds column j-1
-#pragma unroll
-          for (int k = R - 1; k >= 1; k--) {
-            if (UNG) {
-              U[k] = P::add(P::max(U[k - 1], 0), sc[k]);
-              B[k] = P::max(B[k - 1], U[k]);
-            }
-            H[k] = P::add(H[k - 1], sc[k]);
-          }
-          if (UNG) {
-            U[0] = P::add(P::max(udiag, 0), sc[0]);
-            B[0] = P::max(bdiag, U[0]);
-          }
-          H[0] = P::add(hdiag, sc[0]);
-          // vertical / horizontal gaps, first row first
-          T F = in_f;
-#pragma unroll
-          for (int k = 0; k < R; k++) {
-            T h = P::max(P::max(H[k], E[k]), F);
-            if (!P::kFloor) h = P::max(h, 0);
-            H[k] = h;
-            if (P::kFloor) {
-              best = P::max(best, h);
-            } else if (j < len[0] && base + l * R + k < m) {
-              best = P::max(best, h);
-            }
-            const T ho = P::subf(h, vopen);
-            E[k] = P::max(P::subf(E[k], vext), ho);
-            F = P::max(P::subf(F, vext), ho);
-          }
-          out_h = H[R - 1];
-          out_f = F;
-          if (UNG) {
-            out_u = U[R - 1];
-            out_b = B[R - 1];
-          }
-        }
-        hdiag = in_h;
-        if (UNG) {
-          udiag = in_u;
-          bdiag = in_b;
-        }
-        // boundary row for the next pass
-        if (!last && l == G - 1 && j >= 0 && j < n_pad) {
-          uint4 v;
-          v.x = (uint32_t)out_h;
-          v.y = (uint32_t)out_f;
-          v.z = (uint32_t)out_u;
-          v.w = (uint32_t)out_b;
-          wr[j] = v;
-        }
-        if (UNG) {
-          // right edge: park this lane's rows; folded after the pass
-          if (j == n_pad - 1) {
-#pragma unroll
-            for (int k = 0; k < R; k++) edge[lane * R + k] = B[k];
-          }
-          if (last) {
-            // bottom row: the group's last lane parks one value per step
-            const int slot = t % kBottomChunk;
-            if (l == G - 1) bottom[g * kBottomChunk + slot] = (j >= 0 && j < n_pad - 1) ? B[R - 1] : kInvalid;
-            if (slot == kBottomChunk - 1 || t == steps - 1) {
-              const int t0 = t - slot;
-              for (int s = l; s <= slot; s += G) {
-                const T v = bottom[g * kBottomChunk + s];
-                const int d = (t0 + s - (G - 1)) - (m_pad - 1);
-#pragma unroll
-                for (int x = 0; x < kPer; x++) {
-                  const int sc = P::half(v, x);
-                  if (has[x] && sc >= 0 && d >= -(m - 1) && d <= len[x] - 1)
-                    insert4<K>(top[x], P::key(sc, (uint32_t)(d + m - 1)));
-                }
-              }
-            }
-          }
-        }
-      }
-      if (UNG) {
-        // fold this pass's right-edge values (each lane its own rows)
-#pragma unroll
-        for (int k = 0; k < R; k++) {
-          const T v = edge[lane * R + k];
-          const int d = (n_pad - 1) - (base + l * R + k);
-#pragma unroll
-          for (int x = 0; x < kPer; x++)
-            if (has[x] && d >= -(m - 1) && d <= len[x] - 1)
-              insert4<K>(top[x], P::key(P::half(v, x), (uint32_t)(d + m - 1)));
-        }
-      }
+      const uint4 *rd = myb + (uint64_t)((p + 1) & 1) * passbuf_half + (uint64_t)c.g * c.n_pad;
+      uint4 *wr = myb + (uint64_t)(p & 1) * passbuf_half + (uint64_t)c.g * c.n_pad;
+      const bool final_pass = p == (int)u.passes - 1;
+      if (UNG && final_pass)
+        run_pass<P, R, UNG, true>(c, lds, lds_stride, base, p == 0, true, rd, wr, e0, vopen, vext, best, top,
+                                  edge, lane);
+      else
+        run_pass<P, R, UNG, false>(c, lds, lds_stride, base, p == 0, final_pass, rd, wr, e0, vopen, vext, best,
+                                   top, edge, lane);
     }
     // best and top-4 over the group's lanes
-    for (int off = 1; off < G; off <<= 1) {
+    for (int off = 1; off < c.G; off <<= 1) {
       const T o = (T)__shfl_xor((int)best, off);
       best = P::max(best, o);
       if (UNG) {
@@ -417,7 +514,7 @@ __global__ __launch_bounds__(64) void dp_kernel(
         }
       }
     }
-    if (l == 0) {
+    if (c.l == 0) {
 #pragma unroll
       for (int x = 0; x < kPer; x++)
         if (valid[x]) {
@@ -457,15 +554,40 @@ struct DpLaunch {
   PairOut *pout;
   uint64_t *top4;
   uint32_t grid;
+  int lds_stride;  // >= largest G*R of the launch + 2, even
 };
+
+// Resident blocks per CU of a kernel instance with `lds_bytes` of dynamic LDS.
+int dp_occupancy(int kind, int R, bool ung, size_t lds_bytes) {
+  int n = 0;
+#define ST_OCC(PP, RR, UU) (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, dp_kernel<PP, RR, UU>, 64, lds_bytes)
+  if (kind == 0) {
+    if (R == 8) {
+      if (ung) ST_OCC(Pk16, 8, true); else ST_OCC(Pk16, 8, false);
+    } else {
+      if (ung) ST_OCC(Pk16, 16, true); else ST_OCC(Pk16, 16, false);
+    }
+  } else if (kind == 1) {
+    if (R == 8) {
+      if (ung) ST_OCC(I32F, 8, true); else ST_OCC(I32F, 8, false);
+    } else {
+      if (ung) ST_OCC(I32F, 16, true); else ST_OCC(I32F, 16, false);
+    }
+  } else {
+    if (ung) ST_OCC(I32N, 8, true); else ST_OCC(I32N, 8, false);
+  }
+#undef ST_OCC
+  return n;
+}
 
 // kind: 0 = Pk16, 1 = I32F, 2 = I32N; R: 8 or 16 (I32N: 8)
 hipError_t launch_dp(const DpLaunch &a, int kind, int R, bool ung, hipStream_t s) {
   hipLaunchKernelGGL(zero_u32, dim3(1), dim3(64), 0, s, a.counter, 1u);
-#define ST_DP(PP, RR, UU)                                                                  \
-  hipLaunchKernelGGL((dp_kernel<PP, RR, UU>), dim3(a.grid), dim3(64), 0, s, a.units,       \
-                     a.nunits, a.counter, a.pairs, a.queries, a.arena, a.toff, a.tlen,      \
-                     a.open, a.ext, a.passbuf, a.passbuf_half, a.pout, a.top4)
+#define ST_DP(PP, RR, UU)                                                                        \
+  hipLaunchKernelGGL((dp_kernel<PP, RR, UU>), dim3(a.grid), dim3(64),                             \
+                     (size_t)kCodes * a.lds_stride * sizeof(typename PP::L), s, a.units, a.nunits,   \
+                     a.counter, a.pairs, a.queries, a.arena, a.toff, a.tlen, a.open, a.ext, a.passbuf, \
+                     a.passbuf_half, a.pout, a.top4, a.lds_stride)
   if (kind == 0) {
     if (R == 8) {
       if (ung) ST_DP(Pk16, 8, true); else ST_DP(Pk16, 8, false);

@@ -10,14 +10,11 @@ namespace st {
 constexpr int kSentinel = 21;
 // Profile codes held on the device: 0..20 plus the zero row 21.
 constexpr int kCodes = 22;
-// Largest group-rows product G*R of one pass (G <= 16 lanes, R <= 16 rows).
-constexpr int kMaxPassRows = 256;
-// LDS row stride (elements) of a staged profile slice; +2 keeps the code rows
-// of a slice on different banks.
-constexpr int kLdsStride = kMaxPassRows + 2;
-// Bottom-row diagonal ends are staged in LDS and folded into the top-4 lists
-// every kBottomChunk steps.
-constexpr int kBottomChunk = 16;
+// Largest group-rows product G*R of one pass (G <= 64 lanes, R <= 16 rows).
+constexpr int kMaxPassRows = 1024;
+// Per-block right-edge parking area of the DP kernels, in 16-byte units
+// (64 lanes x 16 words); it follows the two pass-boundary halves.
+constexpr uint64_t kEdgeQuadsPerBlock = 64 * 16 * 4 / 16;
 // Largest m + n - 1 whose diagonal index fits the 17-bit field of the packed
 // int16 kernel's top-4 keys.
 constexpr uint32_t kMaxDiagsPk16 = 131071;
@@ -28,7 +25,7 @@ struct Unit {
   uint32_t query;   // index into the QueryDev table
   uint32_t pair0;   // first pair slot
   uint32_t npairs;  // slots in this unit
-  uint32_t G;       // lanes per group (1, 2, 4, 8 or 16)
+  uint32_t G;       // lanes per group (1, 2, 4, ..., 64)
   uint32_t n_pad;   // columns: longest target of the unit
   uint32_t passes;  // ceil(m / (G*R))
   uint32_t R;       // rows per lane (the kernel instance)

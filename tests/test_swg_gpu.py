@@ -117,3 +117,43 @@ def test_pssm_like_wide_rows(ctx):
         rows[:, 20] = 0
         targets = _rand_seqs(rng, 60, 1, 400, 20)
         _check(ctx, rows, targets, GapCosts(44, 4))
+
+
+@pytest.mark.parametrize("G", [1, 2, 4, 8, 16, 32, 64])
+@pytest.mark.parametrize("R", [8, 16])
+def test_every_group_width_and_rows(ctx, monkeypatch, G, R):
+    # every launch shape on multi-pass queries, with all outputs checked
+    monkeypatch.setenv("ST_FORCE_G", str(G))
+    monkeypatch.setenv("ST_FORCE_R", str(R))
+    rng = np.random.default_rng(G * 100 + R)
+    queries = [rng.integers(0, 20, size=L, dtype=np.uint8) for L in (5, 70, 333, 1100)]
+    targets = [rng.integers(0, 20, size=int(L), dtype=np.uint8) for L in rng.integers(1, 700, size=40)]
+    for q in queries:
+        t = q.copy()
+        t[rng.integers(0, len(q), size=len(q) // 5)] = rng.integers(0, 20)
+        targets.append(t)
+    rows = [O.seq_rows(q) for q in queries]
+    db = st.TargetDb.from_sequences(ctx, targets)
+    qps = [st.QueryProfile.from_rows(ctx, r) for r in rows]
+    cands = [np.arange(len(targets), dtype=np.uint32) for _ in queries]
+    b = st.Batch(ctx, db, qps, cands, [GapCosts(11, 1)] * len(queries), [35] * len(queries))
+    b.run()
+    res = b.results()
+    k = 0
+    for qi in range(len(queries)):
+        for t in targets:
+            want = (O.gotoh_score(rows[qi], t, 11, 1),) + O.ung_pair(rows[qi], t, 11, 1, 35)
+            r = res[k]
+            got = (int(r["swg"]), int(r["ung"]), int(r["best_diag"]), int(r["gate_score"]), bool(r["pass"]))
+            assert got == want, (G, R, qi, len(t))
+            k += 1
+
+
+def test_long_query_long_targets_use_wide_groups(ctx):
+    rng = np.random.default_rng(4242)
+    q = rng.integers(0, 20, size=3000, dtype=np.uint8)
+    targets = [rng.integers(0, 20, size=4000, dtype=np.uint8) for _ in range(5)]
+    hom = q.copy()
+    hom[rng.integers(0, 3000, size=900)] = rng.integers(0, 20)
+    targets.append(np.concatenate([targets[0][:500], hom]))
+    _check(ctx, O.seq_rows(q), targets, GapCosts(11, 1))
