@@ -71,8 +71,10 @@ def main():
     gaps = [st.GapCosts(w.gap_open, w.gap_extend)] * len(qps)
     t1 = time.time()
     batch = st.Batch(ctx, db, qps, w.cand_lists, gaps, [w.gate] * len(qps))
-    log(f"plan + upload {time.time() - t1:.1f}s")
     cells = batch.cells()
+    exec_cells, n_units, n_launches = batch.stats()
+    log(f"plan + upload {time.time() - t1:.1f}s: {n_units} units in {n_launches} launches, "
+        f"executed/useful cells {exec_cells / cells:.3f}")
 
     for i in range(a.warmup):
         batch.run()
@@ -111,7 +113,8 @@ def main():
     roof = {"bound": "valu", "kernel": "dp_kernel<Pk16> (int16x2 SWG + Kadane)", "achieved": round(achieved, 2),
             "peak": round(INT16_PACKED_PEAK_TOPS, 2), "unit": "Tops/s (int16 packed)",
             "frac": round(achieved / INT16_PACKED_PEAK_TOPS, 4), "traffic": None,
-            "ops_per_cell": OPS_PER_CELL, "cells_per_launch": cells, "kernel_ms": round(dp * 1e3, 3)}
+            "ops_per_cell": OPS_PER_CELL, "cells_per_launch": cells, "kernel_ms": round(dp * 1e3, 3),
+            "executed_over_useful_cells": round(exec_cells / cells, 4)}
 
     cpu = None
     full_swg = None

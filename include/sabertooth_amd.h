@@ -192,6 +192,10 @@ int st_batch_results(st_batch *b, st_pair_result *out);
  * stream: ms3[0] the main DP launches, ms3[1] the int32 rescoring of
  * saturated pairs, ms3[2] the gate; *rescored (nullable) = pairs rescored. */
 int st_batch_timing(const st_batch *b, double *ms3, uint64_t *rescored);
+/* Plan statistics: DP cells the kernels execute including padding (rows to
+ * whole passes, columns to a unit's longest target, the skew's idle steps,
+ * empty group slots), work units and kernel launches. */
+int st_batch_stats(const st_batch *b, uint64_t *exec_cells, uint64_t *units, uint64_t *launches);
 /* DP cells of the batch (sum of m*n) and pairs. */
 uint64_t st_batch_cells(const st_batch *b);
 uint64_t st_batch_pairs(const st_batch *b);

@@ -65,6 +65,7 @@ _SIGS = [
     ("st_batch_run", C.c_int, [_P]),
     ("st_batch_results", C.c_int, [_P, _P]),
     ("st_batch_timing", C.c_int, [_P, _P, _P]),
+    ("st_batch_stats", C.c_int, [_P, _P, _P, _P]),
     ("st_batch_cells", C.c_uint64, [_P]),
     ("st_batch_pairs", C.c_uint64, [_P]),
     ("st_batch_destroy", C.c_int, [_P]),

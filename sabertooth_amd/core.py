@@ -263,6 +263,12 @@ class Batch:
     def cells(self) -> int:
         return int(lib().st_batch_cells(self._h))
 
+    def stats(self):
+        """(executed cells incl. padding, units, launches) of the plan."""
+        e, u, l = C.c_uint64(0), C.c_uint64(0), C.c_uint64(0)
+        check(lib().st_batch_stats(self._h, C.byref(e), C.byref(u), C.byref(l)))
+        return int(e.value), int(u.value), int(l.value)
+
     def timing(self):
         """(dp_ms, rescore_ms, gate_ms, rescored_pairs) of the last run."""
         ms = np.zeros(3, dtype=np.float64)

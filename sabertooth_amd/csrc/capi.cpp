@@ -438,6 +438,8 @@ static Shape choose_shape(uint32_t m, double n_med, double n_max, Kind kind) {
     for (int G = 1; G <= 64; G <<= 1) {
       if (forced_groups() && G != forced_groups()) continue;
       const int GR = G * R;
+      // the pass's profile slice must fit 64 KB of LDS
+      if ((size_t)kCodes * (GR + 2) * (kind == kPk16 ? 2 : 4) > 65536) continue;
       const uint32_t passes = std::max<uint32_t>(1, (m + GR - 1) / GR);
       const double m_pad = (double)passes * GR;
       double c = (m_pad / std::max<uint32_t>(m, 1)) * ((n_med + G - 1) / std::max(n_med, 1.0));
@@ -476,6 +478,7 @@ struct Plan {
   std::vector<PairDev> pairs;
   std::vector<Launch> launches;
   uint64_t buf_elems = 0;
+  uint64_t exec_cells = 0;  // cells the launches execute, padding included
 };
 
 // Add the pairs of query qi (result rows outs[], target ids tids[]) to `pl`:
@@ -513,6 +516,7 @@ static void plan_query(Plan &pl, std::vector<Unit> &pending, uint32_t qi, const 
       np = std::max(np, db->hlen[tids[o]]);
     }
     u.n_pad = np;
+    pl.exec_cells += (uint64_t)64 * sh.R * per * passes * (np + sh.G - 1);
     pending.push_back(u);
   }
 }
@@ -856,6 +860,14 @@ int st_batch_timing(const st_batch *b, double *ms3, uint64_t *rescored) {
   if (!b || !ms3) return fail(ST_ERR_INVALID, "st_batch_timing: null argument");
   for (int k = 0; k < 3; k++) ms3[k] = b->ms[k];
   if (rescored) *rescored = b->rescored;
+  return ST_OK;
+}
+
+int st_batch_stats(const st_batch *b, uint64_t *exec_cells, uint64_t *units, uint64_t *launches) {
+  if (!b) return fail(ST_ERR_INVALID, "st_batch_stats: null batch");
+  if (exec_cells) *exec_cells = b->plan.exec_cells;
+  if (units) *units = b->plan.units.size();
+  if (launches) *launches = b->plan.launches.size();
   return ST_OK;
 }
 

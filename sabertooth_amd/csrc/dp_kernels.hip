@@ -66,7 +66,11 @@ __device__ __forceinline__ uint4 ldg_u4(const uint4 *p) {
 
 constexpr uint32_t kSent4 = 0x15151515u;  // four zero-score codes
 #ifndef ST_STEP_UNROLL
-#define ST_STEP_UNROLL 1
+#define ST_STEP_UNROLL 4
+#endif
+// 1: a step's profile words are gathered one step ahead
+#ifndef ST_SCORE_PREFETCH
+#define ST_SCORE_PREFETCH 0
 #endif
 // DP steps per loop iteration
 constexpr int kStepUnroll = ST_STEP_UNROLL;
@@ -250,8 +254,10 @@ __device__ __forceinline__ void run_pass(const GroupCtx<P> &c, const typename P:
     code = (cw[0] & 0xffu) | ((cw[1] & 0xffu) << 8);
   }
   // scores of the current step are fetched one step ahead
+#if ST_SCORE_PREFETCH
   uint32_t raw[R];
   load_raw<P, R>(lds, stride, code, l, raw);
+#endif
   // bottom row (LAST): the value the group's last lane produced at step t is
   // kept by lane t mod G and folded every G steps
   T pending = kInvalid;
@@ -278,8 +284,10 @@ __device__ __forceinline__ void run_pass(const GroupCtx<P> &c, const typename P:
           }
         }
       }
+#if ST_SCORE_PREFETCH
       uint32_t nraw[R];
       load_raw<P, R>(lds, stride, nc, l, nraw);
+#endif
 
       // boundary from the upper lane (or the previous pass / the matrix edge)
       T in_h = shr1<P>(out_h, wide), in_f = shr1<P>(out_f, wide);
@@ -306,6 +314,10 @@ __device__ __forceinline__ void run_pass(const GroupCtx<P> &c, const typename P:
       const bool live = P::kFloor || (j >= 0 && j < n_pad);
       if (live) {
         T sc[R];
+#if !ST_SCORE_PREFETCH
+        uint32_t raw[R];
+        load_raw<P, R>(lds, stride, code, l, raw);
+#endif
         unpack_raw<P, R>(raw, sc);
         // diagonal moves, last row first so row k-1 still holds column j-1
 #pragma unroll
@@ -350,8 +362,10 @@ __device__ __forceinline__ void run_pass(const GroupCtx<P> &c, const typename P:
         bdiag = in_b;
       }
       code = nc;
+#if ST_SCORE_PREFETCH
 #pragma unroll
       for (int k = 0; k < R; k++) raw[k] = nraw[k];
+#endif
       // boundary row for the next pass
       if (!final_pass && l == G - 1 && j >= 0 && j < n_pad) {
         uint4 v;
