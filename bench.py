@@ -46,7 +46,22 @@ def parse():
     ap.add_argument("--cands", type=int, default=3000)
     ap.add_argument("--parity-pairs", type=int, default=1500)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="only run a plain device copy of 256 MiB (read 256 MiB, write 256 MiB) three "
+                         "times, to scale the FETCH_SIZE / WRITE_SIZE counters")
     return ap.parse_args()
+
+
+def copy_calibration():
+    import torch
+    n = 256 << 20
+    a = torch.ones(n // 4, dtype=torch.int32, device="cuda")
+    b = torch.empty_like(a)
+    torch.cuda.synchronize()
+    for _ in range(3):
+        torch.add(a, 1, out=b)  # one elementwise kernel: n bytes read, n bytes written
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_calibration_bytes": n, "dispatches": 3, "check": int(b[-1].item())}), flush=True)
 
 
 def main():
@@ -55,6 +70,9 @@ def main():
         print("bench.py: this build runs on one GPU only; the scoring path shards by queries "
               "or by target ranges with no collective (see DESIGN.md)", file=sys.stderr)
         sys.exit(2)
+    if a.copy_calibration:
+        copy_calibration()
+        return
     import oracle as O
     import sabertooth_amd as st
     from sabertooth_amd import synth

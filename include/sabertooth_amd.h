@@ -39,6 +39,7 @@ extern "C" {
 #define ST_ERR_NOMEM 3     /* device or host allocation failed */
 #define ST_ERR_CAPACITY 4  /* admission refused: DP-state memory exhausted */
 #define ST_ERR_CLOSED 5    /* the serving queue was closed */
+#define ST_ERR_TIMEOUT 6   /* a bounded wait ended before the request finished */
 
 /* Message of the calling thread's last error ("" when none). */
 const char *st_last_error(void);
@@ -200,6 +201,67 @@ int st_batch_stats(const st_batch *b, uint64_t *exec_cells, uint64_t *units, uin
 uint64_t st_batch_cells(const st_batch *b);
 uint64_t st_batch_pairs(const st_batch *b);
 int st_batch_destroy(st_batch *b);
+
+/* ---- serving: resident database, continuous batching --------------------- */
+
+/* The reference has no serving mode; its CLI scores fixed query batches
+ * (src/main.rs:255, Engine::search_batch src/search.rs:692-731).  A server
+ * keeps the database resident and runs `workers` slots, each with its own
+ * stream and device buffers.  A free slot takes every queued request (FIFO,
+ * at most max_batch_queries) whose DP state fits the budget beside what is
+ * already running, scores them as one micro-batch and completes them;
+ * requests that arrive meanwhile wait for the next free slot, and the slots'
+ * launches overlap on the device.  Every request's results equal those of
+ * st_batch_create / st_batch_run on the same query and candidates. */
+typedef struct st_server st_server;
+
+typedef struct {
+  uint32_t workers;           /* concurrent micro-batches (slots); 0 = 4, max 16 */
+  uint32_t max_batch_queries; /* requests merged into one micro-batch; 0 = 64 */
+  uint64_t state_budget_bytes;/* device DP state admitted at once; 0 = 4 GiB */
+} st_server_config;
+
+/* One request: `out` receives n_cands results in candidate order and must
+ * stay valid (with `query`) until st_server_wait returns for its ticket. */
+typedef struct {
+  const st_qprof *query;
+  int32_t gap_open;
+  int32_t gap_extend;
+  int32_t min_diag_score;
+  int32_t reserved;
+  const uint32_t *cand_tids;
+  uint64_t n_cands;
+  st_pair_result *out;
+} st_request;
+
+typedef struct {
+  uint64_t batches;             /* micro-batches run */
+  uint64_t queries;             /* requests completed */
+  uint64_t pairs;
+  uint64_t cells;               /* sum of m*n over completed requests */
+  uint64_t rescored;            /* pairs rescored in int32 */
+  uint64_t max_batch_queries;   /* largest micro-batch */
+  uint64_t peak_inflight_bytes; /* largest admitted DP state */
+  double busy_ms;               /* summed slot time spent on micro-batches */
+} st_server_stats;
+
+/* cfg may be NULL (defaults). */
+int st_server_create(st_ctx *ctx, const st_db *db, const st_server_config *cfg,
+                     st_server **out);
+/* Queue a request (candidate ids are copied); *ticket identifies it. */
+int st_server_submit(st_server *s, const st_request *req, uint64_t *ticket);
+/* *done = 1 once the request's results are in its out buffer. */
+int st_server_poll(st_server *s, uint64_t ticket, int *done);
+/* Wait for a request (timeout_ms < 0: no limit; ST_ERR_TIMEOUT otherwise);
+ * *latency_ms (nullable) = submit to completion.  Retires the ticket. */
+int st_server_wait(st_server *s, uint64_t ticket, double timeout_ms, double *latency_ms);
+/* Open-loop replay: submit reqs[k] at arrival_ms[k] (non-decreasing, ms
+ * from the call), then wait for all; latency_ms[k] = submit to completion. */
+int st_server_run_trace(st_server *s, const st_request *reqs, const double *arrival_ms,
+                        uint64_t n, double *latency_ms);
+int st_server_stats_get(st_server *s, st_server_stats *out);
+/* Finish queued requests, stop the workers and free the server. */
+int st_server_destroy(st_server *s);
 
 #ifdef __cplusplus
 }

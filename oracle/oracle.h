@@ -118,6 +118,15 @@ uint64_t or_splitmix_next(uint64_t *state);
  * the BLOSUM62 background using SplitMix64 state *state. */
 void or_background_sequence(uint64_t *state, size_t len, uint8_t *out);
 
+/* ---- host-side rows (host_oracle.c) ----------------------------------- */
+
+/* Profile::from_numeric_msa (src/profile.rs:90-182): msa holds nrows rows of
+ * `stride` codes (0..19, 20 = X, 0xFF = gap), the first qlen are match
+ * columns.  pssm: qlen x 20 int8; prob (qlen x 20), neff (qlen), cons (qlen)
+ * may be NULL.  Returns 0, or -1 for an empty MSA / zero columns. */
+int or_pssm_from_msa(const uint8_t *msa, size_t nrows, size_t stride, size_t qlen, float pca, float pcb,
+                     float score_bias, int8_t *pssm, float *prob, float *neff, uint8_t *cons);
+
 #ifdef __cplusplus
 }
 #endif

@@ -19,6 +19,7 @@ ST_ERR_DEVICE = 2
 ST_ERR_NOMEM = 3
 ST_ERR_CAPACITY = 4
 ST_ERR_CLOSED = 5
+ST_ERR_TIMEOUT = 6
 KERNEL_I16 = 0
 KERNEL_I32 = 1
 
@@ -29,6 +30,12 @@ CAND_DT = np.dtype([("target_id", "<u4"), ("hits", "<u4"), ("best_diag", "<i8"),
 PAIR_RESULT_DT = np.dtype([("swg", "<i4"), ("ung", "<i4"), ("best_diag", "<i4"),
                            ("gate_score", "<i4"), ("pass", "u1"), ("kernel", "u1"),
                            ("reserved", "u1", (2,))])
+SERVER_CONFIG_DT = np.dtype([("workers", "<u4"), ("max_batch_queries", "<u4"), ("state_budget_bytes", "<u8")])
+REQUEST_DT = np.dtype([("query", "<u8"), ("gap_open", "<i4"), ("gap_extend", "<i4"), ("min_diag_score", "<i4"),
+                       ("reserved", "<i4"), ("cand_tids", "<u8"), ("n_cands", "<u8"), ("out", "<u8")])
+SERVER_STATS_DT = np.dtype([("batches", "<u8"), ("queries", "<u8"), ("pairs", "<u8"), ("cells", "<u8"),
+                            ("rescored", "<u8"), ("max_batch_queries", "<u8"), ("peak_inflight_bytes", "<u8"),
+                            ("busy_ms", "<f8")])
 
 
 class StError(RuntimeError):
@@ -69,6 +76,13 @@ _SIGS = [
     ("st_batch_cells", C.c_uint64, [_P]),
     ("st_batch_pairs", C.c_uint64, [_P]),
     ("st_batch_destroy", C.c_int, [_P]),
+    ("st_server_create", C.c_int, [_P, _P, _P, C.POINTER(_P)]),
+    ("st_server_submit", C.c_int, [_P, _P, C.POINTER(C.c_uint64)]),
+    ("st_server_poll", C.c_int, [_P, C.c_uint64, C.POINTER(C.c_int)]),
+    ("st_server_wait", C.c_int, [_P, C.c_uint64, C.c_double, C.POINTER(C.c_double)]),
+    ("st_server_run_trace", C.c_int, [_P, _P, _P, C.c_uint64, _P]),
+    ("st_server_stats_get", C.c_int, [_P, _P]),
+    ("st_server_destroy", C.c_int, [_P]),
 ]
 
 EXPORTED = [s[0] for s in _SIGS]

@@ -14,7 +14,9 @@ import sys
 HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libsabertooth_amd.so")
-SOURCES = ["dp_kernels.hip", "gate_kernels.hip", "capi.cpp", "synth.cpp"]
+SOURCES = ["dp_kernels.hip", "gate_kernels.hip", "capi.cpp", "server.cpp", "synth.cpp", "profile.cpp"]
+# host-only floating-point code keeps the reference's separate roundings
+HOST_FP = {"profile.cpp": ["-ffp-contract=off"]}
 ARCH = "gfx950"
 HIPCC = os.environ.get("HIPCC", shutil.which("hipcc") or "/opt/rocm/bin/hipcc")
 CFLAGS = [
@@ -44,7 +46,7 @@ def build(verbose: bool = False) -> str:
     objs = []
     for src in SOURCES:
         obj = os.path.join(objdir, src + ".o")
-        out = _run([HIPCC, *CFLAGS, "-x", "hip", "-c", os.path.join(CSRC, src), "-o", obj])
+        out = _run([HIPCC, *CFLAGS, *HOST_FP.get(src, []), "-x", "hip", "-c", os.path.join(CSRC, src), "-o", obj])
         if verbose and out.strip():
             print(out)
         objs.append(obj)

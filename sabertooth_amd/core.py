@@ -286,3 +286,91 @@ class Batch:
             self.close()
         except Exception:
             pass
+
+
+class Server:
+    """Serving loop over a resident database (continuous batching).
+
+    Worker slots take every queued request whose DP state fits the memory
+    budget beside what is already running, score them as one micro-batch on
+    their own stream and complete them; results equal those of ``Batch`` on
+    the same query and candidates.
+    """
+
+    def __init__(self, ctx: Context, db: TargetDb, workers: int = 4, max_batch_queries: int = 64,
+                 state_budget_bytes: int = 4 << 30):
+        cfg = np.zeros(1, dtype=_lib.SERVER_CONFIG_DT)
+        cfg["workers"], cfg["max_batch_queries"], cfg["state_budget_bytes"] = (workers, max_batch_queries,
+                                                                              state_budget_bytes)
+        h = C.c_void_p()
+        check(lib().st_server_create(ctx.handle, db.handle, ptr(cfg), C.byref(h)))
+        self._h = h
+        self.ctx, self.db = ctx, db
+        self._live = {}  # ticket -> (request record, candidates, output) kept alive
+
+    @staticmethod
+    def _request(qp: QueryProfile, cands: np.ndarray, out: np.ndarray, gaps: GapCosts, gate: int):
+        r = np.zeros(1, dtype=_lib.REQUEST_DT)
+        r["query"] = qp.handle.value
+        r["gap_open"], r["gap_extend"], r["min_diag_score"] = gaps.open, gaps.extend, gate
+        r["cand_tids"] = cands.ctypes.data if len(cands) else 0
+        r["n_cands"] = len(cands)
+        r["out"] = out.ctypes.data
+        return r
+
+    def submit(self, qp: QueryProfile, cands, gaps: GapCosts, gate: int) -> int:
+        c = np.ascontiguousarray(cands, dtype=np.uint32)
+        out = np.zeros(max(len(c), 1), dtype=PAIR_RESULT_DT)
+        r = self._request(qp, c, out, gaps, gate)
+        t = C.c_uint64(0)
+        check(lib().st_server_submit(self._h, ptr(r), C.byref(t)))
+        self._live[t.value] = (r, c, out, qp)
+        return int(t.value)
+
+    def poll(self, ticket: int) -> bool:
+        d = C.c_int(0)
+        check(lib().st_server_poll(self._h, ticket, C.byref(d)))
+        return bool(d.value)
+
+    def wait(self, ticket: int, timeout_ms: float = -1.0):
+        """(results in candidate order, latency in ms from submit to completion)."""
+        lat = C.c_double(0)
+        rc = lib().st_server_wait(self._h, ticket, timeout_ms, C.byref(lat))
+        if rc != _lib.ST_ERR_TIMEOUT:  # a timed-out ticket stays live
+            rec = self._live.pop(ticket, None)
+        check(rc)
+        _, c, out, _ = rec
+        return out[: len(c)].copy(), float(lat.value)
+
+    def run_trace(self, requests, arrival_ms):
+        """Open-loop replay: requests[k] = (qp, cands, gaps, gate) submitted at
+        arrival_ms[k]; returns (list of result arrays, latencies in ms)."""
+        n = len(requests)
+        recs = np.zeros(max(n, 1), dtype=_lib.REQUEST_DT)
+        keep = []
+        for k, (qp, cands, gaps, gate) in enumerate(requests):
+            c = np.ascontiguousarray(cands, dtype=np.uint32)
+            out = np.zeros(max(len(c), 1), dtype=PAIR_RESULT_DT)
+            recs[k] = self._request(qp, c, out, gaps, gate)[0]
+            keep.append((c, out))
+        arr = np.ascontiguousarray(arrival_ms, dtype=np.float64)
+        lat = np.zeros(max(n, 1), dtype=np.float64)
+        check(lib().st_server_run_trace(self._h, ptr(recs), ptr(arr), n, ptr(lat)))
+        return [out[: len(c)].copy() for c, out in keep], lat[:n].copy()
+
+    def stats(self) -> dict:
+        s = np.zeros(1, dtype=_lib.SERVER_STATS_DT)
+        check(lib().st_server_stats_get(self._h, ptr(s)))
+        return {k: s[k][0].item() for k in s.dtype.names}
+
+    def close(self):
+        if self._h:
+            lib().st_server_destroy(self._h)
+            self._h = None
+            self._live.clear()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -13,141 +13,18 @@
 #include <vector>
 
 #include "../../include/sabertooth_amd.h"
-#include "st_device.h"
+#include "host_internal.h"
 
 namespace st {
-
-// ---- kernel entry points (dp_kernels.hip, gate_kernels.hip) -----------------
-struct DpLaunch {
-  const Unit *units;
-  uint32_t nunits;
-  uint32_t *counter;
-  const PairDev *pairs;
-  const QueryDev *queries;
-  const uint8_t *arena;
-  const uint64_t *toff;
-  const uint32_t *tlen;
-  int32_t open, ext;
-  uint4 *passbuf;
-  uint64_t passbuf_half;
-  PairOut *pout;
-  uint64_t *top4;
-  uint32_t grid;
-  int lds_stride;
-};
-hipError_t launch_dp(const DpLaunch &a, int kind, int R, bool ung, hipStream_t s);
-
-struct GateRes {
-  int32_t score, best, best_diag;
-  uint32_t t_lo, t_hi;
-  uint8_t pass, chain, pad0, pad1;
-};
-struct JobDev {
-  uint32_t tid;
-  int32_t diag;
-  uint32_t lo, hi;
-};
-struct RecDev {
-  uint32_t tid, hits, j0, j1;
-};
-struct PairGateArgs {
-  const PairDev *pairs;
-  uint64_t npairs;
-  const uint32_t *pair_query;
-  const QueryDev *queries;
-  const uint8_t *arena;
-  const uint64_t *toff;
-  const uint32_t *tlen;
-  const uint64_t *top4;
-  const int32_t *open, *ext, *gate;
-  PairOut *pout;
-  void *res;
-};
-hipError_t launch_ung_jobs(const JobDev *jobs, uint64_t njobs, const QueryDev *qd,
-                           const uint8_t *arena, const uint64_t *toff, int32_t *out,
-                           hipStream_t s);
-hipError_t launch_gate_recs(const RecDev *recs, uint64_t nrecs, const JobDev *jobs,
-                            const int32_t *scores, const QueryDev *qd, const uint8_t *arena,
-                            const uint64_t *toff, int32_t open, int32_t ext, int32_t gate,
-                            int regions, GateRes *out, hipStream_t s);
-hipError_t launch_gate_pairs(const PairGateArgs &a, hipStream_t s);
-
-// ---- errors -------------------------------------------------------------------
 static thread_local std::string g_err;
 
-static int fail(int code, const std::string &msg) {
+int fail(int code, const std::string &msg) {
   g_err = msg;
   return code;
 }
-
-#define ST_HIP(call)                                                                   \
-  do {                                                                                 \
-    hipError_t e_ = (call);                                                            \
-    if (e_ != hipSuccess)                                                              \
-      return fail(ST_ERR_DEVICE, std::string(#call) + ": " + hipGetErrorString(e_)); \
-  } while (0)
-
-// Device allocation owned by a host object.
-template <class T>
-struct DevBuf {
-  T *p = nullptr;
-  size_t n = 0;
-  DevBuf() = default;
-  DevBuf(const DevBuf &) = delete;
-  DevBuf &operator=(const DevBuf &) = delete;
-  ~DevBuf() { release(); }
-  void release() {
-    if (p) (void)hipFree(p);
-    p = nullptr;
-    n = 0;
-  }
-  hipError_t alloc(size_t count) {
-    release();
-    n = count;
-    if (count == 0) return hipSuccess;
-    return hipMalloc((void **)&p, count * sizeof(T));
-  }
-  hipError_t upload(const T *src, size_t count, hipStream_t s) {
-    hipError_t e = alloc(count);
-    if (e != hipSuccess || count == 0) return e;
-    e = hipMemcpyAsync(p, src, count * sizeof(T), hipMemcpyHostToDevice, s);
-    if (e != hipSuccess) return e;
-    return hipStreamSynchronize(s);
-  }
-};
-
 }  // namespace st
 
 using namespace st;
-
-struct st_ctx {
-  int device = 0;
-  hipStream_t stream = nullptr;  // main stream: uploads, gate, results
-  hipStream_t slots[4] = {nullptr, nullptr, nullptr, nullptr};  // concurrent DP launches
-  std::mutex mu;
-  int cus = 256;
-  DevBuf<uint32_t> counter;  // one unit counter per launch slot
-};
-
-struct st_db {
-  st_ctx *ctx = nullptr;
-  DevBuf<uint8_t> arena;
-  DevBuf<uint64_t> off;
-  DevBuf<uint32_t> len;
-  std::vector<uint32_t> hlen;
-  uint64_t n_targets = 0, n_res = 0;
-};
-
-struct st_qprof {
-  st_ctx *ctx = nullptr;
-  std::vector<int32_t> rows;
-  uint32_t m = 0, ld = 0;
-  bool fits16 = false;
-  DevBuf<int16_t> prof16;
-  DevBuf<int32_t> prof32;
-  DevBuf<QueryDev> qd;  // a one-entry table for single-query calls
-  QueryDev view{};
-};
 
 // ---- host inputs -----------------------------------------------------------------
 
@@ -391,12 +268,7 @@ uint64_t st_qprof_len(const st_qprof *qp) { return qp ? qp->m : 0; }
 
 namespace st {
 
-int dp_occupancy(int kind, int R, bool ung, size_t lds_bytes);
-
-// Precision class of a query under given gaps.
-enum Kind { kPk16 = 0, kI32F = 1, kI32N = 2 };
-
-static Kind kind_for(const st_qprof *qp, int32_t open, int32_t ext) {
+Kind kind_for(const st_qprof *qp, int32_t open, int32_t ext) {
   if (ext < 0 || open < 0) return kI32N;
   if (qp->fits16 && open <= 32767 && ext <= 32767) return kPk16;
   return kI32F;
@@ -457,33 +329,7 @@ static Shape choose_shape(uint32_t m, double n_med, double n_max, Kind kind) {
   return best;
 }
 
-// Streams (and unit counters) for concurrent DP launches.
-constexpr int kLaunchSlots = 4;
-
-// One kernel launch: units [unit0, unit0 + nunits) of one precision kind, one
-// R, one gap setting and one LDS class; buffer region [buf0, buf0 + grid *
-// (2 half + edge)) of the batch's pass buffer.
-struct Launch {
-  int kind, R;
-  int32_t open, ext;
-  uint32_t unit0, nunits;
-  int gr_max;
-  uint64_t half, buf0;
-  uint32_t grid;
-  int slot;
-};
-
-struct Plan {
-  std::vector<Unit> units;
-  std::vector<PairDev> pairs;
-  std::vector<Launch> launches;
-  uint64_t buf_elems = 0;
-  uint64_t exec_cells = 0;  // cells the launches execute, padding included
-};
-
-// Add the pairs of query qi (result rows outs[], target ids tids[]) to `pl`:
-// targets sorted by length, cut into wave units of 64/G groups.
-static void plan_query(Plan &pl, std::vector<Unit> &pending, uint32_t qi, const st_qprof *qp,
+void plan_query(Plan &pl, std::vector<Unit> &pending, uint32_t qi, const st_qprof *qp,
                        const uint32_t *tids, const uint64_t *outs, size_t count, Kind kind, const st_db *db) {
   if (count == 0) return;
   std::vector<uint32_t> order(count);
@@ -525,7 +371,7 @@ static double unit_cost(const Unit &u) { return (double)u.passes * u.R * (u.n_pa
 
 // Group pending units into launches by (kind, R, gaps, LDS class), longest
 // first, and lay out their pass-buffer regions.
-static void finish_plan(Plan &pl, std::vector<Unit> &pending, const std::vector<Kind> &qkind,
+void finish_plan(Plan &pl, std::vector<Unit> &pending, const std::vector<Kind> &qkind,
                         const std::vector<int32_t> &open, const std::vector<int32_t> &ext, int force_kind,
                         int cus, bool ung) {
   auto kind_of = [&](const Unit &u) { return force_kind >= 0 ? force_kind : (int)qkind[u.query]; };

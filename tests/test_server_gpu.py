@@ -1,0 +1,135 @@
+"""Serving loop: every request's results are the same alone and in the mix
+(bit-exact: integer outputs), equal to the plain batch and to the oracle."""
+import numpy as np
+import pytest
+
+import oracle as O
+import sabertooth_amd as st
+from sabertooth_amd import GapCosts
+
+pytestmark = pytest.mark.gpu
+
+
+def _fields(r):
+    return np.stack([r["swg"], r["ung"], r["best_diag"], r["gate_score"], r["pass"].astype(np.int32),
+                     r["kernel"].astype(np.int32)])
+
+
+@pytest.fixture(scope="module")
+def setup(ctx):
+    rng = np.random.default_rng(2024)
+    s = O.SplitMix64(11)
+    targets = [s.background(int(L)) for L in rng.integers(1, 900, size=2500)]
+    queries = [s.background(int(L)) for L in rng.integers(5, 700, size=20)]
+    # planted homologs so gates pass and scores are large
+    for k, q in enumerate(queries):
+        t = q.copy()
+        t[rng.integers(0, len(q), size=len(q) // 4)] = rng.integers(0, 20)
+        targets[k * 7] = np.concatenate([targets[k * 7][:30], t])
+    rows = [O.seq_rows(q) for q in queries]
+    # a PSSM-like query with profile gap costs
+    prow = rng.integers(-60, 60, size=(230, 21)).astype(np.int32)
+    prow[:, 20] = 0
+    rows.append(prow)
+    # a query whose self-score saturates int16 (rescored in int32)
+    W = O.aa2num()[ord("W")]
+    wq = np.full(3400, W, np.uint8)
+    rows.append(O.seq_rows(wq))
+    targets.append(wq.copy())
+    db = st.TargetDb.from_sequences(ctx, targets)
+    qps = [st.QueryProfile.from_rows(ctx, r) for r in rows]
+    nt = len(targets)
+    cands = [rng.choice(nt - 1, size=int(rng.integers(0, 400)), replace=False).astype(np.uint32) for _ in rows]
+    cands[-1] = np.array([nt - 1, 3, 4], np.uint32)
+    cands[3] = np.zeros(0, np.uint32)  # an empty candidate list
+    gaps = [GapCosts(11, 1)] * len(queries) + [GapCosts(44, 4), GapCosts(11, 1)]
+    gates = [35] * len(queries) + [30, 41]
+    b = st.Batch(ctx, db, qps, cands, gaps, gates)
+    b.run()
+    res = b.results()
+    ref = [res[int(b.offsets[q]):int(b.offsets[q + 1])] for q in range(len(qps))]
+    return dict(db=db, qps=qps, rows=rows, targets=targets, cands=cands, gaps=gaps, gates=gates, ref=ref)
+
+
+def test_batch_reference_matches_oracle(setup):
+    rng = np.random.default_rng(5)
+    for q in range(len(setup["qps"])):
+        c = setup["cands"][q]
+        for k in rng.choice(len(c), size=min(len(c), 12), replace=False) if len(c) else []:
+            t = setup["targets"][int(c[k])]
+            g = setup["gaps"][q]
+            want = (O.gotoh_score(setup["rows"][q], t, g.open, g.extend),) + \
+                O.ung_pair(setup["rows"][q], t, g.open, g.extend, setup["gates"][q])
+            r = setup["ref"][q][k]
+            assert (int(r["swg"]), int(r["ung"]), int(r["best_diag"]), int(r["gate_score"]),
+                    bool(r["pass"])) == want, (q, k)
+    assert setup["ref"][-1]["swg"][0] == 34000 and setup["ref"][-1]["kernel"][0] == st.KERNEL_I32
+
+
+def test_alone_equals_batch(ctx, setup):
+    srv = st.Server(ctx, setup["db"], workers=2)
+    try:
+        for q, qp in enumerate(setup["qps"]):
+            t = srv.submit(qp, setup["cands"][q], setup["gaps"][q], setup["gates"][q])
+            out, lat = srv.wait(t, timeout_ms=60000)
+            assert lat > 0
+            assert (_fields(out) == _fields(setup["ref"][q])).all(), q
+        s = srv.stats()
+        assert s["queries"] == len(setup["qps"]) and s["rescored"] >= 1
+    finally:
+        srv.close()
+
+
+def test_in_the_mix_equals_alone(ctx, setup):
+    srv = st.Server(ctx, setup["db"], workers=3, max_batch_queries=5)
+    try:
+        order = list(range(len(setup["qps"]))) * 3
+        tickets = [(q, srv.submit(setup["qps"][q], setup["cands"][q], setup["gaps"][q], setup["gates"][q]))
+                   for q in order]
+        for q, t in tickets:
+            out, _ = srv.wait(t, timeout_ms=60000)
+            assert (_fields(out) == _fields(setup["ref"][q])).all(), q
+        s = srv.stats()
+        assert s["queries"] == len(order)
+        assert 1 < s["max_batch_queries"] <= 5
+    finally:
+        srv.close()
+
+
+def test_open_loop_trace_equals_alone(ctx, setup):
+    srv = st.Server(ctx, setup["db"], workers=4)
+    try:
+        rng = np.random.default_rng(9)
+        qi = rng.integers(0, len(setup["qps"]), size=60)
+        arrivals = np.cumsum(rng.exponential(0.4, size=len(qi)))
+        reqs = [(setup["qps"][q], setup["cands"][q], setup["gaps"][q], setup["gates"][q]) for q in qi]
+        outs, lat = srv.run_trace(reqs, arrivals)
+        assert (lat > 0).all()
+        for q, out in zip(qi, outs):
+            assert (_fields(out) == _fields(setup["ref"][q])).all(), q
+    finally:
+        srv.close()
+
+
+def test_tiny_budget_admits_one_at_a_time(ctx, setup):
+    srv = st.Server(ctx, setup["db"], workers=4, state_budget_bytes=1)
+    try:
+        tickets = [(q, srv.submit(setup["qps"][q], setup["cands"][q], setup["gaps"][q], setup["gates"][q]))
+                   for q in range(len(setup["qps"]))]
+        for q, t in tickets:
+            out, _ = srv.wait(t, timeout_ms=60000)
+            assert (_fields(out) == _fields(setup["ref"][q])).all(), q
+        assert srv.stats()["max_batch_queries"] == 1
+    finally:
+        srv.close()
+
+
+def test_bad_requests_fail_cleanly(ctx, setup):
+    srv = st.Server(ctx, setup["db"], workers=1)
+    try:
+        with pytest.raises(st.StError):
+            srv.submit(setup["qps"][0], np.array([len(setup["targets"])], np.uint32), GapCosts(11, 1), 35)
+        with pytest.raises(st.StError):
+            srv.wait(123456, timeout_ms=10)
+    finally:
+        srv.close()
