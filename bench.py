@@ -46,6 +46,8 @@ def parse():
     ap.add_argument("--cands", type=int, default=3000)
     ap.add_argument("--parity-pairs", type=int, default=1500)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-serving", action="store_true", help="skip the serving leg")
+    ap.add_argument("--serving-requests", type=int, default=2000)
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a plain device copy of 256 MiB (read 256 MiB, write 256 MiB) three "
                          "times, to scale the FETCH_SIZE / WRITE_SIZE counters")
@@ -64,11 +66,114 @@ def copy_calibration():
     print(json.dumps({"copy_calibration_bytes": n, "dispatches": 3, "check": int(b[-1].item())}), flush=True)
 
 
+def _pct(x, q):
+    return round(float(np.percentile(x, q)), 3) if len(x) else None
+
+
+def serving_leg(a, O, st, synth):
+    """Served vs plain batch on the serving configuration: a resident
+    265 M-residue database, 90% plain / 10% profile requests with 3,000
+    candidates each, Poisson arrivals.  Every served result is compared with
+    the plain batch's (bit-exact)."""
+    t0 = time.time()
+    w = synth.serving(seed=1, n_requests=a.serving_requests)
+    n = len(w.rows)
+    cells = np.array([w.cells(k) for k in range(n)], dtype=np.float64)
+    log(f"serving workload: {len(w.offsets) - 1} targets, {int(w.offsets[-1])} residues, {n} requests "
+        f"({sum(w.profile)} profile), {cells.sum() / 1e9:.1f} G cells ({time.time() - t0:.1f}s)")
+    ctx = st.Context(0)
+    db = st.TargetDb(ctx, w.arena, w.offsets)
+    qps = [st.QueryProfile.from_rows(ctx, r) for r in w.rows]
+    gaps = [st.GapCosts(*g) for g in w.gaps]
+    # plain batch of every request
+    b = st.Batch(ctx, db, qps, w.cands, gaps, w.gates)
+    b.run()
+    ts = time.perf_counter()
+    b.run()
+    t_batch = time.perf_counter() - ts
+    ref = b.results()
+    offs = b.offsets.astype(np.int64)
+    ref_of = [ref[offs[k]:offs[k + 1]] for k in range(n)]
+    b.close()
+    batch_qps = n / t_batch
+    log(f"plain batch: {1e3 * t_batch:.1f} ms, {batch_qps:.0f} q/s, {cells.sum() / t_batch / 1e9:.0f} GCUPS")
+    reqs = [(qps[k], w.cands[k], gaps[k], w.gates[k]) for k in range(n)]
+    srv = st.Server(ctx, db, workers=4, max_batch_queries=64)
+    mism = 0
+
+    def check(idx, outs):
+        nonlocal mism
+        for k, out in zip(idx, outs):
+            r = ref_of[k]
+            mism += int(not all((out[f] == r[f]).all() for f in ("swg", "ung", "best_diag", "gate_score",
+                                                                 "pass", "kernel")))
+
+    # isolated: one request at a time
+    n_iso = min(n, 200)
+    srv.run_trace(reqs[:8], np.zeros(8))  # warm the slots' buffers
+    iso = np.zeros(n_iso)
+    outs = []
+    for k in range(n_iso):
+        o, lat = srv.run_trace([reqs[k]], np.zeros(1))
+        iso[k] = lat[0]
+        outs.append(o[0])
+    check(range(n_iso), outs)
+    log(f"isolated latency over {n_iso} requests: p50 {np.percentile(iso, 50):.2f} ms, "
+        f"p99 {np.percentile(iso, 99):.2f} ms")
+    # saturated: every request queued at once
+    ts = time.perf_counter()
+    outs, lat_sat = srv.run_trace(reqs, np.zeros(n))
+    t_sat = time.perf_counter() - ts
+    check(range(n), outs)
+    sat_qps = n / t_sat
+    log(f"saturated: {n} requests in {1e3 * t_sat:.1f} ms, {sat_qps:.0f} q/s, "
+        f"{cells.sum() / t_sat / 1e9:.0f} GCUPS")
+    rng = np.random.default_rng(3)
+    rates = [50.0, 200.0, 500.0, 1000.0, 2000.0, 0.5 * sat_qps, 0.8 * sat_qps]
+    sweep = []
+    for rate in rates:
+        m = int(min(n, max(100, rate * 1.0)))
+        idx = rng.choice(n, size=m, replace=m > n)
+        arr = np.cumsum(rng.exponential(1e3 / rate, size=m))
+        outs, lat = srv.run_trace([reqs[k] for k in idx], arr)
+        check(idx, outs)
+        span = float(np.max(arr + lat)) / 1e3
+        ent = {"offered_qps": round(rate, 1), "requests": m, "achieved_qps": round(m / span, 1),
+               "gcups": round(float(cells[idx].sum()) / span / 1e9, 1),
+               "p50_ms": _pct(lat, 50), "p99_ms": _pct(lat, 99)}
+        # slowdown of each request against its own isolated latency
+        sel = idx < n_iso
+        if sel.any():
+            ent["p99_over_isolated_same_request"] = round(float(np.percentile(lat[sel] / iso[idx[sel]], 99)), 3)
+        sweep.append(ent)
+        log(f"rate {rate:.0f} q/s: {m} requests, p50 {ent['p50_ms']} ms, p99 {ent['p99_ms']} ms, "
+            f"achieved {ent['achieved_qps']} q/s")
+    stats = srv.stats()
+    srv.close()
+    at80 = sweep[-1]
+    return {
+        "workload": "serving_265M", "requests": n, "profile_requests": int(sum(w.profile)),
+        "db_residues": int(w.offsets[-1]), "db_targets": len(w.offsets) - 1, "candidates_per_request": 3000,
+        "cells": int(cells.sum()), "workers": 4, "max_batch_queries": 64,
+        "plain_batch": {"ms": round(1e3 * t_batch, 3), "queries_per_s": round(batch_qps, 1),
+                        "gcups": round(cells.sum() / t_batch / 1e9, 1)},
+        "saturated": {"ms": round(1e3 * t_sat, 3), "queries_per_s": round(sat_qps, 1),
+                      "gcups": round(cells.sum() / t_sat / 1e9, 1), "p50_ms": _pct(lat_sat, 50),
+                      "p99_ms": _pct(lat_sat, 99)},
+        "isolated": {"requests": n_iso, "p50_ms": _pct(iso, 50), "p99_ms": _pct(iso, 99)},
+        "poisson": sweep,
+        "p99_at_80pct_over_isolated_p50": round(at80["p99_ms"] / float(np.percentile(iso, 50)), 3),
+        "p99_at_80pct_over_isolated_p99": round(at80["p99_ms"] / float(np.percentile(iso, 99)), 3),
+        "served_vs_batch_mismatched_requests": mism,
+        "server_stats": stats,
+    }
+
+
 def main():
     a = parse()
     if a.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py: this build runs on one GPU only; the scoring path shards by queries "
-              "or by target ranges with no collective (see DESIGN.md)", file=sys.stderr)
+              "or by target ranges with no collective, and no multi-GPU figure is measured", file=sys.stderr)
         sys.exit(2)
     if a.copy_calibration:
         copy_calibration()
@@ -157,6 +262,10 @@ def main():
                "seconds": round(secs, 3)}
         log(f"cpu avx2 1 thread: {c1 / s1 / 1e9:.2f} GCUPS")
 
+    serving = None
+    if not a.no_serving:
+        serving = serving_leg(a, O, st, synth)
+
     out = {
         "metric": METRIC, "value": round(gcups, 2), "unit": "GCUPS", "n_gpus": 1, "steps": a.steps,
         "warmup": a.warmup, "ms_per_step": round(step * 1e3, 3), "higher_is_better": True, "scaling": "weak",
@@ -167,7 +276,7 @@ def main():
         "queries_per_s": round(qps_rate, 1),
         "parity": {"sampled_pairs_all_outputs_vs_oracle": int(len(sample)), "mismatches": int(mism),
                    "all_pairs_swg_vs_cpu_avx2_mismatches": full_swg},
-        "roofline": roof, "cpu_baseline": cpu,
+        "roofline": roof, "cpu_baseline": cpu, "serving": serving,
     }
     print(json.dumps(out), flush=True)
 

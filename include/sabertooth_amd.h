@@ -243,6 +243,8 @@ typedef struct {
   uint64_t max_batch_queries;   /* largest micro-batch */
   uint64_t peak_inflight_bytes; /* largest admitted DP state */
   double busy_ms;               /* summed slot time spent on micro-batches */
+  double plan_ms;               /* of which host planning and packing */
+  double device_ms;             /* of which device time (uploads, kernels, results) */
 } st_server_stats;
 
 /* cfg may be NULL (defaults). */

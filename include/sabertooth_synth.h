@@ -25,6 +25,13 @@ uint64_t sy_homolog(uint64_t *state, const uint8_t *src, uint64_t m, double iden
                     double indel_rate, uint32_t indel_max, uint64_t out_len, uint8_t *out);
 /* n ids uniform in [0, range). */
 void sy_pick(uint64_t *state, uint64_t n, uint64_t range, uint32_t *out);
+/* One a3m member row around `root`: per column the root residue with
+ * probability identity (else a P(a|b) substitute), gap runs of 1..10 starting
+ * with probability gap_rate, lowercase insertions of 1..5 residues after a
+ * column with probability insert_rate.  Returns the text length (written up
+ * to cap bytes, no newline). */
+uint64_t sy_msa_member(uint64_t *state, const uint8_t *root, uint64_t m, double identity,
+                       double gap_rate, double insert_rate, char *out, uint64_t cap);
 #ifdef __cplusplus
 }
 #endif

@@ -108,3 +108,240 @@ def read_pssm_table(text: str):
         rows.append([int(x) for x in tok[2:22]])
         neff.append(tok[22])
     return np.array(rows, dtype=np.int32), neff, "".join(res)
+
+
+# ---- E-value statistics (src/evalue.rs), f64 with the C library's exp/log ----
+
+import math  # noqa: E402
+
+# SEQUENCE_TABLE (src/evalue.rs:274-305): (open, ext, comp_bias) -> (lambda, K, alpha/lambda, beta)
+SEQUENCE_TABLE = {(11, 1, False): (0.3666, 0.1177, 2.7, -13.0), (11, 1, True): (0.3649, 0.1076, 2.4, -12.0)}
+
+
+def _fmax(a, b):
+    if a != a:
+        return b
+    if b != b:
+        return a
+    return a if a > b else b
+
+
+def length_adjustment(k, adl, beta, m, n_res, n_seqs):
+    """length_adjustment (src/evalue.rs:193-237)."""
+    log_k = math.log(k)
+    a = n_seqs
+    mb = m * n_seqs + n_res
+    c = n_res * m - _fmax(m, n_res) / k
+    if c < 0.0:
+        return 0.0
+    ell_max = 2.0 * c / (mb + math.sqrt(_fmax(mb * mb - 4.0 * a * c, 0.0)))
+    lo, hi, ell, conv = 0.0, ell_max, 0.0, False
+    for i in range(1, 21):
+        ss = (m - ell) * (n_res - n_seqs * ell)
+        bar = adl * (log_k + math.log(_fmax(ss, 1.0))) + beta
+        if bar >= ell:
+            lo = ell
+            if bar - lo <= 1.0:
+                conv = True
+                break
+            if lo == hi:
+                break
+        else:
+            hi = ell
+        if lo <= bar <= hi:
+            ell = bar
+        elif i == 1:
+            ell = hi
+        else:
+            ell = 0.5 * (lo + hi)
+    out = float(math.floor(lo))
+    if conv:
+        up = float(math.ceil(lo))
+        if up <= hi:
+            ss = (m - up) * (n_res - n_seqs * up)
+            if adl * (log_k + math.log(_fmax(ss, 1.0))) + beta >= up:
+                out = up
+    return _fmax(out, 0.0)
+
+
+class EValueParams:
+    """EValueParams (src/evalue.rs:311-461)."""
+
+    def __init__(self, lam, k, db_residues, db_seqs, length=None):
+        self.lam, self.k, self.log_k = lam, k, math.log(k)
+        self.N = float(max(db_residues, 1))
+        self.n = float(max(db_seqs, 1))
+        self.length = length
+
+    @staticmethod
+    def sequence(open_, ext, comp_bias, db_residues, db_seqs):
+        lam, k, adl, beta = SEQUENCE_TABLE[(open_, ext, bool(comp_bias))]
+        return EValueParams(lam, k, db_residues, db_seqs, (adl, beta))
+
+    def search_space(self, qlen):
+        m = float(max(qlen, 1))
+        if self.length is None:
+            return m * self.N
+        ell = length_adjustment(self.k, self.length[0], self.length[1], m, self.N, self.n)
+        return _fmax(_fmax(m - ell, 1.0 / self.k), 1.0) * _fmax(_fmax(self.N - self.n * ell, self.N * 0.01), 1.0)
+
+    def bit_score(self, raw):
+        return (self.lam * float(raw) - self.log_k) / math.log(2.0)
+
+    def evalue_in_space(self, raw, space):
+        return self.k * space * math.exp(-self.lam * float(raw))
+
+    def evalue(self, raw, qlen):
+        return self.evalue_in_space(raw, self.search_space(qlen))
+
+    def min_score_for(self, max_e, space):
+        try:
+            s = math.ceil(math.log(self.k * space / max_e) / self.lam)
+        except (ValueError, OverflowError, ZeroDivisionError):
+            return 1
+        return int(min(max(float(s), 1.0), 2147483647 / 2.0))
+
+
+def gumbel_mu_given_lambda(xs, lam):
+    xmin = min(xs)
+    s = 0.0
+    for x in xs:
+        s += math.exp(-lam * (x - xmin))
+    return xmin - math.log(s / len(xs)) / lam
+
+
+def fit_gumbel_ml(xs):
+    """fit_gumbel_ml (src/evalue.rs:139-182): (lambda, mu) or None."""
+    n = len(xs)
+    if n < 10:
+        return None
+    mean = 0.0
+    for x in xs:
+        mean += x
+    mean /= n
+    var = 0.0
+    for x in xs:
+        var += (x - mean) * (x - mean)
+    var /= (n - 1.0)
+    if var != var or var <= 0.0:
+        return None
+    xmin = min(xs)
+
+    def g(lam):
+        s0 = s1 = 0.0
+        for x in xs:
+            w = math.exp(-lam * (x - xmin))
+            s0 += w
+            s1 += x * w
+        return 1.0 / lam - mean + s1 / s0
+
+    lam0 = math.pi / (math.sqrt(var) * math.sqrt(6.0))
+    lo, hi = lam0 / 10.0, lam0 * 10.0
+    if g(lo) < 0.0 or g(hi) > 0.0:
+        return None
+    for _ in range(200):
+        mid = 0.5 * (lo + hi)
+        if g(mid) > 0.0:
+            lo = mid
+        else:
+            hi = mid
+        if hi - lo < 1e-12 * hi:
+            break
+    lam = 0.5 * (lo + hi)
+    return lam, gumbel_mu_given_lambda(xs, lam)
+
+
+def rust_e2(x: float) -> str:
+    """Rust's {:.2e}: C's %.2e mantissa (correctly rounded, ties to even),
+    exponent without '+' or zero padding (src/search.rs:203-216)."""
+    s = "%.2e" % x
+    if "e" not in s:
+        return {"nan": "NaN", "inf": "inf", "-inf": "-inf"}.get(s, s)
+    mant, e = s.split("e")
+    return f"{mant}e{int(e)}"
+
+
+# ---- composition bias and low-complexity mask (pure Python, f64) -------------
+
+def _round_half_away(x: float) -> int:
+    return int(math.floor(x + 0.5)) if x >= 0.0 else -int(math.floor(-x + 0.5))
+
+
+def local_bias_correction(q, penalize_only=False):
+    """local_bias_correction_mode (src/compbias.rs:38-70)."""
+    from . import blosum62
+    sc, pb, _, _ = blosum62()
+    sc = np.asarray(sc, dtype=np.int64).reshape(21, 21)
+    exp_ = []
+    for a in range(21):
+        e = 0.0
+        for b in range(20):
+            e += float(pb[b]) * float(sc[a, b])
+        exp_.append(e)
+    n = len(q)
+    out = []
+    for i in range(n):
+        lo, hi = max(i - 20, 0), min(i + 20, n)
+        a = int(q[i])
+        s = sum(int(sc[a, int(q[j])]) for j in range(lo, hi) if j != i)
+        d = _round_half_away(exp_[min(a, 20)] - s / float(hi - lo))
+        out.append(min(d, 0) if penalize_only else d)
+    return np.array(out, dtype=np.int32)
+
+
+def low_complexity_mask(seq, window=12, locut=2.2, hicut=2.5):
+    """low_complexity_mask_into (src/mask.rs:70-134)."""
+    n, w = len(seq), window
+    mask = np.zeros(n, bool)
+    if w < 2 or n < w:
+        return mask
+    clog = [0.0] + [c * math.log2(c) for c in range(1, w + 1)]
+    log_w = math.log2(w)
+    nwin = n - w + 1
+    ent = [0.0] * nwin
+    counts = [0] * 32
+    s = 0.0
+    sym = lambda r: min(int(r), 31)  # noqa: E731
+    for r in seq[:w]:
+        k = sym(r)
+        s -= clog[counts[k]]
+        counts[k] += 1
+        s += clog[counts[k]]
+    ent[0] = log_w - s / w
+    for i in range(1, nwin):
+        o = sym(seq[i - 1])
+        s -= clog[counts[o]]
+        counts[o] -= 1
+        s += clog[counts[o]]
+        k = sym(seq[i + w - 1])
+        s -= clog[counts[k]]
+        counts[k] += 1
+        s += clog[counts[k]]
+        ent[i] = log_w - s / w
+    i = 0
+    while i < nwin:
+        if ent[i] <= locut + 1e-12:
+            lo = i
+            while lo > 0 and ent[lo - 1] <= hicut + 1e-12:
+                lo -= 1
+            hi = i
+            while hi + 1 < nwin and ent[hi + 1] <= hicut + 1e-12:
+                hi += 1
+            mask[lo:hi + w] = True
+            i = hi + 1
+        else:
+            i += 1
+    return mask
+
+
+def query_rows(q, comp_bias=True, penalize_only=False, mask=(15, 2.3, 2.3)):
+    """search_query's rows (src/search.rs:432-463)."""
+    from . import blosum62, seq_rows
+    rows = np.asarray(seq_rows(q), dtype=np.int32).copy()
+    if comp_bias:
+        rows += local_bias_correction(q, penalize_only)[:, None]
+    if mask is not None:
+        mk = low_complexity_mask(q, *mask)
+        sc = np.asarray(blosum62()[0], dtype=np.int32).reshape(21, 21)
+        rows[mk] = sc[20]
+    return rows

@@ -274,10 +274,6 @@ Kind kind_for(const st_qprof *qp, int32_t open, int32_t ext) {
   return kI32F;
 }
 
-struct Shape {
-  int G, R;
-};
-
 // ST_FORCE_R=8|16 and ST_FORCE_G=1..64 pin the rows per lane and the lanes
 // per group (tuning runs and tests); read at every plan
 static int forced_rows() {
@@ -292,79 +288,105 @@ static int forced_groups() {
   return (v >= 1 && v <= 64 && (v & (v - 1)) == 0) ? v : 0;
 }
 
-// A unit longer than this many DP steps is penalised: one wave runs a unit
-// alone, so very long units set the tail of a persistent launch.
-constexpr double kUnitSteps = 12000.0;
+// Device time per executed cell of each launch shape (picoseconds, packed
+// int16 kernel with the ungapped evidence), measured on MI355X with uniform
+// workloads where every unit has the shape; rows R = 8, 16 by lanes per group
+// G = 1..64.  Narrow groups pay for their many distinct pass-boundary and
+// residue streams (memory-pipeline bound), 64-lane groups for cross-row
+// shuffles.
+static const double kShapePs[2][7] = {
+    {2.00, 0.87, 0.33, 0.26, 0.255, 0.26, 0.41},  // R = 8
+    {0.85, 0.38, 0.27, 0.27, 0.27, 0.315, 0.68},  // R = 16
+};
 
-// Launch shape for a query of length m against targets of median length
-// n_med (longest n_max): minimise the modelled cost per useful cell (row
-// padding to passes x G x R, the skew's G-1 idle steps per pass, the
-// pass-boundary round trip, the fixed work of a step) with a penalty on
-// units whose step count would dominate the launch's tail.
-static Shape choose_shape(uint32_t m, double n_med, double n_max, Kind kind) {
-  Shape best{16, 16};
-  double bc = 1e300;
-  for (int R = 8; R <= 16; R += 8) {
-    if (kind == kI32N && R != 8) continue;
-    if (forced_rows() && R != forced_rows() && kind != kI32N) continue;
-    for (int G = 1; G <= 64; G <<= 1) {
-      if (forced_groups() && G != forced_groups()) continue;
-      const int GR = G * R;
-      // the pass's profile slice must fit 64 KB of LDS
-      if ((size_t)kCodes * (GR + 2) * (kind == kPk16 ? 2 : 4) > 65536) continue;
-      const uint32_t passes = std::max<uint32_t>(1, (m + GR - 1) / GR);
-      const double m_pad = (double)passes * GR;
-      double c = (m_pad / std::max<uint32_t>(m, 1)) * ((n_med + G - 1) / std::max(n_med, 1.0));
-      c *= 1.0 + 1.6 / R;
-      if (passes > 1) c *= 1.0 + 6.0 / GR;
-      if (G > 16) c *= 1.04;  // cross-row lane shuffles
-      const double steps = (double)passes * (n_max + G - 1) * R / 16.0;
-      if (steps > kUnitSteps) c *= steps / kUnitSteps;
-      if (c < bc) {
-        bc = c;
-        best = Shape{G, R};
-      }
-    }
-  }
-  return best;
+double unit_steps_cap(uint64_t batch_cells) {
+  // half the mean work of a resident wave (256 CUs x 8 waves, 2048 cells per
+  // packed step), within [1500, 12000] steps
+  const double mean = (double)batch_cells / 2048.0 / 2048.0;
+  return std::min(12000.0, std::max(1500.0, mean / 2.0));
 }
 
-void plan_query(Plan &pl, std::vector<Unit> &pending, uint32_t qi, const st_qprof *qp,
-                       const uint32_t *tids, const uint64_t *outs, size_t count, Kind kind, const st_db *db) {
+// Cut the sorted candidates of one query into wave units.  Each unit takes
+// the launch shape (G, R) with the least modelled device time per useful
+// cell: executed cells (rows padded to passes x G x R, columns to the unit's
+// longest target plus the skew's G-1 steps, empty group slots) times the
+// shape's measured cost, with a penalty on units longer than `cap` steps
+// (normalised to R = 16), whose single wave would set a launch's tail.
+void plan_query(Plan &pl, std::vector<Unit> &pending, uint32_t qi, const st_qprof *qp, const uint32_t *tids,
+                const uint64_t *outs, size_t count, Kind kind, const st_db *db, double cap) {
   if (count == 0) return;
-  std::vector<uint32_t> order(count);
-  std::iota(order.begin(), order.end(), 0u);
-  std::stable_sort(order.begin(), order.end(),
-                   [&](uint32_t a, uint32_t b) { return db->hlen[tids[a]] < db->hlen[tids[b]]; });
-  const double nmed = (double)db->hlen[tids[order[count / 2]]];
-  const double nmax = (double)db->hlen[tids[order[count - 1]]];
-  const Shape sh = choose_shape(qp->m, nmed, nmax, kind);
+  // stable length order: (length << 32 | index) keys
+  std::vector<uint64_t> key(count);
+  for (size_t k = 0; k < count; k++) key[k] = ((uint64_t)db->hlen[tids[k]] << 32) | k;
+  std::sort(key.begin(), key.end());
+  std::vector<uint64_t> csum(count + 1, 0);
+  for (size_t k = 0; k < count; k++) csum[k + 1] = csum[k] + (key[k] >> 32);
   const int per = kind == kPk16 ? 2 : 1;
-  const uint32_t cap = (uint32_t)(64 / sh.G * per);
-  const uint32_t GR = (uint32_t)(sh.G * sh.R);
-  const uint32_t passes = std::max<uint32_t>(1, (qp->m + GR - 1) / GR);
-  for (size_t s = 0; s < count; s += cap) {
-    const size_t e = std::min(count, s + cap);
+  const uint64_t m = qp->m;
+  const int fr = forced_rows(), fg = forced_groups();
+  for (size_t s = 0; s < count;) {
+    int bG = 16, bR = kind == kI32N ? 8 : 16;
+    double bc = 1e300;
+    for (int ri = 0; ri < 2; ri++) {
+      const int R = 8 * (ri + 1);
+      if (kind == kI32N && R != 8) continue;
+      if (fr && R != fr && kind != kI32N) continue;
+      for (int gi = 0; gi < 7; gi++) {
+        const int G = 1 << gi;
+        if (fg && G != fg) continue;
+        const uint64_t GR = (uint64_t)G * R;
+        // the pass's profile slice must fit 64 KB of LDS
+        if ((uint64_t)kCodes * (GR + 2) * (kind == kPk16 ? 2 : 4) > 65536) continue;
+        const size_t e = std::min(count, s + (size_t)(64 / G * per));
+        const uint64_t np = key[e - 1] >> 32;
+        const uint64_t passes = std::max<uint64_t>(1, (m + GR - 1) / GR);
+        const double exec = 64.0 * R * per * passes * (np + G - 1);
+        const double useful = std::max(1.0, (double)m * (double)(csum[e] - csum[s]));
+        double c = exec * kShapePs[ri][gi] / useful;
+        const double steps = (double)passes * (np + G - 1) * R / 16.0;
+        if (steps > cap) c *= steps / cap;
+        if (c < bc) {
+          bc = c;
+          bG = G;
+          bR = R;
+        }
+      }
+    }
+    const size_t e = std::min(count, s + (size_t)(64 / bG * per));
+    const uint32_t GR = (uint32_t)(bG * bR);
     Unit u{};
     u.query = qi;
     u.pair0 = (uint32_t)pl.pairs.size();
     u.npairs = (uint32_t)(e - s);
-    u.G = (uint32_t)sh.G;
-    u.R = (uint32_t)sh.R;
-    u.passes = passes;
-    uint32_t np = 0;
+    u.G = (uint32_t)bG;
+    u.R = (uint32_t)bR;
+    u.passes = std::max<uint32_t>(1, (uint32_t)((m + GR - 1) / GR));
     for (size_t k = s; k < e; k++) {
-      const uint32_t o = order[k];
+      const uint32_t o = (uint32_t)(key[k] & 0xffffffffu);
       PairDev pd;
       pd.tid = tids[o];
       pd.out = (uint32_t)outs[o];
       pl.pairs.push_back(pd);
-      np = std::max(np, db->hlen[tids[o]]);
     }
-    u.n_pad = np;
-    pl.exec_cells += (uint64_t)64 * sh.R * per * passes * (np + sh.G - 1);
+    u.n_pad = (uint32_t)(key[e - 1] >> 32);
+    pl.exec_cells += (uint64_t)64 * bR * per * u.passes * (u.n_pad + bG - 1);
     pending.push_back(u);
+    s = e;
   }
+}
+
+// dp_occupancy memoised per (kind, R, ung, LDS bytes): the runtime query is
+// slow next to a serving micro-batch's planning.
+static int occupancy_cached(int kind, int R, bool ung, size_t lds) {
+  static std::mutex mu;
+  static std::vector<std::pair<uint64_t, int>> memo;
+  const uint64_t key = ((uint64_t)kind << 40) | ((uint64_t)R << 32) | ((uint64_t)ung << 31) | lds;
+  std::lock_guard<std::mutex> lk(mu);
+  for (const auto &e : memo)
+    if (e.first == key) return e.second;
+  const int n = dp_occupancy(kind, R, ung, lds);
+  memo.emplace_back(key, n);
+  return n;
 }
 
 static double unit_cost(const Unit &u) { return (double)u.passes * u.R * (u.n_pad + u.G - 1); }
@@ -407,7 +429,7 @@ void finish_plan(Plan &pl, std::vector<Unit> &pending, const std::vector<Kind> &
     }
     L.nunits = (uint32_t)(j - i);
     const size_t lds = (size_t)kCodes * (L.gr_max + 2) * (L.kind == kPk16 ? 2 : 4);
-    const int occ = std::max(1, dp_occupancy(L.kind, L.R, ung, lds));
+    const int occ = std::max(1, occupancy_cached(L.kind, L.R, ung, lds));
     L.grid = std::max<uint32_t>(1, std::min<uint32_t>(L.nunits, (uint32_t)(occ * cus)));
     L.slot = (int)(pl.launches.size() % kLaunchSlots);
     L.buf0 = pl.buf_elems;
@@ -551,6 +573,7 @@ static int batch_create_impl(st_ctx *ctx, const st_db *db, const st_qprof *const
     return fail(ST_ERR_DEVICE, "st_batch_create: hipSetDevice failed");
   }
   std::vector<Unit> pending;
+  const double cap = unit_steps_cap(cells);
   for (uint64_t q = 0; q < nq; q++) {
     b->qkind[q] = kind_for(b->qps[q], b->open[q], b->ext[q]);
     const uint64_t s = cand_off[q], e = cand_off[q + 1];
@@ -560,7 +583,8 @@ static int batch_create_impl(st_ctx *ctx, const st_db *db, const st_qprof *const
     if (b->qkind[q] == kPk16 && (uint64_t)b->qps[q]->m + nmax - 1 > kMaxDiagsPk16) b->qkind[q] = kI32F;
     std::vector<uint64_t> outs(e - s);
     std::iota(outs.begin(), outs.end(), s);
-    plan_query(b->plan, pending, (uint32_t)q, b->qps[q], cand_tids + s, outs.data(), e - s, b->qkind[q], db);
+    plan_query(b->plan, pending, (uint32_t)q, b->qps[q], cand_tids + s, outs.data(), e - s, b->qkind[q], db,
+               cap);
   }
   finish_plan(b->plan, pending, b->qkind, b->open, b->ext, -1, ctx->cus, ung);
   std::vector<QueryDev> qv(nq);
@@ -629,7 +653,8 @@ static int batch_run_locked(st_batch *b) {
       std::vector<uint32_t> tids;
       for (uint64_t o : perq[q]) tids.push_back(b->cand_tids[o]);
       b->rescored += perq[q].size();
-      plan_query(b->resc, pending, (uint32_t)q, b->qps[q], tids.data(), perq[q].data(), tids.size(), kI32F, b->db);
+      plan_query(b->resc, pending, (uint32_t)q, b->qps[q], tids.data(), perq[q].data(), tids.size(), kI32F, b->db,
+                 unit_steps_cap(0));
     }
     finish_plan(b->resc, pending, b->qkind, b->open, b->ext, (int)kI32F, ctx->cus, b->ung);
     if (!b->resc.units.empty()) {

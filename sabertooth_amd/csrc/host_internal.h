@@ -174,9 +174,14 @@ struct Plan {
 };
 
 // Add the pairs of query qi (result rows outs[], target ids tids[]) to the
-// pending units of `pl`: targets sorted by length, cut into wave units.
+// pending units of `pl`: targets sorted by length, cut into wave units whose
+// launch shape is chosen per unit; `cap` bounds a unit's DP steps.
 void plan_query(Plan &pl, std::vector<Unit> &pending, uint32_t qi, const st_qprof *qp, const uint32_t *tids,
-                const uint64_t *outs, size_t count, Kind kind, const st_db *db);
+                const uint64_t *outs, size_t count, Kind kind, const st_db *db, double cap);
+
+// Unit step cap for a batch of `batch_cells` DP cells: short units when the
+// batch is small (latency), up to 12000 steps for large batches.
+double unit_steps_cap(uint64_t batch_cells);
 
 // Group pending units into launches (kind, R, gaps, LDS class), longest
 // first, size their grids for `cus` compute units and lay out their

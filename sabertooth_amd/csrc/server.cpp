@@ -93,6 +93,7 @@ struct Slot {
   int index = 0;
   hipStream_t stream = nullptr;
   hipEvent_t done = nullptr;
+  hipEvent_t t_start = nullptr, t_end = nullptr;  // device time of a micro-batch
   DevBuf<uint32_t> counter;
   GrowBuf<QueryDev> d_q;
   GrowBuf<int32_t> d_gaps;  // open[nq], ext[nq], gate[nq]
@@ -171,6 +172,7 @@ hipError_t h2d(GrowBuf<T> &d, const T *h, size_t n, hipStream_t s) {
 
 // Score one micro-batch on the slot's stream and fill every request's output.
 int run_micro_batch(Slot &sl, std::vector<std::shared_ptr<Req>> &reqs) {
+  const double t_begin = now_ms();
   st_server *srv = sl.srv;
   const st_db *db = srv->db;
   const size_t nq = reqs.size();
@@ -179,6 +181,9 @@ int run_micro_batch(Slot &sl, std::vector<std::shared_ptr<Req>> &reqs) {
   const uint64_t np = off[nq];
   std::vector<Kind> qkind(nq);
   std::vector<int32_t> open(nq), ext(nq);
+  uint64_t batch_cells = 0;
+  for (const auto &r : reqs) batch_cells += r->cells;
+  const double cap = unit_steps_cap(batch_cells);
   Plan pl;
   std::vector<Unit> pending;
   ST_HIP(sl.h_q.reserve(nq));
@@ -195,7 +200,7 @@ int run_micro_batch(Slot &sl, std::vector<std::shared_ptr<Req>> &reqs) {
     if (qkind[q] == kPk16 && (uint64_t)r.qp->m + nmax - 1 > kMaxDiagsPk16) qkind[q] = kI32F;
     std::vector<uint64_t> outs(r.cands.size());
     std::iota(outs.begin(), outs.end(), off[q]);
-    plan_query(pl, pending, (uint32_t)q, r.qp, r.cands.data(), outs.data(), r.cands.size(), qkind[q], db);
+    plan_query(pl, pending, (uint32_t)q, r.qp, r.cands.data(), outs.data(), r.cands.size(), qkind[q], db, cap);
     sl.h_q.p[q] = r.qp->view;
     sl.h_gaps.p[q] = r.open;
     sl.h_gaps.p[nq + q] = r.ext;
@@ -207,11 +212,13 @@ int run_micro_batch(Slot &sl, std::vector<std::shared_ptr<Req>> &reqs) {
     }
   }
   finish_plan(pl, pending, qkind, open, ext, -1, srv->ctx->cus, true);
+  const double t_planned = now_ms();
   ST_HIP(sl.h_units.reserve(pl.units.size()));
   ST_HIP(sl.h_pairs.reserve(pl.pairs.size()));
   std::copy(pl.units.begin(), pl.units.end(), sl.h_units.p);
   std::copy(pl.pairs.begin(), pl.pairs.end(), sl.h_pairs.p);
   hipStream_t s = sl.stream;
+  ST_HIP(hipEventRecord(sl.t_start, s));
   ST_HIP(h2d(sl.d_q, sl.h_q.p, nq, s));
   ST_HIP(h2d(sl.d_gaps, sl.h_gaps.p, 3 * nq, s));
   ST_HIP(h2d(sl.d_units, sl.h_units.p, pl.units.size(), s));
@@ -246,8 +253,10 @@ int run_micro_batch(Slot &sl, std::vector<std::shared_ptr<Req>> &reqs) {
     ST_HIP(hipMemcpyAsync(sl.h_pout.p, sl.d_pout.p(), np * sizeof(PairOut), hipMemcpyDeviceToHost, s));
     ST_HIP(hipMemcpyAsync(sl.h_res.p, sl.d_res.p(), np * sizeof(st_pair_result), hipMemcpyDeviceToHost, s));
   }
-  ST_HIP(hipEventRecord(sl.done, s));
-  ST_HIP(hipEventSynchronize(sl.done));
+  ST_HIP(hipEventRecord(sl.t_end, s));
+  ST_HIP(hipEventSynchronize(sl.t_end));
+  float dev_ms = 0;
+  ST_HIP(hipEventElapsedTime(&dev_ms, sl.t_start, sl.t_end));
   // int16 pairs whose best reached 32767: exact int32 rescoring, then the
   // gate again over the micro-batch
   Plan rs;
@@ -263,7 +272,8 @@ int run_micro_batch(Slot &sl, std::vector<std::shared_ptr<Req>> &reqs) {
       }
     if (outs.empty()) continue;
     nres += outs.size();
-    plan_query(rs, rs_pending, (uint32_t)q, reqs[q]->qp, tids.data(), outs.data(), tids.size(), kI32F, db);
+    plan_query(rs, rs_pending, (uint32_t)q, reqs[q]->qp, tids.data(), outs.data(), tids.size(), kI32F, db,
+               unit_steps_cap(0));
   }
   if (nres) {
     finish_plan(rs, rs_pending, qkind, open, ext, (int)kI32F, srv->ctx->cus, true);
@@ -287,6 +297,8 @@ int run_micro_batch(Slot &sl, std::vector<std::shared_ptr<Req>> &reqs) {
       std::memcpy(reqs[q]->out, sl.h_res.p + off[q], reqs[q]->cands.size() * sizeof(st_pair_result));
   std::lock_guard<std::mutex> lk(srv->mu);
   srv->stats.rescored += nres;
+  srv->stats.plan_ms += t_planned - t_begin;
+  srv->stats.device_ms += dev_ms;
   return ST_OK;
 }
 
@@ -374,6 +386,7 @@ int st_server_create(st_ctx *ctx, const st_db *db, const st_server_config *cfg, 
     sl->index = (int)k;
     if (hipStreamCreateWithFlags(&sl->stream, hipStreamNonBlocking) != hipSuccess ||
         hipEventCreateWithFlags(&sl->done, hipEventDisableTiming) != hipSuccess ||
+        hipEventCreate(&sl->t_start) != hipSuccess || hipEventCreate(&sl->t_end) != hipSuccess ||
         sl->counter.alloc(1) != hipSuccess) {
       if (sl->stream) (void)hipStreamDestroy(sl->stream);
       if (sl->done) (void)hipEventDestroy(sl->done);
@@ -493,6 +506,8 @@ int st_server_destroy(st_server *s) {
   for (auto &sl : s->slots) {
     (void)hipStreamSynchronize(sl->stream);
     (void)hipEventDestroy(sl->done);
+    (void)hipEventDestroy(sl->t_start);
+    (void)hipEventDestroy(sl->t_end);
     (void)hipStreamDestroy(sl->stream);
   }
   delete s;

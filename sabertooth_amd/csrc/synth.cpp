@@ -160,4 +160,38 @@ void sy_pick(uint64_t *state, uint64_t n, uint64_t range, uint32_t *out) {
   *state = g.s;
 }
 
+// One a3m member row of an MSA built around `root` (m codes): each root
+// column keeps its residue with probability `identity` (else a P(a|b)
+// substitute) as an uppercase letter; a gap run of 1..10 columns ('-')
+// starts with probability gap_rate; after a column, an insertion of 1..5
+// lowercase background residues follows with probability insert_rate.
+// Writes at most cap bytes of text (no newline) and returns the length.
+uint64_t sy_msa_member(uint64_t *state, const uint8_t *root, uint64_t m, double identity, double gap_rate,
+                       double insert_rate, char *out, uint64_t cap) {
+  static const char kAA[] = "ACDEFGHIKLMNPQRSTVWYX";
+  Rng g{*state};
+  uint64_t n = 0;
+  auto put = [&](char c) {
+    if (n < cap) out[n] = c;
+    n++;
+  };
+  for (uint64_t i = 0; i < m;) {
+    if (g.f64() < gap_rate) {
+      const uint64_t L = std::min<uint64_t>(1 + g.below(10), m - i);
+      for (uint64_t k = 0; k < L; k++) put('-');
+      i += L;
+      continue;
+    }
+    const uint8_t r = g.f64() < identity ? root[i] : substitute(g, root[i]);
+    put(kAA[r < 21 ? r : 20]);
+    i++;
+    if (i < m && g.f64() < insert_rate) {
+      const uint64_t L = 1 + g.below(5);
+      for (uint64_t k = 0; k < L; k++) put((char)(kAA[background(g)] + 32));
+    }
+  }
+  *state = g.s;
+  return n;
+}
+
 }  // extern "C"

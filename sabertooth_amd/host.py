@@ -96,3 +96,158 @@ def parse_msa(text: str):
     out = np.zeros(max(n.value * L.value, 1), np.uint8)
     check(_L().st_parse_msa(buf, len(b), ptr(out), out.size, C.byref(n), C.byref(L)))
     return out[: n.value * L.value].reshape(n.value, L.value), int(L.value)
+
+
+# ---- hit statistics (src/evalue.rs) ------------------------------------------
+
+GAPPED_PARAMS_DT = np.dtype([("lambda", "<f8"), ("k", "<f8"), ("has_length", "<i4"), ("reserved", "<i4"),
+                             ("alpha_d_lambda", "<f8"), ("beta", "<f8")])
+EVALUE_PARAMS_DT = np.dtype([("lambda", "<f8"), ("k", "<f8"), ("log_k", "<f8"), ("db_residues", "<f8"),
+                             ("db_seqs", "<f8"), ("has_length", "<i4"), ("reserved", "<i4"),
+                             ("alpha_d_lambda", "<f8"), ("beta", "<f8")])
+GUMBEL_DT = np.dtype([("lambda", "<f8"), ("mu", "<f8"), ("n", "<u8")])
+_D = C.c_double
+_SIGS += [
+    ("st_evalue_table", C.c_int, [C.c_int32, C.c_int32, C.c_int, _P]),
+    ("st_evalue_params_from", C.c_int, [_P, C.c_uint64, C.c_uint64, _P]),
+    ("st_evalue_params_profile", C.c_int, [C.c_uint64, _P]),
+    ("st_length_adjustment", _D, [_D, _D, _D, _D, _D, _D]),
+    ("st_search_space", _D, [_P, C.c_uint64]),
+    ("st_bit_score", _D, [_P, C.c_int32]),
+    ("st_evalue", _D, [_P, C.c_int32, C.c_uint64]),
+    ("st_evalue_in_space", _D, [_P, C.c_int32, _D]),
+    ("st_min_score_for", C.c_int32, [_P, _D, _D]),
+    ("st_gumbel_mu_given_lambda", _D, [_P, C.c_uint64, _D]),
+    ("st_fit_gumbel_ml", C.c_int, [_P, C.c_uint64, _P]),
+    ("st_ungapped_lambda", _D, []),
+]
+EXPORTED = [s[0] for s in _SIGS]
+
+
+def length_adjustment(k, alpha_d_lambda, beta, m, n_res, n_seqs) -> float:
+    return float(_L().st_length_adjustment(k, alpha_d_lambda, beta, m, n_res, n_seqs))
+
+
+class EValueParams:
+    """EValueParams (src/evalue.rs:311-461) bound to a database size."""
+
+    def __init__(self, rec: np.ndarray):
+        self._r = rec
+
+    @staticmethod
+    def sequence(open_: int, ext: int, comp_bias: bool, db_residues: int, db_seqs: int) -> "EValueParams":
+        g = np.zeros(1, GAPPED_PARAMS_DT)
+        check(_L().st_evalue_table(open_, ext, 1 if comp_bias else 0, ptr(g)))
+        return EValueParams.from_params(g, db_residues, db_seqs)
+
+    @staticmethod
+    def from_params(g: np.ndarray, db_residues: int, db_seqs: int) -> "EValueParams":
+        r = np.zeros(1, EVALUE_PARAMS_DT)
+        check(_L().st_evalue_params_from(ptr(g), db_residues, db_seqs, ptr(r)))
+        return EValueParams(r)
+
+    @staticmethod
+    def gapped(lam: float, k: float, db_residues: int, db_seqs: int, length=None) -> "EValueParams":
+        g = np.zeros(1, GAPPED_PARAMS_DT)
+        g["lambda"], g["k"] = lam, k
+        if length is not None:
+            g["has_length"], g["alpha_d_lambda"], g["beta"] = 1, length[0], length[1]
+        return EValueParams.from_params(g, db_residues, db_seqs)
+
+    @staticmethod
+    def for_profile(db_residues: int) -> "EValueParams":
+        r = np.zeros(1, EVALUE_PARAMS_DT)
+        check(_L().st_evalue_params_profile(db_residues, ptr(r)))
+        return EValueParams(r)
+
+    @property
+    def lam(self) -> float:
+        return float(self._r["lambda"][0])
+
+    @property
+    def k(self) -> float:
+        return float(self._r["k"][0])
+
+    def search_space(self, qlen: int) -> float:
+        return float(_L().st_search_space(ptr(self._r), qlen))
+
+    def bit_score(self, raw: int) -> float:
+        return float(_L().st_bit_score(ptr(self._r), raw))
+
+    def evalue(self, raw: int, qlen: int) -> float:
+        return float(_L().st_evalue(ptr(self._r), raw, qlen))
+
+    def evalue_in_space(self, raw: int, space: float) -> float:
+        return float(_L().st_evalue_in_space(ptr(self._r), raw, space))
+
+    def min_score_for(self, max_evalue: float, space: float) -> int:
+        return int(_L().st_min_score_for(ptr(self._r), max_evalue, space))
+
+
+def fit_gumbel_ml(xs):
+    """(lambda, mu) of the ML Gumbel fit, or None (src/evalue.rs:139-182)."""
+    x = np.ascontiguousarray(xs, dtype=np.float64)
+    out = np.zeros(1, GUMBEL_DT)
+    rc = _L().st_fit_gumbel_ml(ptr(x), len(x), ptr(out))
+    if rc != 0:
+        return None
+    return float(out["lambda"][0]), float(out["mu"][0])
+
+
+def ungapped_lambda() -> float:
+    return float(_L().st_ungapped_lambda())
+
+
+# ---- sequence-query rows (src/compbias.rs, src/mask.rs, src/search.rs) -------
+
+@dataclass(frozen=True)
+class MaskParams:
+    """MaskParams (src/mask.rs:27-66)."""
+    window: int = 12
+    locut: float = 2.2
+    hicut: float = 2.5
+
+    @staticmethod
+    def scoring_default() -> "MaskParams":
+        return MaskParams(15, 2.3, 2.3)
+
+
+_SIGS += [
+    ("st_local_bias_correction", C.c_int, [_P, C.c_uint64, C.c_int, _P]),
+    ("st_low_complexity_mask", C.c_int, [_P, C.c_uint64, C.c_uint32, _D, _D, _P, C.POINTER(C.c_uint64)]),
+    ("st_query_rows", C.c_int, [_P, C.c_uint64, C.c_int, C.c_int, C.c_uint32, _D, _D, _P,
+                                C.POINTER(C.c_uint64)]),
+]
+EXPORTED = [s[0] for s in _SIGS]
+
+
+def _codes(q) -> np.ndarray:
+    return np.ascontiguousarray(np.asarray(q, dtype=np.uint8).reshape(-1))
+
+
+def local_bias_correction(query, penalize_only: bool = False) -> np.ndarray:
+    q = _codes(query)
+    out = np.zeros(max(len(q), 1), np.int32)
+    check(_L().st_local_bias_correction(ptr(q), len(q), 1 if penalize_only else 0, ptr(out)))
+    return out[: len(q)]
+
+
+def low_complexity_mask(seq, p: MaskParams = MaskParams()) -> np.ndarray:
+    q = _codes(seq)
+    out = np.zeros(max(len(q), 1), np.uint8)
+    n = C.c_uint64(0)
+    check(_L().st_low_complexity_mask(ptr(q), len(q), p.window, p.locut, p.hicut, ptr(out), C.byref(n)))
+    return out[: len(q)].astype(bool)
+
+
+def query_rows(query, comp_bias: bool = True, penalize_only: bool = False,
+               mask: MaskParams | None = MaskParams.scoring_default()) -> np.ndarray:
+    """Rows int32 [m][21] of a sequence query as the default search builds
+    them (src/search.rs:432-463)."""
+    q = _codes(query)
+    rows = np.zeros((max(len(q), 1), 21), np.int32)
+    n = C.c_uint64(0)
+    w, lo, hi = (mask.window, mask.locut, mask.hicut) if mask is not None else (0, 0.0, 0.0)
+    check(_L().st_query_rows(ptr(q), len(q), 1 if comp_bias else 0, 1 if penalize_only else 0, w, lo, hi,
+                             ptr(rows), C.byref(n)))
+    return rows[: len(q)]
