@@ -98,7 +98,7 @@ def serving_leg(a, O, st, synth):
     batch_qps = n / t_batch
     log(f"plain batch: {1e3 * t_batch:.1f} ms, {batch_qps:.0f} q/s, {cells.sum() / t_batch / 1e9:.0f} GCUPS")
     reqs = [(qps[k], w.cands[k], gaps[k], w.gates[k]) for k in range(n)]
-    srv = st.Server(ctx, db, workers=4, max_batch_queries=64)
+    srv = st.Server(ctx, db, workers=3, max_batch_queries=128)
     mism = 0
 
     def check(idx, outs):
@@ -154,7 +154,7 @@ def serving_leg(a, O, st, synth):
     return {
         "workload": "serving_265M", "requests": n, "profile_requests": int(sum(w.profile)),
         "db_residues": int(w.offsets[-1]), "db_targets": len(w.offsets) - 1, "candidates_per_request": 3000,
-        "cells": int(cells.sum()), "workers": 4, "max_batch_queries": 64,
+        "cells": int(cells.sum()), "workers": 3, "max_batch_queries": 128,
         "plain_batch": {"ms": round(1e3 * t_batch, 3), "queries_per_s": round(batch_qps, 1),
                         "gcups": round(cells.sum() / t_batch / 1e9, 1)},
         "saturated": {"ms": round(1e3 * t_sat, 3), "queries_per_s": round(sat_qps, 1),

@@ -5,8 +5,9 @@
  * are restated here in C++ (no GPU involved) so a caller gets the same
  * inputs and outputs as the reference's Rust: the PSSM builder of profile
  * queries, the row modifiers of sequence queries (composition-bias
- * correction, low-complexity masking) and the hit statistics (E-values, bit
- * scores, search spaces).  Floating-point code follows the reference's operation order and
+ * correction, low-complexity masking), the exact traceback of reported hits,
+ * the hit statistics (E-values, bit scores, search spaces), the gapped stage
+ * of a query's search and its .m8 rows.  Floating-point code follows the reference's operation order and
  * libm calls; the functions are deterministic and thread-safe.
  *
  * Status codes and st_last_error() are those of sabertooth_amd.h.
@@ -64,6 +65,51 @@ int st_low_complexity_mask(const uint8_t *seq, uint64_t n, uint32_t window, doub
 int st_query_rows(const uint8_t *query, uint64_t m, int comp_bias, int penalize_only,
                   uint32_t mask_window, double mask_locut, double mask_hicut, int32_t *rows,
                   uint64_t *n_masked);
+
+/* ---- exact traceback of reported hits (src/align.rs) ---------------------- */
+
+/* Alignment (src/align.rs:224-248); coordinates 0-based inclusive. */
+typedef struct {
+  int32_t raw_score;
+  int32_t reserved;
+  uint64_t query_start, query_end;
+  uint64_t target_start, target_end;
+  uint64_t aln_len;    /* aligned pairs + gap columns */
+  uint64_t identities; /* pairs of identical real residues */
+  uint64_t mismatches;
+  uint64_t gaps;       /* gap columns */
+  uint64_t gap_opens;  /* gap runs (BLAST gapopen) */
+} st_alignment;
+
+/* A score and its end cell (query_end / target_end = -1 when score is 0). */
+typedef struct {
+  int32_t score;
+  int32_t reserved;
+  int64_t query_end;
+  int64_t target_end;
+} st_score_end;
+
+/* scalar_gotoh_score_end (src/simd.rs:347-390): exact score and the first
+ * optimum in column-major order (smallest target position, then smallest
+ * query position); *qend = *tend = -1 when the score is 0. */
+int st_sw_score_end_host(const int32_t *rows, uint64_t m, const uint8_t *t, uint64_t n,
+                         int32_t gap_open, int32_t gap_extend, int32_t *score, int64_t *qend,
+                         int64_t *tend);
+/* align_canonical (src/align.rs:840-882) of rows [m][21] (qres: the query's
+ * residues for identity counts, NULL for none) against t.  `known` (NULL:
+ * computed here) is the forward score and end cell, e.g. from
+ * st_sw_score_end_batch.  *found = 0 when no positive alignment exists.  The
+ * CIGAR (M, I = target residue against a query gap, D) is written
+ * NUL-terminated up to cigar_cap bytes; *cigar_len receives its length. */
+int st_align_canonical(const int32_t *rows, const uint8_t *qres, uint64_t m, const uint8_t *t,
+                       uint64_t n, int32_t gap_open, int32_t gap_extend, uint32_t band_radius,
+                       const st_score_end *known, st_alignment *out, char *cigar,
+                       uint64_t cigar_cap, uint64_t *cigar_len, int *found);
+/* rescore_alignment (src/align.rs:1064-1135): recompute an alignment's
+ * score and counts from its CIGAR; ST_ERR_INVALID on any disagreement. */
+int st_rescore_alignment(const int32_t *rows, const uint8_t *qres, uint64_t m, const uint8_t *t,
+                         uint64_t n, int32_t gap_open, int32_t gap_extend,
+                         const st_alignment *aln, const char *cigar, int32_t *score);
 
 /* ---- hit statistics (src/evalue.rs) ------------------------------------ */
 
@@ -125,6 +171,56 @@ int st_fit_gumbel_ml(const double *xs, uint64_t n, st_gumbel *out);
 /* solve_lambda (src/evalue.rs:466-498): ungapped lambda of the integer
  * BLOSUM62 (diagnostic). */
 double st_ungapped_lambda(void);
+
+/* ---- gapped stage, hits and .m8 rows (src/search.rs) --------------------- */
+
+/* Hit (src/search.rs:162-186) without the identifier strings. */
+typedef struct {
+  uint32_t target;   /* target ordinal */
+  int32_t raw_score;
+  double pident;     /* 100 * identities / aln_len */
+  uint64_t aln_len, mismatches, gap_opens;
+  uint64_t q_start, q_end, t_start, t_end; /* 0-based inclusive */
+  double evalue;
+  double bit_score;
+  uint64_t qlen, tlen;
+} st_hit;
+
+/* SearchStats counters of the gapped stage (src/search.rs:565-666). */
+typedef struct {
+  uint64_t scored;            /* gapped_scored */
+  uint64_t passed_score;      /* passed_score */
+  uint64_t tracebacks;
+  uint64_t score_mismatches;  /* traceback score != gapped score (expected 0) */
+  uint64_t rejected_coverage;
+  uint64_t hits;
+} st_search_counters;
+
+/* hit_order (src/search.rs:191-197): <0, 0, >0 as a sorts before b. */
+int st_hit_order(const st_hit *a, const st_hit *b);
+/* The pass filter and order of the gapped stage (src/search.rs:537-580):
+ * order[0 .. *n_pass) = indices with score >= min_score and score > 0,
+ * by score descending then target ascending. */
+int st_gapped_stage(const int32_t *scores, const uint32_t *tids, uint64_t n, int32_t min_score,
+                    uint64_t *order, uint64_t *n_pass);
+/* One query's search after scoring (src/search.rs:532-666): min_score from
+ * the E-value cut-off and the query's search space, the gapped stage,
+ * deferred canonical traceback until max_hits (0 = unlimited) hits pass the
+ * coverage filters, E-values and bit scores, hits in hit_order.  arena /
+ * offsets: the database residues on the host.  Writes up to cap hits (and
+ * their CIGARs, cigar_stride bytes each, when cigars is non-NULL);
+ * *n_hits = all hits; counters (nullable) are accumulated. */
+int st_search_hits(const int32_t *rows, const uint8_t *qres, uint64_t m, const uint8_t *arena,
+                   const uint64_t *offsets, uint64_t n_targets, const uint32_t *cand_tids,
+                   const int32_t *scores, uint64_t n_cands, int32_t gap_open, int32_t gap_extend,
+                   const st_evalue_params *ev, double max_evalue, uint64_t max_hits,
+                   double min_query_cov, double min_target_cov, uint32_t band_radius, st_hit *hits,
+                   char *cigars, uint64_t cigar_stride, uint64_t cap, uint64_t *n_hits,
+                   st_search_counters *counters);
+/* Hit::to_m8 (src/search.rs:201-216) with Rust's number formatting: pident
+ * {:.3}, E-value {:.2e} (exponent without '+' or padding), bits {:.1}. */
+int st_format_m8(const char *query_id, const char *target_id, const st_hit *h, char *buf,
+                 uint64_t cap, uint64_t *len);
 
 #ifdef __cplusplus
 }

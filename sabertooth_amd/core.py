@@ -348,11 +348,19 @@ class Server:
         n = len(requests)
         recs = np.zeros(max(n, 1), dtype=_lib.REQUEST_DT)
         keep = []
+        cols = {f: np.zeros(n, dtype=recs.dtype[f]) for f in ("query", "gap_open", "gap_extend", "min_diag_score",
+                                                                 "cand_tids", "n_cands", "out")}
         for k, (qp, cands, gaps, gate) in enumerate(requests):
             c = np.ascontiguousarray(cands, dtype=np.uint32)
             out = np.zeros(max(len(c), 1), dtype=PAIR_RESULT_DT)
-            recs[k] = self._request(qp, c, out, gaps, gate)[0]
+            cols["query"][k] = qp.handle.value
+            cols["gap_open"][k], cols["gap_extend"][k], cols["min_diag_score"][k] = gaps.open, gaps.extend, gate
+            cols["cand_tids"][k] = c.ctypes.data if len(c) else 0
+            cols["n_cands"][k] = len(c)
+            cols["out"][k] = out.ctypes.data
             keep.append((c, out))
+        for f, v in cols.items():
+            recs[f][:n] = v
         arr = np.ascontiguousarray(arrival_ms, dtype=np.float64)
         lat = np.zeros(max(n, 1), dtype=np.float64)
         check(lib().st_server_run_trace(self._h, ptr(recs), ptr(arr), n, ptr(lat)))

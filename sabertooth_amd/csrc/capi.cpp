@@ -301,9 +301,10 @@ static const double kShapePs[2][7] = {
 
 double unit_steps_cap(uint64_t batch_cells) {
   // half the mean work of a resident wave (256 CUs x 8 waves, 2048 cells per
-  // packed step), within [1500, 12000] steps
+  // packed step), within [300, 12000] steps: a lone query spreads over many
+  // short units (latency), a large batch keeps wide units (padding)
   const double mean = (double)batch_cells / 2048.0 / 2048.0;
-  return std::min(12000.0, std::max(1500.0, mean / 2.0));
+  return std::min(12000.0, std::max(300.0, mean / 2.0));
 }
 
 // Cut the sorted candidates of one query into wave units.  Each unit takes
@@ -315,10 +316,28 @@ double unit_steps_cap(uint64_t batch_cells) {
 void plan_query(Plan &pl, std::vector<Unit> &pending, uint32_t qi, const st_qprof *qp, const uint32_t *tids,
                 const uint64_t *outs, size_t count, Kind kind, const st_db *db, double cap) {
   if (count == 0) return;
-  // stable length order: (length << 32 | index) keys
+  // stable length order: (length << 32 | index) keys; two 8-bit LSD radix
+  // passes when every length is below 65536
   std::vector<uint64_t> key(count);
-  for (size_t k = 0; k < count; k++) key[k] = ((uint64_t)db->hlen[tids[k]] << 32) | k;
-  std::sort(key.begin(), key.end());
+  uint32_t lmax = 0;
+  for (size_t k = 0; k < count; k++) {
+    const uint32_t l = db->hlen[tids[k]];
+    lmax = std::max(lmax, l);
+    key[k] = ((uint64_t)l << 32) | k;
+  }
+  if (lmax < 65536) {
+    std::vector<uint64_t> tmp(count);
+    for (int pass = 0; pass < 2; pass++) {
+      const int sh = 32 + 8 * pass;
+      size_t cnt[257] = {0};
+      for (size_t k = 0; k < count; k++) cnt[((key[k] >> sh) & 0xff) + 1]++;
+      for (int b = 0; b < 256; b++) cnt[b + 1] += cnt[b];
+      for (size_t k = 0; k < count; k++) tmp[cnt[(key[k] >> sh) & 0xff]++] = key[k];
+      key.swap(tmp);
+    }
+  } else {
+    std::sort(key.begin(), key.end());
+  }
   std::vector<uint64_t> csum(count + 1, 0);
   for (size_t k = 0; k < count; k++) csum[k + 1] = csum[k] + (key[k] >> 32);
   const int per = kind == kPk16 ? 2 : 1;

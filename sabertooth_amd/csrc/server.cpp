@@ -88,28 +88,30 @@ uint64_t state_bytes_of(const Req &r, const st_db *db) {
 
 }  // namespace
 
+// Concurrent launch streams of one slot: a micro-batch's launches of
+// different kernel instances (rows per lane, precision, LDS class) run side
+// by side.
+constexpr int kSubStreams = 3;
+
 struct Slot {
   st_server *srv = nullptr;
   int index = 0;
   hipStream_t stream = nullptr;
-  hipEvent_t done = nullptr;
+  hipStream_t sub[kSubStreams] = {nullptr, nullptr, nullptr};
+  hipEvent_t fork = nullptr, join[kSubStreams] = {nullptr, nullptr, nullptr};
   hipEvent_t t_start = nullptr, t_end = nullptr;  // device time of a micro-batch
-  DevBuf<uint32_t> counter;
-  GrowBuf<QueryDev> d_q;
-  GrowBuf<int32_t> d_gaps;  // open[nq], ext[nq], gate[nq]
-  GrowBuf<Unit> d_units, d_rs_units;
-  GrowBuf<PairDev> d_pairs, d_rs_pairs, d_all;
-  GrowBuf<uint32_t> d_candq;
+  DevBuf<uint32_t> counter;                       // one per sub-stream
+  // inputs packed into one pinned block and one device block (one copy)
+  Pinned<uint8_t> h_in;
+  GrowBuf<uint8_t> d_in;
+  GrowBuf<Unit> d_rs_units;
+  GrowBuf<PairDev> d_rs_pairs;
   GrowBuf<PairOut> d_pout;
   GrowBuf<uint64_t> d_top4;
   GrowBuf<st_pair_result> d_res;
   GrowBuf<uint4> d_buf, d_rs_buf;
-  Pinned<QueryDev> h_q;
-  Pinned<int32_t> h_gaps;
   Pinned<Unit> h_units;
-  Pinned<PairDev> h_pairs, h_all;
-  Pinned<uint32_t> h_candq;
-  Pinned<PairOut> h_pout;
+  Pinned<PairDev> h_pairs;
   Pinned<st_pair_result> h_res;
   std::thread th;
 };
@@ -136,15 +138,25 @@ struct st_server {
 namespace st {
 namespace {
 
-// Launch a plan's DP kernels one after another on stream s (the slot's own
-// counter is cleared before each launch).
-int launch_on(const Plan &pl, const st_db *db, const QueryDev *d_q, const Unit *d_units, const PairDev *d_pairs,
-              uint4 *d_buf, PairOut *d_pout, uint64_t *d_top4, uint32_t *counter, hipStream_t s) {
-  for (const Launch &L : pl.launches) {
+// Launch a plan's DP kernels: one launch runs on the slot's stream, several
+// fan out over the slot's sub-streams (each with its own unit counter) and
+// join back before the slot stream continues.
+int launch_on(Slot &sl, const Plan &pl, const QueryDev *d_q, const Unit *d_units, const PairDev *d_pairs,
+              uint4 *d_buf, PairOut *d_pout, uint64_t *d_top4) {
+  const st_db *db = sl.srv->db;
+  const size_t nl = pl.launches.size();
+  const bool fan = nl > 1;
+  if (fan) {
+    ST_HIP(hipEventRecord(sl.fork, sl.stream));
+    for (int k = 0; k < kSubStreams && k < (int)nl; k++) ST_HIP(hipStreamWaitEvent(sl.sub[k], sl.fork, 0));
+  }
+  for (size_t i = 0; i < nl; i++) {
+    const Launch &L = pl.launches[i];
+    const int k = (int)(i % kSubStreams);
     DpLaunch a;
     a.units = d_units + L.unit0;
     a.nunits = L.nunits;
-    a.counter = counter;
+    a.counter = sl.counter.p + (fan ? k : 0);
     a.pairs = d_pairs;
     a.queries = d_q;
     a.arena = db->arena.p;
@@ -158,8 +170,13 @@ int launch_on(const Plan &pl, const st_db *db, const QueryDev *d_q, const Unit *
     a.pout = d_pout;
     a.top4 = d_top4;
     a.lds_stride = L.gr_max + 2;
-    ST_HIP(launch_dp(a, L.kind, L.R, true, s));
+    ST_HIP(launch_dp(a, L.kind, L.R, true, fan ? sl.sub[k] : sl.stream));
   }
+  if (fan)
+    for (int k = 0; k < kSubStreams && k < (int)nl; k++) {
+      ST_HIP(hipEventRecord(sl.join[k], sl.sub[k]));
+      ST_HIP(hipStreamWaitEvent(sl.stream, sl.join[k], 0));
+    }
   return ST_OK;
 }
 
@@ -170,7 +187,9 @@ hipError_t h2d(GrowBuf<T> &d, const T *h, size_t n, hipStream_t s) {
   return hipMemcpyAsync(d.p(), h, n * sizeof(T), hipMemcpyHostToDevice, s);
 }
 
-// Score one micro-batch on the slot's stream and fill every request's output.
+size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
+
+// Score one micro-batch on the slot's streams and fill every request's output.
 int run_micro_batch(Slot &sl, std::vector<std::shared_ptr<Req>> &reqs) {
   const double t_begin = now_ms();
   st_server *srv = sl.srv;
@@ -185,11 +204,8 @@ int run_micro_batch(Slot &sl, std::vector<std::shared_ptr<Req>> &reqs) {
   for (const auto &r : reqs) batch_cells += r->cells;
   const double cap = unit_steps_cap(batch_cells);
   Plan pl;
+  pl.pairs.reserve(np);
   std::vector<Unit> pending;
-  ST_HIP(sl.h_q.reserve(nq));
-  ST_HIP(sl.h_gaps.reserve(3 * nq));
-  ST_HIP(sl.h_all.reserve(np));
-  ST_HIP(sl.h_candq.reserve(np));
   for (size_t q = 0; q < nq; q++) {
     const Req &r = *reqs[q];
     open[q] = r.open;
@@ -201,64 +217,75 @@ int run_micro_batch(Slot &sl, std::vector<std::shared_ptr<Req>> &reqs) {
     std::vector<uint64_t> outs(r.cands.size());
     std::iota(outs.begin(), outs.end(), off[q]);
     plan_query(pl, pending, (uint32_t)q, r.qp, r.cands.data(), outs.data(), r.cands.size(), qkind[q], db, cap);
-    sl.h_q.p[q] = r.qp->view;
-    sl.h_gaps.p[q] = r.open;
-    sl.h_gaps.p[nq + q] = r.ext;
-    sl.h_gaps.p[2 * nq + q] = r.gate;
-    for (size_t k = 0; k < r.cands.size(); k++) {
-      sl.h_all.p[off[q] + k].tid = r.cands[k];
-      sl.h_all.p[off[q] + k].out = (uint32_t)(off[q] + k);
-      sl.h_candq.p[off[q] + k] = (uint32_t)q;
-    }
   }
   finish_plan(pl, pending, qkind, open, ext, -1, srv->ctx->cus, true);
+  // packed inputs: query table, gaps (open, ext, gate), units, unit pair
+  // slots, candidate-order pairs, candidate query index
+  const size_t o_q = 0, o_g = align256(o_q + nq * sizeof(QueryDev)),
+               o_u = align256(o_g + 3 * nq * sizeof(int32_t)),
+               o_p = align256(o_u + pl.units.size() * sizeof(Unit)),
+               o_a = align256(o_p + pl.pairs.size() * sizeof(PairDev)), o_c = align256(o_a + np * sizeof(PairDev)),
+               total = align256(o_c + np * sizeof(uint32_t));
+  ST_HIP(sl.h_in.reserve(total));
+  uint8_t *h = sl.h_in.p;
+  auto *hq = (QueryDev *)(h + o_q);
+  auto *hg = (int32_t *)(h + o_g);
+  auto *ha = (PairDev *)(h + o_a);
+  auto *hc = (uint32_t *)(h + o_c);
+  for (size_t q = 0; q < nq; q++) {
+    const Req &r = *reqs[q];
+    hq[q] = r.qp->view;
+    hg[q] = r.open;
+    hg[nq + q] = r.ext;
+    hg[2 * nq + q] = r.gate;
+    for (size_t k = 0; k < r.cands.size(); k++) {
+      ha[off[q] + k].tid = r.cands[k];
+      ha[off[q] + k].out = (uint32_t)(off[q] + k);
+      hc[off[q] + k] = (uint32_t)q;
+    }
+  }
+  std::copy(pl.units.begin(), pl.units.end(), (Unit *)(h + o_u));
+  std::copy(pl.pairs.begin(), pl.pairs.end(), (PairDev *)(h + o_p));
   const double t_planned = now_ms();
-  ST_HIP(sl.h_units.reserve(pl.units.size()));
-  ST_HIP(sl.h_pairs.reserve(pl.pairs.size()));
-  std::copy(pl.units.begin(), pl.units.end(), sl.h_units.p);
-  std::copy(pl.pairs.begin(), pl.pairs.end(), sl.h_pairs.p);
   hipStream_t s = sl.stream;
-  ST_HIP(hipEventRecord(sl.t_start, s));
-  ST_HIP(h2d(sl.d_q, sl.h_q.p, nq, s));
-  ST_HIP(h2d(sl.d_gaps, sl.h_gaps.p, 3 * nq, s));
-  ST_HIP(h2d(sl.d_units, sl.h_units.p, pl.units.size(), s));
-  ST_HIP(h2d(sl.d_pairs, sl.h_pairs.p, pl.pairs.size(), s));
-  ST_HIP(h2d(sl.d_all, sl.h_all.p, np, s));
-  ST_HIP(h2d(sl.d_candq, sl.h_candq.p, np, s));
+  ST_HIP(sl.d_in.reserve(total));
   ST_HIP(sl.d_pout.reserve(np));
   ST_HIP(sl.d_top4.reserve(np * 4));
   ST_HIP(sl.d_res.reserve(np));
   ST_HIP(sl.d_buf.reserve(pl.buf_elems));
-  ST_HIP(sl.h_pout.reserve(np));
   ST_HIP(sl.h_res.reserve(np));
-  int rc = launch_on(pl, db, sl.d_q.p(), sl.d_units.p(), sl.d_pairs.p(), sl.d_buf.p(), sl.d_pout.p(),
-                     sl.d_top4.p(), sl.counter.p, s);
+  uint8_t *d = sl.d_in.p();
+  const auto *dq = (const QueryDev *)(d + o_q);
+  const auto *dg = (const int32_t *)(d + o_g);
+  ST_HIP(hipEventRecord(sl.t_start, s));
+  ST_HIP(hipMemcpyAsync(d, h, total, hipMemcpyHostToDevice, s));
+  int rc = launch_on(sl, pl, dq, (const Unit *)(d + o_u), (const PairDev *)(d + o_p), sl.d_buf.p(), sl.d_pout.p(),
+                     sl.d_top4.p());
   if (rc) return rc;
   PairGateArgs ga;
-  ga.pairs = sl.d_all.p();
+  ga.pairs = (const PairDev *)(d + o_a);
   ga.npairs = np;
-  ga.pair_query = sl.d_candq.p();
-  ga.queries = sl.d_q.p();
+  ga.pair_query = (const uint32_t *)(d + o_c);
+  ga.queries = dq;
   ga.arena = db->arena.p;
   ga.toff = db->off.p;
   ga.tlen = db->len.p;
   ga.top4 = sl.d_top4.p();
-  ga.open = sl.d_gaps.p();
-  ga.ext = sl.d_gaps.p() + nq;
-  ga.gate = sl.d_gaps.p() + 2 * nq;
+  ga.open = dg;
+  ga.ext = dg + nq;
+  ga.gate = dg + 2 * nq;
   ga.pout = sl.d_pout.p();
   ga.res = sl.d_res.p();
-  if (np) ST_HIP(launch_gate_pairs(ga, s));
   if (np) {
-    ST_HIP(hipMemcpyAsync(sl.h_pout.p, sl.d_pout.p(), np * sizeof(PairOut), hipMemcpyDeviceToHost, s));
+    ST_HIP(launch_gate_pairs(ga, s));
     ST_HIP(hipMemcpyAsync(sl.h_res.p, sl.d_res.p(), np * sizeof(st_pair_result), hipMemcpyDeviceToHost, s));
   }
   ST_HIP(hipEventRecord(sl.t_end, s));
   ST_HIP(hipEventSynchronize(sl.t_end));
   float dev_ms = 0;
   ST_HIP(hipEventElapsedTime(&dev_ms, sl.t_start, sl.t_end));
-  // int16 pairs whose best reached 32767: exact int32 rescoring, then the
-  // gate again over the micro-batch
+  // int16 pairs whose best reached 32767 (an int16 result at the clamp):
+  // exact int32 rescoring, then the gate again over the micro-batch
   Plan rs;
   std::vector<Unit> rs_pending;
   uint64_t nres = 0;
@@ -266,7 +293,7 @@ int run_micro_batch(Slot &sl, std::vector<std::shared_ptr<Req>> &reqs) {
     std::vector<uint64_t> outs;
     std::vector<uint32_t> tids;
     for (uint64_t k = off[q]; k < off[q + 1]; k++)
-      if (sl.h_pout.p[k].flag == 1) {
+      if (sl.h_res.p[k].kernel == ST_KERNEL_I16 && sl.h_res.p[k].swg >= 32767) {
         outs.push_back(k);
         tids.push_back(reqs[q]->cands[k - off[q]]);
       }
@@ -284,13 +311,12 @@ int run_micro_batch(Slot &sl, std::vector<std::shared_ptr<Req>> &reqs) {
     ST_HIP(h2d(sl.d_rs_units, sl.h_units.p, rs.units.size(), s));
     ST_HIP(h2d(sl.d_rs_pairs, sl.h_pairs.p, rs.pairs.size(), s));
     ST_HIP(sl.d_rs_buf.reserve(rs.buf_elems));
-    rc = launch_on(rs, db, sl.d_q.p(), sl.d_rs_units.p(), sl.d_rs_pairs.p(), sl.d_rs_buf.p(), sl.d_pout.p(),
-                   sl.d_top4.p(), sl.counter.p, s);
+    rc = launch_on(sl, rs, dq, sl.d_rs_units.p(), sl.d_rs_pairs.p(), sl.d_rs_buf.p(), sl.d_pout.p(), sl.d_top4.p());
     if (rc) return rc;
     ST_HIP(launch_gate_pairs(ga, s));
     ST_HIP(hipMemcpyAsync(sl.h_res.p, sl.d_res.p(), np * sizeof(st_pair_result), hipMemcpyDeviceToHost, s));
-    ST_HIP(hipEventRecord(sl.done, s));
-    ST_HIP(hipEventSynchronize(sl.done));
+    ST_HIP(hipEventRecord(sl.t_end, s));
+    ST_HIP(hipEventSynchronize(sl.t_end));
   }
   for (size_t q = 0; q < nq; q++)
     if (reqs[q]->out && !reqs[q]->cands.empty())
@@ -355,6 +381,29 @@ void worker(Slot *sl) {
 }  // namespace
 }  // namespace st
 
+namespace {
+
+// Wait for and release every slot's streams and events (threads joined).
+void release_slots(st_server *s) {
+  for (auto &sl : s->slots) {
+    if (sl->stream) (void)hipStreamSynchronize(sl->stream);
+    for (hipStream_t &x : sl->sub)
+      if (x) {
+        (void)hipStreamSynchronize(x);
+        (void)hipStreamDestroy(x);
+      }
+    for (hipEvent_t &e : sl->join)
+      if (e) (void)hipEventDestroy(e);
+    if (sl->fork) (void)hipEventDestroy(sl->fork);
+    if (sl->t_start) (void)hipEventDestroy(sl->t_start);
+    if (sl->t_end) (void)hipEventDestroy(sl->t_end);
+    if (sl->stream) (void)hipStreamDestroy(sl->stream);
+  }
+  s->slots.clear();
+}
+
+}  // namespace
+
 extern "C" {
 
 int st_server_create(st_ctx *ctx, const st_db *db, const st_server_config *cfg, st_server **out) {
@@ -384,20 +433,19 @@ int st_server_create(st_ctx *ctx, const st_db *db, const st_server_config *cfg, 
     auto sl = std::make_unique<Slot>();
     sl->srv = s;
     sl->index = (int)k;
-    if (hipStreamCreateWithFlags(&sl->stream, hipStreamNonBlocking) != hipSuccess ||
-        hipEventCreateWithFlags(&sl->done, hipEventDisableTiming) != hipSuccess ||
-        hipEventCreate(&sl->t_start) != hipSuccess || hipEventCreate(&sl->t_end) != hipSuccess ||
-        sl->counter.alloc(1) != hipSuccess) {
-      if (sl->stream) (void)hipStreamDestroy(sl->stream);
-      if (sl->done) (void)hipEventDestroy(sl->done);
-      for (auto &o : s->slots) {
-        (void)hipEventDestroy(o->done);
-        (void)hipStreamDestroy(o->stream);
-      }
+    bool ok = hipStreamCreateWithFlags(&sl->stream, hipStreamNonBlocking) == hipSuccess &&
+              hipEventCreateWithFlags(&sl->fork, hipEventDisableTiming) == hipSuccess &&
+              hipEventCreate(&sl->t_start) == hipSuccess && hipEventCreate(&sl->t_end) == hipSuccess &&
+              sl->counter.alloc(kSubStreams) == hipSuccess;
+    for (int k = 0; ok && k < kSubStreams; k++)
+      ok = hipStreamCreateWithFlags(&sl->sub[k], hipStreamNonBlocking) == hipSuccess &&
+           hipEventCreateWithFlags(&sl->join[k], hipEventDisableTiming) == hipSuccess;
+    s->slots.push_back(std::move(sl));
+    if (!ok) {
+      release_slots(s);
       delete s;
       return fail(ST_ERR_DEVICE, "st_server_create: stream/event/counter creation failed");
     }
-    s->slots.push_back(std::move(sl));
   }
   for (auto &sl : s->slots) sl->th = std::thread(worker, sl.get());
   *out = s;
@@ -503,13 +551,7 @@ int st_server_destroy(st_server *s) {
   for (auto &sl : s->slots)
     if (sl->th.joinable()) sl->th.join();
   (void)hipSetDevice(s->ctx->device);
-  for (auto &sl : s->slots) {
-    (void)hipStreamSynchronize(sl->stream);
-    (void)hipEventDestroy(sl->done);
-    (void)hipEventDestroy(sl->t_start);
-    (void)hipEventDestroy(sl->t_end);
-    (void)hipStreamDestroy(sl->stream);
-  }
+  release_slots(s);
   delete s;
   return ST_OK;
 }

@@ -251,3 +251,175 @@ def query_rows(query, comp_bias: bool = True, penalize_only: bool = False,
     check(_L().st_query_rows(ptr(q), len(q), 1 if comp_bias else 0, 1 if penalize_only else 0, w, lo, hi,
                              ptr(rows), C.byref(n)))
     return rows[: len(q)]
+
+
+# ---- exact traceback of reported hits (src/align.rs) -------------------------
+
+ALIGNMENT_DT = np.dtype([("raw_score", "<i4"), ("reserved", "<i4"), ("query_start", "<u8"), ("query_end", "<u8"),
+                         ("target_start", "<u8"), ("target_end", "<u8"), ("aln_len", "<u8"),
+                         ("identities", "<u8"), ("mismatches", "<u8"), ("gaps", "<u8"), ("gap_opens", "<u8")])
+SCORE_END_DT = np.dtype([("score", "<i4"), ("reserved", "<i4"), ("query_end", "<i8"), ("target_end", "<i8")])
+_SIGS += [
+    ("st_sw_score_end_host", C.c_int, [_P, C.c_uint64, _P, C.c_uint64, C.c_int32, C.c_int32, _P,
+                                       C.POINTER(C.c_int64), C.POINTER(C.c_int64)]),
+    ("st_align_canonical", C.c_int, [_P, _P, C.c_uint64, _P, C.c_uint64, C.c_int32, C.c_int32, C.c_uint32, _P, _P,
+                                     _P, C.c_uint64, C.POINTER(C.c_uint64), C.POINTER(C.c_int)]),
+    ("st_rescore_alignment", C.c_int, [_P, _P, C.c_uint64, _P, C.c_uint64, C.c_int32, C.c_int32, _P, _P, _P]),
+]
+EXPORTED = [s[0] for s in _SIGS]
+DEFAULT_BAND_RADIUS = 16
+
+
+@dataclass
+class Alignment:
+    """Alignment (src/align.rs:224-270)."""
+    raw_score: int
+    query_start: int
+    query_end: int
+    target_start: int
+    target_end: int
+    aln_len: int
+    identities: int
+    mismatches: int
+    gaps: int
+    gap_opens: int
+    cigar: str
+
+    def pct_identity(self) -> float:
+        return 0.0 if self.aln_len == 0 else 100.0 * self.identities / self.aln_len
+
+    def query_cov(self, qlen: int) -> float:
+        return (self.query_end + 1 - self.query_start) / max(qlen, 1)
+
+    def target_cov(self, tlen: int) -> float:
+        return (self.target_end + 1 - self.target_start) / max(tlen, 1)
+
+    def _rec(self) -> np.ndarray:
+        r = np.zeros(1, ALIGNMENT_DT)
+        for f in ALIGNMENT_DT.names:
+            if f != "reserved":
+                r[f] = getattr(self, f)
+        return r
+
+
+def _rows(rows) -> np.ndarray:
+    return np.ascontiguousarray(np.asarray(rows, dtype=np.int32).reshape(-1, 21))
+
+
+def sw_score_end_host(rows, target, gaps):
+    """(score, query_end, target_end) with the column-major-first end, or
+    (0, -1, -1) (src/simd.rs:347-390)."""
+    r, t = _rows(rows), _codes(target)
+    s, qe, te = C.c_int32(0), C.c_int64(0), C.c_int64(0)
+    check(_L().st_sw_score_end_host(ptr(r), len(r), ptr(t), len(t), gaps.open, gaps.extend, C.byref(s),
+                                    C.byref(qe), C.byref(te)))
+    return int(s.value), int(qe.value), int(te.value)
+
+
+def align_canonical(rows, qres, target, gaps, band_radius: int = DEFAULT_BAND_RADIUS, known=None):
+    """align_canonical (src/align.rs:840-882): an Alignment or None.
+    known = (score, query_end, target_end) of the forward pass, if at hand."""
+    r, t = _rows(rows), _codes(target)
+    q = _codes(qres) if qres is not None else None
+    kn = None
+    if known is not None:
+        kn = np.zeros(1, SCORE_END_DT)
+        kn["score"], kn["query_end"], kn["target_end"] = known
+    out = np.zeros(1, ALIGNMENT_DT)
+    cap = 2 * (len(r) + len(t)) + 64
+    buf = C.create_string_buffer(cap)
+    n, found = C.c_uint64(0), C.c_int(0)
+    check(_L().st_align_canonical(ptr(r), ptr(q) if q is not None else None, len(r), ptr(t), len(t), gaps.open,
+                                  gaps.extend, band_radius, ptr(kn) if kn is not None else None, ptr(out), buf, cap,
+                                  C.byref(n), C.byref(found)))
+    if not found.value:
+        return None
+    o = out[0]
+    return Alignment(int(o["raw_score"]), int(o["query_start"]), int(o["query_end"]), int(o["target_start"]),
+                     int(o["target_end"]), int(o["aln_len"]), int(o["identities"]), int(o["mismatches"]),
+                     int(o["gaps"]), int(o["gap_opens"]), buf.value.decode())
+
+
+def rescore_alignment(rows, qres, target, gaps, aln: Alignment) -> int:
+    """rescore_alignment (src/align.rs:1064-1135): the recomputed score;
+    raises StError on any inconsistency."""
+    r, t = _rows(rows), _codes(target)
+    q = _codes(qres) if qres is not None else None
+    s = C.c_int32(0)
+    check(_L().st_rescore_alignment(ptr(r), ptr(q) if q is not None else None, len(r), ptr(t), len(t), gaps.open,
+                                    gaps.extend, ptr(aln._rec()), aln.cigar.encode(), C.byref(s)))
+    return int(s.value)
+
+
+# ---- gapped stage, hits and .m8 rows (src/search.rs) -------------------------
+
+HIT_DT = np.dtype([("target", "<u4"), ("raw_score", "<i4"), ("pident", "<f8"), ("aln_len", "<u8"),
+                   ("mismatches", "<u8"), ("gap_opens", "<u8"), ("q_start", "<u8"), ("q_end", "<u8"),
+                   ("t_start", "<u8"), ("t_end", "<u8"), ("evalue", "<f8"), ("bit_score", "<f8"), ("qlen", "<u8"),
+                   ("tlen", "<u8")])
+COUNTERS_DT = np.dtype([(f, "<u8") for f in ("scored", "passed_score", "tracebacks", "score_mismatches",
+                                             "rejected_coverage", "hits")])
+M8_HEADER = "query\ttarget\tpident\talnlen\tmismatch\tgapopen\tqstart\tqend\ttstart\ttend\tevalue\tbits"
+_SIGS += [
+    ("st_hit_order", C.c_int, [_P, _P]),
+    ("st_gapped_stage", C.c_int, [_P, _P, C.c_uint64, C.c_int32, _P, C.POINTER(C.c_uint64)]),
+    ("st_search_hits", C.c_int, [_P, _P, C.c_uint64, _P, _P, C.c_uint64, _P, _P, C.c_uint64, C.c_int32, C.c_int32,
+                                 _P, _D, C.c_uint64, _D, _D, C.c_uint32, _P, _P, C.c_uint64, C.c_uint64,
+                                 C.POINTER(C.c_uint64), _P]),
+    ("st_format_m8", C.c_int, [C.c_char_p, C.c_char_p, _P, _P, C.c_uint64, C.POINTER(C.c_uint64)]),
+]
+EXPORTED = [s[0] for s in _SIGS]
+
+
+@dataclass
+class SearchParams:
+    """The gapped-stage settings of SearchParams (src/search.rs:59-141)."""
+    max_evalue: float = 1e-3
+    max_hits: int = 300
+    min_query_cov: float = 0.0
+    min_target_cov: float = 0.0
+    band_radius: int = DEFAULT_BAND_RADIUS
+
+
+def gapped_stage(scores, tids, min_score: int) -> np.ndarray:
+    s = np.ascontiguousarray(scores, dtype=np.int32)
+    t = np.ascontiguousarray(tids, dtype=np.uint32)
+    order = np.zeros(max(len(s), 1), np.uint64)
+    n = C.c_uint64(0)
+    check(_L().st_gapped_stage(ptr(s), ptr(t), len(s), min_score, ptr(order), C.byref(n)))
+    return order[: n.value].astype(np.int64)
+
+
+def search_hits(rows, qres, arena, offsets, cand_tids, scores, gaps, ev: EValueParams,
+                params: SearchParams = SearchParams(), counters: np.ndarray | None = None):
+    """Hits (HIT_DT array, hit_order) and their CIGARs for one query."""
+    r = _rows(rows)
+    q = _codes(qres) if qres is not None else None
+    ar = np.ascontiguousarray(arena, dtype=np.uint8)
+    of = np.ascontiguousarray(offsets, dtype=np.uint64)
+    ct = np.ascontiguousarray(cand_tids, dtype=np.uint32)
+    sc = np.ascontiguousarray(scores, dtype=np.int32)
+    cap = len(ct) if params.max_hits == 0 else min(len(ct), params.max_hits)
+    hits = np.zeros(max(cap, 1), HIT_DT)
+    maxlen = int((of[1:] - of[:-1]).max()) if len(of) > 1 else 0
+    stride = 2 * (len(r) + maxlen) + 64
+    cig = C.create_string_buffer(max(cap, 1) * stride)
+    n = C.c_uint64(0)
+    cnt = counters if counters is not None else np.zeros(1, COUNTERS_DT)
+    check(_L().st_search_hits(ptr(r), ptr(q) if q is not None else None, len(r), ptr(ar), ptr(of), len(of) - 1,
+                              ptr(ct), ptr(sc), len(ct), gaps.open, gaps.extend, ptr(ev._r), params.max_evalue,
+                              params.max_hits, params.min_query_cov, params.min_target_cov, params.band_radius,
+                              ptr(hits), cig, stride, cap, C.byref(n), ptr(cnt)))
+    k = min(cap, n.value)
+    raw = cig.raw
+    cigars = [raw[i * stride:(i + 1) * stride].split(b"\0", 1)[0].decode() for i in range(k)]
+    return hits[:k].copy(), cigars
+
+
+def format_m8(query_id: str, target_id: str, hit) -> str:
+    h = np.zeros(1, HIT_DT)
+    h[0] = hit
+    buf = C.create_string_buffer(len(query_id) + len(target_id) + 256)
+    n = C.c_uint64(0)
+    check(_L().st_format_m8(query_id.encode(), target_id.encode(), ptr(h), buf, len(buf), C.byref(n)))
+    return buf.value.decode()
