@@ -202,7 +202,7 @@ def main():
     for i in range(a.warmup):
         batch.run()
         log(f"warmup {i} done")
-    times, dp_ms = [], []
+    times, dp_ms, lms = [], [], []
     for i in range(a.steps):
         ts = time.perf_counter()
         batch.run()
@@ -210,10 +210,13 @@ def main():
         times.append(te - ts)
         d, r, g, nres = batch.timing()
         dp_ms.append(d)
+        lms.append(batch.launches()["ms"].copy())
         log(f"step {i}: {1e3 * (te - ts):.1f} ms (dp {d:.1f} ms, rescore {r:.1f} ms / {nres} pairs, gate {g:.1f} ms)")
     step = float(np.mean(times))
     gcups = cells / step / 1e9
     qps_rate = len(qps) / step
+    launches = batch.launches()
+    launch_ms = np.mean(np.stack(lms), axis=0)
 
     # parity on a seeded sample of pairs (full outputs vs the oracle)
     res = batch.results()
@@ -232,12 +235,35 @@ def main():
     log(f"parity: {len(sample) - mism}/{len(sample)} sampled pairs bit-exact")
 
     dp = float(np.mean(dp_ms)) / 1e3
-    achieved = cells * OPS_PER_CELL / dp / 1e12
-    roof = {"bound": "valu", "kernel": "dp_kernel<Pk16> (int16x2 SWG + Kadane)", "achieved": round(achieved, 2),
-            "peak": round(INT16_PACKED_PEAK_TOPS, 2), "unit": "Tops/s (int16 packed)",
-            "frac": round(achieved / INT16_PACKED_PEAK_TOPS, 4), "traffic": None,
-            "ops_per_cell": OPS_PER_CELL, "cells_per_launch": cells, "kernel_ms": round(dp * 1e3, 3),
-            "executed_over_useful_cells": round(exec_cells / cells, 4)}
+    kinds = {0: "Pk16", 1: "I32F", 2: "I32N"}
+    per_launch = []
+    for k, L in enumerate(launches):
+        ms = float(launch_ms[k])
+        ach = float(L["useful_cells"]) * OPS_PER_CELL / (ms / 1e3) / 1e12 if ms > 0 else 0.0
+        per_launch.append({"kernel": f"dp_kernel<{kinds[int(L['kind'])]}, {int(L['rows_per_lane'])}, true>",
+                           "units": int(L["units"]), "grid": int(L["grid"]), "useful_cells": int(L["useful_cells"]),
+                           "exec_over_useful": round(float(L["exec_cells"]) / max(float(L["useful_cells"]), 1.0), 4),
+                           "ms": round(ms, 3), "achieved": round(ach, 2),
+                           "frac": round(ach / INT16_PACKED_PEAK_TOPS, 4)})
+    dom = max(range(len(per_launch)), key=lambda k: per_launch[k]["useful_cells"])
+    d0 = per_launch[dom]
+    traffic, traffic_src = None, None
+    tf = os.path.join(ROOT, "profiles", "dp_kernel_batched_uneven_traffic.json")
+    if os.path.exists(tf):
+        tj = json.load(open(tf))
+        if tj.get("kernel") == d0["kernel"]:
+            traffic, traffic_src = tj["bytes_per_launch"], os.path.relpath(tf, ROOT)
+    phase = cells * OPS_PER_CELL / dp / 1e12
+    roof = {"bound": "valu", "kernel": d0["kernel"], "achieved": d0["achieved"],
+            "peak": round(INT16_PACKED_PEAK_TOPS, 2), "unit": "Tops/s (int16 packed)", "frac": d0["frac"],
+            "traffic": traffic, "traffic_source": traffic_src, "ops_per_cell": OPS_PER_CELL,
+            "cells_per_launch": d0["useful_cells"], "launch_ms": d0["ms"],
+            "launches": per_launch,
+            "dp_phase": {"ms": round(dp * 1e3, 3), "achieved": round(phase, 2),
+                         "frac": round(phase / INT16_PACKED_PEAK_TOPS, 4),
+                         "executed_over_useful_cells": round(exec_cells / cells, 4)}}
+    log(f"dominant launch {d0['kernel']}: {d0['ms']:.2f} ms, {d0['achieved']:.2f} Tops/s ({100 * d0['frac']:.1f}% of "
+        f"peak); dp phase {1e3 * dp:.2f} ms ({100 * phase / INT16_PACKED_PEAK_TOPS:.1f}%)")
 
     cpu = None
     full_swg = None

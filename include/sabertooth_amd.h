@@ -197,6 +197,21 @@ int st_batch_timing(const st_batch *b, double *ms3, uint64_t *rescored);
  * whole passes, columns to a unit's longest target, the skew's idle steps,
  * empty group slots), work units and kernel launches. */
 int st_batch_stats(const st_batch *b, uint64_t *exec_cells, uint64_t *units, uint64_t *launches);
+/* One DP kernel launch of a batch's plan: precision (0 packed int16, 1
+ * int32 floored, 2 int32 unfloored), rows per lane, units, grid, useful and
+ * executed cells, and its device time in the last st_batch_run (HIP events
+ * around the launch on its stream). */
+typedef struct {
+  int32_t kind;
+  int32_t rows_per_lane;
+  uint64_t units;
+  uint64_t grid;
+  uint64_t useful_cells;
+  uint64_t exec_cells;
+  double ms;
+} st_launch_info;
+/* Writes up to cap launches; *n = the plan's launch count. */
+int st_batch_launches(const st_batch *b, st_launch_info *out, uint64_t cap, uint64_t *n);
 /* DP cells of the batch (sum of m*n) and pairs. */
 uint64_t st_batch_cells(const st_batch *b);
 uint64_t st_batch_pairs(const st_batch *b);

@@ -4,7 +4,8 @@
  * The reference computes these on the host around its scoring kernels; they
  * are restated here in C++ (no GPU involved) so a caller gets the same
  * inputs and outputs as the reference's Rust: the PSSM builder of profile
- * queries, the row modifiers of sequence queries (composition-bias
+ * queries (and their per-profile statistics, whose random-target scores run
+ * on the GPU), the row modifiers of sequence queries (composition-bias
  * correction, low-complexity masking), the exact traceback of reported hits,
  * the hit statistics (E-values, bit scores, search spaces), the gapped stage
  * of a query's search and its .m8 rows.  Floating-point code follows the reference's operation order and
@@ -17,6 +18,8 @@
 
 #include <stddef.h>
 #include <stdint.h>
+
+#include "sabertooth_amd.h"
 
 #ifdef __cplusplus
 extern "C" {
@@ -221,6 +224,26 @@ int st_search_hits(const int32_t *rows, const uint8_t *qres, uint64_t m, const u
  * {:.3}, E-value {:.2e} (exponent without '+' or padding), bits {:.1}. */
 int st_format_m8(const char *query_id, const char *target_id, const st_hit *h, char *buf,
                  uint64_t cap, uint64_t *len);
+
+/* ---- per-profile statistics (src/evalue.rs:714-749) ----------------------- */
+
+/* The random targets of calibrate_profile: sample i draws its length from
+ * lengths[] and its residues from the BLOSUM62 background with
+ * SplitMix64(stream_seed(seed, 0xBEEF, i)) (src/evalue.rs:52-114, 727-731).
+ * offsets[0 .. samples] always; residues into arena when it holds
+ * offsets[samples] bytes (cap). */
+int st_calibration_targets(const uint64_t *lengths, uint64_t n_lengths, uint64_t samples,
+                           uint64_t seed, uint8_t *arena, uint64_t cap, uint64_t *offsets);
+
+/* calibrate_profile (src/evalue.rs:714-749): `samples` (reference: 2000)
+ * random targets from st_calibration_targets scored against the profile on
+ * the GPU with the given gaps, a maximum-likelihood Gumbel fit, lambda and
+ * K = e^(lambda (mu + 1/2)) / (m * mean length); out->has_length = 0; fit
+ * (nullable) receives the Gumbel.  ST_ERR_INVALID where the reference
+ * returns None. */
+int st_calibrate_profile(st_ctx *ctx, const st_qprof *qp, int32_t gap_open, int32_t gap_extend,
+                         const uint64_t *lengths, uint64_t n_lengths, uint64_t samples, uint64_t seed,
+                         st_gapped_params *out, st_gumbel *fit);
 
 #ifdef __cplusplus
 }

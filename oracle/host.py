@@ -345,3 +345,36 @@ def query_rows(q, comp_bias=True, penalize_only=False, mask=(15, 2.3, 2.3)):
         sc = np.asarray(blosum62()[0], dtype=np.int32).reshape(21, 21)
         rows[mk] = sc[20]
     return rows
+
+
+# ---- calibrate_profile's random targets (src/evalue.rs:52-114, 714-749) -------
+
+def stream_seed(seed, c, i):
+    from . import SplitMix64
+    M = 0xFFFFFFFFFFFFFFFF
+    return SplitMix64((seed ^ ((c * 0xA24BAED4963EE407) & M) ^ ((i * 0x9FB21C651E98DF25) & M)) & M).next_u64()
+
+
+def calibration_targets(lengths, samples, seed):
+    """The reference's per-sample targets: list of uint8 arrays."""
+    from . import _p as P, lib as L
+    out = []
+    for i in range(samples):
+        st = C.c_uint64(stream_seed(seed, 0xBEEF, i))
+        n = max(int(lengths[L().or_splitmix_next(C.byref(st)) % len(lengths)]), 1)
+        t = np.zeros(n, np.uint8)
+        L().or_background_sequence(C.byref(st), n, P(t))
+        out.append(t)
+    return out
+
+
+def calibrate_profile(rows, open_, ext, lengths, samples, seed):
+    """(lambda, K, (lambda, mu)) with the oracle's exact scores and fit."""
+    from . import gotoh_score
+    xs = [float(gotoh_score(rows, t, open_, ext)) for t in calibration_targets(lengths, samples, seed)]
+    f = fit_gumbel_ml(xs)
+    if f is None:
+        return None
+    lam, mu = f
+    mean_len = float(sum(int(x) for x in lengths)) / len(lengths)
+    return lam, math.exp(lam * (mu + 0.5)) / (len(rows) * mean_len), (lam, mu)

@@ -30,6 +30,8 @@ CAND_DT = np.dtype([("target_id", "<u4"), ("hits", "<u4"), ("best_diag", "<i8"),
 PAIR_RESULT_DT = np.dtype([("swg", "<i4"), ("ung", "<i4"), ("best_diag", "<i4"),
                            ("gate_score", "<i4"), ("pass", "u1"), ("kernel", "u1"),
                            ("reserved", "u1", (2,))])
+LAUNCH_INFO_DT = np.dtype([("kind", "<i4"), ("rows_per_lane", "<i4"), ("units", "<u8"), ("grid", "<u8"),
+                           ("useful_cells", "<u8"), ("exec_cells", "<u8"), ("ms", "<f8")])
 SERVER_CONFIG_DT = np.dtype([("workers", "<u4"), ("max_batch_queries", "<u4"), ("state_budget_bytes", "<u8")])
 REQUEST_DT = np.dtype([("query", "<u8"), ("gap_open", "<i4"), ("gap_extend", "<i4"), ("min_diag_score", "<i4"),
                        ("reserved", "<i4"), ("cand_tids", "<u8"), ("n_cands", "<u8"), ("out", "<u8")])
@@ -73,6 +75,7 @@ _SIGS = [
     ("st_batch_results", C.c_int, [_P, _P]),
     ("st_batch_timing", C.c_int, [_P, _P, _P]),
     ("st_batch_stats", C.c_int, [_P, _P, _P, _P]),
+    ("st_batch_launches", C.c_int, [_P, _P, C.c_uint64, C.POINTER(C.c_uint64)]),
     ("st_batch_cells", C.c_uint64, [_P]),
     ("st_batch_pairs", C.c_uint64, [_P]),
     ("st_batch_destroy", C.c_int, [_P]),

@@ -269,6 +269,14 @@ class Batch:
         check(lib().st_batch_stats(self._h, C.byref(e), C.byref(u), C.byref(l)))
         return int(e.value), int(u.value), int(l.value)
 
+    def launches(self) -> np.ndarray:
+        """Per-launch plan figures and device time of the last run."""
+        n = C.c_uint64(0)
+        check(lib().st_batch_launches(self._h, None, 0, C.byref(n)))
+        out = np.zeros(max(n.value, 1), dtype=_lib.LAUNCH_INFO_DT)
+        check(lib().st_batch_launches(self._h, ptr(out), n.value, C.byref(n)))
+        return out[: n.value]
+
     def timing(self):
         """(dp_ms, rescore_ms, gate_ms, rescored_pairs) of the last run."""
         ms = np.zeros(3, dtype=np.float64)

@@ -493,10 +493,15 @@ struct st_batch {
   hipEvent_t ev[6] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
   hipEvent_t join[kLaunchSlots] = {nullptr, nullptr, nullptr, nullptr};
   double ms[3] = {0, 0, 0};
+  // per-launch start/end events of the main plan and their last durations
+  std::vector<hipEvent_t> lev;
+  std::vector<double> launch_ms;
   ~st_batch() {
     for (hipEvent_t &e : ev)
       if (e) (void)hipEventDestroy(e);
     for (hipEvent_t &e : join)
+      if (e) (void)hipEventDestroy(e);
+    for (hipEvent_t &e : lev)
       if (e) (void)hipEventDestroy(e);
   }
 };
@@ -532,7 +537,11 @@ static int launch_plan(st_batch *b, const Plan &pl, const Unit *d_units, const P
     a.pout = b->d_pout.p;
     a.top4 = b->d_top4.p;
     a.lds_stride = L.gr_max + 2;
+    const bool timed = &pl == &b->plan && b->lev.size() == 2 * pl.launches.size();
+    const size_t li = (size_t)(&L - pl.launches.data());
+    if (timed) ST_HIP(hipEventRecord(b->lev[2 * li], s));
     ST_HIP(launch_dp(a, L.kind, L.R, b->ung, s));
+    if (timed) ST_HIP(hipEventRecord(b->lev[2 * li + 1], s));
   }
   for (int k = 0; k < kLaunchSlots; k++)
     if (used[k]) {
@@ -606,6 +615,17 @@ static int batch_create_impl(st_ctx *ctx, const st_db *db, const st_qprof *const
                cap);
   }
   finish_plan(b->plan, pending, b->qkind, b->open, b->ext, -1, ctx->cus, ung);
+  // useful and executed cells of every launch (roofline accounting)
+  for (Launch &L : b->plan.launches) {
+    L.useful_cells = L.exec_cells = 0;
+    for (uint32_t u = L.unit0; u < L.unit0 + L.nunits; u++) {
+      const Unit &U = b->plan.units[u];
+      uint64_t cols = 0;
+      for (uint32_t k = 0; k < U.npairs; k++) cols += db->hlen[b->plan.pairs[U.pair0 + k].tid];
+      L.useful_cells += (uint64_t)b->qps[U.query]->m * cols;
+      L.exec_cells += (uint64_t)64 * U.R * (L.kind == kPk16 ? 2 : 1) * U.passes * (U.n_pad + U.G - 1);
+    }
+  }
   std::vector<QueryDev> qv(nq);
   for (uint64_t q = 0; q < nq; q++) qv[q] = b->qps[q]->view;
   std::vector<PairDev> all(np);
@@ -647,6 +667,10 @@ static int batch_run_locked(st_batch *b) {
     if (!e) ST_HIP(hipEventCreate(&e));
   for (hipEvent_t &e : b->join)
     if (!e) ST_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+  if (b->lev.size() != 2 * b->plan.launches.size()) {
+    b->lev.assign(2 * b->plan.launches.size(), nullptr);
+    for (hipEvent_t &e : b->lev) ST_HIP(hipEventCreate(&e));
+  }
   ST_HIP(hipEventRecord(b->ev[0], ctx->stream));
   int rc = launch_plan(b, b->plan, b->d_units.p, b->d_pairs.p, b->d_passbuf.p, b->ev[0]);
   if (rc) return rc;
@@ -714,6 +738,11 @@ static int batch_run_locked(st_batch *b) {
   b->ms[1] = t;
   ST_HIP(hipEventElapsedTime(&t, b->ev[4], b->ev[5]));
   b->ms[2] = t;
+  b->launch_ms.assign(b->plan.launches.size(), 0.0);
+  for (size_t i = 0; i < b->plan.launches.size(); i++) {
+    ST_HIP(hipEventElapsedTime(&t, b->lev[2 * i], b->lev[2 * i + 1]));
+    b->launch_ms[i] = t;
+  }
   return ST_OK;
 }
 
@@ -758,6 +787,24 @@ int st_batch_stats(const st_batch *b, uint64_t *exec_cells, uint64_t *units, uin
   if (exec_cells) *exec_cells = b->plan.exec_cells;
   if (units) *units = b->plan.units.size();
   if (launches) *launches = b->plan.launches.size();
+  return ST_OK;
+}
+
+int st_batch_launches(const st_batch *b, st_launch_info *out, uint64_t cap, uint64_t *n) {
+  if (!b || !n || (cap && !out)) return fail(ST_ERR_INVALID, "st_batch_launches: null argument");
+  *n = b->plan.launches.size();
+  for (uint64_t i = 0; i < std::min<uint64_t>(cap, *n); i++) {
+    const Launch &L = b->plan.launches[i];
+    st_launch_info &o = out[i];
+    std::memset(&o, 0, sizeof o);
+    o.kind = L.kind;
+    o.rows_per_lane = L.R;
+    o.units = L.nunits;
+    o.grid = L.grid;
+    o.useful_cells = L.useful_cells;
+    o.exec_cells = L.exec_cells;
+    o.ms = i < b->launch_ms.size() ? b->launch_ms[i] : 0.0;
+  }
   return ST_OK;
 }
 

@@ -163,6 +163,7 @@ struct Launch {
   uint64_t half, buf0;
   uint32_t grid;
   int slot;
+  uint64_t useful_cells = 0, exec_cells = 0;  // filled by the batch planner
 };
 
 struct Plan {

@@ -423,3 +423,38 @@ def format_m8(query_id: str, target_id: str, hit) -> str:
     n = C.c_uint64(0)
     check(_L().st_format_m8(query_id.encode(), target_id.encode(), ptr(h), buf, len(buf), C.byref(n)))
     return buf.value.decode()
+
+
+# ---- per-profile statistics (src/evalue.rs:714-749) --------------------------
+
+_SIGS += [
+    ("st_calibration_targets", C.c_int, [_P, C.c_uint64, C.c_uint64, C.c_uint64, _P, C.c_uint64, _P]),
+    ("st_calibrate_profile", C.c_int, [_P, _P, C.c_int32, C.c_int32, _P, C.c_uint64, C.c_uint64, C.c_uint64, _P,
+                                       _P]),
+]
+EXPORTED = [s[0] for s in _SIGS]
+CALIBRATION_SEED = 0x5EED  # SearchParams::calibration_seed (src/search.rs:121)
+
+
+def calibration_targets(lengths, samples: int, seed: int = CALIBRATION_SEED):
+    """(arena, offsets) of calibrate_profile's random targets."""
+    ln = np.ascontiguousarray(lengths, dtype=np.uint64)
+    off = np.zeros(samples + 1, np.uint64)
+    check(_L().st_calibration_targets(ptr(ln), len(ln), samples, seed, None, 0, ptr(off)))
+    arena = np.zeros(max(int(off[-1]), 1), np.uint8)
+    check(_L().st_calibration_targets(ptr(ln), len(ln), samples, seed, ptr(arena), arena.size, ptr(off)))
+    return arena[: int(off[-1])], off
+
+
+def calibrate_profile(qp, gaps, lengths, samples: int = 2000, seed: int = CALIBRATION_SEED):
+    """(lambda, K, (gumbel lambda, mu)) of a profile query (st QueryProfile),
+    scored on the GPU; None where the reference returns None."""
+    ln = np.ascontiguousarray(lengths, dtype=np.uint64)
+    g = np.zeros(1, GAPPED_PARAMS_DT)
+    fit = np.zeros(1, GUMBEL_DT)
+    rc = _L().st_calibrate_profile(qp.ctx.handle, qp.handle, gaps.open, gaps.extend, ptr(ln), len(ln), samples, seed,
+                                   ptr(g), ptr(fit))
+    if rc == 1:  # ST_ERR_INVALID: no fit
+        return None
+    check(rc)
+    return float(g["lambda"][0]), float(g["k"][0]), (float(fit["lambda"][0]), float(fit["mu"][0]))
