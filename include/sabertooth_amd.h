@@ -113,6 +113,23 @@ int st_sw_score_batch(st_ctx *ctx, const st_qprof *qp, const st_db *db,
 /* One diagonal to score: target, diagonal (target_pos - query_pos) and the
  * query positions [lo, hi) (Job, src/prefilter.rs:1362-1369; Lane,
  * src/ungapped.rs:265-270). */
+/* A score and its end cell (0-based; query_end = target_end = -1 when the
+ * score is 0). */
+typedef struct {
+  int32_t score;
+  int32_t reserved;
+  int64_t query_end;
+  int64_t target_end;
+} st_score_end;
+
+/* sw_score_end_with (src/simd.rs:294-326) for target ids tids[0..count):
+ * the exact score and the first optimum in column-major order (smallest
+ * target position, then smallest query position; src/simd.rs:977-998),
+ * computed by the int32 end-tracking DP kernel. */
+int st_sw_score_end_batch(st_ctx *ctx, const st_qprof *qp, const st_db *db,
+                          const uint32_t *tids, uint64_t count, int32_t gap_open,
+                          int32_t gap_extend, st_score_end *out);
+
 typedef struct {
   uint32_t tid;
   int32_t diag;

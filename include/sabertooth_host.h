@@ -84,14 +84,6 @@ typedef struct {
   uint64_t gap_opens;  /* gap runs (BLAST gapopen) */
 } st_alignment;
 
-/* A score and its end cell (query_end / target_end = -1 when score is 0). */
-typedef struct {
-  int32_t score;
-  int32_t reserved;
-  int64_t query_end;
-  int64_t target_end;
-} st_score_end;
-
 /* scalar_gotoh_score_end (src/simd.rs:347-390): exact score and the first
  * optimum in column-major order (smallest target position, then smallest
  * query position); *qend = *tend = -1 when the score is 0. */
@@ -103,7 +95,8 @@ int st_sw_score_end_host(const int32_t *rows, uint64_t m, const uint8_t *t, uint
  * computed here) is the forward score and end cell, e.g. from
  * st_sw_score_end_batch.  *found = 0 when no positive alignment exists.  The
  * CIGAR (M, I = target residue against a query gap, D) is written
- * NUL-terminated up to cigar_cap bytes; *cigar_len receives its length. */
+ * NUL-terminated up to cigar_cap bytes; *cigar_len receives its length.
+ * st_score_end is declared in sabertooth_amd.h. */
 int st_align_canonical(const int32_t *rows, const uint8_t *qres, uint64_t m, const uint8_t *t,
                        uint64_t n, int32_t gap_open, int32_t gap_extend, uint32_t band_radius,
                        const st_score_end *known, st_alignment *out, char *cigar,

@@ -179,6 +179,15 @@ class QueryProfile:
                                       gaps.open, gaps.extend, ptr(sc), ptr(kn)))
         return sc, kn
 
+    def sw_score_end_with(self, db: TargetDb, tids, gaps: GapCosts) -> np.ndarray:
+        """Score and column-major-first end cell per target id
+        (sw_score_end_with, src/simd.rs:294-326): SCORE_END_DT records."""
+        t = np.ascontiguousarray(tids, dtype=np.uint32)
+        out = np.zeros(max(len(t), 1), dtype=_lib.SCORE_END_DT)
+        check(lib().st_sw_score_end_batch(self.ctx.handle, self._h, db.handle, ptr(t), len(t),
+                                          gaps.open, gaps.extend, ptr(out)))
+        return out[: len(t)]
+
     def sw_score_batch(self, db: TargetDb, tids, gaps: GapCosts) -> np.ndarray:
         return self.sw_score_with(db, tids, gaps)[0]
 

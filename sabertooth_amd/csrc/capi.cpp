@@ -396,14 +396,14 @@ void plan_query(Plan &pl, std::vector<Unit> &pending, uint32_t qi, const st_qpro
 
 // dp_occupancy memoised per (kind, R, ung, LDS bytes): the runtime query is
 // slow next to a serving micro-batch's planning.
-static int occupancy_cached(int kind, int R, bool ung, size_t lds) {
+static int occupancy_cached(int kind, int R, bool ung, size_t lds, bool end) {
   static std::mutex mu;
   static std::vector<std::pair<uint64_t, int>> memo;
-  const uint64_t key = ((uint64_t)kind << 40) | ((uint64_t)R << 32) | ((uint64_t)ung << 31) | lds;
+  const uint64_t key = ((uint64_t)end << 48) | ((uint64_t)kind << 40) | ((uint64_t)R << 32) | ((uint64_t)ung << 31) | lds;
   std::lock_guard<std::mutex> lk(mu);
   for (const auto &e : memo)
     if (e.first == key) return e.second;
-  const int n = dp_occupancy(kind, R, ung, lds);
+  const int n = dp_occupancy(kind, R, ung, lds, end);
   memo.emplace_back(key, n);
   return n;
 }
@@ -414,7 +414,7 @@ static double unit_cost(const Unit &u) { return (double)u.passes * u.R * (u.n_pa
 // first, and lay out their pass-buffer regions.
 void finish_plan(Plan &pl, std::vector<Unit> &pending, const std::vector<Kind> &qkind,
                         const std::vector<int32_t> &open, const std::vector<int32_t> &ext, int force_kind,
-                        int cus, bool ung) {
+                        int cus, bool ung, bool end) {
   auto kind_of = [&](const Unit &u) { return force_kind >= 0 ? force_kind : (int)qkind[u.query]; };
   auto big = [](const Unit &u) { return u.G * u.R > 256 ? 1 : 0; };
   std::stable_sort(pending.begin(), pending.end(), [&](const Unit &a, const Unit &b) {
@@ -448,7 +448,7 @@ void finish_plan(Plan &pl, std::vector<Unit> &pending, const std::vector<Kind> &
     }
     L.nunits = (uint32_t)(j - i);
     const size_t lds = (size_t)kCodes * (L.gr_max + 2) * (L.kind == kPk16 ? 2 : 4);
-    const int occ = std::max(1, occupancy_cached(L.kind, L.R, ung, lds));
+    const int occ = std::max(1, occupancy_cached(L.kind, L.R, ung, lds, end));
     L.grid = std::max<uint32_t>(1, std::min<uint32_t>(L.nunits, (uint32_t)(occ * cus)));
     L.slot = (int)(pl.launches.size() % kLaunchSlots);
     L.buf0 = pl.buf_elems;
@@ -466,6 +466,8 @@ struct st_batch {
   st_ctx *ctx = nullptr;
   const st_db *db = nullptr;
   bool ung = true;
+  bool end = false;  // score + end-cell batch (int32, no evidence)
+  DevBuf<int32_t> d_qend, d_tend;
   uint64_t npairs = 0, cells = 0;
   std::vector<const st_qprof *> qps;
   std::vector<int32_t> open, ext, gate;
@@ -536,11 +538,13 @@ static int launch_plan(st_batch *b, const Plan &pl, const Unit *d_units, const P
     a.passbuf_half = L.half;
     a.pout = b->d_pout.p;
     a.top4 = b->d_top4.p;
+    a.qend = b->d_qend.p;
+    a.tend = b->d_tend.p;
     a.lds_stride = L.gr_max + 2;
     const bool timed = &pl == &b->plan && b->lev.size() == 2 * pl.launches.size();
     const size_t li = (size_t)(&L - pl.launches.data());
     if (timed) ST_HIP(hipEventRecord(b->lev[2 * li], s));
-    ST_HIP(launch_dp(a, L.kind, L.R, b->ung, s));
+    ST_HIP(launch_dp(a, L.kind, L.R, b->ung, s, b->end));
     if (timed) ST_HIP(hipEventRecord(b->lev[2 * li + 1], s));
   }
   for (int k = 0; k < kLaunchSlots; k++)
@@ -558,7 +562,7 @@ extern "C" {
 static int batch_create_impl(st_ctx *ctx, const st_db *db, const st_qprof *const *queries, uint64_t nq,
                              const uint64_t *cand_off, const uint32_t *cand_tids, const int32_t *gap_open,
                              const int32_t *gap_extend, const int32_t *min_diag_score, bool ung,
-                             st_batch **out) {
+                             st_batch **out, bool end = false) {
   if (!ctx || !db || !out || (nq && (!queries || !cand_off || !gap_open || !gap_extend)))
     return fail(ST_ERR_INVALID, "st_batch_create: null argument");
   *out = nullptr;
@@ -577,6 +581,7 @@ static int batch_create_impl(st_ctx *ctx, const st_db *db, const st_qprof *const
   b->ctx = ctx;
   b->db = db;
   b->ung = ung;
+  b->end = end;
   b->npairs = np;
   b->qps.assign(queries, queries + nq);
   b->open.assign(gap_open, gap_open + nq);
@@ -609,12 +614,14 @@ static int batch_create_impl(st_ctx *ctx, const st_db *db, const st_qprof *const
     uint32_t nmax = 0;
     for (uint64_t k = s; k < e; k++) nmax = std::max(nmax, db->hlen[cand_tids[k]]);
     if (b->qkind[q] == kPk16 && (uint64_t)b->qps[q]->m + nmax - 1 > kMaxDiagsPk16) b->qkind[q] = kI32F;
+    // the end cell is tracked by the int32 kernels
+    if (end && b->qkind[q] == kPk16) b->qkind[q] = kI32F;
     std::vector<uint64_t> outs(e - s);
     std::iota(outs.begin(), outs.end(), s);
     plan_query(b->plan, pending, (uint32_t)q, b->qps[q], cand_tids + s, outs.data(), e - s, b->qkind[q], db,
                cap);
   }
-  finish_plan(b->plan, pending, b->qkind, b->open, b->ext, -1, ctx->cus, ung);
+  finish_plan(b->plan, pending, b->qkind, b->open, b->ext, -1, ctx->cus, ung, end);
   // useful and executed cells of every launch (roofline accounting)
   for (Launch &L : b->plan.launches) {
     L.useful_cells = L.exec_cells = 0;
@@ -644,7 +651,9 @@ static int batch_create_impl(st_ctx *ctx, const st_db *db, const st_qprof *const
             b->d_pout.alloc(std::max<uint64_t>(np, 1)) == hipSuccess &&
             b->d_top4.alloc(ung ? std::max<uint64_t>(np, 1) * 4 : 1) == hipSuccess &&
             b->d_res.alloc(std::max<uint64_t>(np, 1) * sizeof(st_pair_result)) == hipSuccess &&
-            b->d_passbuf.alloc(std::max<uint64_t>(b->plan.buf_elems, 1)) == hipSuccess;
+            b->d_passbuf.alloc(std::max<uint64_t>(b->plan.buf_elems, 1)) == hipSuccess &&
+            (!end || (b->d_qend.alloc(std::max<uint64_t>(np, 1)) == hipSuccess &&
+                      b->d_tend.alloc(std::max<uint64_t>(np, 1)) == hipSuccess));
   if (!ok) {
     delete b;
     return fail(ST_ERR_NOMEM, "st_batch_create: device allocation failed");
@@ -699,7 +708,7 @@ static int batch_run_locked(st_batch *b) {
       plan_query(b->resc, pending, (uint32_t)q, b->qps[q], tids.data(), perq[q].data(), tids.size(), kI32F, b->db,
                  unit_steps_cap(0));
     }
-    finish_plan(b->resc, pending, b->qkind, b->open, b->ext, (int)kI32F, ctx->cus, b->ung);
+    finish_plan(b->resc, pending, b->qkind, b->open, b->ext, (int)kI32F, ctx->cus, b->ung, b->end);
     if (!b->resc.units.empty()) {
       if (b->d_rs_units.upload(b->resc.units.data(), b->resc.units.size(), ctx->stream) != hipSuccess ||
           b->d_rs_pairs.upload(b->resc.pairs.data(), b->resc.pairs.size(), ctx->stream) != hipSuccess ||
@@ -841,6 +850,39 @@ int st_sw_score_batch(st_ctx *ctx, const st_qprof *qp, const st_db *db, const ui
         scores[k] = res[k].swg;
         if (kernel) kernel[k] = res[k].kernel;
       }
+  }
+  st_batch_destroy(b);
+  return rc;
+}
+
+int st_sw_score_end_batch(st_ctx *ctx, const st_qprof *qp, const st_db *db, const uint32_t *tids, uint64_t count,
+                          int32_t gap_open, int32_t gap_extend, st_score_end *out) {
+  if (!ctx || !qp || !db || (count && (!tids || !out)))
+    return fail(ST_ERR_INVALID, "st_sw_score_end_batch: null argument");
+  if (count == 0) return ST_OK;
+  const uint64_t off[2] = {0, count};
+  st_batch *b = nullptr;
+  int rc = batch_create_impl(ctx, db, &qp, 1, off, tids, &gap_open, &gap_extend, nullptr, false, &b, true);
+  if (rc) return rc;
+  rc = st_batch_run(b);
+  if (!rc) {
+    std::lock_guard<std::mutex> lk(ctx->mu);
+    std::vector<PairOut> po(count);
+    std::vector<int32_t> qe(count), te(count);
+    hipError_t e = hipMemcpyAsync(po.data(), b->d_pout.p, count * sizeof(PairOut), hipMemcpyDeviceToHost, ctx->stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(qe.data(), b->d_qend.p, count * 4, hipMemcpyDeviceToHost, ctx->stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(te.data(), b->d_tend.p, count * 4, hipMemcpyDeviceToHost, ctx->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+    if (e != hipSuccess) {
+      rc = fail(ST_ERR_DEVICE, std::string("st_sw_score_end_batch: ") + hipGetErrorString(e));
+    } else {
+      for (uint64_t k = 0; k < count; k++) {
+        out[k].score = po[k].swg;
+        out[k].reserved = 0;
+        out[k].query_end = po[k].swg > 0 ? qe[k] : -1;
+        out[k].target_end = po[k].swg > 0 ? te[k] : -1;
+      }
+    }
   }
   st_batch_destroy(b);
   return rc;

@@ -216,12 +216,12 @@ struct GroupCtx {
 
 // One pass of G*R rows over all n_pad columns.  LAST: the pass holding the
 // matrix's last padded row (bottom-row diagonal ends are folded here).
-template <class P, int R, bool UNG, bool LAST>
+template <class P, int R, bool UNG, bool LAST, bool END>
 __device__ __forceinline__ void run_pass(const GroupCtx<P> &c, const typename P::L *lds, int stride, int base,
                                          bool first, bool final_pass, const uint4 *rd, uint4 *wr,
                                          typename P::T e0, typename P::T vopen, typename P::T vext,
                                          typename P::T &best, typename P::K (&top)[P::kPer][4],
-                                         typename P::T *edge, int lane) {
+                                         typename P::T *edge, int lane, int &bj, int &bi) {
   using T = typename P::T;
   using K = typename P::K;
   constexpr int kPer = P::kPer;
@@ -340,7 +340,16 @@ __device__ __forceinline__ void run_pass(const GroupCtx<P> &c, const typename P:
           T h = P::max(P::max(H[k], E[k]), F);
           if (!P::kFloor) h = P::max(h, 0);
           H[k] = h;
-          if (P::kFloor) {
+          if (END) {
+            // first optimum in column-major order: a lane sees its columns in
+            // increasing j and, within a column, its rows in increasing i
+            const int row = base + l * R + k;
+            if ((P::kFloor || (j < c.len[0] && row < m)) && (h > best || (h == best && h > 0 && j < bj))) {
+              best = h;
+              bj = j;
+              bi = row;
+            }
+          } else if (P::kFloor) {
             best = P::max(best, h);
           } else if (j < c.len[0] && base + l * R + k < m) {
             best = P::max(best, h);
@@ -417,14 +426,16 @@ __device__ __forceinline__ void run_pass(const GroupCtx<P> &c, const typename P:
 }
 
 // Persistent DP kernel: every block (one wave) pulls units from *counter.
-template <class P, int R, bool UNG>
+// END (int32 kernels without the evidence): also the end cell of the
+// optimum, first in column-major order, into qend / tend.
+template <class P, int R, bool UNG, bool END>
 __global__ __launch_bounds__(64) void dp_kernel(
     const Unit *__restrict__ units, uint32_t nunits, uint32_t *__restrict__ counter,
     const PairDev *__restrict__ pairs, const QueryDev *__restrict__ queries,
     const uint8_t *__restrict__ arena, const uint64_t *__restrict__ toff,
     const uint32_t *__restrict__ tlen, int32_t gap_open, int32_t gap_ext,
     uint4 *__restrict__ passbuf, uint64_t passbuf_half, PairOut *__restrict__ pout,
-    uint64_t *__restrict__ top4, int lds_stride) {
+    uint64_t *__restrict__ top4, int32_t *__restrict__ qend, int32_t *__restrict__ tend, int lds_stride) {
   using T = typename P::T;
   using L = typename P::L;
   using K = typename P::K;
@@ -485,6 +496,7 @@ __global__ __launch_bounds__(64) void dp_kernel(
       for (int k = 0; k < 4; k++) top[x][k] = 0;
 
     T best = 0;
+    int bj = 0x7fffffff, bi = 0;
     for (int p = 0; p < (int)u.passes; p++) {
       const int base = p * GR;
       // stage the profile rows [base, base + GR) of all codes into LDS
@@ -507,15 +519,23 @@ __global__ __launch_bounds__(64) void dp_kernel(
       uint4 *wr = myb + (uint64_t)(p & 1) * passbuf_half + (uint64_t)c.g * c.n_pad;
       const bool final_pass = p == (int)u.passes - 1;
       if (UNG && final_pass)
-        run_pass<P, R, UNG, true>(c, lds, lds_stride, base, p == 0, true, rd, wr, e0, vopen, vext, best, top,
-                                  edge, lane);
+        run_pass<P, R, UNG, true, END>(c, lds, lds_stride, base, p == 0, true, rd, wr, e0, vopen, vext, best, top,
+                                       edge, lane, bj, bi);
       else
-        run_pass<P, R, UNG, false>(c, lds, lds_stride, base, p == 0, final_pass, rd, wr, e0, vopen, vext, best,
-                                   top, edge, lane);
+        run_pass<P, R, UNG, false, END>(c, lds, lds_stride, base, p == 0, final_pass, rd, wr, e0, vopen, vext,
+                                        best, top, edge, lane, bj, bi);
     }
     // best and top-4 over the group's lanes
     for (int off = 1; off < c.G; off <<= 1) {
       const T o = (T)__shfl_xor((int)best, off);
+      if (END) {
+        // (score desc, column asc, row asc)
+        const int oj = __shfl_xor(bj, off), oi = __shfl_xor(bi, off);
+        if ((int)o > (int)best || ((int)o == (int)best && (oj < bj || (oj == bj && oi < bi)))) {
+          bj = oj;
+          bi = oi;
+        }
+      }
       best = P::max(best, o);
       if (UNG) {
 #pragma unroll
@@ -537,6 +557,10 @@ __global__ __launch_bounds__(64) void dp_kernel(
           // 0: exact int16; 1: int16 saturated (rescore); 2: exact int32
           po.flag = kPer == 2 ? (po.swg >= 32767 ? 1 : 0) : 2;
           pout[outi[x]] = po;
+          if (END) {
+            qend[outi[x]] = (int)best > 0 ? bi : -1;
+            tend[outi[x]] = (int)best > 0 ? bj : -1;
+          }
           if (UNG) {
 #pragma unroll
             for (int k = 0; k < 4; k++) top4[(uint64_t)outi[x] * 4 + k] = P::key64(top[x][k]);
@@ -567,55 +591,66 @@ struct DpLaunch {
   uint64_t passbuf_half;
   PairOut *pout;
   uint64_t *top4;
+  int32_t *qend, *tend;  // end cells (END launches)
   uint32_t grid;
   int lds_stride;  // >= largest G*R of the launch + 2, even
 };
 
 // Resident blocks per CU of a kernel instance with `lds_bytes` of dynamic LDS.
-int dp_occupancy(int kind, int R, bool ung, size_t lds_bytes) {
+int dp_occupancy(int kind, int R, bool ung, size_t lds_bytes, bool end) {
   int n = 0;
-#define ST_OCC(PP, RR, UU) (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, dp_kernel<PP, RR, UU>, 64, lds_bytes)
-  if (kind == 0) {
+#define ST_OCC(PP, RR, UU, EE) \
+  (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, dp_kernel<PP, RR, UU, EE>, 64, lds_bytes)
+  if (end) {
+    if (kind == 2) ST_OCC(I32N, 8, false, true);
+    else if (R == 8) ST_OCC(I32F, 8, false, true);
+    else ST_OCC(I32F, 16, false, true);
+  } else if (kind == 0) {
     if (R == 8) {
-      if (ung) ST_OCC(Pk16, 8, true); else ST_OCC(Pk16, 8, false);
+      if (ung) ST_OCC(Pk16, 8, true, false); else ST_OCC(Pk16, 8, false, false);
     } else {
-      if (ung) ST_OCC(Pk16, 16, true); else ST_OCC(Pk16, 16, false);
+      if (ung) ST_OCC(Pk16, 16, true, false); else ST_OCC(Pk16, 16, false, false);
     }
   } else if (kind == 1) {
     if (R == 8) {
-      if (ung) ST_OCC(I32F, 8, true); else ST_OCC(I32F, 8, false);
+      if (ung) ST_OCC(I32F, 8, true, false); else ST_OCC(I32F, 8, false, false);
     } else {
-      if (ung) ST_OCC(I32F, 16, true); else ST_OCC(I32F, 16, false);
+      if (ung) ST_OCC(I32F, 16, true, false); else ST_OCC(I32F, 16, false, false);
     }
   } else {
-    if (ung) ST_OCC(I32N, 8, true); else ST_OCC(I32N, 8, false);
+    if (ung) ST_OCC(I32N, 8, true, false); else ST_OCC(I32N, 8, false, false);
   }
 #undef ST_OCC
   return n;
 }
 
-// kind: 0 = Pk16, 1 = I32F, 2 = I32N; R: 8 or 16 (I32N: 8)
-hipError_t launch_dp(const DpLaunch &a, int kind, int R, bool ung, hipStream_t s) {
+// kind: 0 = Pk16, 1 = I32F, 2 = I32N; R: 8 or 16 (I32N: 8); end: the
+// score + end-cell instances (int32, no evidence)
+hipError_t launch_dp(const DpLaunch &a, int kind, int R, bool ung, hipStream_t s, bool end) {
   hipLaunchKernelGGL(zero_u32, dim3(1), dim3(64), 0, s, a.counter, 1u);
-#define ST_DP(PP, RR, UU)                                                                        \
-  hipLaunchKernelGGL((dp_kernel<PP, RR, UU>), dim3(a.grid), dim3(64),                             \
+#define ST_DP(PP, RR, UU, EE)                                                                       \
+  hipLaunchKernelGGL((dp_kernel<PP, RR, UU, EE>), dim3(a.grid), dim3(64),                            \
                      (size_t)kCodes * a.lds_stride * sizeof(typename PP::L), s, a.units, a.nunits,   \
                      a.counter, a.pairs, a.queries, a.arena, a.toff, a.tlen, a.open, a.ext, a.passbuf, \
-                     a.passbuf_half, a.pout, a.top4, a.lds_stride)
-  if (kind == 0) {
+                     a.passbuf_half, a.pout, a.top4, a.qend, a.tend, a.lds_stride)
+  if (end) {
+    if (kind == 2) ST_DP(I32N, 8, false, true);
+    else if (R == 8) ST_DP(I32F, 8, false, true);
+    else ST_DP(I32F, 16, false, true);
+  } else if (kind == 0) {
     if (R == 8) {
-      if (ung) ST_DP(Pk16, 8, true); else ST_DP(Pk16, 8, false);
+      if (ung) ST_DP(Pk16, 8, true, false); else ST_DP(Pk16, 8, false, false);
     } else {
-      if (ung) ST_DP(Pk16, 16, true); else ST_DP(Pk16, 16, false);
+      if (ung) ST_DP(Pk16, 16, true, false); else ST_DP(Pk16, 16, false, false);
     }
   } else if (kind == 1) {
     if (R == 8) {
-      if (ung) ST_DP(I32F, 8, true); else ST_DP(I32F, 8, false);
+      if (ung) ST_DP(I32F, 8, true, false); else ST_DP(I32F, 8, false, false);
     } else {
-      if (ung) ST_DP(I32F, 16, true); else ST_DP(I32F, 16, false);
+      if (ung) ST_DP(I32F, 16, true, false); else ST_DP(I32F, 16, false, false);
     }
   } else {
-    if (ung) ST_DP(I32N, 8, true); else ST_DP(I32N, 8, false);
+    if (ung) ST_DP(I32N, 8, true, false); else ST_DP(I32N, 8, false, false);
   }
 #undef ST_DP
   return hipGetLastError();

@@ -29,10 +29,12 @@ struct DpLaunch {
   uint64_t passbuf_half;
   PairOut *pout;
   uint64_t *top4;
+  int32_t *qend, *tend;  // end cells (score + end launches), else null
   uint32_t grid;
   int lds_stride;
 };
-hipError_t launch_dp(const DpLaunch &a, int kind, int R, bool ung, hipStream_t s);
+// end: the int32 score + end-cell instances (no evidence)
+hipError_t launch_dp(const DpLaunch &a, int kind, int R, bool ung, hipStream_t s, bool end = false);
 
 struct GateRes {
   int32_t score, best, best_diag;
@@ -142,7 +144,7 @@ struct st_qprof {
 
 namespace st {
 
-int dp_occupancy(int kind, int R, bool ung, size_t lds_bytes);
+int dp_occupancy(int kind, int R, bool ung, size_t lds_bytes, bool end = false);
 
 // Precision class of a query under given gaps.
 enum Kind { kPk16 = 0, kI32F = 1, kI32N = 2 };
@@ -189,6 +191,6 @@ double unit_steps_cap(uint64_t batch_cells);
 // pass-buffer regions.  force_kind >= 0 overrides every query's kind.
 void finish_plan(Plan &pl, std::vector<Unit> &pending, const std::vector<Kind> &qkind,
                  const std::vector<int32_t> &open, const std::vector<int32_t> &ext, int force_kind, int cus,
-                 bool ung);
+                 bool ung, bool end = false);
 
 }  // namespace st

@@ -169,6 +169,7 @@ int launch_on(Slot &sl, const Plan &pl, const QueryDev *d_q, const Unit *d_units
     a.passbuf_half = L.half;
     a.pout = d_pout;
     a.top4 = d_top4;
+    a.qend = a.tend = nullptr;
     a.lds_stride = L.gr_max + 2;
     ST_HIP(launch_dp(a, L.kind, L.R, true, fan ? sl.sub[k] : sl.stream));
   }
