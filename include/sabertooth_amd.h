@@ -238,7 +238,11 @@ int st_batch_destroy(st_batch *b);
 
 /* The reference has no serving mode; its CLI scores fixed query batches
  * (src/main.rs:255, Engine::search_batch src/search.rs:692-731).  A server
- * keeps the database resident and runs `workers` slots, each with its own
+ * keeps the database resident.  With `persistent`, each admitted request is
+ * planned into work units that are appended to the queue of one running
+ * persistent kernel (scoring and gate fused per unit), so requests merge into
+ * the work already on the device.  Otherwise (and for requests the
+ * persistent kernel cannot take) it runs `workers` slots, each with its own
  * stream and device buffers.  A free slot takes every queued request (FIFO,
  * at most max_batch_queries) whose DP state fits the budget beside what is
  * already running, scores them as one micro-batch and completes them;
@@ -251,6 +255,11 @@ typedef struct {
   uint32_t workers;           /* concurrent micro-batches (slots); 0 = 4, max 16 */
   uint32_t max_batch_queries; /* requests merged into one micro-batch; 0 = 64 */
   uint64_t state_budget_bytes;/* device DP state admitted at once; 0 = 4 GiB */
+  uint32_t persistent;        /* 1: merge requests into one persistent serving
+                                 kernel through a device work queue; requests it
+                                 cannot take (int32 precision, very long
+                                 targets) use the micro-batch slots */
+  uint32_t reserved;
 } st_server_config;
 
 /* One request: `out` receives n_cands results in candidate order and must

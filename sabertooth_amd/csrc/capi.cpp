@@ -313,9 +313,9 @@ double unit_steps_cap(uint64_t batch_cells) {
 // longest target plus the skew's G-1 steps, empty group slots) times the
 // shape's measured cost, with a penalty on units longer than `cap` steps
 // (normalised to R = 16), whose single wave would set a launch's tail.
-void plan_query(Plan &pl, std::vector<Unit> &pending, uint32_t qi, const st_qprof *qp, const uint32_t *tids,
-                const uint64_t *outs, size_t count, Kind kind, const st_db *db, double cap) {
-  if (count == 0) return;
+bool plan_query(Plan &pl, std::vector<Unit> &pending, uint32_t qi, const st_qprof *qp, const uint32_t *tids,
+                const uint64_t *outs, size_t count, Kind kind, const st_db *db, double cap, const PlanLimits *lim) {
+  if (count == 0) return true;
   // stable length order: (length << 32 | index) keys; two 8-bit LSD radix
   // passes when every length is below 65536
   std::vector<uint64_t> key(count);
@@ -359,6 +359,9 @@ void plan_query(Plan &pl, std::vector<Unit> &pending, uint32_t qi, const st_qpro
         const size_t e = std::min(count, s + (size_t)(64 / G * per));
         const uint64_t np = key[e - 1] >> 32;
         const uint64_t passes = std::max<uint64_t>(1, (m + GR - 1) / GR);
+        if (lim && ((lim->only_R && R != lim->only_R) || GR > (uint64_t)lim->max_GR ||
+                    (passes > 1 && (uint64_t)(64 / G) * np > lim->max_half)))
+          continue;
         const double exec = 64.0 * R * per * passes * (np + G - 1);
         const double useful = std::max(1.0, (double)m * (double)(csum[e] - csum[s]));
         double c = exec * kShapePs[ri][gi] / useful;
@@ -371,6 +374,7 @@ void plan_query(Plan &pl, std::vector<Unit> &pending, uint32_t qi, const st_qpro
         }
       }
     }
+    if (bc >= 1e300) return false;  // the limits admit no shape
     const size_t e = std::min(count, s + (size_t)(64 / bG * per));
     const uint32_t GR = (uint32_t)(bG * bR);
     Unit u{};
@@ -392,6 +396,7 @@ void plan_query(Plan &pl, std::vector<Unit> &pending, uint32_t qi, const st_qpro
     pending.push_back(u);
     s = e;
   }
+  return true;
 }
 
 // dp_occupancy memoised per (kind, R, ung, LDS bytes): the runtime query is

@@ -1,0 +1,516 @@
+// Device core of the DP kernels (included by the batch and serving kernels):
+// arithmetic policies, lane shuffles, profile gathers and one pass of the
+// fused SWG + Kadane recurrence.  The mapping and the exactness argument are
+// described in dp_kernels.hip.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include "st_device.h"
+
+namespace st {
+
+typedef short s16x2 __attribute__((ext_vector_type(2)));
+typedef unsigned short u16x2 __attribute__((ext_vector_type(2)));
+typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
+typedef const __attribute__((address_space(1))) uint32_t *gptr_u32;
+typedef const __attribute__((address_space(1))) u32x4 *gptr_u32x4;
+
+template <class To, class From>
+__device__ __forceinline__ To bc(From v) {
+  return __builtin_bit_cast(To, v);
+}
+
+// global-address-space loads (plain global_load, never flat)
+__device__ __forceinline__ uint32_t ldg_u32(const uint8_t *p) { return *(gptr_u32)(p); }
+__device__ __forceinline__ uint4 ldg_u4(const uint4 *p) {
+  const u32x4 v = *(gptr_u32x4)(p);
+  return make_uint4(v.x, v.y, v.z, v.w);
+}
+
+constexpr uint32_t kSent4 = 0x15151515u;  // four zero-score codes
+#ifndef ST_STEP_UNROLL
+#define ST_STEP_UNROLL 4
+#endif
+// 1: a step's profile words are gathered one step ahead
+#ifndef ST_SCORE_PREFETCH
+#define ST_SCORE_PREFETCH 0
+#endif
+// DP steps per loop iteration
+constexpr int kStepUnroll = ST_STEP_UNROLL;
+
+// ---- arithmetic policies -----------------------------------------------------
+
+// Two pairs per group in the int16 halves (low = first slot, high = second).
+struct Pk16 {
+  static constexpr int kPer = 2;
+  static constexpr bool kFloor = true;
+  using T = uint32_t;
+  using L = int16_t;
+  using K = uint32_t;  // score << 17 | (0x1FFFF - x)
+  __device__ static T splat(int v) { return (uint32_t)(uint16_t)v * 0x10001u; }
+  __device__ static T add(T a, T b) {
+    return bc<T>(__builtin_elementwise_add_sat(bc<s16x2>(a), bc<s16x2>(b)));
+  }
+  __device__ static T max(T a, T b) {
+    return bc<T>(__builtin_elementwise_max(bc<s16x2>(a), bc<s16x2>(b)));
+  }
+  // a - c floored at 0 (a, c >= 0)
+  __device__ static T subf(T a, T c) {
+    return bc<T>(__builtin_elementwise_sub_sat(bc<u16x2>(a), bc<u16x2>(c)));
+  }
+  __device__ static T score(const L *rowA, const L *rowB, int k) {
+    return (uint32_t)(uint16_t)rowA[k] | ((uint32_t)(uint16_t)rowB[k] << 16);
+  }
+  __device__ static int half(T v, int x) { return x ? (int)(int16_t)(v >> 16) : (int)(int16_t)(v & 0xffffu); }
+  __device__ static K key(int score, uint32_t x) { return ((uint32_t)score << 17) | (0x1FFFFu - x); }
+  __device__ static uint64_t key64(K k) {
+    return k ? diag_key((int32_t)(k >> 17), 0x1FFFFu - (k & 0x1FFFFu)) : 0ull;
+  }
+  __device__ static K shfl_xor(K k, int off) { return (K)__shfl_xor((int)k, off); }
+};
+
+// One pair per group in int32, E/F floored at 0 (needs ext >= 0, open >= 0).
+struct I32F {
+  static constexpr int kPer = 1;
+  static constexpr bool kFloor = true;
+  using T = int32_t;
+  using L = int32_t;
+  using K = uint64_t;
+  __device__ static T splat(int v) { return v; }
+  __device__ static T add(T a, T b) { return a + b; }
+  __device__ static T max(T a, T b) { return a > b ? a : b; }
+  __device__ static T subf(T a, T c) {
+    return (int32_t)__builtin_elementwise_sub_sat((uint32_t)a, (uint32_t)c);
+  }
+  __device__ static T score(const L *rowA, const L *, int k) { return rowA[k]; }
+  __device__ static int half(T v, int) { return v; }
+  __device__ static K key(int score, uint32_t x) { return diag_key(score, x); }
+  __device__ static uint64_t key64(K k) { return k; }
+  __device__ static K shfl_xor(K k, int off) {
+    const uint32_t lo = (uint32_t)__shfl_xor((int)(uint32_t)k, off);
+    const uint32_t hi = (uint32_t)__shfl_xor((int)(uint32_t)(k >> 32), off);
+    return ((uint64_t)hi << 32) | lo;
+  }
+};
+
+// One pair per group in int32 with the reference's unfloored E/F (any gap
+// costs); cells outside the real matrix are masked.
+struct I32N : I32F {
+  static constexpr bool kFloor = false;
+  __device__ static T subf(T a, T c) { return a - c; }
+};
+
+template <class K>
+__device__ __forceinline__ void insert4(K (&k)[4], K v) {
+  // k[0] >= k[1] >= k[2] >= k[3]; keys are distinct
+  k[3] = k[3] > (k[2] < v ? k[2] : v) ? k[3] : (k[2] < v ? k[2] : v);
+  k[2] = k[2] > (k[1] < v ? k[1] : v) ? k[2] : (k[1] < v ? k[1] : v);
+  k[1] = k[1] > (k[0] < v ? k[0] : v) ? k[1] : (k[0] < v ? k[0] : v);
+  k[0] = k[0] > v ? k[0] : v;
+}
+
+__device__ __forceinline__ uint32_t shr1u(uint32_t v) {
+  // lane i <- lane i-1 inside each 16-lane row; row starts keep their value
+  // and are overwritten by the caller (they are always group starts)
+  return (uint32_t)__builtin_amdgcn_update_dpp((int)v, (int)v, 0x111, 0xf, 0xf, false);
+}
+
+// lane i <- lane i-1 across the whole wave (groups of 32 or 64 lanes)
+__device__ __forceinline__ uint32_t shr1w(uint32_t v) { return (uint32_t)__shfl_up((int)v, 1); }
+
+template <class P>
+__device__ __forceinline__ typename P::T shr1(typename P::T v, bool wide) {
+  return (typename P::T)(wide ? shr1w((uint32_t)v) : shr1u((uint32_t)v));
+}
+
+template <class P>
+__device__ __forceinline__ const typename P::L *prof_of(const QueryDev &q);
+template <>
+__device__ __forceinline__ const int16_t *prof_of<Pk16>(const QueryDev &q) { return q.prof16; }
+template <>
+__device__ __forceinline__ const int32_t *prof_of<I32F>(const QueryDev &q) { return q.prof32; }
+template <>
+__device__ __forceinline__ const int32_t *prof_of<I32N>(const QueryDev &q) { return q.prof32; }
+
+// One step's profile words: Pk16 reads R/2 dwords (row pairs) per target,
+// the int32 kernels R dwords.  Row pointers are dword aligned (even stride,
+// even R).
+template <class P, int R>
+__device__ __forceinline__ void load_raw(const typename P::L *lds, int stride, uint32_t code, int l,
+                                         uint32_t (&raw)[R]) {
+  if (sizeof(typename P::L) == 2) {
+    const uint32_t *ra = (const uint32_t *)(lds + (int)(code & 0xff) * stride + l * R);
+    const uint32_t *rb = (const uint32_t *)(lds + (int)((code >> 8) & 0xff) * stride + l * R);
+#pragma unroll
+    for (int i = 0; i < R / 2; i++) {
+      raw[i] = ra[i];
+      raw[R / 2 + i] = rb[i];
+    }
+  } else {
+    const uint32_t *ra = (const uint32_t *)(lds + (int)(code & 0xff) * stride + l * R);
+#pragma unroll
+    for (int i = 0; i < R; i++) raw[i] = ra[i];
+  }
+}
+
+// raw words -> per-row scores (Pk16: first target low half, second high)
+template <class P, int R>
+__device__ __forceinline__ void unpack_raw(const uint32_t (&raw)[R], typename P::T (&sc)[R]) {
+  if (sizeof(typename P::L) == 2) {
+#pragma unroll
+    for (int i = 0; i < R / 2; i++) {
+      sc[2 * i] = (typename P::T)__builtin_amdgcn_perm(raw[R / 2 + i], raw[i], 0x05040100u);
+      sc[2 * i + 1] = (typename P::T)__builtin_amdgcn_perm(raw[R / 2 + i], raw[i], 0x07060302u);
+    }
+  } else {
+#pragma unroll
+    for (int i = 0; i < R; i++) sc[i] = (typename P::T)raw[i];
+  }
+}
+
+// Per-unit, per-group state shared by the passes.
+template <class P>
+struct GroupCtx {
+  int G, l, g, n_pad, m, m_pad;
+  int len[2];
+  const uint8_t *tp[2];
+  bool has[2];
+};
+
+// One pass of G*R rows over all n_pad columns.  LAST: the pass holding the
+// matrix's last padded row (bottom-row diagonal ends are folded here).
+template <class P, int R, bool UNG, bool LAST, bool END>
+__device__ __forceinline__ void run_pass(const GroupCtx<P> &c, const typename P::L *lds, int stride, int base,
+                                         bool first, bool final_pass, const uint4 *rd, uint4 *wr,
+                                         typename P::T e0, typename P::T vopen, typename P::T vext,
+                                         typename P::T &best, typename P::K (&top)[P::kPer][4],
+                                         typename P::T *edge, int lane, int &bj, int &bi) {
+  using T = typename P::T;
+  using K = typename P::K;
+  constexpr int kPer = P::kPer;
+  const int l = c.l, G = c.G, n_pad = c.n_pad, m = c.m;
+  const bool wide = G > 16;  // groups wider than a 16-lane DPP row
+  const T kInvalid = (T)0xFFFFFFFFu;  // a negative B: not a diagonal end
+
+  T H[R], E[R], U[R], B[R];
+#pragma unroll
+  for (int k = 0; k < R; k++) {
+    H[k] = 0;
+    E[k] = e0;
+    U[k] = 0;
+    B[k] = 0;
+  }
+  T hdiag = 0, udiag = 0, bdiag = 0;
+  T out_h = 0, out_f = e0, out_u = 0, out_b = 0;
+  // the group's first lane streams residues: cw = columns [4u, 4u + 4),
+  // nw = the next four; the boundary row of the previous pass one ahead
+  uint32_t cw[2] = {kSent4, kSent4}, nw[2] = {kSent4, kSent4};
+  uint4 pb = make_uint4(0, (uint32_t)e0, 0, 0);
+  uint32_t code = kSentinel | (kSentinel << 8);
+  if (l == 0) {
+#pragma unroll
+    for (int x = 0; x < kPer; x++) {
+      if (c.len[x] > 0) cw[x] = ldg_u32(c.tp[x]);
+      if (c.len[x] > 4) nw[x] = ldg_u32(c.tp[x] + 4);
+    }
+    if (!first && n_pad > 0) pb = ldg_u4(rd);
+    code = (cw[0] & 0xffu) | ((cw[1] & 0xffu) << 8);
+  }
+  // scores of the current step are fetched one step ahead
+#if ST_SCORE_PREFETCH
+  uint32_t raw[R];
+  load_raw<P, R>(lds, stride, code, l, raw);
+#endif
+  // bottom row (LAST): the value the group's last lane produced at step t is
+  // kept by lane t mod G and folded every G steps
+  T pending = kInvalid;
+
+  const int steps = n_pad + G - 1;
+  for (int t0 = 0; t0 < steps; t0 += kStepUnroll) {
+#pragma unroll
+    for (int s = 0; s < kStepUnroll; s++) {
+      const int t = t0 + s;
+      if (t >= steps) break;
+      // next step's code: the upper lane's current one, or the stream
+      uint32_t nc = wide ? shr1w(code) : shr1u(code);
+      if (l == 0) {
+        if ((t & 3) < 3) {
+          const int sh = ((t + 1) & 3) * 8;
+          nc = ((cw[0] >> sh) & 0xffu) | (((cw[1] >> sh) & 0xffu) << 8);
+        } else {
+          nc = (nw[0] & 0xffu) | ((nw[1] & 0xffu) << 8);
+          const int nx = t + 5;
+#pragma unroll
+          for (int x = 0; x < kPer; x++) {
+            cw[x] = nw[x];
+            nw[x] = nx < c.len[x] ? ldg_u32(c.tp[x] + nx) : kSent4;
+          }
+        }
+      }
+#if ST_SCORE_PREFETCH
+      uint32_t nraw[R];
+      load_raw<P, R>(lds, stride, nc, l, nraw);
+#endif
+
+      // boundary from the upper lane (or the previous pass / the matrix edge)
+      T in_h = shr1<P>(out_h, wide), in_f = shr1<P>(out_f, wide);
+      T in_u = 0, in_b = 0;
+      if (UNG) {
+        in_u = shr1<P>(out_u, wide);
+        in_b = shr1<P>(out_b, wide);
+      }
+      if (l == 0) {
+        if (!first && t < n_pad) {
+          in_h = (T)pb.x;
+          in_f = (T)pb.y;
+          in_u = (T)pb.z;
+          in_b = (T)pb.w;
+        } else {
+          in_h = 0;
+          in_f = e0;
+          in_u = 0;
+          in_b = 0;
+        }
+        if (!first && t + 1 < n_pad) pb = ldg_u4(rd + t + 1);
+      }
+      const int j = t - l;
+      const bool live = P::kFloor || (j >= 0 && j < n_pad);
+      if (live) {
+        T sc[R];
+#if !ST_SCORE_PREFETCH
+        uint32_t raw[R];
+        load_raw<P, R>(lds, stride, code, l, raw);
+#endif
+        unpack_raw<P, R>(raw, sc);
+        // diagonal moves, last row first so row k-1 still holds column j-1
+#pragma unroll
+        for (int k = R - 1; k >= 1; k--) {
+          if (UNG) {
+            U[k] = P::add(P::max(U[k - 1], 0), sc[k]);
+            B[k] = P::max(B[k - 1], U[k]);
+          }
+          H[k] = P::add(H[k - 1], sc[k]);
+        }
+        if (UNG) {
+          U[0] = P::add(P::max(udiag, 0), sc[0]);
+          B[0] = P::max(bdiag, U[0]);
+        }
+        H[0] = P::add(hdiag, sc[0]);
+        // vertical / horizontal gaps, first row first
+        T F = in_f;
+#pragma unroll
+        for (int k = 0; k < R; k++) {
+          T h = P::max(P::max(H[k], E[k]), F);
+          if (!P::kFloor) h = P::max(h, 0);
+          H[k] = h;
+          if (END) {
+            // first optimum in column-major order: a lane sees its columns in
+            // increasing j and, within a column, its rows in increasing i
+            const int row = base + l * R + k;
+            if ((P::kFloor || (j < c.len[0] && row < m)) && (h > best || (h == best && h > 0 && j < bj))) {
+              best = h;
+              bj = j;
+              bi = row;
+            }
+          } else if (P::kFloor) {
+            best = P::max(best, h);
+          } else if (j < c.len[0] && base + l * R + k < m) {
+            best = P::max(best, h);
+          }
+          const T ho = P::subf(h, vopen);
+          E[k] = P::max(P::subf(E[k], vext), ho);
+          F = P::max(P::subf(F, vext), ho);
+        }
+        out_h = H[R - 1];
+        out_f = F;
+        if (UNG) {
+          out_u = U[R - 1];
+          out_b = B[R - 1];
+        }
+      }
+      hdiag = in_h;
+      if (UNG) {
+        udiag = in_u;
+        bdiag = in_b;
+      }
+      code = nc;
+#if ST_SCORE_PREFETCH
+#pragma unroll
+      for (int k = 0; k < R; k++) raw[k] = nraw[k];
+#endif
+      // boundary row for the next pass
+      if (!final_pass && l == G - 1 && j >= 0 && j < n_pad) {
+        uint4 v;
+        v.x = (uint32_t)out_h;
+        v.y = (uint32_t)out_f;
+        v.z = (uint32_t)out_u;
+        v.w = (uint32_t)out_b;
+        wr[j] = v;
+      }
+      if (UNG) {
+        // right edge: park this lane's rows; folded after the pass
+        if (j == n_pad - 1) {
+#pragma unroll
+          for (int k = 0; k < R; k++) edge[lane * R + k] = B[k];
+        }
+        if (LAST) {
+          // bottom row: the group's last lane produces one value per step
+          const T mine = (j >= 0 && j < n_pad - 1) ? B[R - 1] : kInvalid;
+          const int slot = t & (G - 1);
+          const T v = G > 1 ? (T)__shfl((int)mine, (lane & ~(G - 1)) + G - 1) : mine;
+          if (slot == l) pending = v;
+          if (slot == G - 1 || t == steps - 1) {
+            // lane l's pending value came from step t - slot + l
+            const int d = (t - slot + l - (G - 1)) - (c.m_pad - 1);
+#pragma unroll
+            for (int x = 0; x < kPer; x++) {
+              const int sv = P::half(pending, x);
+              if (l <= slot && c.has[x] && sv >= 0 && d >= -(m - 1) && d <= c.len[x] - 1)
+                insert4<K>(top[x], P::key(sv, (uint32_t)(d + m - 1)));
+            }
+            pending = kInvalid;
+          }
+        }
+      }
+    }
+  }
+  if (UNG) {
+    // fold this pass's right-edge values (each lane its own rows)
+#pragma unroll
+    for (int k = 0; k < R; k++) {
+      const T v = edge[lane * R + k];
+      const int d = (n_pad - 1) - (base + l * R + k);
+#pragma unroll
+      for (int x = 0; x < kPer; x++)
+        if (c.has[x] && d >= -(m - 1) && d <= c.len[x] - 1)
+          insert4<K>(top[x], P::key(P::half(v, x), (uint32_t)(d + m - 1)));
+    }
+  }
+}
+
+// Outputs of one unit, identical in every lane of a group after the merge:
+// the best H (two int16 halves in Pk16), the 4 best diagonal keys per pair,
+// the end cell (END) and the group's pair slots.
+template <class P>
+struct UnitResult {
+  typename P::T best;
+  typename P::K top[P::kPer][4];
+  int bj, bi;
+  uint32_t outi[2];
+  bool valid[2];
+  GroupCtx<P> c;
+};
+
+// Score one work unit: a query, npairs pair slots, G lanes per group, n_pad
+// columns, `passes` passes of G*R rows.  lds holds kCodes rows of lds_stride
+// entries; myb is the block's pass-boundary region (two halves of
+// passbuf_half quads, then the right-edge parking area).
+template <class P, int R, bool UNG, bool END>
+__device__ __forceinline__ void score_unit(const QueryDev q, const PairDev *upairs, uint32_t npairs, uint32_t G,
+                                           uint32_t n_pad, uint32_t passes, int32_t gap_open, int32_t gap_ext,
+                                           const uint8_t *arena, const uint64_t *toff, const uint32_t *tlen,
+                                           typename P::L *lds, int lds_stride, uint4 *myb, uint64_t passbuf_half,
+                                           int lane, UnitResult<P> &r) {
+  using T = typename P::T;
+  using L = typename P::L;
+  using K = typename P::K;
+  constexpr int kPer = P::kPer;
+  T *const edge = (T *)(myb + 2 * passbuf_half);
+  const T vopen = P::splat(gap_open);
+  const T vext = P::splat(gap_ext);
+  // boundary state of the row above the matrix / column left of it
+  const T e0 = P::kFloor ? T(0) : T(max(-gap_open, (INT32_MIN / 4) - gap_ext));
+  const L *prof = prof_of<P>(q);
+  GroupCtx<P> &c = r.c;
+  c.G = (int)G;
+  c.l = lane & (c.G - 1);
+  c.g = lane / c.G;
+  const int GR = c.G * R;
+  c.n_pad = (int)n_pad;
+  c.m = (int)q.m;
+  c.m_pad = (int)passes * GR;
+
+  // this group's pair slots
+  r.outi[0] = r.outi[1] = 0;
+  r.valid[0] = r.valid[1] = false;
+  c.len[0] = c.len[1] = 0;
+  c.tp[0] = c.tp[1] = arena;
+#pragma unroll
+  for (int x = 0; x < kPer; x++) {
+    const uint32_t slot = (uint32_t)(c.g * kPer + x);
+    if (slot < npairs) {
+      const PairDev pd = upairs[slot];
+      r.valid[x] = true;
+      c.len[x] = (int)tlen[pd.tid];
+      c.tp[x] = arena + toff[pd.tid];
+      r.outi[x] = pd.out;
+    }
+  }
+#pragma unroll
+  for (int x = 0; x < 2; x++) c.has[x] = r.valid[x] && c.len[x] > 0 && c.m > 0;
+
+#pragma unroll
+  for (int x = 0; x < kPer; x++)
+#pragma unroll
+    for (int k = 0; k < 4; k++) r.top[x][k] = 0;
+  T best = 0;
+  int bj = 0x7fffffff, bi = 0;
+  for (int p = 0; p < (int)passes; p++) {
+    const int base = p * GR;
+    // stage the profile rows [base, base + GR) of all codes into LDS
+    __syncthreads();
+    if (sizeof(L) == 2) {
+      const int dw = GR / 2;
+      for (int idx = lane; idx < kCodes * dw; idx += 64) {
+        const int cc = idx / dw, rr = idx - cc * dw;
+        const uint32_t v = *(const uint32_t *)(prof + (uint64_t)cc * q.ld + base + 2 * rr);
+        *(uint32_t *)(lds + cc * lds_stride + 2 * rr) = v;
+      }
+    } else {
+      for (int idx = lane; idx < kCodes * GR; idx += 64) {
+        const int cc = idx / GR, rr = idx - cc * GR;
+        lds[cc * lds_stride + rr] = prof[(uint64_t)cc * q.ld + base + rr];
+      }
+    }
+    __syncthreads();
+    const uint4 *rd = myb + (uint64_t)((p + 1) & 1) * passbuf_half + (uint64_t)c.g * c.n_pad;
+    uint4 *wr = myb + (uint64_t)(p & 1) * passbuf_half + (uint64_t)c.g * c.n_pad;
+    const bool final_pass = p == (int)passes - 1;
+    if (UNG && final_pass)
+      run_pass<P, R, UNG, true, END>(c, lds, lds_stride, base, p == 0, true, rd, wr, e0, vopen, vext, best, r.top,
+                                     edge, lane, bj, bi);
+    else
+      run_pass<P, R, UNG, false, END>(c, lds, lds_stride, base, p == 0, final_pass, rd, wr, e0, vopen, vext, best,
+                                      r.top, edge, lane, bj, bi);
+  }
+  // best and top-4 over the group's lanes
+  for (int off = 1; off < c.G; off <<= 1) {
+    const T o = (T)__shfl_xor((int)best, off);
+    if (END) {
+      // (score desc, column asc, row asc)
+      const int oj = __shfl_xor(bj, off), oi = __shfl_xor(bi, off);
+      if ((int)o > (int)best || ((int)o == (int)best && (oj < bj || (oj == bj && oi < bi)))) {
+        bj = oj;
+        bi = oi;
+      }
+    }
+    best = P::max(best, o);
+    if (UNG) {
+#pragma unroll
+      for (int x = 0; x < kPer; x++) {
+        K other[4];
+#pragma unroll
+        for (int k = 0; k < 4; k++) other[k] = P::shfl_xor(r.top[x][k], off);
+#pragma unroll
+        for (int k = 0; k < 4; k++) insert4<K>(r.top[x], other[k]);
+      }
+    }
+  }
+  r.best = best;
+  if (END) {
+    r.bj = bj;
+    r.bi = bi;
+  }
+}
+
+}  // namespace st

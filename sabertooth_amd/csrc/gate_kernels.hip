@@ -16,158 +16,9 @@
 // query that fits) or in the int32 profile in HBM.
 #include <hip/hip_runtime.h>
 
-#include "st_device.h"
+#include "gate_core.h"
 
 namespace st {
-
-struct GateRes {
-  int32_t score;      // best single or chain
-  int32_t best;       // best single segment
-  int32_t best_diag;  // diagonal of the best single segment
-  uint32_t t_lo, t_hi;
-  uint8_t pass, chain, pad0, pad1;
-};
-
-// score lookups: code 21 rows of both profiles are zero
-struct ProfGlobal {
-  const int32_t *p;
-  uint32_t ld;
-  __device__ int32_t operator()(uint32_t code, int q) const { return p[(uint64_t)code * ld + q]; }
-};
-struct ProfLds {
-  const int16_t *p;
-  int ld;
-  __device__ int32_t operator()(uint32_t code, int q) const { return p[(int)code * ld + q]; }
-};
-
-// kadane_segment (src/ungapped.rs:69-88) over query rows [lo, hi) of
-// diagonal d: the first maximum in query order, restarting when cur <= 0
-template <class Acc>
-__device__ int32_t kadane_segment_dev(const Acc &acc, const uint8_t *t, int d, int lo, int hi, int *qs,
-                                      int *qe) {
-  int32_t best = 0, cur = 0;
-  int bs = 0, be = 0, cs = lo;
-  int q = lo;
-  if (q < hi) {
-    const int64_t p0 = (int64_t)q + d;
-    const int64_t pa = p0 & ~(int64_t)15;
-    int skip = (int)(p0 - pa);
-    for (int64_t pos = pa; q < hi; pos += 16) {
-      const uint4 v = *(const uint4 *)(t + pos);
-      const uint32_t w[4] = {v.x, v.y, v.z, v.w};
-      int32_t s[16];
-#pragma unroll
-      for (int b = 0; b < 16; b++) {
-        const int qq = q + b - skip;
-        s[b] = (b >= skip && qq < hi) ? acc((w[b >> 2] >> ((b & 3) * 8)) & 0xffu, qq) : 0;
-      }
-#pragma unroll
-      for (int b = 0; b < 16; b++) {
-        if (b >= skip && q < hi) {
-          if (cur <= 0) {
-            cur = s[b];
-            cs = q;
-          } else {
-            cur += s[b];
-          }
-          if (cur > best) {
-            best = cur;
-            bs = cs;
-            be = q;
-          }
-          q++;
-        }
-      }
-      skip = 0;
-    }
-  }
-  *qs = bs;
-  *qe = be;
-  return best;
-}
-
-// chain + gate of one target over ns <= 16 scored diagonals
-// (src/prefilter.rs:1577-1657, region 1269-1278)
-template <class Acc>
-__device__ GateRes gate_eval(const Acc &acc, const uint8_t *t, const int32_t *sc, const int32_t *dg,
-                             const int32_t *lo, const int32_t *hi, int ns, int32_t open, int32_t ext,
-                             int32_t min_diag_score, bool regions) {
-  GateRes r;
-  r.pass = 0;
-  r.chain = 0;
-  r.t_lo = 0;
-  r.t_hi = 0xffffffffu;
-  int32_t best_s = 0;
-  int best_k = 0;
-  for (int k = 0; k < ns; k++)
-    if (sc[k] > best_s) {
-      best_s = sc[k];
-      best_k = k;
-    }
-  r.best = best_s;
-  r.best_diag = ns > 0 ? dg[best_k] : 0;
-  r.score = best_s;
-  if (ns == 0) {
-    r.pass = 0 >= min_diag_score;
-    return r;
-  }
-  if (ns == 1 && sc[0] < min_diag_score) return r;
-  int qs[16], qe[16];
-  uint32_t have = 0;
-  int32_t chain_s = INT32_MIN;
-  int cx = 0, cy = 0;
-  if (ns > 1)
-    for (int x = 0; x < ns; x++)
-      for (int y = 0; y < ns; y++) {
-        if (x == y) continue;
-        const int32_t sx = sc[x], sy = sc[y];
-        const int dx = dg[x], dy = dg[y];
-        if (sx <= 0 || sy <= 0 || dx == dy) continue;
-        const int ad = dy > dx ? dy - dx : dx - dy;
-        const int32_t c = sx + sy - (open + (ad - 1) * ext);
-        if (c <= best_s || c <= chain_s) continue;
-        if (!(have >> x & 1)) {
-          kadane_segment_dev(acc, t, dx, lo[x], hi[x], &qs[x], &qe[x]);
-          have |= 1u << x;
-        }
-        if (!(have >> y & 1)) {
-          kadane_segment_dev(acc, t, dy, lo[y], hi[y], &qs[y], &qe[y]);
-          have |= 1u << y;
-        }
-        if (qe[x] < qs[y] && qe[x] + dx < qs[y] + dy) {
-          chain_s = c;
-          cx = x;
-          cy = y;
-        }
-      }
-  int used0 = best_k, used1 = best_k;
-  if (chain_s > best_s) {
-    r.chain = 1;
-    r.score = chain_s;
-    used0 = cx;
-    used1 = cy;
-  }
-  if (r.score < min_diag_score) return r;
-  r.pass = 1;
-  if (regions) {
-    uint32_t a = 0xffffffffu, b = 0;
-    const int us[2] = {used0, used1};
-    for (int u = 0; u < 2; u++) {
-      const int k = us[u];
-      if (!(have >> k & 1)) {
-        kadane_segment_dev(acc, t, dg[k], lo[k], hi[k], &qs[k], &qe[k]);
-        have |= 1u << k;
-      }
-      const int s0 = qs[k] + dg[k], s1 = qe[k] + dg[k];
-      const uint32_t ua = (uint32_t)(s0 > 0 ? s0 : 0), ub = (uint32_t)(s1 > 0 ? s1 : 0);
-      a = ua < a ? ua : a;
-      b = ub > b ? ub : b;
-    }
-    r.t_lo = a;
-    r.t_hi = b;
-  }
-  return r;
-}
 
 struct JobDev {
   uint32_t tid;
@@ -268,11 +119,6 @@ struct PairGateArgs {
   const int32_t *open, *ext, *gate;  // per query
   PairOut *pout;
   void *res;  // st_pair_result rows (by PairDev::out)
-};
-
-struct PairResultDev {
-  int32_t swg, ung, best_diag, gate_score;
-  uint8_t pass, kernel, r0, r1;
 };
 
 // query rows held in LDS by gate_pairs_kernel

@@ -179,8 +179,19 @@ struct Plan {
 // Add the pairs of query qi (result rows outs[], target ids tids[]) to the
 // pending units of `pl`: targets sorted by length, cut into wave units whose
 // launch shape is chosen per unit; `cap` bounds a unit's DP steps.
-void plan_query(Plan &pl, std::vector<Unit> &pending, uint32_t qi, const st_qprof *qp, const uint32_t *tids,
-                const uint64_t *outs, size_t count, Kind kind, const st_db *db, double cap);
+// Shape limits of a kernel that cannot take every shape (the serving
+// kernel: one R, bounded G*R and pass-boundary rows).
+struct PlanLimits {
+  int only_R = 0;              // 0: any
+  int max_GR = 1 << 30;
+  uint64_t max_half = ~0ull;   // (64 / G) * n_pad of multi-pass units
+};
+
+// Returns false (and plans nothing further) when `lim` admits no shape for
+// some unit.
+bool plan_query(Plan &pl, std::vector<Unit> &pending, uint32_t qi, const st_qprof *qp, const uint32_t *tids,
+                const uint64_t *outs, size_t count, Kind kind, const st_db *db, double cap,
+                const PlanLimits *lim = nullptr);
 
 // Unit step cap for a batch of `batch_cells` DP cells: short units when the
 // batch is small (latency), up to 12000 steps for large batches.

@@ -14,6 +14,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <atomic>
 #include <chrono>
 #include <condition_variable>
 #include <cstring>
@@ -26,6 +27,30 @@
 #include "host_internal.h"
 
 namespace st {
+
+struct ServeArgs {
+  const ServeUnit *ring;
+  uint64_t ring_mask;
+  const uint64_t *tail;
+  const uint32_t *stop;
+  uint64_t *head;
+  const PairDev *pairs;
+  PairResultDev *results;
+  uint32_t *remaining;
+  uint32_t *error;
+  uint64_t pair_rows;
+  uint32_t req_slots;
+  const uint8_t *arena;
+  const uint64_t *toff;
+  const uint32_t *tlen;
+  uint4 *passbuf;
+  uint64_t passbuf_half;
+  uint64_t idle_ticks;
+  int lds_stride;
+};
+int serve_occupancy(size_t lds_bytes);
+hipError_t launch_serve(const ServeArgs &a, uint32_t grid, hipStream_t s);
+
 namespace {
 
 double now_ms() {
@@ -116,6 +141,124 @@ struct Slot {
   std::thread th;
 };
 
+// Host-coherent pinned memory (fine-grained: the running kernel and the host
+// see each other's writes without a kernel boundary).
+template <class T>
+struct Coherent {
+  T *p = nullptr;  // host address
+  T *d = nullptr;  // device address of the same memory
+  size_t n = 0;
+  Coherent() = default;
+  Coherent(const Coherent &) = delete;
+  Coherent &operator=(const Coherent &) = delete;
+  ~Coherent() {
+    if (p) (void)hipHostFree(p);
+  }
+  hipError_t alloc(size_t count) {
+    n = count;
+    hipError_t e = hipHostMalloc((void **)&p, count * sizeof(T), hipHostMallocCoherent | hipHostMallocMapped);
+    if (e == hipSuccess) std::memset((void *)p, 0, count * sizeof(T));
+    if (e == hipSuccess) e = hipHostGetDevicePointer((void **)&d, p, 0);
+    return e;
+  }
+};
+
+// Circular allocator of contiguous ranges, freed in any order and reclaimed
+// in allocation order.
+struct RangeRing {
+  uint64_t cap = 0, wpos = 0;
+  struct A {
+    uint64_t off, len;
+    bool done;
+  };
+  std::deque<A> live;
+  bool alloc(uint64_t len, uint64_t *off) {
+    len = std::max<uint64_t>(len, 1);
+    if (len > cap) return false;
+    if (live.empty()) wpos = 0;
+    uint64_t o;
+    if (live.empty()) {
+      o = 0;
+    } else {
+      // live ranges lie in [oldest, wpos) unless they wrapped (wpos <= oldest)
+      const uint64_t oldest = live.front().off;
+      if (wpos > oldest) {
+        if (wpos + len <= cap)
+          o = wpos;
+        else if (len <= oldest)
+          o = 0;
+        else
+          return false;
+      } else {
+        if (wpos + len <= oldest)
+          o = wpos;
+        else
+          return false;
+      }
+    }
+    live.push_back(A{o, len, false});
+    wpos = o + len;
+    *off = o;
+    return true;
+  }
+  void release(uint64_t off) {
+    for (A &a : live)
+      if (a.off == off && !a.done) {
+        a.done = true;
+        break;
+      }
+    while (!live.empty() && live.front().done) live.pop_front();
+  }
+};
+
+// A request in the persistent kernel's queue.
+struct Flight {
+  std::shared_ptr<Req> req;
+  uint32_t slot;       // completion counter
+  uint64_t r0, n;      // pair / result rows
+  uint64_t u0, units;  // unit indices
+};
+
+// The persistent serving engine: work queue, pools and the kernel's launches.
+struct Persist {
+  static constexpr uint64_t kRingUnits = 1ull << 20;
+  static constexpr uint64_t kPairRows = 1ull << 23;
+  static constexpr uint32_t kReqSlots = 1u << 16;
+  static constexpr uint64_t kHalfQuads = 20480;  // pass-boundary half of one block
+  static constexpr uint64_t kArenaBytes = 1ull << 30;
+  Coherent<ServeUnit> ring;
+  Coherent<uint64_t> ctl;  // [0] tail
+  Coherent<uint32_t> stop;
+  Coherent<PairDev> pairs;
+  Coherent<PairResultDev> results;
+  Coherent<uint32_t> remaining;
+  DevBuf<uint64_t> head;
+  DevBuf<uint4> passbuf;
+  DevBuf<uint8_t> arena;
+  uint64_t arena_used = 0;
+  double close_started = 0;
+  std::atomic<bool> draining{false};
+  bool feeder_done = false;  // guarded by the server mutex
+  bool broken = false;       // requests were failed; guarded by the server mutex
+  hipStream_t kstream = nullptr, cstream = nullptr;
+  hipEvent_t kdone = nullptr;
+  uint32_t grid = 0;
+  int lds_stride = 258;
+  uint64_t published = 0;  // units published (host copy of the tail)
+  uint64_t launches = 0;
+  RangeRing rows;
+  std::deque<uint64_t> unit_live;  // u0 of requests in flight, in order
+  std::vector<uint32_t> free_slots;
+  std::vector<Flight> flights;
+  std::mutex kmu;  // launches
+  std::thread feeder, completer;
+  ~Persist() {
+    if (kdone) (void)hipEventDestroy(kdone);
+    if (kstream) (void)hipStreamDestroy(kstream);
+    if (cstream) (void)hipStreamDestroy(cstream);
+  }
+};
+
 }  // namespace st
 
 using namespace st;
@@ -126,6 +269,7 @@ struct st_server {
   st_server_config cfg{};
   std::mutex mu;
   std::condition_variable cv_work, cv_done;
+  std::condition_variable cv_feed;  // the persistent path's feeder
   std::deque<std::shared_ptr<Req>> queue;
   std::unordered_map<uint64_t, std::shared_ptr<Req>> live;
   uint64_t next_ticket = 1;
@@ -133,6 +277,9 @@ struct st_server {
   uint64_t inflight_bytes = 0;
   std::vector<std::unique_ptr<Slot>> slots;
   st_server_stats stats{};
+  // persistent path: requests it cannot take go to fb_queue (the slots)
+  std::unique_ptr<Persist> pe;
+  std::deque<std::shared_ptr<Req>> fb_queue;
 };
 
 namespace st {
@@ -337,16 +484,17 @@ void worker(Slot *sl) {
     uint64_t bytes = 0;
     {
       std::unique_lock<std::mutex> lk(srv->mu);
+      auto &Q = srv->pe ? srv->fb_queue : srv->queue;
       for (;;) {
-        if (srv->closing && srv->queue.empty()) return;
+        if (srv->closing && Q.empty() && (!srv->pe || srv->pe->feeder_done)) return;
         // admit FIFO while the DP-state budget allows; a request larger than
         // the whole budget runs alone once nothing else is in flight
-        while (!srv->queue.empty() && take.size() < srv->cfg.max_batch_queries) {
-          const uint64_t need = srv->queue.front()->state_bytes;
+        while (!Q.empty() && take.size() < srv->cfg.max_batch_queries) {
+          const uint64_t need = Q.front()->state_bytes;
           const bool alone = take.empty() && srv->inflight_bytes == 0;
           if (!alone && srv->inflight_bytes + bytes + need > srv->cfg.state_budget_bytes) break;
-          take.push_back(srv->queue.front());
-          srv->queue.pop_front();
+          take.push_back(Q.front());
+          Q.pop_front();
           bytes += need;
         }
         if (!take.empty()) break;
@@ -377,6 +525,327 @@ void worker(Slot *sl) {
     srv->cv_done.notify_all();
     srv->cv_work.notify_all();
   }
+}
+
+// ---- persistent serving kernel -------------------------------------------------
+
+// Launch the serving kernel unless it is running.  Caller holds pe.kmu.
+int ensure_running(st_server *srv) {
+  Persist &pe = *srv->pe;
+  if (hipEventQuery(pe.kdone) == hipErrorNotReady) return ST_OK;
+  ServeArgs a;
+  a.ring = pe.ring.d;
+  a.ring_mask = Persist::kRingUnits - 1;
+  a.tail = pe.ctl.d;
+  a.stop = pe.stop.d;
+  a.head = pe.head.p;
+  a.pairs = pe.pairs.d;
+  a.results = pe.results.d;
+  a.remaining = pe.remaining.d;
+  a.error = pe.stop.d + 4;
+  a.pair_rows = Persist::kPairRows;
+  a.req_slots = Persist::kReqSlots;
+  a.arena = srv->db->arena.p;
+  a.toff = srv->db->off.p;
+  a.tlen = srv->db->len.p;
+  a.passbuf = pe.passbuf.p;
+  a.passbuf_half = Persist::kHalfQuads;
+  a.idle_ticks = 500000;  // 5 ms of the 100 MHz wall clock
+  a.lds_stride = pe.lds_stride;
+  ST_HIP(launch_serve(a, pe.grid, pe.kstream));
+  ST_HIP(hipEventRecord(pe.kdone, pe.kstream));
+  pe.launches++;
+  return ST_OK;
+}
+
+void complete(st_server *srv, const std::shared_ptr<Req> &r, int rc, const std::string &err) {
+  std::lock_guard<std::mutex> lk(srv->mu);
+  srv->stats.queries++;
+  srv->stats.pairs += r->cands.size();
+  srv->stats.cells += r->cells;
+  r->t_done = now_ms();
+  r->status = rc;
+  r->err = err;
+}
+
+// Plan one request for the serving kernel and publish its units; false when
+// the kernel cannot take it (the caller routes it to the slots).
+bool publish(st_server *srv, const std::shared_ptr<Req> &r, int *rc) {
+  Persist &pe = *srv->pe;
+  const st_db *db = srv->db;
+  const st_qprof *qp = r->qp;
+  *rc = ST_OK;
+  const uint64_t n = r->cands.size();
+  if (kind_for(qp, r->open, r->ext) != kPk16 || !qp->prof16.p || qp->m == 0) return false;
+  uint32_t nmax = 0;
+  for (uint32_t t : r->cands) nmax = std::max(nmax, db->hlen[t]);
+  if ((uint64_t)qp->m + nmax - 1 > kMaxDiagsPk16) return false;
+  Plan pl;
+  std::vector<Unit> units;
+  std::vector<uint64_t> outs(n);
+  std::iota(outs.begin(), outs.end(), 0);
+  PlanLimits lim;
+  lim.only_R = 8;
+  lim.max_GR = 256;
+  lim.max_half = Persist::kHalfQuads;
+  if (!plan_query(pl, units, 0, qp, r->cands.data(), outs.data(), n, kPk16, db, unit_steps_cap(r->cells), &lim))
+    return false;
+  // longest units first: they set the request's latency
+  std::stable_sort(units.begin(), units.end(), [](const Unit &a, const Unit &b) {
+    return (uint64_t)a.passes * (a.n_pad + a.G - 1) > (uint64_t)b.passes * (b.n_pad + b.G - 1);
+  });
+  uint32_t ld = std::max<uint32_t>(qp->m, 8);
+  for (const Unit &u : units) ld = std::max<uint32_t>(ld, u.passes * u.G * u.R);
+  ld = (ld + 7) & ~7u;
+  if (ld > qp->ld) return false;
+  const uint64_t b16 = ((uint64_t)kCodes * ld * 2 + 255) & ~255ull, b32 = ((uint64_t)kCodes * ld * 4 + 255) & ~255ull;
+  if (b16 + b32 > Persist::kArenaBytes || units.size() > Persist::kRingUnits / 2 || n > Persist::kPairRows / 2)
+    return false;
+  // wait for room: rows, units, a counter slot, the arena (draining the
+  // kernel when the arena must restart) and the DP-state budget
+  uint64_t r0 = 0;
+  uint32_t slot = 0;
+  {
+    std::unique_lock<std::mutex> lk(srv->mu);
+    for (;;) {
+      if (pe.broken) {
+        *rc = fail(ST_ERR_DEVICE, "serving kernel: earlier requests did not finish");
+        return true;
+      }
+      const bool arena_ok = pe.arena_used + b16 + b32 <= Persist::kArenaBytes;
+      if (!arena_ok && pe.flights.empty()) {
+        // every launch-lifetime address was used: restart the kernel
+        pe.draining = true;
+        lk.unlock();
+        {
+          std::lock_guard<std::mutex> kl(pe.kmu);
+          __atomic_store_n(pe.stop.p, 1u, __ATOMIC_RELEASE);
+          (void)hipEventSynchronize(pe.kdone);
+          __atomic_store_n(pe.stop.p, 0u, __ATOMIC_RELEASE);
+        }
+        lk.lock();
+        pe.arena_used = 0;
+        pe.draining = false;
+        continue;
+      }
+      const uint64_t oldest = pe.unit_live.empty() ? pe.published : pe.unit_live.front();
+      const bool units_ok = pe.published + units.size() - oldest <= Persist::kRingUnits;
+      const bool budget_ok = pe.flights.empty() || srv->inflight_bytes + r->state_bytes <= srv->cfg.state_budget_bytes;
+      if (arena_ok && units_ok && budget_ok && !pe.free_slots.empty() && pe.rows.alloc(n ? n : 1, &r0)) {
+        slot = pe.free_slots.back();
+        pe.free_slots.pop_back();
+        srv->inflight_bytes += r->state_bytes;
+        srv->stats.peak_inflight_bytes = std::max(srv->stats.peak_inflight_bytes, srv->inflight_bytes);
+        break;
+      }
+      srv->cv_work.wait_for(lk, std::chrono::milliseconds(1));
+    }
+  }
+  const double t0 = now_ms();
+  // the query's profiles, at addresses fresh for the running launch
+  uint8_t *dst = pe.arena.p + pe.arena_used;
+  pe.arena_used += b16 + b32;
+  int16_t *p16 = (int16_t *)dst;
+  int32_t *p32 = (int32_t *)(dst + b16);
+  hipError_t e = hipMemcpy2DAsync(p16, (size_t)ld * 2, qp->prof16.p, (size_t)qp->ld * 2, (size_t)ld * 2, kCodes,
+                                  hipMemcpyDeviceToDevice, pe.cstream);
+  if (e == hipSuccess)
+    e = hipMemcpy2DAsync(p32, (size_t)ld * 4, qp->prof32.p, (size_t)qp->ld * 4, (size_t)ld * 4, kCodes,
+                         hipMemcpyDeviceToDevice, pe.cstream);
+  if (e == hipSuccess) e = hipStreamSynchronize(pe.cstream);
+  if (e != hipSuccess) {
+    *rc = fail(ST_ERR_DEVICE, std::string("serving profile copy: ") + hipGetErrorString(e));
+    return true;
+  }
+  // pair slots (result rows r0 + k) and unit descriptors
+  for (size_t k = 0; k < pl.pairs.size(); k++) {
+    PairDev pd = pl.pairs[k];
+    pd.out = (uint32_t)(r0 + pd.out);
+    pe.pairs.p[r0 + k] = pd;
+  }
+  const uint64_t u0 = pe.published;
+  for (size_t i = 0; i < units.size(); i++) {
+    const Unit &u = units[i];
+    ServeUnit su{};
+    su.prof16 = p16;
+    su.prof32 = p32;
+    su.m = qp->m;
+    su.ld = ld;
+    su.pair0 = (uint32_t)(r0 + u.pair0);
+    su.npairs = u.npairs;
+    su.G = u.G;
+    su.n_pad = u.n_pad;
+    su.passes = u.passes;
+    su.open = r->open;
+    su.ext = r->ext;
+    su.gate = r->gate;
+    su.req = slot;
+    pe.ring.p[(u0 + i) & (Persist::kRingUnits - 1)] = su;
+  }
+  __atomic_store_n(pe.remaining.p + slot, (uint32_t)units.size(), __ATOMIC_RELAXED);
+  {
+    std::lock_guard<std::mutex> lk(srv->mu);
+    pe.flights.push_back(Flight{r, slot, r0, n, u0, units.size()});
+    pe.unit_live.push_back(u0);
+    pe.published = u0 + units.size();
+    srv->stats.plan_ms += now_ms() - t0;
+  }
+  std::atomic_thread_fence(std::memory_order_release);
+  __atomic_store_n(pe.ctl.p, u0 + units.size(), __ATOMIC_RELEASE);
+  std::lock_guard<std::mutex> kl(pe.kmu);
+  *rc = ensure_running(srv);
+  return true;
+}
+
+void feeder(st_server *srv) {
+  (void)hipSetDevice(srv->ctx->device);
+  for (;;) {
+    std::shared_ptr<Req> r;
+    {
+      std::unique_lock<std::mutex> lk(srv->mu);
+      srv->cv_feed.wait(lk, [&] { return srv->closing || !srv->queue.empty(); });
+      if (srv->queue.empty()) {  // closing
+        srv->pe->feeder_done = true;
+        lk.unlock();
+        srv->cv_work.notify_all();
+        return;
+      }
+      r = srv->queue.front();
+      srv->queue.pop_front();
+    }
+    if (r->cands.empty()) {
+      complete(srv, r, ST_OK, "");
+      srv->cv_done.notify_all();
+      continue;
+    }
+    int rc = ST_OK;
+    if (!publish(srv, r, &rc)) {
+      std::lock_guard<std::mutex> lk(srv->mu);
+      srv->fb_queue.push_back(r);
+    } else if (rc) {
+      complete(srv, r, rc, st_last_error());
+      srv->cv_done.notify_all();
+    }
+    srv->cv_work.notify_all();
+  }
+}
+
+// Exact int32 rescoring of results at the int16 clamp (the batch path).
+int rescore_clamped(st_server *srv, const Req &r, st_pair_result *rows) {
+  std::vector<uint32_t> tids;
+  std::vector<uint64_t> idx;
+  for (uint64_t k = 0; k < r.cands.size(); k++)
+    if (rows[k].kernel == ST_KERNEL_I16 && rows[k].swg >= 32767) {
+      tids.push_back(r.cands[k]);
+      idx.push_back(k);
+    }
+  if (tids.empty()) return ST_OK;
+  const uint64_t off[2] = {0, tids.size()};
+  st_batch *b = nullptr;
+  int rc = st_batch_create(srv->ctx, srv->db, &r.qp, 1, off, tids.data(), &r.open, &r.ext, &r.gate, &b);
+  if (rc) return rc;
+  rc = st_batch_run(b);
+  std::vector<st_pair_result> res(tids.size());
+  if (!rc) rc = st_batch_results(b, res.data());
+  st_batch_destroy(b);
+  if (rc) return rc;
+  for (size_t i = 0; i < idx.size(); i++) rows[idx[i]] = res[i];
+  std::lock_guard<std::mutex> lk(srv->mu);
+  srv->stats.rescored += idx.size();
+  return ST_OK;
+}
+
+void completer(st_server *srv) {
+  (void)hipSetDevice(srv->ctx->device);
+  Persist &pe = *srv->pe;
+  for (;;) {
+    std::vector<Flight> done;
+    bool busy, quit;
+    {
+      std::lock_guard<std::mutex> lk(srv->mu);
+      for (size_t i = 0; i < pe.flights.size();) {
+        if (__atomic_load_n(pe.remaining.p + pe.flights[i].slot, __ATOMIC_ACQUIRE) == 0) {
+          done.push_back(pe.flights[i]);
+          pe.flights[i] = pe.flights.back();
+          pe.flights.pop_back();
+        } else {
+          i++;
+        }
+      }
+      busy = !pe.flights.empty();
+      quit = srv->closing && pe.feeder_done && pe.flights.empty() && done.empty();
+      // a malformed descriptor (reported by the kernel) or a server closing
+      // with requests that no longer progress fails those requests
+      const bool broken = __atomic_load_n(pe.stop.p + 4, __ATOMIC_ACQUIRE) != 0;
+      if (srv->closing && pe.feeder_done && busy) {
+        if (pe.close_started == 0) pe.close_started = now_ms();
+      }
+      if (busy && (broken || (pe.close_started > 0 && now_ms() - pe.close_started > 10000.0))) {
+        for (Flight &f : pe.flights) {
+          f.req->t_done = now_ms();
+          f.req->status = ST_ERR_DEVICE;
+          f.req->err = broken ? "serving kernel: malformed work unit" : "serving kernel: requests did not finish";
+        }
+        pe.flights.clear();
+        pe.broken = true;
+        busy = false;
+        quit = srv->closing && pe.feeder_done;
+        srv->cv_done.notify_all();
+      }
+    }
+    if (quit) return;
+    for (const Flight &f : done) {
+      Req &r = *f.req;
+      std::memcpy(r.out, pe.results.p + f.r0, f.n * sizeof(st_pair_result));
+      const int rc = rescore_clamped(srv, r, r.out);
+      {
+        std::lock_guard<std::mutex> lk(srv->mu);
+        pe.rows.release(f.r0);
+        for (auto it = pe.unit_live.begin(); it != pe.unit_live.end(); ++it)
+          if (*it == f.u0) {
+            pe.unit_live.erase(it);
+            break;
+          }
+        pe.free_slots.push_back(f.slot);
+        srv->inflight_bytes -= r.state_bytes;
+      }
+      complete(srv, f.req, rc, rc ? std::string(st_last_error()) : std::string());
+    }
+    if (!done.empty()) {
+      srv->cv_done.notify_all();
+      srv->cv_work.notify_all();
+    }
+    if (busy) {
+      // a launch that went idle and ended while units were being published
+      std::lock_guard<std::mutex> kl(pe.kmu);
+      if (!pe.draining && hipEventQuery(pe.kdone) != hipErrorNotReady) (void)ensure_running(srv);
+    }
+    std::this_thread::sleep_for(std::chrono::microseconds(busy ? 10 : 200));
+  }
+}
+
+int setup_persistent(st_server *s) {
+  auto pe = std::make_unique<Persist>();
+  pe->rows.cap = Persist::kPairRows;
+  const size_t lds = (size_t)kCodes * pe->lds_stride * 2;
+  const int occ = serve_occupancy(lds);
+  // leave one block per CU to the other launches (rescoring, int32 requests)
+  pe->grid = (uint32_t)std::max(1, occ - 1) * (uint32_t)s->ctx->cus;
+  bool ok = pe->ring.alloc(Persist::kRingUnits) == hipSuccess && pe->ctl.alloc(8) == hipSuccess &&
+            pe->stop.alloc(8) == hipSuccess && pe->pairs.alloc(Persist::kPairRows) == hipSuccess &&
+            pe->results.alloc(Persist::kPairRows) == hipSuccess &&
+            pe->remaining.alloc(Persist::kReqSlots) == hipSuccess && pe->head.alloc(1) == hipSuccess &&
+            pe->passbuf.alloc((size_t)pe->grid * (2 * Persist::kHalfQuads + kEdgeQuadsPerBlock)) == hipSuccess &&
+            pe->arena.alloc(Persist::kArenaBytes) == hipSuccess &&
+            hipMemset(pe->head.p, 0, sizeof(uint64_t)) == hipSuccess &&
+            hipStreamCreateWithFlags(&pe->kstream, hipStreamNonBlocking) == hipSuccess &&
+            hipStreamCreateWithFlags(&pe->cstream, hipStreamNonBlocking) == hipSuccess &&
+            hipEventCreateWithFlags(&pe->kdone, hipEventDisableTiming) == hipSuccess &&
+            hipDeviceSynchronize() == hipSuccess;
+  if (!ok) return fail(ST_ERR_NOMEM, "st_server_create: persistent serving buffers could not be allocated");
+  for (uint32_t k = 0; k < Persist::kReqSlots; k++) pe->free_slots.push_back(Persist::kReqSlots - 1 - k);
+  s->pe = std::move(pe);
+  return ST_OK;
 }
 
 }  // namespace
@@ -448,7 +917,20 @@ int st_server_create(st_ctx *ctx, const st_db *db, const st_server_config *cfg, 
       return fail(ST_ERR_DEVICE, "st_server_create: stream/event/counter creation failed");
     }
   }
+  if (cfg && cfg->persistent) {
+    s->cfg.persistent = 1;
+    const int rc = setup_persistent(s);
+    if (rc) {
+      release_slots(s);
+      delete s;
+      return rc;
+    }
+  }
   for (auto &sl : s->slots) sl->th = std::thread(worker, sl.get());
+  if (s->pe) {
+    s->pe->feeder = std::thread(feeder, s);
+    s->pe->completer = std::thread(completer, s);
+  }
   *out = s;
   return ST_OK;
 }
@@ -477,7 +959,10 @@ int st_server_submit(st_server *s, const st_request *req, uint64_t *ticket) {
     s->live[r->ticket] = r;
     s->queue.push_back(r);
   }
-  s->cv_work.notify_one();
+  if (s->pe)
+    s->cv_feed.notify_one();
+  else
+    s->cv_work.notify_one();
   *ticket = r->ticket;
   return ST_OK;
 }
@@ -549,8 +1034,20 @@ int st_server_destroy(st_server *s) {
     s->closing = true;
   }
   s->cv_work.notify_all();
+  s->cv_feed.notify_all();
+  if (s->pe) {
+    if (s->pe->feeder.joinable()) s->pe->feeder.join();
+    if (s->pe->completer.joinable()) s->pe->completer.join();
+  }
+  s->cv_work.notify_all();
   for (auto &sl : s->slots)
     if (sl->th.joinable()) sl->th.join();
+  if (s->pe) {
+    (void)hipSetDevice(s->ctx->device);
+    __atomic_store_n(s->pe->stop.p, 1u, __ATOMIC_RELEASE);
+    (void)hipStreamSynchronize(s->pe->kstream);
+    s->pe.reset();
+  }
   (void)hipSetDevice(s->ctx->device);
   release_slots(s);
   delete s;

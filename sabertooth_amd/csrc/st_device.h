@@ -52,6 +52,26 @@ struct PairOut {
   int32_t flag;  // 0: exact int16; 1: int16 saturated (rescore); 2: exact int32
 };
 
+// Per-pair result row (the C ABI's st_pair_result).
+struct PairResultDev {
+  int32_t swg, ung, best_diag, gate_score;
+  uint8_t pass, kernel, r0, r1;
+};
+
+// One unit of the serving kernel's work queue: a request's query (profile
+// copies in the serving arena), npairs consecutive slots of the pair pool,
+// the launch shape and the request's gaps and gate.
+struct ServeUnit {
+  const int16_t *prof16;
+  const int32_t *prof32;
+  uint32_t m, ld;
+  uint32_t pair0, npairs;  // pair pool slots
+  uint32_t G, n_pad, passes;
+  int32_t open, ext, gate;
+  uint32_t req;            // request slot: its completion counter
+  uint32_t pad;
+};
+
 // Top-diagonal key: best ungapped score of a diagonal in the high word, and
 // 0xFFFFFFFF - (d + m - 1) in the low word, so a larger key means a higher
 // score, then a smaller diagonal d; 0 = no diagonal.
