@@ -286,6 +286,11 @@ typedef struct {
   double busy_ms;               /* summed slot time spent on micro-batches */
   double plan_ms;               /* of which host planning and packing */
   double device_ms;             /* of which device time (uploads, kernels, results) */
+  /* persistent serving kernel */
+  uint64_t kernel_launches;     /* launches of the serving kernel */
+  uint64_t published;           /* requests it took */
+  double wait_ms;               /* summed submit -> publication time */
+  double run_ms;                /* summed publication -> completion time */
 } st_server_stats;
 
 /* cfg may be NULL (defaults). */

@@ -39,7 +39,9 @@ REQUEST_DT = np.dtype([("query", "<u8"), ("gap_open", "<i4"), ("gap_extend", "<i
                        ("reserved", "<i4"), ("cand_tids", "<u8"), ("n_cands", "<u8"), ("out", "<u8")])
 SERVER_STATS_DT = np.dtype([("batches", "<u8"), ("queries", "<u8"), ("pairs", "<u8"), ("cells", "<u8"),
                             ("rescored", "<u8"), ("max_batch_queries", "<u8"), ("peak_inflight_bytes", "<u8"),
-                            ("busy_ms", "<f8"), ("plan_ms", "<f8"), ("device_ms", "<f8")])
+                            ("busy_ms", "<f8"), ("plan_ms", "<f8"), ("device_ms", "<f8"),
+                            ("kernel_launches", "<u8"), ("published", "<u8"), ("wait_ms", "<f8"),
+                            ("run_ms", "<f8")])
 
 
 class StError(RuntimeError):

@@ -25,31 +25,10 @@
 #include <unordered_map>
 
 #include "host_internal.h"
+#include "serve_args.h"
 
 namespace st {
 
-struct ServeArgs {
-  const ServeUnit *ring;
-  uint64_t ring_mask;
-  const uint64_t *tail;
-  const uint32_t *stop;
-  uint64_t *head;
-  const PairDev *pairs;
-  PairResultDev *results;
-  uint32_t *remaining;
-  uint32_t *error;
-  uint64_t pair_rows;
-  uint32_t req_slots;
-  const uint8_t *arena;
-  const uint64_t *toff;
-  const uint32_t *tlen;
-  uint4 *passbuf;
-  uint64_t passbuf_half;
-  uint64_t idle_ticks;
-  int lds_stride;
-};
-int serve_occupancy(size_t lds_bytes);
-hipError_t launch_serve(const ServeArgs &a, uint32_t grid, hipStream_t s);
 
 namespace {
 
@@ -215,27 +194,30 @@ struct RangeRing {
 struct Flight {
   std::shared_ptr<Req> req;
   uint32_t slot;       // completion counter
-  uint64_t r0, n;      // pair / result rows
+  uint64_t r0, n;      // result rows (plan-slot order)
   uint64_t u0, units;  // unit indices
+  double t_pub;        // publication time
+  std::vector<uint32_t> cand_of;  // plan slot -> candidate index
 };
 
 // The persistent serving engine: work queue, pools and the kernel's launches.
 struct Persist {
   static constexpr uint64_t kRingUnits = 1ull << 20;
-  static constexpr uint64_t kPairRows = 1ull << 23;
+  static constexpr uint64_t kResultRows = 1ull << 22;
   static constexpr uint32_t kReqSlots = 1u << 16;
   static constexpr uint64_t kHalfQuads = 20480;  // pass-boundary half of one block
-  static constexpr uint64_t kArenaBytes = 1ull << 30;
+  static constexpr uint64_t kArenaBytes = 4ull << 30;
   Coherent<ServeUnit> ring;
   Coherent<uint64_t> ctl;  // [0] tail
-  Coherent<uint32_t> stop;
-  Coherent<PairDev> pairs;
-  Coherent<PairResultDev> results;
+  Coherent<uint32_t> stop;  // [0] stop, [4] malformed-unit flag
+  Coherent<ServeRow> results;
   Coherent<uint32_t> remaining;
-  DevBuf<uint64_t> head;
+  DevBuf<uint64_t> dctl;  // the kernel's device control words
   DevBuf<uint4> passbuf;
-  DevBuf<uint8_t> arena;
+  DevBuf<uint8_t> arena;  // profiles and pair slots of published requests
+  Pinned<uint8_t> h_stage;  // a request's arena image
   uint64_t arena_used = 0;
+  uint64_t inflight_cells = 0;  // guarded by the server mutex
   double close_started = 0;
   std::atomic<bool> draining{false};
   bool feeder_done = false;  // guarded by the server mutex
@@ -538,23 +520,34 @@ int ensure_running(st_server *srv) {
   a.ring_mask = Persist::kRingUnits - 1;
   a.tail = pe.ctl.d;
   a.stop = pe.stop.d;
-  a.head = pe.head.p;
-  a.pairs = pe.pairs.d;
+  a.ctl = pe.dctl.p;
   a.results = pe.results.d;
   a.remaining = pe.remaining.d;
   a.error = pe.stop.d + 4;
-  a.pair_rows = Persist::kPairRows;
+  a.result_rows = Persist::kResultRows;
   a.req_slots = Persist::kReqSlots;
+  a.sarena = pe.arena.p;
+  a.sarena_bytes = Persist::kArenaBytes;
   a.arena = srv->db->arena.p;
   a.toff = srv->db->off.p;
   a.tlen = srv->db->len.p;
+  a.n_targets = (uint32_t)srv->db->n_targets;
   a.passbuf = pe.passbuf.p;
   a.passbuf_half = Persist::kHalfQuads;
   a.idle_ticks = 500000;  // 5 ms of the 100 MHz wall clock
+  a.epoch = (uint32_t)(pe.launches % 65535) + 1;
+  a.zero = 0;
+  // tickets restart at the first unit the previous launch did not finish
+  ST_HIP(hipMemcpyAsync(pe.dctl.p + kCtlHead * kCtlStride, pe.dctl.p + kCtlDone * kCtlStride, sizeof(uint64_t),
+                        hipMemcpyDeviceToDevice, pe.kstream));
   a.lds_stride = pe.lds_stride;
   ST_HIP(launch_serve(a, pe.grid, pe.kstream));
   ST_HIP(hipEventRecord(pe.kdone, pe.kstream));
   pe.launches++;
+  {
+    std::lock_guard<std::mutex> lk(srv->mu);
+    srv->stats.kernel_launches++;
+  }
   return ST_OK;
 }
 
@@ -588,7 +581,14 @@ bool publish(st_server *srv, const std::shared_ptr<Req> &r, int *rc) {
   lim.only_R = 8;
   lim.max_GR = 256;
   lim.max_half = Persist::kHalfQuads;
-  if (!plan_query(pl, units, 0, qp, r->cands.data(), outs.data(), n, kPk16, db, unit_steps_cap(r->cells), &lim))
+  uint64_t load;
+  {
+    std::lock_guard<std::mutex> lk(srv->mu);
+    load = pe.inflight_cells + r->cells;
+  }
+  // unit size from the cells in flight: small units when the kernel is
+  // nearly idle (latency), larger ones under load (fewer claims)
+  if (!plan_query(pl, units, 0, qp, r->cands.data(), outs.data(), n, kPk16, db, unit_steps_cap(load), &lim))
     return false;
   // longest units first: they set the request's latency
   std::stable_sort(units.begin(), units.end(), [](const Unit &a, const Unit &b) {
@@ -599,7 +599,9 @@ bool publish(st_server *srv, const std::shared_ptr<Req> &r, int *rc) {
   ld = (ld + 7) & ~7u;
   if (ld > qp->ld) return false;
   const uint64_t b16 = ((uint64_t)kCodes * ld * 2 + 255) & ~255ull, b32 = ((uint64_t)kCodes * ld * 4 + 255) & ~255ull;
-  if (b16 + b32 > Persist::kArenaBytes || units.size() > Persist::kRingUnits / 2 || n > Persist::kPairRows / 2)
+  const uint64_t bp = (pl.pairs.size() * sizeof(PairDev) + 255) & ~255ull;
+  const uint64_t need = b16 + b32 + bp;
+  if (need > Persist::kArenaBytes / 2 || units.size() > Persist::kRingUnits / 2 || n > Persist::kResultRows / 2)
     return false;
   // wait for room: rows, units, a counter slot, the arena (draining the
   // kernel when the arena must restart) and the DP-state budget
@@ -612,7 +614,7 @@ bool publish(st_server *srv, const std::shared_ptr<Req> &r, int *rc) {
         *rc = fail(ST_ERR_DEVICE, "serving kernel: earlier requests did not finish");
         return true;
       }
-      const bool arena_ok = pe.arena_used + b16 + b32 <= Persist::kArenaBytes;
+      const bool arena_ok = pe.arena_used + need <= Persist::kArenaBytes;
       if (!arena_ok && pe.flights.empty()) {
         // every launch-lifetime address was used: restart the kernel
         pe.draining = true;
@@ -635,6 +637,7 @@ bool publish(st_server *srv, const std::shared_ptr<Req> &r, int *rc) {
         slot = pe.free_slots.back();
         pe.free_slots.pop_back();
         srv->inflight_bytes += r->state_bytes;
+        pe.inflight_cells += r->cells;
         srv->stats.peak_inflight_bytes = std::max(srv->stats.peak_inflight_bytes, srv->inflight_bytes);
         break;
       }
@@ -642,36 +645,55 @@ bool publish(st_server *srv, const std::shared_ptr<Req> &r, int *rc) {
     }
   }
   const double t0 = now_ms();
-  // the query's profiles, at addresses fresh for the running launch
+  // the query's profiles (code-major [kCodes][ld], zero beyond row m and for
+  // code 21) and pair slots (result rows r0 + k), built on the host and
+  // copied in one transfer to arena addresses fresh for the running launch
   uint8_t *dst = pe.arena.p + pe.arena_used;
-  pe.arena_used += b16 + b32;
+  pe.arena_used += need;
   int16_t *p16 = (int16_t *)dst;
   int32_t *p32 = (int32_t *)(dst + b16);
-  hipError_t e = hipMemcpy2DAsync(p16, (size_t)ld * 2, qp->prof16.p, (size_t)qp->ld * 2, (size_t)ld * 2, kCodes,
-                                  hipMemcpyDeviceToDevice, pe.cstream);
-  if (e == hipSuccess)
-    e = hipMemcpy2DAsync(p32, (size_t)ld * 4, qp->prof32.p, (size_t)qp->ld * 4, (size_t)ld * 4, kCodes,
-                         hipMemcpyDeviceToDevice, pe.cstream);
+  PairDev *pp = (PairDev *)(dst + b16 + b32);
+  hipError_t e = pe.h_stage.reserve(need);
+  if (e == hipSuccess) {
+    uint8_t *hs = pe.h_stage.p;
+    std::memset(hs, 0, b16 + b32);
+    int16_t *s16 = (int16_t *)hs;
+    int32_t *s32 = (int32_t *)(hs + b16);
+    const int32_t *rows = qp->rows.data();
+    for (int c = 0; c < 21; c++)
+      for (uint32_t i = 0; i < qp->m; i++) {
+        const int32_t v = rows[(size_t)i * 21 + c];
+        s16[(size_t)c * ld + i] = (int16_t)v;
+        s32[(size_t)c * ld + i] = v;
+      }
+    // slot k's result row is r0 + k: a unit's rows are consecutive
+    PairDev *sp = (PairDev *)(hs + b16 + b32);
+    for (size_t k = 0; k < pl.pairs.size(); k++) {
+      sp[k] = pl.pairs[k];
+      sp[k].out = (uint32_t)(r0 + k);
+    }
+    e = hipMemcpyAsync(dst, hs, need, hipMemcpyHostToDevice, pe.cstream);
+  }
   if (e == hipSuccess) e = hipStreamSynchronize(pe.cstream);
   if (e != hipSuccess) {
-    *rc = fail(ST_ERR_DEVICE, std::string("serving profile copy: ") + hipGetErrorString(e));
+    *rc = fail(ST_ERR_DEVICE, std::string("serving upload: ") + hipGetErrorString(e));
+    std::lock_guard<std::mutex> lk(srv->mu);
+    pe.rows.release(r0);
+    pe.free_slots.push_back(slot);
+    srv->inflight_bytes -= r->state_bytes;
+    pe.inflight_cells -= r->cells;
     return true;
   }
-  // pair slots (result rows r0 + k) and unit descriptors
-  for (size_t k = 0; k < pl.pairs.size(); k++) {
-    PairDev pd = pl.pairs[k];
-    pd.out = (uint32_t)(r0 + pd.out);
-    pe.pairs.p[r0 + k] = pd;
-  }
+  // unit descriptors
   const uint64_t u0 = pe.published;
   for (size_t i = 0; i < units.size(); i++) {
     const Unit &u = units[i];
     ServeUnit su{};
     su.prof16 = p16;
     su.prof32 = p32;
+    su.pairs = pp + u.pair0;
     su.m = qp->m;
     su.ld = ld;
-    su.pair0 = (uint32_t)(r0 + u.pair0);
     su.npairs = u.npairs;
     su.G = u.G;
     su.n_pad = u.n_pad;
@@ -685,7 +707,12 @@ bool publish(st_server *srv, const std::shared_ptr<Req> &r, int *rc) {
   __atomic_store_n(pe.remaining.p + slot, (uint32_t)units.size(), __ATOMIC_RELAXED);
   {
     std::lock_guard<std::mutex> lk(srv->mu);
-    pe.flights.push_back(Flight{r, slot, r0, n, u0, units.size()});
+    const double tp = now_ms();
+    std::vector<uint32_t> cand_of(pl.pairs.size());
+    for (size_t k = 0; k < pl.pairs.size(); k++) cand_of[k] = pl.pairs[k].out;
+    pe.flights.push_back(Flight{r, slot, r0, n, u0, units.size(), tp, std::move(cand_of)});
+    srv->stats.published++;
+    srv->stats.wait_ms += tp - r->t_submit;
     pe.unit_live.push_back(u0);
     pe.published = u0 + units.size();
     srv->stats.plan_ms += now_ms() - t0;
@@ -765,8 +792,8 @@ void completer(st_server *srv) {
       std::lock_guard<std::mutex> lk(srv->mu);
       for (size_t i = 0; i < pe.flights.size();) {
         if (__atomic_load_n(pe.remaining.p + pe.flights[i].slot, __ATOMIC_ACQUIRE) == 0) {
-          done.push_back(pe.flights[i]);
-          pe.flights[i] = pe.flights.back();
+          done.push_back(std::move(pe.flights[i]));
+          pe.flights[i] = std::move(pe.flights.back());
           pe.flights.pop_back();
         } else {
           i++;
@@ -796,7 +823,17 @@ void completer(st_server *srv) {
     if (quit) return;
     for (const Flight &f : done) {
       Req &r = *f.req;
-      std::memcpy(r.out, pe.results.p + f.r0, f.n * sizeof(st_pair_result));
+      for (uint64_t k = 0; k < f.n; k++) {
+        const ServeRow row = pe.results.p[f.r0 + k];
+        st_pair_result &o = r.out[f.cand_of[k]];
+        o.swg = row.swg;
+        o.ung = row.ung;
+        o.best_diag = row.best_diag;
+        o.gate_score = row.gate_score;
+        o.pass = row.gate_score >= r.gate;
+        o.kernel = ST_KERNEL_I16;
+        o.reserved[0] = o.reserved[1] = 0;
+      }
       const int rc = rescore_clamped(srv, r, r.out);
       {
         std::lock_guard<std::mutex> lk(srv->mu);
@@ -808,6 +845,11 @@ void completer(st_server *srv) {
           }
         pe.free_slots.push_back(f.slot);
         srv->inflight_bytes -= r.state_bytes;
+        pe.inflight_cells -= r.cells;
+      }
+      {
+        std::lock_guard<std::mutex> lk(srv->mu);
+        srv->stats.run_ms += now_ms() - f.t_pub;
       }
       complete(srv, f.req, rc, rc ? std::string(st_last_error()) : std::string());
     }
@@ -826,18 +868,18 @@ void completer(st_server *srv) {
 
 int setup_persistent(st_server *s) {
   auto pe = std::make_unique<Persist>();
-  pe->rows.cap = Persist::kPairRows;
+  pe->rows.cap = Persist::kResultRows;
   const size_t lds = (size_t)kCodes * pe->lds_stride * 2;
   const int occ = serve_occupancy(lds);
   // leave one block per CU to the other launches (rescoring, int32 requests)
   pe->grid = (uint32_t)std::max(1, occ - 1) * (uint32_t)s->ctx->cus;
   bool ok = pe->ring.alloc(Persist::kRingUnits) == hipSuccess && pe->ctl.alloc(8) == hipSuccess &&
-            pe->stop.alloc(8) == hipSuccess && pe->pairs.alloc(Persist::kPairRows) == hipSuccess &&
-            pe->results.alloc(Persist::kPairRows) == hipSuccess &&
-            pe->remaining.alloc(Persist::kReqSlots) == hipSuccess && pe->head.alloc(1) == hipSuccess &&
+            pe->stop.alloc(8) == hipSuccess && pe->results.alloc(Persist::kResultRows) == hipSuccess &&
+            pe->remaining.alloc(Persist::kReqSlots) == hipSuccess &&
+            pe->dctl.alloc((size_t)kCtlWords * kCtlStride) == hipSuccess &&
             pe->passbuf.alloc((size_t)pe->grid * (2 * Persist::kHalfQuads + kEdgeQuadsPerBlock)) == hipSuccess &&
             pe->arena.alloc(Persist::kArenaBytes) == hipSuccess &&
-            hipMemset(pe->head.p, 0, sizeof(uint64_t)) == hipSuccess &&
+            hipMemset(pe->dctl.p, 0, (size_t)kCtlWords * kCtlStride * sizeof(uint64_t)) == hipSuccess &&
             hipStreamCreateWithFlags(&pe->kstream, hipStreamNonBlocking) == hipSuccess &&
             hipStreamCreateWithFlags(&pe->cstream, hipStreamNonBlocking) == hipSuccess &&
             hipEventCreateWithFlags(&pe->kdone, hipEventDisableTiming) == hipSuccess &&

@@ -62,15 +62,16 @@ struct PairResultDev {
 // copies in the serving arena), npairs consecutive slots of the pair pool,
 // the launch shape and the request's gaps and gate.
 struct ServeUnit {
-  const int16_t *prof16;
+  const int16_t *prof16;   // profile copies in the serving arena
   const int32_t *prof32;
+  const PairDev *pairs;    // the unit's pair slots in the serving arena
   uint32_t m, ld;
-  uint32_t pair0, npairs;  // pair pool slots
+  uint32_t npairs;
   uint32_t G, n_pad, passes;
   int32_t open, ext, gate;
   uint32_t req;            // request slot: its completion counter
-  uint32_t pad;
 };
+static_assert(sizeof(ServeUnit) == 64, "one 64-byte descriptor per unit");
 
 // Top-diagonal key: best ungapped score of a diagonal in the high word, and
 // 0xFFFFFFFF - (d + m - 1) in the low word, so a larger key means a higher
