@@ -70,11 +70,83 @@ def _pct(x, q):
     return round(float(np.percentile(x, q)), 3) if len(x) else None
 
 
+def _serve_mode(st, ctx, db, reqs, cells, ref_of, n_iso, label, **cfg):
+    """Isolated, saturated and Poisson-arrival latencies of one server
+    configuration; every served result is compared with the plain batch's."""
+    n = len(reqs)
+    srv = st.Server(ctx, db, **cfg)
+    mism = 0
+
+    def check(idx, outs):
+        nonlocal mism
+        for k, out in zip(idx, outs):
+            r = ref_of[k]
+            mism += int(not all((out[f] == r[f]).all() for f in ("swg", "ung", "best_diag", "gate_score",
+                                                                 "pass", "kernel")))
+
+    srv.run_trace(reqs[:8], np.zeros(8))  # warm buffers (and start the serving kernel)
+    # isolated: one request at a time
+    iso = np.zeros(n_iso)
+    outs = []
+    for k in range(n_iso):
+        o, lat = srv.run_trace([reqs[k]], np.zeros(1))
+        iso[k] = lat[0]
+        outs.append(o[0])
+    check(range(n_iso), outs)
+    log(f"[{label}] isolated latency over {n_iso} requests: p50 {np.percentile(iso, 50):.2f} ms, "
+        f"p99 {np.percentile(iso, 99):.2f} ms")
+    # saturated: every request queued at once
+    ts = time.perf_counter()
+    outs, lat_sat = srv.run_trace(reqs, np.zeros(n))
+    t_sat = time.perf_counter() - ts
+    check(range(n), outs)
+    sat_qps = n / t_sat
+    log(f"[{label}] saturated: {n} requests in {1e3 * t_sat:.1f} ms, {sat_qps:.0f} q/s, "
+        f"{cells.sum() / t_sat / 1e9:.0f} GCUPS")
+    rng = np.random.default_rng(3)
+    rates = [50.0, 200.0, 500.0, 1000.0, 2000.0, 0.5 * sat_qps, 0.8 * sat_qps]
+    sweep = []
+    for rate in rates:
+        m = int(min(n, max(100, rate * 1.0)))
+        idx = rng.choice(n, size=m, replace=m > n)
+        arr = np.cumsum(rng.exponential(1e3 / rate, size=m))
+        outs, lat = srv.run_trace([reqs[k] for k in idx], arr)
+        check(idx, outs)
+        span = float(np.max(arr + lat)) / 1e3
+        ent = {"offered_qps": round(rate, 1), "requests": m, "achieved_qps": round(m / span, 1),
+               "gcups": round(float(cells[idx].sum()) / span / 1e9, 1),
+               "p50_ms": _pct(lat, 50), "p99_ms": _pct(lat, 99)}
+        # slowdown of each request against its own isolated latency
+        sel = idx < n_iso
+        if sel.any():
+            ent["p99_over_isolated_same_request"] = round(float(np.percentile(lat[sel] / iso[idx[sel]], 99)), 3)
+        sweep.append(ent)
+        log(f"[{label}] rate {rate:.0f} q/s: {m} requests, p50 {ent['p50_ms']} ms, p99 {ent['p99_ms']} ms, "
+            f"achieved {ent['achieved_qps']} q/s")
+    stats = srv.stats()
+    srv.close()
+    at80 = sweep[-1]
+    return {
+        "config": cfg,
+        "saturated": {"ms": round(1e3 * t_sat, 3), "queries_per_s": round(sat_qps, 1),
+                      "gcups": round(cells.sum() / t_sat / 1e9, 1), "p50_ms": _pct(lat_sat, 50),
+                      "p99_ms": _pct(lat_sat, 99)},
+        "isolated": {"requests": n_iso, "p50_ms": _pct(iso, 50), "p99_ms": _pct(iso, 99)},
+        "poisson": sweep,
+        "p99_at_80pct_over_isolated_p50": round(at80["p99_ms"] / float(np.percentile(iso, 50)), 3),
+        "p99_at_80pct_over_isolated_p99": round(at80["p99_ms"] / float(np.percentile(iso, 99)), 3),
+        "served_vs_batch_mismatched_requests": mism,
+        "server_stats": stats,
+    }
+
+
 def serving_leg(a, O, st, synth):
     """Served vs plain batch on the serving configuration: a resident
     265 M-residue database, 90% plain / 10% profile requests with 3,000
-    candidates each, Poisson arrivals.  Every served result is compared with
-    the plain batch's (bit-exact)."""
+    candidates each, Poisson arrivals.  Two servers are measured: the
+    persistent serving kernel (requests merged into one running launch) and
+    micro-batching slots (a launch per micro-batch).  Every served result is
+    compared with the plain batch's (bit-exact)."""
     t0 = time.time()
     w = synth.serving(seed=1, n_requests=a.serving_requests)
     n = len(w.rows)
@@ -98,75 +170,27 @@ def serving_leg(a, O, st, synth):
     batch_qps = n / t_batch
     log(f"plain batch: {1e3 * t_batch:.1f} ms, {batch_qps:.0f} q/s, {cells.sum() / t_batch / 1e9:.0f} GCUPS")
     reqs = [(qps[k], w.cands[k], gaps[k], w.gates[k]) for k in range(n)]
-    srv = st.Server(ctx, db, workers=3, max_batch_queries=128)
-    mism = 0
-
-    def check(idx, outs):
-        nonlocal mism
-        for k, out in zip(idx, outs):
-            r = ref_of[k]
-            mism += int(not all((out[f] == r[f]).all() for f in ("swg", "ung", "best_diag", "gate_score",
-                                                                 "pass", "kernel")))
-
-    # isolated: one request at a time
     n_iso = min(n, 200)
-    srv.run_trace(reqs[:8], np.zeros(8))  # warm the slots' buffers
-    iso = np.zeros(n_iso)
-    outs = []
-    for k in range(n_iso):
-        o, lat = srv.run_trace([reqs[k]], np.zeros(1))
-        iso[k] = lat[0]
-        outs.append(o[0])
-    check(range(n_iso), outs)
-    log(f"isolated latency over {n_iso} requests: p50 {np.percentile(iso, 50):.2f} ms, "
-        f"p99 {np.percentile(iso, 99):.2f} ms")
-    # saturated: every request queued at once
-    ts = time.perf_counter()
-    outs, lat_sat = srv.run_trace(reqs, np.zeros(n))
-    t_sat = time.perf_counter() - ts
-    check(range(n), outs)
-    sat_qps = n / t_sat
-    log(f"saturated: {n} requests in {1e3 * t_sat:.1f} ms, {sat_qps:.0f} q/s, "
-        f"{cells.sum() / t_sat / 1e9:.0f} GCUPS")
-    rng = np.random.default_rng(3)
-    rates = [50.0, 200.0, 500.0, 1000.0, 2000.0, 0.5 * sat_qps, 0.8 * sat_qps]
-    sweep = []
-    for rate in rates:
-        m = int(min(n, max(100, rate * 1.0)))
-        idx = rng.choice(n, size=m, replace=m > n)
-        arr = np.cumsum(rng.exponential(1e3 / rate, size=m))
-        outs, lat = srv.run_trace([reqs[k] for k in idx], arr)
-        check(idx, outs)
-        span = float(np.max(arr + lat)) / 1e3
-        ent = {"offered_qps": round(rate, 1), "requests": m, "achieved_qps": round(m / span, 1),
-               "gcups": round(float(cells[idx].sum()) / span / 1e9, 1),
-               "p50_ms": _pct(lat, 50), "p99_ms": _pct(lat, 99)}
-        # slowdown of each request against its own isolated latency
-        sel = idx < n_iso
-        if sel.any():
-            ent["p99_over_isolated_same_request"] = round(float(np.percentile(lat[sel] / iso[idx[sel]], 99)), 3)
-        sweep.append(ent)
-        log(f"rate {rate:.0f} q/s: {m} requests, p50 {ent['p50_ms']} ms, p99 {ent['p99_ms']} ms, "
-            f"achieved {ent['achieved_qps']} q/s")
-    stats = srv.stats()
-    srv.close()
-    at80 = sweep[-1]
-    return {
+    pers = _serve_mode(st, ctx, db, reqs, cells, ref_of, n_iso, "persistent", workers=3, persistent=True)
+    micro = _serve_mode(st, ctx, db, reqs, cells, ref_of, n_iso, "micro-batch", workers=3, max_batch_queries=128,
+                        persistent=False)
+    out = {
         "workload": "serving_265M", "requests": n, "profile_requests": int(sum(w.profile)),
         "db_residues": int(w.offsets[-1]), "db_targets": len(w.offsets) - 1, "candidates_per_request": 3000,
-        "cells": int(cells.sum()), "workers": 3, "max_batch_queries": 128,
+        "cells": int(cells.sum()),
         "plain_batch": {"ms": round(1e3 * t_batch, 3), "queries_per_s": round(batch_qps, 1),
                         "gcups": round(cells.sum() / t_batch / 1e9, 1)},
-        "saturated": {"ms": round(1e3 * t_sat, 3), "queries_per_s": round(sat_qps, 1),
-                      "gcups": round(cells.sum() / t_sat / 1e9, 1), "p50_ms": _pct(lat_sat, 50),
-                      "p99_ms": _pct(lat_sat, 99)},
-        "isolated": {"requests": n_iso, "p50_ms": _pct(iso, 50), "p99_ms": _pct(iso, 99)},
-        "poisson": sweep,
-        "p99_at_80pct_over_isolated_p50": round(at80["p99_ms"] / float(np.percentile(iso, 50)), 3),
-        "p99_at_80pct_over_isolated_p99": round(at80["p99_ms"] / float(np.percentile(iso, 99)), 3),
-        "served_vs_batch_mismatched_requests": mism,
-        "server_stats": stats,
+        "served_mode": "persistent",
     }
+    # headline figures: the persistent serving kernel
+    for k in ("saturated", "isolated", "poisson", "p99_at_80pct_over_isolated_p50", "p99_at_80pct_over_isolated_p99",
+              "served_vs_batch_mismatched_requests", "server_stats"):
+        out[k] = pers[k]
+    out["micro_batch"] = micro
+    for q in qps:
+        q.close()
+    db.close()
+    return out
 
 
 def main():

@@ -308,14 +308,18 @@ class Batch:
 class Server:
     """Serving loop over a resident database (continuous batching).
 
-    Worker slots take every queued request whose DP state fits the memory
-    budget beside what is already running, score them as one micro-batch on
-    their own stream and complete them; results equal those of ``Batch`` on
-    the same query and candidates.
+    persistent=True (default): requests are planned into work units and
+    appended to the queue of one running serving kernel, which merges them
+    with the work already on the device; requests it cannot take (int32
+    precision, negative gap costs, very long diagonals) go to the worker
+    slots.  persistent=False: worker slots take every queued request whose
+    DP state fits the memory budget beside what is already running and score
+    them as one micro-batch on their own stream.  Either way each request's
+    results equal those of ``Batch`` on the same query and candidates.
     """
 
     def __init__(self, ctx: Context, db: TargetDb, workers: int = 4, max_batch_queries: int = 64,
-                 state_budget_bytes: int = 4 << 30, persistent: bool = False):
+                 state_budget_bytes: int = 4 << 30, persistent: bool = True):
         cfg = np.zeros(1, dtype=_lib.SERVER_CONFIG_DT)
         cfg["workers"], cfg["max_batch_queries"], cfg["state_budget_bytes"] = (workers, max_batch_queries,
                                                                               state_budget_bytes)
