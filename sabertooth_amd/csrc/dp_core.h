@@ -110,17 +110,17 @@ __device__ __forceinline__ void insert4(K (&k)[4], K v) {
 }
 
 __device__ __forceinline__ uint32_t shr1u(uint32_t v) {
-  // lane i <- lane i-1 inside each 16-lane row; row starts keep their value
-  // and are overwritten by the caller (they are always group starts)
-  return (uint32_t)__builtin_amdgcn_update_dpp((int)v, (int)v, 0x111, 0xf, 0xf, false);
+  // lane i <- lane i-1 inside each 16-lane row; row starts read 0 and are
+  // overwritten by the caller (they are always group starts)
+  return (uint32_t)__builtin_amdgcn_mov_dpp((int)v, 0x111, 0xf, 0xf, true);
 }
 
 // lane i <- lane i-1 across the whole wave (groups of 32 or 64 lanes)
 __device__ __forceinline__ uint32_t shr1w(uint32_t v) { return (uint32_t)__shfl_up((int)v, 1); }
 
-template <class P>
-__device__ __forceinline__ typename P::T shr1(typename P::T v, bool wide) {
-  return (typename P::T)(wide ? shr1w((uint32_t)v) : shr1u((uint32_t)v));
+template <class P, bool WIDE>
+__device__ __forceinline__ typename P::T shr1(typename P::T v) {
+  return (typename P::T)(WIDE ? shr1w((uint32_t)v) : shr1u((uint32_t)v));
 }
 
 template <class P>
@@ -138,16 +138,17 @@ __device__ __forceinline__ const int32_t *prof_of<I32N>(const QueryDev &q) { ret
 template <class P, int R>
 __device__ __forceinline__ void load_raw(const typename P::L *lds, int stride, uint32_t code, int l,
                                          uint32_t (&raw)[R]) {
+  // code * stride < 2^24: full-rate 24-bit multiplies
   if (sizeof(typename P::L) == 2) {
-    const uint32_t *ra = (const uint32_t *)(lds + (int)(code & 0xff) * stride + l * R);
-    const uint32_t *rb = (const uint32_t *)(lds + (int)((code >> 8) & 0xff) * stride + l * R);
+    const uint32_t *ra = (const uint32_t *)(lds + (int)__umul24(code & 0xff, (uint32_t)stride) + l * R);
+    const uint32_t *rb = (const uint32_t *)(lds + (int)__umul24((code >> 8) & 0xff, (uint32_t)stride) + l * R);
 #pragma unroll
     for (int i = 0; i < R / 2; i++) {
       raw[i] = ra[i];
       raw[R / 2 + i] = rb[i];
     }
   } else {
-    const uint32_t *ra = (const uint32_t *)(lds + (int)(code & 0xff) * stride + l * R);
+    const uint32_t *ra = (const uint32_t *)(lds + (int)__umul24(code & 0xff, (uint32_t)stride) + l * R);
 #pragma unroll
     for (int i = 0; i < R; i++) raw[i] = ra[i];
   }
@@ -179,7 +180,8 @@ struct GroupCtx {
 
 // One pass of G*R rows over all n_pad columns.  LAST: the pass holding the
 // matrix's last padded row (bottom-row diagonal ends are folded here).
-template <class P, int R, bool UNG, bool LAST, bool END>
+// WIDE: groups of 32 or 64 lanes (boundaries cross 16-lane DPP rows).
+template <class P, int R, bool UNG, bool LAST, bool END, bool WIDE>
 __device__ __forceinline__ void run_pass(const GroupCtx<P> &c, const typename P::L *lds, int stride, int base,
                                          bool first, bool final_pass, const uint4 *rd, uint4 *wr,
                                          typename P::T e0, typename P::T vopen, typename P::T vext,
@@ -189,7 +191,7 @@ __device__ __forceinline__ void run_pass(const GroupCtx<P> &c, const typename P:
   using K = typename P::K;
   constexpr int kPer = P::kPer;
   const int l = c.l, G = c.G, n_pad = c.n_pad, m = c.m;
-  const bool wide = G > 16;  // groups wider than a 16-lane DPP row
+  constexpr bool wide = WIDE;
   const T kInvalid = (T)0xFFFFFFFFu;  // a negative B: not a diagonal end
 
   T H[R], E[R], U[R], B[R];
@@ -253,11 +255,11 @@ __device__ __forceinline__ void run_pass(const GroupCtx<P> &c, const typename P:
 #endif
 
       // boundary from the upper lane (or the previous pass / the matrix edge)
-      T in_h = shr1<P>(out_h, wide), in_f = shr1<P>(out_f, wide);
+      T in_h = shr1<P, WIDE>(out_h), in_f = shr1<P, WIDE>(out_f);
       T in_u = 0, in_b = 0;
       if (UNG) {
-        in_u = shr1<P>(out_u, wide);
-        in_b = shr1<P>(out_b, wide);
+        in_u = shr1<P, WIDE>(out_u);
+        in_b = shr1<P, WIDE>(out_b);
       }
       if (l == 0) {
         if (!first && t < n_pad) {
@@ -388,6 +390,31 @@ __device__ __forceinline__ void run_pass(const GroupCtx<P> &c, const typename P:
   }
 }
 
+// One pass with the instance for the group width and the pass position
+// chosen once (uniform per unit), not per step.
+template <class P, int R, bool UNG, bool END>
+__device__ __forceinline__ void run_pass_sel(bool last, const GroupCtx<P> &c, const typename P::L *lds, int stride,
+                                             int base, bool first, bool final_pass, const uint4 *rd, uint4 *wr,
+                                             typename P::T e0, typename P::T vopen, typename P::T vext,
+                                             typename P::T &best, typename P::K (&top)[P::kPer][4],
+                                             typename P::T *edge, int lane, int &bj, int &bi) {
+  if (c.G > 16) {
+    if (last)
+      run_pass<P, R, UNG, true, END, true>(c, lds, stride, base, first, true, rd, wr, e0, vopen, vext, best, top,
+                                           edge, lane, bj, bi);
+    else
+      run_pass<P, R, UNG, false, END, true>(c, lds, stride, base, first, final_pass, rd, wr, e0, vopen, vext, best,
+                                            top, edge, lane, bj, bi);
+  } else {
+    if (last)
+      run_pass<P, R, UNG, true, END, false>(c, lds, stride, base, first, true, rd, wr, e0, vopen, vext, best, top,
+                                            edge, lane, bj, bi);
+    else
+      run_pass<P, R, UNG, false, END, false>(c, lds, stride, base, first, final_pass, rd, wr, e0, vopen, vext, best,
+                                             top, edge, lane, bj, bi);
+  }
+}
+
 // Outputs of one unit, identical in every lane of a group after the merge:
 // the best H (two int16 halves in Pk16), the 4 best diagonal keys per pair,
 // the end cell (END) and the group's pair slots.
@@ -476,12 +503,8 @@ __device__ __forceinline__ void score_unit(const QueryDev q, const PairDev *upai
     const uint4 *rd = myb + (uint64_t)((p + 1) & 1) * passbuf_half + (uint64_t)c.g * c.n_pad;
     uint4 *wr = myb + (uint64_t)(p & 1) * passbuf_half + (uint64_t)c.g * c.n_pad;
     const bool final_pass = p == (int)passes - 1;
-    if (UNG && final_pass)
-      run_pass<P, R, UNG, true, END>(c, lds, lds_stride, base, p == 0, true, rd, wr, e0, vopen, vext, best, r.top,
-                                     edge, lane, bj, bi);
-    else
-      run_pass<P, R, UNG, false, END>(c, lds, lds_stride, base, p == 0, final_pass, rd, wr, e0, vopen, vext, best,
-                                      r.top, edge, lane, bj, bi);
+    run_pass_sel<P, R, UNG, END>(UNG && final_pass, c, lds, lds_stride, base, p == 0, final_pass, rd, wr, e0, vopen,
+                                 vext, best, r.top, edge, lane, bj, bi);
   }
   // best and top-4 over the group's lanes
   for (int off = 1; off < c.G; off <<= 1) {

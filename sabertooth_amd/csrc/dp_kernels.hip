@@ -50,7 +50,7 @@ namespace st {
 // END (int32 kernels without the evidence): also the end cell of the
 // optimum, first in column-major order, into qend / tend.
 template <class P, int R, bool UNG, bool END>
-__global__ __launch_bounds__(64) void dp_kernel(
+__global__ __launch_bounds__(64, (R == 8 && !END) ? 4 : 2) void dp_kernel(
     const Unit *__restrict__ units, uint32_t nunits, uint32_t *__restrict__ counter,
     const PairDev *__restrict__ pairs, const QueryDev *__restrict__ queries,
     const uint8_t *__restrict__ arena, const uint64_t *__restrict__ toff,
@@ -139,12 +139,8 @@ __global__ __launch_bounds__(64) void dp_kernel(
       const uint4 *rd = myb + (uint64_t)((p + 1) & 1) * passbuf_half + (uint64_t)c.g * c.n_pad;
       uint4 *wr = myb + (uint64_t)(p & 1) * passbuf_half + (uint64_t)c.g * c.n_pad;
       const bool final_pass = p == (int)u.passes - 1;
-      if (UNG && final_pass)
-        run_pass<P, R, UNG, true, END>(c, lds, lds_stride, base, p == 0, true, rd, wr, e0, vopen, vext, best, top,
-                                       edge, lane, bj, bi);
-      else
-        run_pass<P, R, UNG, false, END>(c, lds, lds_stride, base, p == 0, final_pass, rd, wr, e0, vopen, vext,
-                                        best, top, edge, lane, bj, bi);
+      run_pass_sel<P, R, UNG, END>(UNG && final_pass, c, lds, lds_stride, base, p == 0, final_pass, rd, wr, e0,
+                                   vopen, vext, best, top, edge, lane, bj, bi);
     }
     // best and top-4 over the group's lanes
     for (int off = 1; off < c.G; off <<= 1) {
