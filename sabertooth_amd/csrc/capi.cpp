@@ -295,8 +295,8 @@ static int forced_groups() {
 // residue streams (memory-pipeline bound), 64-lane groups for cross-row
 // shuffles.
 static const double kShapePs[2][7] = {
-    {2.00, 0.87, 0.33, 0.26, 0.255, 0.26, 0.41},  // R = 8
-    {0.85, 0.38, 0.27, 0.27, 0.27, 0.315, 0.68},  // R = 16
+    {2.00, 0.87, 0.33, 0.255, 0.246, 0.252, 0.375},  // R = 8
+    {0.83, 0.36, 0.248, 0.247, 0.255, 0.29, 0.60},    // R = 16
 };
 
 double unit_steps_cap(uint64_t batch_cells) {
@@ -422,10 +422,15 @@ void finish_plan(Plan &pl, std::vector<Unit> &pending, const std::vector<Kind> &
                         int cus, bool ung, bool end) {
   auto kind_of = [&](const Unit &u) { return force_kind >= 0 ? force_kind : (int)qkind[u.query]; };
   auto big = [](const Unit &u) { return u.G * u.R > 256 ? 1 : 0; };
+  // packed int16 units of both heights share a launch (the kernel instance
+  // with R = 0 takes each unit's own R); the int32 kernels are per R
+  auto r_of = [&](const Unit &u) { return kind_of(u) == kPk16 ? 0 : (int)u.R; };
   std::stable_sort(pending.begin(), pending.end(), [&](const Unit &a, const Unit &b) {
     if (kind_of(a) != kind_of(b)) return kind_of(a) < kind_of(b);
-    if (a.R != b.R) return a.R < b.R;
-    if (big(a) != big(b)) return big(a) < big(b);
+    if (r_of(a) != r_of(b)) return r_of(a) < r_of(b);
+    // wide-group launches first: their few long units must start before the
+    // main launch fills the device
+    if (big(a) != big(b)) return big(a) > big(b);
     if (open[a.query] != open[b.query]) return open[a.query] < open[b.query];
     if (ext[a.query] != ext[b.query]) return ext[a.query] < ext[b.query];
     return unit_cost(a) > unit_cost(b);
@@ -434,7 +439,7 @@ void finish_plan(Plan &pl, std::vector<Unit> &pending, const std::vector<Kind> &
     const Unit &u = pending[i];
     Launch L{};
     L.kind = kind_of(u);
-    L.R = (int)u.R;
+    L.R = r_of(u);
     L.open = open[u.query];
     L.ext = ext[u.query];
     L.unit0 = (uint32_t)pl.units.size();
@@ -443,7 +448,7 @@ void finish_plan(Plan &pl, std::vector<Unit> &pending, const std::vector<Kind> &
     L.half = 0;
     while (j < pending.size()) {
       const Unit &v = pending[j];
-      if (kind_of(v) != L.kind || (int)v.R != L.R || big(v) != big(u) || open[v.query] != L.open ||
+      if (kind_of(v) != L.kind || r_of(v) != L.R || big(v) != big(u) || open[v.query] != L.open ||
           ext[v.query] != L.ext)
         break;
       L.gr_max = std::max(L.gr_max, (int)(v.G * v.R));

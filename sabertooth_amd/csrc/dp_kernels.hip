@@ -46,11 +46,132 @@
 
 namespace st {
 
+// One unit: every pass of its groups, then the merged outputs.
+template <class P, int R, bool UNG, bool END>
+__device__ __forceinline__ void dp_unit(const Unit &u, const QueryDev *__restrict__ queries,
+                                        const PairDev *__restrict__ pairs, const uint8_t *__restrict__ arena,
+                                        const uint64_t *__restrict__ toff, const uint32_t *__restrict__ tlen,
+                                        typename P::L *lds, int lds_stride, uint4 *myb, typename P::T *edge,
+                                        uint64_t passbuf_half, typename P::T e0, typename P::T vopen,
+                                        typename P::T vext, PairOut *__restrict__ pout, uint64_t *__restrict__ top4,
+                                        int32_t *__restrict__ qend, int32_t *__restrict__ tend, int lane) {
+  using T = typename P::T;
+  using L = typename P::L;
+  using K = typename P::K;
+  constexpr int kPer = P::kPer;
+  const QueryDev q = queries[u.query];
+  const L *prof = prof_of<P>(q);
+  GroupCtx<P> c;
+  c.G = (int)u.G;
+  c.l = lane & (c.G - 1);
+  c.g = lane / c.G;
+  const int GR = c.G * R;
+  c.n_pad = (int)u.n_pad;
+  c.m = (int)q.m;
+  c.m_pad = (int)u.passes * GR;
+
+  // this group's pair slots
+  uint32_t outi[2] = {0, 0};
+  bool valid[2] = {false, false};
+  c.len[0] = c.len[1] = 0;
+  c.tp[0] = c.tp[1] = arena;
+#pragma unroll
+  for (int x = 0; x < kPer; x++) {
+    const uint32_t slot = (uint32_t)(c.g * kPer + x);
+    if (slot < u.npairs) {
+      const PairDev pd = pairs[u.pair0 + slot];
+      valid[x] = true;
+      c.len[x] = (int)tlen[pd.tid];
+      c.tp[x] = arena + toff[pd.tid];
+      outi[x] = pd.out;
+    }
+  }
+#pragma unroll
+  for (int x = 0; x < 2; x++) c.has[x] = valid[x] && c.len[x] > 0 && c.m > 0;
+
+  K top[kPer][4];
+#pragma unroll
+  for (int x = 0; x < kPer; x++)
+#pragma unroll
+    for (int k = 0; k < 4; k++) top[x][k] = 0;
+
+  T best = 0;
+  int bj = 0x7fffffff, bi = 0;
+  for (int p = 0; p < (int)u.passes; p++) {
+    const int base = p * GR;
+    // stage the profile rows [base, base + GR) of all codes into LDS
+    __syncthreads();
+    if (sizeof(L) == 2) {
+      const int dw = GR / 2;
+      for (int idx = lane; idx < kCodes * dw; idx += 64) {
+        const int cc = idx / dw, r = idx - cc * dw;
+        const uint32_t v = *(const uint32_t *)(prof + (uint64_t)cc * q.ld + base + 2 * r);
+        *(uint32_t *)(lds + cc * lds_stride + 2 * r) = v;
+      }
+    } else {
+      for (int idx = lane; idx < kCodes * GR; idx += 64) {
+        const int cc = idx / GR, r = idx - cc * GR;
+        lds[cc * lds_stride + r] = prof[(uint64_t)cc * q.ld + base + r];
+      }
+    }
+    __syncthreads();
+    const uint4 *rd = myb + (uint64_t)((p + 1) & 1) * passbuf_half + (uint64_t)c.g * c.n_pad;
+    uint4 *wr = myb + (uint64_t)(p & 1) * passbuf_half + (uint64_t)c.g * c.n_pad;
+    const bool final_pass = p == (int)u.passes - 1;
+    run_pass_sel<P, R, UNG, END>(UNG && final_pass, c, lds, lds_stride, base, p == 0, final_pass, rd, wr, e0,
+                                 vopen, vext, best, top, edge, lane, bj, bi);
+  }
+  // best and top-4 over the group's lanes
+  for (int off = 1; off < c.G; off <<= 1) {
+    const T o = (T)__shfl_xor((int)best, off);
+    if (END) {
+      // (score desc, column asc, row asc)
+      const int oj = __shfl_xor(bj, off), oi = __shfl_xor(bi, off);
+      if ((int)o > (int)best || ((int)o == (int)best && (oj < bj || (oj == bj && oi < bi)))) {
+        bj = oj;
+        bi = oi;
+      }
+    }
+    best = P::max(best, o);
+    if (UNG) {
+#pragma unroll
+      for (int x = 0; x < kPer; x++) {
+        K other[4];
+#pragma unroll
+        for (int k = 0; k < 4; k++) other[k] = P::shfl_xor(top[x][k], off);
+#pragma unroll
+        for (int k = 0; k < 4; k++) insert4<K>(top[x], other[k]);
+      }
+    }
+  }
+  if (c.l == 0) {
+#pragma unroll
+    for (int x = 0; x < kPer; x++)
+      if (valid[x]) {
+        PairOut po;
+        po.swg = P::half(best, x);
+        // 0: exact int16; 1: int16 saturated (rescore); 2: exact int32
+        po.flag = kPer == 2 ? (po.swg >= 32767 ? 1 : 0) : 2;
+        pout[outi[x]] = po;
+        if (END) {
+          qend[outi[x]] = (int)best > 0 ? bi : -1;
+          tend[outi[x]] = (int)best > 0 ? bj : -1;
+        }
+        if (UNG) {
+#pragma unroll
+          for (int k = 0; k < 4; k++) top4[(uint64_t)outi[x] * 4 + k] = P::key64(top[x][k]);
+        }
+      }
+  }
+}
+
 // Persistent DP kernel: every block (one wave) pulls units from *counter.
+// R = 8 or 16: rows per lane of every unit; R = 0: each unit's own R (8 or
+// 16), so one launch holds units of both heights.
 // END (int32 kernels without the evidence): also the end cell of the
 // optimum, first in column-major order, into qend / tend.
 template <class P, int R, bool UNG, bool END>
-__global__ __launch_bounds__(64, (R == 8 && !END) ? 4 : 2) void dp_kernel(
+__global__ __launch_bounds__(64, (R == 8 && !END) ? 4 : (R == 0 ? 3 : 2)) void dp_kernel(
     const Unit *__restrict__ units, uint32_t nunits, uint32_t *__restrict__ counter,
     const PairDev *__restrict__ pairs, const QueryDev *__restrict__ queries,
     const uint8_t *__restrict__ arena, const uint64_t *__restrict__ toff,
@@ -59,8 +180,6 @@ __global__ __launch_bounds__(64, (R == 8 && !END) ? 4 : 2) void dp_kernel(
     uint64_t *__restrict__ top4, int32_t *__restrict__ qend, int32_t *__restrict__ tend, int lds_stride) {
   using T = typename P::T;
   using L = typename P::L;
-  using K = typename P::K;
-  constexpr int kPer = P::kPer;
   // the pass's profile slice: kCodes rows of lds_stride (>= G*R + 2) entries
   extern __shared__ __align__(16) unsigned char lds_raw[];
   L *const lds = (L *)lds_raw;
@@ -80,109 +199,16 @@ __global__ __launch_bounds__(64, (R == 8 && !END) ? 4 : 2) void dp_kernel(
     const uint32_t ui = __builtin_amdgcn_readfirstlane(__shfl(got, 0));
     if (ui >= nunits) break;
     const Unit u = units[ui];
-    const QueryDev q = queries[u.query];
-    const L *prof = prof_of<P>(q);
-    GroupCtx<P> c;
-    c.G = (int)u.G;
-    c.l = lane & (c.G - 1);
-    c.g = lane / c.G;
-    const int GR = c.G * R;
-    c.n_pad = (int)u.n_pad;
-    c.m = (int)q.m;
-    c.m_pad = (int)u.passes * GR;
-
-    // this group's pair slots
-    uint32_t outi[2] = {0, 0};
-    bool valid[2] = {false, false};
-    c.len[0] = c.len[1] = 0;
-    c.tp[0] = c.tp[1] = arena;
-#pragma unroll
-    for (int x = 0; x < kPer; x++) {
-      const uint32_t slot = (uint32_t)(c.g * kPer + x);
-      if (slot < u.npairs) {
-        const PairDev pd = pairs[u.pair0 + slot];
-        valid[x] = true;
-        c.len[x] = (int)tlen[pd.tid];
-        c.tp[x] = arena + toff[pd.tid];
-        outi[x] = pd.out;
-      }
-    }
-#pragma unroll
-    for (int x = 0; x < 2; x++) c.has[x] = valid[x] && c.len[x] > 0 && c.m > 0;
-
-    K top[kPer][4];
-#pragma unroll
-    for (int x = 0; x < kPer; x++)
-#pragma unroll
-      for (int k = 0; k < 4; k++) top[x][k] = 0;
-
-    T best = 0;
-    int bj = 0x7fffffff, bi = 0;
-    for (int p = 0; p < (int)u.passes; p++) {
-      const int base = p * GR;
-      // stage the profile rows [base, base + GR) of all codes into LDS
-      __syncthreads();
-      if (sizeof(L) == 2) {
-        const int dw = GR / 2;
-        for (int idx = lane; idx < kCodes * dw; idx += 64) {
-          const int cc = idx / dw, r = idx - cc * dw;
-          const uint32_t v = *(const uint32_t *)(prof + (uint64_t)cc * q.ld + base + 2 * r);
-          *(uint32_t *)(lds + cc * lds_stride + 2 * r) = v;
-        }
-      } else {
-        for (int idx = lane; idx < kCodes * GR; idx += 64) {
-          const int cc = idx / GR, r = idx - cc * GR;
-          lds[cc * lds_stride + r] = prof[(uint64_t)cc * q.ld + base + r];
-        }
-      }
-      __syncthreads();
-      const uint4 *rd = myb + (uint64_t)((p + 1) & 1) * passbuf_half + (uint64_t)c.g * c.n_pad;
-      uint4 *wr = myb + (uint64_t)(p & 1) * passbuf_half + (uint64_t)c.g * c.n_pad;
-      const bool final_pass = p == (int)u.passes - 1;
-      run_pass_sel<P, R, UNG, END>(UNG && final_pass, c, lds, lds_stride, base, p == 0, final_pass, rd, wr, e0,
-                                   vopen, vext, best, top, edge, lane, bj, bi);
-    }
-    // best and top-4 over the group's lanes
-    for (int off = 1; off < c.G; off <<= 1) {
-      const T o = (T)__shfl_xor((int)best, off);
-      if (END) {
-        // (score desc, column asc, row asc)
-        const int oj = __shfl_xor(bj, off), oi = __shfl_xor(bi, off);
-        if ((int)o > (int)best || ((int)o == (int)best && (oj < bj || (oj == bj && oi < bi)))) {
-          bj = oj;
-          bi = oi;
-        }
-      }
-      best = P::max(best, o);
-      if (UNG) {
-#pragma unroll
-        for (int x = 0; x < kPer; x++) {
-          K other[4];
-#pragma unroll
-          for (int k = 0; k < 4; k++) other[k] = P::shfl_xor(top[x][k], off);
-#pragma unroll
-          for (int k = 0; k < 4; k++) insert4<K>(top[x], other[k]);
-        }
-      }
-    }
-    if (c.l == 0) {
-#pragma unroll
-      for (int x = 0; x < kPer; x++)
-        if (valid[x]) {
-          PairOut po;
-          po.swg = P::half(best, x);
-          // 0: exact int16; 1: int16 saturated (rescore); 2: exact int32
-          po.flag = kPer == 2 ? (po.swg >= 32767 ? 1 : 0) : 2;
-          pout[outi[x]] = po;
-          if (END) {
-            qend[outi[x]] = (int)best > 0 ? bi : -1;
-            tend[outi[x]] = (int)best > 0 ? bj : -1;
-          }
-          if (UNG) {
-#pragma unroll
-            for (int k = 0; k < 4; k++) top4[(uint64_t)outi[x] * 4 + k] = P::key64(top[x][k]);
-          }
-        }
+    if constexpr (R == 0) {
+      if (u.R == 16)
+        dp_unit<P, 16, UNG, END>(u, queries, pairs, arena, toff, tlen, lds, lds_stride, myb, edge, passbuf_half, e0,
+                                 vopen, vext, pout, top4, qend, tend, lane);
+      else
+        dp_unit<P, 8, UNG, END>(u, queries, pairs, arena, toff, tlen, lds, lds_stride, myb, edge, passbuf_half, e0,
+                                vopen, vext, pout, top4, qend, tend, lane);
+    } else {
+      dp_unit<P, R, UNG, END>(u, queries, pairs, arena, toff, tlen, lds, lds_stride, myb, edge, passbuf_half, e0,
+                              vopen, vext, pout, top4, qend, tend, lane);
     }
   }
 }
@@ -223,7 +249,9 @@ int dp_occupancy(int kind, int R, bool ung, size_t lds_bytes, bool end) {
     else if (R == 8) ST_OCC(I32F, 8, false, true);
     else ST_OCC(I32F, 16, false, true);
   } else if (kind == 0) {
-    if (R == 8) {
+    if (R == 0) {
+      if (ung) ST_OCC(Pk16, 0, true, false); else ST_OCC(Pk16, 0, false, false);
+    } else if (R == 8) {
       if (ung) ST_OCC(Pk16, 8, true, false); else ST_OCC(Pk16, 8, false, false);
     } else {
       if (ung) ST_OCC(Pk16, 16, true, false); else ST_OCC(Pk16, 16, false, false);
@@ -255,7 +283,9 @@ hipError_t launch_dp(const DpLaunch &a, int kind, int R, bool ung, hipStream_t s
     else if (R == 8) ST_DP(I32F, 8, false, true);
     else ST_DP(I32F, 16, false, true);
   } else if (kind == 0) {
-    if (R == 8) {
+    if (R == 0) {
+      if (ung) ST_DP(Pk16, 0, true, false); else ST_DP(Pk16, 0, false, false);
+    } else if (R == 8) {
       if (ung) ST_DP(Pk16, 8, true, false); else ST_DP(Pk16, 8, false, false);
     } else {
       if (ung) ST_DP(Pk16, 16, true, false); else ST_DP(Pk16, 16, false, false);
