@@ -106,7 +106,7 @@ __device__ void serve_poller(const ServeArgs &a, int lane) {
 }
 
 template <int R>
-__global__ __launch_bounds__(64) void serve_kernel(ServeArgs a) {
+__global__ __launch_bounds__(64, 3) void serve_kernel(ServeArgs a) {
   using P = Pk16;
   using T = P::T;
   using K = P::K;
