@@ -457,7 +457,7 @@ void finish_plan(Plan &pl, std::vector<Unit> &pending, const std::vector<Kind> &
       j++;
     }
     L.nunits = (uint32_t)(j - i);
-    const size_t lds = (size_t)kCodes * (L.gr_max + 2) * (L.kind == kPk16 ? 2 : 4);
+    const size_t lds = dp_lds_bytes(L.gr_max + 2, L.kind == kPk16 ? 2 : 4, ung && !end);
     const int occ = std::max(1, occupancy_cached(L.kind, L.R, ung, lds, end));
     L.grid = std::max<uint32_t>(1, std::min<uint32_t>(L.nunits, (uint32_t)(occ * cus)));
     L.slot = (int)(pl.launches.size() % kLaunchSlots);
@@ -489,11 +489,9 @@ struct st_batch {
   DevBuf<int32_t> d_open, d_ext, d_gate;
   DevBuf<Unit> d_units;
   DevBuf<PairDev> d_pairs;
-  DevBuf<PairDev> d_allpairs;  // candidate order, for the gate kernel
-  DevBuf<uint32_t> d_candq;
-  DevBuf<PairOut> d_pout;
-  DevBuf<uint64_t> d_top4;
-  DevBuf<uint8_t> d_res;
+  DevBuf<PairOut> d_pout;       // score-only batches
+  DevBuf<uint8_t> d_res;        // evidence batches: st_pair_result rows
+  DevBuf<uint32_t> d_sat;       // int16 pairs at the clamp: count, then rows
   DevBuf<uint4> d_passbuf;
   // exact int32 rescoring of saturated int16 pairs (built at run time)
   Plan resc;
@@ -501,7 +499,8 @@ struct st_batch {
   DevBuf<PairDev> d_rs_pairs;
   DevBuf<uint4> d_rs_passbuf;
   uint64_t rescored = 0;
-  // device time of the last run: main DP launches, int32 rescoring, gate
+  // device time of the last run: main DP launches, int32 rescoring, and a
+  // third phase kept at 0 (the gate runs inside the DP launches)
   hipEvent_t ev[6] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
   hipEvent_t join[kLaunchSlots] = {nullptr, nullptr, nullptr, nullptr};
   double ms[3] = {0, 0, 0};
@@ -546,10 +545,14 @@ static int launch_plan(st_batch *b, const Plan &pl, const Unit *d_units, const P
     a.grid = L.grid;
     a.passbuf = d_buf + L.buf0;
     a.passbuf_half = L.half;
-    a.pout = b->d_pout.p;
-    a.top4 = b->d_top4.p;
-    a.qend = b->d_qend.p;
-    a.tend = b->d_tend.p;
+    a.out.pout = b->d_pout.p;
+    a.out.res = (PairResultDev *)b->d_res.p;
+    a.out.qgate = b->d_gate.p;
+    // only the main plan's packed int16 pairs can reach the clamp
+    a.out.sat = &pl == &b->plan ? b->d_sat.p : nullptr;
+    a.out.sat_cap = (uint32_t)b->npairs;
+    a.out.qend = b->d_qend.p;
+    a.out.tend = b->d_tend.p;
     a.lds_stride = L.gr_max + 2;
     const bool timed = &pl == &b->plan && b->lev.size() == 2 * pl.launches.size();
     const size_t li = (size_t)(&L - pl.launches.data());
@@ -645,22 +648,15 @@ static int batch_create_impl(st_ctx *ctx, const st_db *db, const st_qprof *const
   }
   std::vector<QueryDev> qv(nq);
   for (uint64_t q = 0; q < nq; q++) qv[q] = b->qps[q]->view;
-  std::vector<PairDev> all(np);
-  for (uint64_t k = 0; k < np; k++) {
-    all[k].tid = cand_tids[k];
-    all[k].out = (uint32_t)k;
-  }
   bool ok = b->d_queries.upload(qv.data(), qv.size(), ctx->stream) == hipSuccess &&
             b->d_open.upload(b->open.data(), nq, ctx->stream) == hipSuccess &&
             b->d_ext.upload(b->ext.data(), nq, ctx->stream) == hipSuccess &&
             b->d_gate.upload(b->gate.data(), nq, ctx->stream) == hipSuccess &&
             b->d_units.upload(b->plan.units.data(), b->plan.units.size(), ctx->stream) == hipSuccess &&
             b->d_pairs.upload(b->plan.pairs.data(), b->plan.pairs.size(), ctx->stream) == hipSuccess &&
-            b->d_allpairs.upload(all.data(), all.size(), ctx->stream) == hipSuccess &&
-            b->d_candq.upload(b->cand_q.data(), b->cand_q.size(), ctx->stream) == hipSuccess &&
             b->d_pout.alloc(std::max<uint64_t>(np, 1)) == hipSuccess &&
-            b->d_top4.alloc(ung ? std::max<uint64_t>(np, 1) * 4 : 1) == hipSuccess &&
             b->d_res.alloc(std::max<uint64_t>(np, 1) * sizeof(st_pair_result)) == hipSuccess &&
+            b->d_sat.alloc(np + 1) == hipSuccess &&
             b->d_passbuf.alloc(std::max<uint64_t>(b->plan.buf_elems, 1)) == hipSuccess &&
             (!end || (b->d_qend.alloc(std::max<uint64_t>(np, 1)) == hipSuccess &&
                       b->d_tend.alloc(std::max<uint64_t>(np, 1)) == hipSuccess));
@@ -690,6 +686,7 @@ static int batch_run_locked(st_batch *b) {
     b->lev.assign(2 * b->plan.launches.size(), nullptr);
     for (hipEvent_t &e : b->lev) ST_HIP(hipEventCreate(&e));
   }
+  ST_HIP(clear_u32(b->d_sat.p, 1, ctx->stream));
   ST_HIP(hipEventRecord(b->ev[0], ctx->stream));
   int rc = launch_plan(b, b->plan, b->d_units.p, b->d_pairs.p, b->d_passbuf.p, b->ev[0]);
   if (rc) return rc;
@@ -702,13 +699,20 @@ static int batch_run_locked(st_batch *b) {
   bool any16 = false;
   for (const Launch &L : b->plan.launches) any16 |= L.kind == kPk16;
   if (any16) {
-    std::vector<PairOut> po(b->npairs);
-    ST_HIP(hipMemcpyAsync(po.data(), b->d_pout.p, b->npairs * sizeof(PairOut), hipMemcpyDeviceToHost,
-                          ctx->stream));
+    // the DP launches listed the result rows of int16 pairs at the clamp
+    uint32_t nsat = 0;
+    ST_HIP(hipMemcpyAsync(&nsat, b->d_sat.p, sizeof nsat, hipMemcpyDeviceToHost, ctx->stream));
     ST_HIP(hipStreamSynchronize(ctx->stream));
+    nsat = (uint32_t)std::min<uint64_t>(nsat, b->npairs);
+    std::vector<uint32_t> rows(nsat);
+    if (nsat) {
+      ST_HIP(hipMemcpyAsync(rows.data(), b->d_sat.p + 1, nsat * sizeof(uint32_t), hipMemcpyDeviceToHost,
+                            ctx->stream));
+      ST_HIP(hipStreamSynchronize(ctx->stream));
+      std::sort(rows.begin(), rows.end());
+    }
     std::vector<std::vector<uint64_t>> perq(b->qps.size());
-    for (uint64_t k = 0; k < b->npairs; k++)
-      if (po[k].flag == 1) perq[b->cand_q[k]].push_back(k);
+    for (uint32_t k : rows) perq[b->cand_q[k]].push_back(k);
     std::vector<Unit> pending;
     for (size_t q = 0; q < perq.size(); q++) {
       if (perq[q].empty()) continue;
@@ -731,23 +735,6 @@ static int batch_run_locked(st_batch *b) {
     }
   }
   ST_HIP(hipEventRecord(b->ev[4], ctx->stream));
-  if (b->ung && b->npairs) {
-    PairGateArgs ga;
-    ga.pairs = b->d_allpairs.p;
-    ga.npairs = b->npairs;
-    ga.pair_query = b->d_candq.p;
-    ga.queries = b->d_queries.p;
-    ga.arena = b->db->arena.p;
-    ga.toff = b->db->off.p;
-    ga.tlen = b->db->len.p;
-    ga.top4 = b->d_top4.p;
-    ga.open = b->d_open.p;
-    ga.ext = b->d_ext.p;
-    ga.gate = b->d_gate.p;
-    ga.pout = b->d_pout.p;
-    ga.res = b->d_res.p;
-    ST_HIP(launch_gate_pairs(ga, ctx->stream));
-  }
   ST_HIP(hipEventRecord(b->ev[5], ctx->stream));
   ST_HIP(hipStreamSynchronize(ctx->stream));
   float t = 0;

@@ -67,6 +67,7 @@ struct Pk16 {
     return k ? diag_key((int32_t)(k >> 17), 0x1FFFFu - (k & 0x1FFFFu)) : 0ull;
   }
   __device__ static K shfl_xor(K k, int off) { return (K)__shfl_xor((int)k, off); }
+  __device__ static K shfl(K k, int src) { return (K)__shfl((int)k, src); }
 };
 
 // One pair per group in int32, E/F floored at 0 (needs ext >= 0, open >= 0).
@@ -89,6 +90,11 @@ struct I32F {
   __device__ static K shfl_xor(K k, int off) {
     const uint32_t lo = (uint32_t)__shfl_xor((int)(uint32_t)k, off);
     const uint32_t hi = (uint32_t)__shfl_xor((int)(uint32_t)(k >> 32), off);
+    return ((uint64_t)hi << 32) | lo;
+  }
+  __device__ static K shfl(K k, int src) {
+    const uint32_t lo = (uint32_t)__shfl((int)(uint32_t)k, src);
+    const uint32_t hi = (uint32_t)__shfl((int)(uint32_t)(k >> 32), src);
     return ((uint64_t)hi << 32) | lo;
   }
 };

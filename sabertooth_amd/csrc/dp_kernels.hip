@@ -43,8 +43,12 @@
 #include <hip/hip_runtime.h>
 
 #include "dp_core.h"
+#include "dp_launch.h"
+#include "gate_core.h"
 
 namespace st {
+
+static_assert(kGateLdsBytes <= 6144, "the gate scratch must fit the evidence launches' minimum LDS");
 
 // One unit: every pass of its groups, then the merged outputs.
 template <class P, int R, bool UNG, bool END>
@@ -53,8 +57,8 @@ __device__ __forceinline__ void dp_unit(const Unit &u, const QueryDev *__restric
                                         const uint64_t *__restrict__ toff, const uint32_t *__restrict__ tlen,
                                         typename P::L *lds, int lds_stride, uint4 *myb, typename P::T *edge,
                                         uint64_t passbuf_half, typename P::T e0, typename P::T vopen,
-                                        typename P::T vext, PairOut *__restrict__ pout, uint64_t *__restrict__ top4,
-                                        int32_t *__restrict__ qend, int32_t *__restrict__ tend, int lane) {
+                                        typename P::T vext, int32_t gap_open, int32_t gap_ext, const DpOut &o,
+                                        int lane) {
   using T = typename P::T;
   using L = typename P::L;
   using K = typename P::K;
@@ -144,7 +148,42 @@ __device__ __forceinline__ void dp_unit(const Unit &u, const QueryDev *__restric
       }
     }
   }
-  if (c.l == 0) {
+  if (UNG) {
+    // the gate of each pair slot s on lane s mod 64, from the merged keys of
+    // its group (every lane of a group holds the merge), then the pair's
+    // final result row
+    for (int s0 = 0; s0 < (int)u.npairs; s0 += 64) {
+      const int s = s0 + lane;
+      const int x = s % kPer;
+      const int src = ((s / kPer) * c.G) & 63;
+      const T bs = (T)__shfl((int)best, src);
+      uint64_t key[4];
+#pragma unroll
+      for (int k = 0; k < 4; k++) {
+        const K k0 = P::shfl(top[0][k], src);
+        const K k1 = kPer == 2 ? P::shfl(top[kPer - 1][k], src) : k0;
+        key[k] = P::key64(x ? k1 : k0);
+      }
+      const bool act = s < (int)u.npairs;
+      const PairDev pd = act ? pairs[u.pair0 + s] : PairDev{0, 0};
+      PairResultDev r;
+      r.swg = P::half(bs, x);
+      // the profile region of LDS is free until the next unit's first pass
+      gate_wave(ProfGlobal{q.prof32, q.ld}, act, arena + toff[pd.tid], c.m, act ? (int)tlen[pd.tid] : 0, key,
+                gap_open, gap_ext, lane, (GateLds *)lds, r.ung, r.best_diag, r.gate_score);
+      if (act) {
+        r.pass = r.gate_score >= o.qgate[u.query];
+        r.kernel = kPer == 2 ? 0 : 1;  // ST_KERNEL_I16 / ST_KERNEL_I32
+        r.r0 = r.r1 = 0;
+        o.res[pd.out] = r;
+        if (kPer == 2 && r.swg >= 32767 && o.sat) {
+          // at the int16 clamp: listed for exact int32 rescoring
+          const uint32_t k = atomicAdd(o.sat, 1u);
+          if (k < o.sat_cap) o.sat[1 + k] = pd.out;
+        }
+      }
+    }
+  } else if (c.l == 0) {
 #pragma unroll
     for (int x = 0; x < kPer; x++)
       if (valid[x]) {
@@ -152,14 +191,14 @@ __device__ __forceinline__ void dp_unit(const Unit &u, const QueryDev *__restric
         po.swg = P::half(best, x);
         // 0: exact int16; 1: int16 saturated (rescore); 2: exact int32
         po.flag = kPer == 2 ? (po.swg >= 32767 ? 1 : 0) : 2;
-        pout[outi[x]] = po;
-        if (END) {
-          qend[outi[x]] = (int)best > 0 ? bi : -1;
-          tend[outi[x]] = (int)best > 0 ? bj : -1;
+        o.pout[outi[x]] = po;
+        if (po.flag == 1 && o.sat) {
+          const uint32_t k = atomicAdd(o.sat, 1u);
+          if (k < o.sat_cap) o.sat[1 + k] = outi[x];
         }
-        if (UNG) {
-#pragma unroll
-          for (int k = 0; k < 4; k++) top4[(uint64_t)outi[x] * 4 + k] = P::key64(top[x][k]);
+        if (END) {
+          o.qend[outi[x]] = (int)best > 0 ? bi : -1;
+          o.tend[outi[x]] = (int)best > 0 ? bj : -1;
         }
       }
   }
@@ -176,8 +215,7 @@ __global__ __launch_bounds__(64, (R == 8 && !END) ? 4 : (R == 0 ? 3 : 2)) void d
     const PairDev *__restrict__ pairs, const QueryDev *__restrict__ queries,
     const uint8_t *__restrict__ arena, const uint64_t *__restrict__ toff,
     const uint32_t *__restrict__ tlen, int32_t gap_open, int32_t gap_ext,
-    uint4 *__restrict__ passbuf, uint64_t passbuf_half, PairOut *__restrict__ pout,
-    uint64_t *__restrict__ top4, int32_t *__restrict__ qend, int32_t *__restrict__ tend, int lds_stride) {
+    uint4 *__restrict__ passbuf, uint64_t passbuf_half, DpOut out, int lds_stride) {
   using T = typename P::T;
   using L = typename P::L;
   // the pass's profile slice: kCodes rows of lds_stride (>= G*R + 2) entries
@@ -202,13 +240,13 @@ __global__ __launch_bounds__(64, (R == 8 && !END) ? 4 : (R == 0 ? 3 : 2)) void d
     if constexpr (R == 0) {
       if (u.R == 16)
         dp_unit<P, 16, UNG, END>(u, queries, pairs, arena, toff, tlen, lds, lds_stride, myb, edge, passbuf_half, e0,
-                                 vopen, vext, pout, top4, qend, tend, lane);
+                                 vopen, vext, gap_open, gap_ext, out, lane);
       else
         dp_unit<P, 8, UNG, END>(u, queries, pairs, arena, toff, tlen, lds, lds_stride, myb, edge, passbuf_half, e0,
-                                vopen, vext, pout, top4, qend, tend, lane);
+                                vopen, vext, gap_open, gap_ext, out, lane);
     } else {
       dp_unit<P, R, UNG, END>(u, queries, pairs, arena, toff, tlen, lds, lds_stride, myb, edge, passbuf_half, e0,
-                              vopen, vext, pout, top4, qend, tend, lane);
+                              vopen, vext, gap_open, gap_ext, out, lane);
     }
   }
 }
@@ -220,24 +258,6 @@ __global__ void zero_u32(uint32_t *p, uint32_t n) {
   if (i < n) p[i] = 0;
 }
 
-struct DpLaunch {
-  const Unit *units;
-  uint32_t nunits;
-  uint32_t *counter;
-  const PairDev *pairs;
-  const QueryDev *queries;
-  const uint8_t *arena;
-  const uint64_t *toff;
-  const uint32_t *tlen;
-  int32_t open, ext;
-  uint4 *passbuf;
-  uint64_t passbuf_half;
-  PairOut *pout;
-  uint64_t *top4;
-  int32_t *qend, *tend;  // end cells (END launches)
-  uint32_t grid;
-  int lds_stride;  // >= largest G*R of the launch + 2, even
-};
 
 // Resident blocks per CU of a kernel instance with `lds_bytes` of dynamic LDS.
 int dp_occupancy(int kind, int R, bool ung, size_t lds_bytes, bool end) {
@@ -271,13 +291,19 @@ int dp_occupancy(int kind, int R, bool ung, size_t lds_bytes, bool end) {
 
 // kind: 0 = Pk16, 1 = I32F, 2 = I32N; R: 8 or 16 (I32N: 8); end: the
 // score + end-cell instances (int32, no evidence)
+hipError_t clear_u32(uint32_t *p, uint32_t n, hipStream_t s) {
+  if (n == 0) return hipSuccess;
+  hipLaunchKernelGGL(zero_u32, dim3((n + 63) / 64), dim3(64), 0, s, p, n);
+  return hipGetLastError();
+}
+
 hipError_t launch_dp(const DpLaunch &a, int kind, int R, bool ung, hipStream_t s, bool end) {
   hipLaunchKernelGGL(zero_u32, dim3(1), dim3(64), 0, s, a.counter, 1u);
 #define ST_DP(PP, RR, UU, EE)                                                                       \
   hipLaunchKernelGGL((dp_kernel<PP, RR, UU, EE>), dim3(a.grid), dim3(64),                            \
-                     (size_t)kCodes * a.lds_stride * sizeof(typename PP::L), s, a.units, a.nunits,   \
+                     dp_lds_bytes(a.lds_stride, sizeof(typename PP::L), UU), s, a.units, a.nunits,   \
                      a.counter, a.pairs, a.queries, a.arena, a.toff, a.tlen, a.open, a.ext, a.passbuf, \
-                     a.passbuf_half, a.pout, a.top4, a.qend, a.tend, a.lds_stride)
+                     a.passbuf_half, a.out, a.lds_stride)
   if (end) {
     if (kind == 2) ST_DP(I32N, 8, false, true);
     else if (R == 8) ST_DP(I32F, 8, false, true);

@@ -1,0 +1,51 @@
+// Arguments of one DP kernel launch, shared by the kernels and the host code
+// that plans and issues them.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+#include "st_device.h"
+
+namespace st {
+
+// Where a DP launch puts its per-pair outputs.
+struct DpOut {
+  PairOut *pout;         // score-only launches: score and precision flag per result row
+  PairResultDev *res;    // evidence launches: the final result row (the gate runs in the kernel)
+  const int32_t *qgate;  // evidence launches: min_diag_score per query
+  uint32_t *sat;         // packed int16 pairs at the clamp: sat[0] = count, sat[1 + k] = their result rows
+  uint32_t sat_cap;      // entries after sat[0]
+  int32_t *qend, *tend;  // score + end launches: end cell per result row
+};
+
+struct DpLaunch {
+  const Unit *units;
+  uint32_t nunits;
+  uint32_t *counter;
+  const PairDev *pairs;
+  const QueryDev *queries;
+  const uint8_t *arena;
+  const uint64_t *toff;
+  const uint32_t *tlen;
+  int32_t open, ext;
+  uint4 *passbuf;
+  uint64_t passbuf_half;
+  DpOut out;
+  uint32_t grid;
+  int lds_stride;  // >= largest G*R of the launch + 2, even
+};
+// Dynamic LDS of a DP launch: the pass's profile slice (kCodes rows of
+// lds_stride entries), and for the evidence kernels at least the gate's
+// scratch (GateLds, 6 KB), which reuses it after a unit's passes.
+inline size_t dp_lds_bytes(int lds_stride, size_t entry_bytes, bool ung) {
+  const size_t prof = (size_t)kCodes * (size_t)lds_stride * entry_bytes;
+  return ung && prof < 6144 ? 6144 : prof;
+}
+
+// end: the int32 score + end-cell instances (no evidence)
+hipError_t launch_dp(const DpLaunch &a, int kind, int R, bool ung, hipStream_t s, bool end = false);
+// n words at p set to 0 by a kernel (not a memset: safe under graph capture)
+hipError_t clear_u32(uint32_t *p, uint32_t n, hipStream_t s);
+
+}  // namespace st

@@ -1,5 +1,5 @@
-// Ungapped job scoring, the chain/gate rule and the all-diagonal reduction on
-// gfx950.
+// Ungapped job scoring and the chain/gate rule on gfx950 (the job-list
+// drop-ins; the all-pairs evidence and gate run inside the DP kernels).
 //
 //  * ung_jobs_kernel: kadane_score of explicit (target, diagonal, [lo, hi))
 //    jobs (ungapped::score_batch, src/ungapped.rs:285-386; kadane_score,
@@ -7,13 +7,9 @@
 //    residues with 16-byte loads (the arena is 16-byte aligned per target).
 //  * gate_recs_kernel: score_and_gate's per-target chain and gate
 //    (src/prefilter.rs:1577-1657) over already scored jobs.
-//  * gate_pairs_kernel: the seedless evidence of a (query, candidate) pair
-//    from its 4 best diagonals (keys left by the DP kernel): best diagonal
-//    and the same chain/gate rule over their whole ranges.
 //
-// kadane_segment scans read 16 residues per load and look scores up either
-// in an LDS copy of the query's int16 profile (when a block's pairs share one
-// query that fits) or in the int32 profile in HBM.
+// kadane_segment scans read 16 residues per load and look scores up in the
+// query's int32 profile.
 #include <hip/hip_runtime.h>
 
 #include "gate_core.h"
@@ -104,81 +100,6 @@ __global__ __launch_bounds__(128) void gate_recs_kernel(const RecDev *__restrict
   out[i] = r;
 }
 
-// All-diagonal evidence of each pair: the DP kernel leaves the 4 best
-// diagonals as sorted keys (score desc, diagonal asc); the best one is the
-// pair's ungapped score and best diagonal, and the gate runs over all four.
-struct PairGateArgs {
-  const PairDev *pairs;
-  uint64_t npairs;
-  const uint32_t *pair_query;  // query index of each pair slot
-  const QueryDev *queries;
-  const uint8_t *arena;
-  const uint64_t *toff;
-  const uint32_t *tlen;
-  const uint64_t *top4;              // 4 keys per result row
-  const int32_t *open, *ext, *gate;  // per query
-  PairOut *pout;
-  void *res;  // st_pair_result rows (by PairDev::out)
-};
-
-// query rows held in LDS by gate_pairs_kernel
-constexpr int kGateLdsRows = 512;
-
-__global__ __launch_bounds__(64) void gate_pairs_kernel(PairGateArgs a) {
-  __shared__ int16_t lp[kCodes * kGateLdsRows];
-  const uint64_t base = (uint64_t)blockIdx.x * 64;
-  const uint64_t i = base + threadIdx.x;
-  const uint64_t last = base + 63 < a.npairs ? base + 63 : a.npairs - 1;
-  const uint32_t q0 = a.pair_query[base];
-  const QueryDev Q = a.queries[q0];
-  // one query for the whole block, short enough, with an int16 profile
-  const bool uni = a.pair_query[last] == q0 && Q.prof16 != nullptr && Q.m <= (uint32_t)kGateLdsRows;
-  if (uni) {
-    const int m = (int)Q.m;
-    for (int idx = threadIdx.x; idx < kCodes * m; idx += 64) {
-      const int c = idx / m, r = idx - c * m;
-      lp[c * m + r] = Q.prof16[(uint64_t)c * Q.ld + r];
-    }
-  }
-  __syncthreads();
-  if (i >= a.npairs) return;
-  const PairDev pd = a.pairs[i];
-  const uint32_t qi = a.pair_query[i];
-  const QueryDev q = uni ? Q : a.queries[qi];
-  const int m = (int)q.m;
-  const int n = (int)a.tlen[pd.tid];
-  int32_t ts[4], td[4], lo[4], hi[4];
-  int nt = 0;
-  for (int k = 0; k < 4; k++) {
-    const uint64_t key = a.top4[(uint64_t)pd.out * 4 + k];
-    if (key == 0) break;
-    ts[k] = (int32_t)(key >> 32);
-    td[k] = (int32_t)(0xFFFFFFFFu - (uint32_t)key) - (m - 1);
-    // whole diagonal (diag_range, src/ungapped.rs:38-46)
-    const int d = td[k];
-    const int a0 = -d > 0 ? -d : 0;
-    const int b0 = m < n - d ? m : n - d;
-    lo[k] = a0;
-    hi[k] = b0 > a0 ? b0 : a0;
-    nt++;
-  }
-  const uint8_t *t = a.arena + a.toff[pd.tid];
-  const GateRes r = uni ? gate_eval(ProfLds{lp, m}, t, ts, td, lo, hi, nt, a.open[qi], a.ext[qi], INT32_MIN, false)
-                        : gate_eval(ProfGlobal{q.prof32, q.ld}, t, ts, td, lo, hi, nt, a.open[qi], a.ext[qi],
-                                    INT32_MIN, false);
-  const int32_t g = a.gate[qi];
-  PairResultDev o;
-  const PairOut po = a.pout[pd.out];
-  o.swg = po.swg;
-  o.ung = nt ? ts[0] : 0;
-  o.best_diag = nt ? td[0] : 0;
-  o.gate_score = nt ? r.score : 0;
-  o.pass = (nt ? r.score : 0) >= g;
-  o.kernel = po.flag == 2 ? 1 : 0;  // ST_KERNEL_I32 when the int32 pass produced swg
-  o.r0 = o.r1 = 0;
-  ((PairResultDev *)a.res)[pd.out] = o;
-}
-
 hipError_t launch_ung_jobs(const JobDev *jobs, uint64_t njobs, const QueryDev *qd, const uint8_t *arena,
                            const uint64_t *toff, int32_t *out, hipStream_t s) {
   if (njobs == 0) return hipSuccess;
@@ -194,13 +115,6 @@ hipError_t launch_gate_recs(const RecDev *recs, uint64_t nrecs, const JobDev *jo
   const uint64_t blocks = (nrecs + 127) / 128;
   hipLaunchKernelGGL(gate_recs_kernel, dim3((uint32_t)blocks), dim3(128), 0, s, recs, nrecs, jobs, scores, qd,
                      arena, toff, open, ext, gate, regions, out);
-  return hipGetLastError();
-}
-
-hipError_t launch_gate_pairs(const PairGateArgs &a, hipStream_t s) {
-  if (a.npairs == 0) return hipSuccess;
-  const uint64_t blocks = (a.npairs + 63) / 64;
-  hipLaunchKernelGGL(gate_pairs_kernel, dim3((uint32_t)blocks), dim3(64), 0, s, a);
   return hipGetLastError();
 }
 

@@ -11,30 +11,11 @@
 
 #include "../../include/sabertooth_amd.h"
 #include "st_device.h"
+#include "dp_launch.h"
 
 namespace st {
 
 // ---- kernel entry points (dp_kernels.hip, gate_kernels.hip) -----------------
-struct DpLaunch {
-  const Unit *units;
-  uint32_t nunits;
-  uint32_t *counter;
-  const PairDev *pairs;
-  const QueryDev *queries;
-  const uint8_t *arena;
-  const uint64_t *toff;
-  const uint32_t *tlen;
-  int32_t open, ext;
-  uint4 *passbuf;
-  uint64_t passbuf_half;
-  PairOut *pout;
-  uint64_t *top4;
-  int32_t *qend, *tend;  // end cells (score + end launches), else null
-  uint32_t grid;
-  int lds_stride;
-};
-// end: the int32 score + end-cell instances (no evidence)
-hipError_t launch_dp(const DpLaunch &a, int kind, int R, bool ung, hipStream_t s, bool end = false);
 
 struct GateRes {
   int32_t score, best, best_diag;
@@ -49,19 +30,6 @@ struct JobDev {
 struct RecDev {
   uint32_t tid, hits, j0, j1;
 };
-struct PairGateArgs {
-  const PairDev *pairs;
-  uint64_t npairs;
-  const uint32_t *pair_query;
-  const QueryDev *queries;
-  const uint8_t *arena;
-  const uint64_t *toff;
-  const uint32_t *tlen;
-  const uint64_t *top4;
-  const int32_t *open, *ext, *gate;
-  PairOut *pout;
-  void *res;
-};
 hipError_t launch_ung_jobs(const JobDev *jobs, uint64_t njobs, const QueryDev *qd,
                            const uint8_t *arena, const uint64_t *toff, int32_t *out,
                            hipStream_t s);
@@ -69,7 +37,6 @@ hipError_t launch_gate_recs(const RecDev *recs, uint64_t nrecs, const JobDev *jo
                             const int32_t *scores, const QueryDev *qd, const uint8_t *arena,
                             const uint64_t *toff, int32_t open, int32_t ext, int32_t gate,
                             int regions, GateRes *out, hipStream_t s);
-hipError_t launch_gate_pairs(const PairGateArgs &a, hipStream_t s);
 
 // ---- errors -------------------------------------------------------------------
 // Record msg as this thread's last error (st_last_error) and return code.

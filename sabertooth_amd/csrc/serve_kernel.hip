@@ -23,7 +23,7 @@
 //  * The claimed unit's descriptor is read with system-scope loads (coherent
 //    with the host's writes, which preceded the tail), the unit is scored
 //    (the batch kernels' DP pass: SWG + all-diagonal Kadane, packed int16),
-//    each pair's gate runs from its group's merged keys (the gate_pairs rule)
+//    each pair's gate runs from its group's merged keys (as in the DP kernels)
 //    and the unit's result rows (consecutive, 16 bytes each) are written to
 //    host-coherent memory with system-scope (write-through) stores arranged
 //    so that each store instruction covers 256 contiguous bytes; once those
@@ -198,38 +198,18 @@ __global__ __launch_bounds__(64, 3) void serve_kernel(ServeArgs a) {
         const K k1 = (K)__shfl((int)r.top[1][k], src);
         key[k] = x ? k1 : k0;
       }
-      const PairDev pd = s < (int)u.npairs ? u.pairs[s] : PairDev{0, 0};
-      uint32_t orow = ~0u;  // this slot's result row, ~0u: none
-      int32_t val[4] = {0, 0, 0, 0};
-      if (s < (int)u.npairs && pd.out < a.result_rows && pd.tid < a.n_targets) {
-        const int m = (int)u.m;
-        const int n = (int)a.tlen[pd.tid];
-        int32_t ts[4], td[4], lo[4], hi[4];
-        int nt = 0;
+      const bool act = s < (int)u.npairs;
+      const PairDev pd = act ? u.pairs[s] : PairDev{0, 0};
+      const bool ok_row = act && pd.out < a.result_rows && pd.tid < a.n_targets;
+      uint64_t k64[4];
 #pragma unroll
-        for (int k = 0; k < 4; k++) {
-          const uint64_t k64 = P::key64(key[k]);
-          if (k64 == 0 || nt < k) continue;
-          ts[nt] = (int32_t)(k64 >> 32);
-          td[nt] = (int32_t)(0xFFFFFFFFu - (uint32_t)k64) - (m - 1);
-          // whole diagonal (diag_range, src/ungapped.rs:38-46)
-          const int d = td[nt];
-          const int a0 = -d > 0 ? -d : 0;
-          const int b0 = m < n - d ? m : n - d;
-          lo[nt] = a0;
-          hi[nt] = b0 > a0 ? b0 : a0;
-          nt++;
-        }
-        const uint8_t *t = a.arena + a.toff[pd.tid];
-        const GateRes g = gate_eval<ProfGlobal, 4>(ProfGlobal{u.prof32, u.ld}, t, ts, td, lo, hi, nt, u.open,
-                                                   u.ext, INT32_MIN, false);
-        // int16 pass; a result at the clamp (32767) is rescored by the host
-        val[0] = P::half(bs, x);
-        val[1] = nt ? ts[0] : 0;
-        val[2] = nt ? td[0] : 0;
-        val[3] = nt ? g.score : 0;
-        orow = pd.out;
-      }
+      for (int k = 0; k < 4; k++) k64[k] = P::key64(key[k]);
+      // int16 pass; a result at the clamp (32767) is rescored by the host
+      int32_t val[4] = {P::half(bs, x), 0, 0, 0};
+      // the profile region of LDS is free until the next unit's first pass
+      gate_wave(ProfGlobal{u.prof32, u.ld}, ok_row, a.arena + (ok_row ? a.toff[pd.tid] : 0), (int)u.m,
+                ok_row ? (int)a.tlen[pd.tid] : 0, k64, u.open, u.ext, lane, (GateLds *)lds, val[1], val[2], val[3]);
+      const uint32_t orow = ok_row ? pd.out : ~0u;  // this slot's result row, ~0u: none
       // store instruction w writes words 64w .. 64w + 63 of these 64 rows:
       // word 4r + f (row r, field f) comes from lane r
       uint32_t *const words = (uint32_t *)a.results;

@@ -110,8 +110,6 @@ struct Slot {
   GrowBuf<uint8_t> d_in;
   GrowBuf<Unit> d_rs_units;
   GrowBuf<PairDev> d_rs_pairs;
-  GrowBuf<PairOut> d_pout;
-  GrowBuf<uint64_t> d_top4;
   GrowBuf<st_pair_result> d_res;
   GrowBuf<uint4> d_buf, d_rs_buf;
   Pinned<Unit> h_units;
@@ -271,7 +269,7 @@ namespace {
 // fan out over the slot's sub-streams (each with its own unit counter) and
 // join back before the slot stream continues.
 int launch_on(Slot &sl, const Plan &pl, const QueryDev *d_q, const Unit *d_units, const PairDev *d_pairs,
-              uint4 *d_buf, PairOut *d_pout, uint64_t *d_top4) {
+              uint4 *d_buf, st_pair_result *d_res, const int32_t *d_gate) {
   const st_db *db = sl.srv->db;
   const size_t nl = pl.launches.size();
   const bool fan = nl > 1;
@@ -296,9 +294,9 @@ int launch_on(Slot &sl, const Plan &pl, const QueryDev *d_q, const Unit *d_units
     a.grid = L.grid;
     a.passbuf = d_buf + L.buf0;
     a.passbuf_half = L.half;
-    a.pout = d_pout;
-    a.top4 = d_top4;
-    a.qend = a.tend = nullptr;
+    a.out = DpOut{};
+    a.out.res = (PairResultDev *)d_res;  // the gate runs in the DP launches
+    a.out.qgate = d_gate;
     a.lds_stride = L.gr_max + 2;
     ST_HIP(launch_dp(a, L.kind, L.R, true, fan ? sl.sub[k] : sl.stream));
   }
@@ -350,37 +348,27 @@ int run_micro_batch(Slot &sl, std::vector<std::shared_ptr<Req>> &reqs) {
   }
   finish_plan(pl, pending, qkind, open, ext, -1, srv->ctx->cus, true);
   // packed inputs: query table, gaps (open, ext, gate), units, unit pair
-  // slots, candidate-order pairs, candidate query index
+  // slots
   const size_t o_q = 0, o_g = align256(o_q + nq * sizeof(QueryDev)),
                o_u = align256(o_g + 3 * nq * sizeof(int32_t)),
                o_p = align256(o_u + pl.units.size() * sizeof(Unit)),
-               o_a = align256(o_p + pl.pairs.size() * sizeof(PairDev)), o_c = align256(o_a + np * sizeof(PairDev)),
-               total = align256(o_c + np * sizeof(uint32_t));
+               total = align256(o_p + pl.pairs.size() * sizeof(PairDev));
   ST_HIP(sl.h_in.reserve(total));
   uint8_t *h = sl.h_in.p;
   auto *hq = (QueryDev *)(h + o_q);
   auto *hg = (int32_t *)(h + o_g);
-  auto *ha = (PairDev *)(h + o_a);
-  auto *hc = (uint32_t *)(h + o_c);
   for (size_t q = 0; q < nq; q++) {
     const Req &r = *reqs[q];
     hq[q] = r.qp->view;
     hg[q] = r.open;
     hg[nq + q] = r.ext;
     hg[2 * nq + q] = r.gate;
-    for (size_t k = 0; k < r.cands.size(); k++) {
-      ha[off[q] + k].tid = r.cands[k];
-      ha[off[q] + k].out = (uint32_t)(off[q] + k);
-      hc[off[q] + k] = (uint32_t)q;
-    }
   }
   std::copy(pl.units.begin(), pl.units.end(), (Unit *)(h + o_u));
   std::copy(pl.pairs.begin(), pl.pairs.end(), (PairDev *)(h + o_p));
   const double t_planned = now_ms();
   hipStream_t s = sl.stream;
   ST_HIP(sl.d_in.reserve(total));
-  ST_HIP(sl.d_pout.reserve(np));
-  ST_HIP(sl.d_top4.reserve(np * 4));
   ST_HIP(sl.d_res.reserve(np));
   ST_HIP(sl.d_buf.reserve(pl.buf_elems));
   ST_HIP(sl.h_res.reserve(np));
@@ -389,33 +377,16 @@ int run_micro_batch(Slot &sl, std::vector<std::shared_ptr<Req>> &reqs) {
   const auto *dg = (const int32_t *)(d + o_g);
   ST_HIP(hipEventRecord(sl.t_start, s));
   ST_HIP(hipMemcpyAsync(d, h, total, hipMemcpyHostToDevice, s));
-  int rc = launch_on(sl, pl, dq, (const Unit *)(d + o_u), (const PairDev *)(d + o_p), sl.d_buf.p(), sl.d_pout.p(),
-                     sl.d_top4.p());
+  int rc = launch_on(sl, pl, dq, (const Unit *)(d + o_u), (const PairDev *)(d + o_p), sl.d_buf.p(), sl.d_res.p(),
+                     dg + 2 * nq);
   if (rc) return rc;
-  PairGateArgs ga;
-  ga.pairs = (const PairDev *)(d + o_a);
-  ga.npairs = np;
-  ga.pair_query = (const uint32_t *)(d + o_c);
-  ga.queries = dq;
-  ga.arena = db->arena.p;
-  ga.toff = db->off.p;
-  ga.tlen = db->len.p;
-  ga.top4 = sl.d_top4.p();
-  ga.open = dg;
-  ga.ext = dg + nq;
-  ga.gate = dg + 2 * nq;
-  ga.pout = sl.d_pout.p();
-  ga.res = sl.d_res.p();
-  if (np) {
-    ST_HIP(launch_gate_pairs(ga, s));
-    ST_HIP(hipMemcpyAsync(sl.h_res.p, sl.d_res.p(), np * sizeof(st_pair_result), hipMemcpyDeviceToHost, s));
-  }
+  if (np) ST_HIP(hipMemcpyAsync(sl.h_res.p, sl.d_res.p(), np * sizeof(st_pair_result), hipMemcpyDeviceToHost, s));
   ST_HIP(hipEventRecord(sl.t_end, s));
   ST_HIP(hipEventSynchronize(sl.t_end));
   float dev_ms = 0;
   ST_HIP(hipEventElapsedTime(&dev_ms, sl.t_start, sl.t_end));
   // int16 pairs whose best reached 32767 (an int16 result at the clamp):
-  // exact int32 rescoring, then the gate again over the micro-batch
+  // exact int32 rescoring, whose launches rewrite those pairs' rows
   Plan rs;
   std::vector<Unit> rs_pending;
   uint64_t nres = 0;
@@ -441,9 +412,8 @@ int run_micro_batch(Slot &sl, std::vector<std::shared_ptr<Req>> &reqs) {
     ST_HIP(h2d(sl.d_rs_units, sl.h_units.p, rs.units.size(), s));
     ST_HIP(h2d(sl.d_rs_pairs, sl.h_pairs.p, rs.pairs.size(), s));
     ST_HIP(sl.d_rs_buf.reserve(rs.buf_elems));
-    rc = launch_on(sl, rs, dq, sl.d_rs_units.p(), sl.d_rs_pairs.p(), sl.d_rs_buf.p(), sl.d_pout.p(), sl.d_top4.p());
+    rc = launch_on(sl, rs, dq, sl.d_rs_units.p(), sl.d_rs_pairs.p(), sl.d_rs_buf.p(), sl.d_res.p(), dg + 2 * nq);
     if (rc) return rc;
-    ST_HIP(launch_gate_pairs(ga, s));
     ST_HIP(hipMemcpyAsync(sl.h_res.p, sl.d_res.p(), np * sizeof(st_pair_result), hipMemcpyDeviceToHost, s));
     ST_HIP(hipEventRecord(sl.t_end, s));
     ST_HIP(hipEventSynchronize(sl.t_end));
