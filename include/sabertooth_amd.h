@@ -260,7 +260,10 @@ typedef struct {
                                  kernel through a device work queue; requests it
                                  cannot take (int32 precision, very long
                                  targets) use the micro-batch slots */
-  uint32_t reserved;
+  uint32_t arena_mib;         /* persistent: device arena for the published
+                                 requests' profiles and pair slots, MiB; the
+                                 kernel restarts (after draining) when it is
+                                 used up; 0 = 4096 */
 } st_server_config;
 
 /* One request: `out` receives n_cands results in candidate order and must

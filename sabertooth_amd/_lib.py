@@ -34,7 +34,7 @@ LAUNCH_INFO_DT = np.dtype([("kind", "<i4"), ("rows_per_lane", "<i4"), ("units", 
                            ("useful_cells", "<u8"), ("exec_cells", "<u8"), ("ms", "<f8")])
 SCORE_END_DT = np.dtype([("score", "<i4"), ("reserved", "<i4"), ("query_end", "<i8"), ("target_end", "<i8")])
 SERVER_CONFIG_DT = np.dtype([("workers", "<u4"), ("max_batch_queries", "<u4"), ("state_budget_bytes", "<u8"),
-                             ("persistent", "<u4"), ("reserved", "<u4")])
+                             ("persistent", "<u4"), ("arena_mib", "<u4")])
 REQUEST_DT = np.dtype([("query", "<u8"), ("gap_open", "<i4"), ("gap_extend", "<i4"), ("min_diag_score", "<i4"),
                        ("reserved", "<i4"), ("cand_tids", "<u8"), ("n_cands", "<u8"), ("out", "<u8")])
 SERVER_STATS_DT = np.dtype([("batches", "<u8"), ("queries", "<u8"), ("pairs", "<u8"), ("cells", "<u8"),

@@ -319,11 +319,12 @@ class Server:
     """
 
     def __init__(self, ctx: Context, db: TargetDb, workers: int = 4, max_batch_queries: int = 64,
-                 state_budget_bytes: int = 4 << 30, persistent: bool = True):
+                 state_budget_bytes: int = 4 << 30, persistent: bool = True, arena_mib: int = 0):
         cfg = np.zeros(1, dtype=_lib.SERVER_CONFIG_DT)
         cfg["workers"], cfg["max_batch_queries"], cfg["state_budget_bytes"] = (workers, max_batch_queries,
                                                                               state_budget_bytes)
         cfg["persistent"] = 1 if persistent else 0
+        cfg["arena_mib"] = arena_mib
         h = C.c_void_p()
         check(lib().st_server_create(ctx.handle, db.handle, ptr(cfg), C.byref(h)))
         self._h = h

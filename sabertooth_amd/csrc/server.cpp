@@ -204,7 +204,7 @@ struct Persist {
   static constexpr uint64_t kResultRows = 1ull << 22;
   static constexpr uint32_t kReqSlots = 1u << 16;
   static constexpr uint64_t kHalfQuads = 20480;  // pass-boundary half of one block
-  static constexpr uint64_t kArenaBytes = 4ull << 30;
+  uint64_t arena_bytes = 4ull << 30;
   Coherent<ServeUnit> ring;
   Coherent<uint64_t> ctl;  // [0] tail
   Coherent<uint32_t> stop;  // [0] stop, [4] malformed-unit flag
@@ -497,7 +497,7 @@ int ensure_running(st_server *srv) {
   a.result_rows = Persist::kResultRows;
   a.req_slots = Persist::kReqSlots;
   a.sarena = pe.arena.p;
-  a.sarena_bytes = Persist::kArenaBytes;
+  a.sarena_bytes = pe.arena_bytes;
   a.arena = srv->db->arena.p;
   a.toff = srv->db->off.p;
   a.tlen = srv->db->len.p;
@@ -571,7 +571,7 @@ bool publish(st_server *srv, const std::shared_ptr<Req> &r, int *rc) {
   const uint64_t b16 = ((uint64_t)kCodes * ld * 2 + 255) & ~255ull, b32 = ((uint64_t)kCodes * ld * 4 + 255) & ~255ull;
   const uint64_t bp = (pl.pairs.size() * sizeof(PairDev) + 255) & ~255ull;
   const uint64_t need = b16 + b32 + bp;
-  if (need > Persist::kArenaBytes / 2 || units.size() > Persist::kRingUnits / 2 || n > Persist::kResultRows / 2)
+  if (need > pe.arena_bytes / 2 || units.size() > Persist::kRingUnits / 2 || n > Persist::kResultRows / 2)
     return false;
   // wait for room: rows, units, a counter slot, the arena (draining the
   // kernel when the arena must restart) and the DP-state budget
@@ -584,7 +584,7 @@ bool publish(st_server *srv, const std::shared_ptr<Req> &r, int *rc) {
         *rc = fail(ST_ERR_DEVICE, "serving kernel: earlier requests did not finish");
         return true;
       }
-      const bool arena_ok = pe.arena_used + need <= Persist::kArenaBytes;
+      const bool arena_ok = pe.arena_used + need <= pe.arena_bytes;
       if (!arena_ok && pe.flights.empty()) {
         // every launch-lifetime address was used: restart the kernel
         pe.draining = true;
@@ -839,6 +839,7 @@ void completer(st_server *srv) {
 int setup_persistent(st_server *s) {
   auto pe = std::make_unique<Persist>();
   pe->rows.cap = Persist::kResultRows;
+  if (s->cfg.arena_mib) pe->arena_bytes = (uint64_t)s->cfg.arena_mib << 20;
   const size_t lds = (size_t)kCodes * pe->lds_stride * 2;
   const int occ = serve_occupancy(lds);
   // leave one block per CU to the other launches (rescoring, int32 requests)
@@ -848,7 +849,7 @@ int setup_persistent(st_server *s) {
             pe->remaining.alloc(Persist::kReqSlots) == hipSuccess &&
             pe->dctl.alloc((size_t)kCtlWords * kCtlStride) == hipSuccess &&
             pe->passbuf.alloc((size_t)pe->grid * (2 * Persist::kHalfQuads + kEdgeQuadsPerBlock)) == hipSuccess &&
-            pe->arena.alloc(Persist::kArenaBytes) == hipSuccess &&
+            pe->arena.alloc(pe->arena_bytes) == hipSuccess &&
             hipMemset(pe->dctl.p, 0, (size_t)kCtlWords * kCtlStride * sizeof(uint64_t)) == hipSuccess &&
             hipStreamCreateWithFlags(&pe->kstream, hipStreamNonBlocking) == hipSuccess &&
             hipStreamCreateWithFlags(&pe->cstream, hipStreamNonBlocking) == hipSuccess &&
@@ -931,6 +932,7 @@ int st_server_create(st_ctx *ctx, const st_db *db, const st_server_config *cfg, 
   }
   if (cfg && cfg->persistent) {
     s->cfg.persistent = 1;
+    s->cfg.arena_mib = cfg->arena_mib;
     const int rc = setup_persistent(s);
     if (rc) {
       release_slots(s);

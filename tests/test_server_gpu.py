@@ -131,6 +131,37 @@ def test_tiny_budget_admits_one_at_a_time(ctx, setup, persistent):
         srv.close()
 
 
+def test_small_arena_restarts_the_serving_kernel(ctx, setup):
+    # a 1 MiB arena holds a few dozen requests: publishing drains and
+    # relaunches the serving kernel many times; results do not change
+    srv = st.Server(ctx, setup["db"], workers=2, persistent=True, arena_mib=1)
+    try:
+        order = list(range(len(setup["qps"]))) * 4
+        tickets = [(q, srv.submit(setup["qps"][q], setup["cands"][q], setup["gaps"][q], setup["gates"][q]))
+                   for q in order]
+        for q, t in tickets:
+            out, _ = srv.wait(t, timeout_ms=60000)
+            assert (_fields(out) == _fields(setup["ref"][q])).all(), q
+        s = srv.stats()
+        assert s["kernel_launches"] >= 2 and s["published"] > 0
+    finally:
+        srv.close()
+
+
+def test_serving_kernel_relaunches_after_idling(ctx, setup):
+    import time
+    srv = st.Server(ctx, setup["db"], workers=1, persistent=True)
+    try:
+        for q in (0, 1):
+            out, _ = srv.wait(srv.submit(setup["qps"][q], setup["cands"][q], setup["gaps"][q], setup["gates"][q]),
+                              timeout_ms=60000)
+            assert (_fields(out) == _fields(setup["ref"][q])).all(), q
+            time.sleep(0.05)  # longer than the kernel's 5 ms idle limit
+        assert srv.stats()["kernel_launches"] >= 2
+    finally:
+        srv.close()
+
+
 def test_bad_requests_fail_cleanly(ctx, setup):
     srv = st.Server(ctx, setup["db"], workers=1)
     try:
