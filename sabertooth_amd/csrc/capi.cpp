@@ -274,12 +274,12 @@ Kind kind_for(const st_qprof *qp, int32_t open, int32_t ext) {
   return kI32F;
 }
 
-// ST_FORCE_R=8|16 and ST_FORCE_G=1..64 pin the rows per lane and the lanes
-// per group (tuning runs and tests); read at every plan
+// ST_FORCE_R=8|12|16 and ST_FORCE_G=1..64 pin the rows per lane and the
+// lanes per group (tuning runs and tests); read at every plan
 static int forced_rows() {
   const char *e = std::getenv("ST_FORCE_R");
   const int v = e ? std::atoi(e) : 0;
-  return (v == 8 || v == 16) ? v : 0;
+  return (v == 8 || v == 12 || v == 16) ? v : 0;
 }
 
 static int forced_groups() {
@@ -289,14 +289,17 @@ static int forced_groups() {
 }
 
 // Device time per executed cell of each launch shape (picoseconds, packed
-// int16 kernel with the ungapped evidence), measured on MI355X with uniform
-// workloads where every unit has the shape; rows R = 8, 16 by lanes per group
-// G = 1..64.  Narrow groups pay for their many distinct pass-boundary and
-// residue streams (memory-pipeline bound), 64-lane groups for cross-row
-// shuffles.
-static const double kShapePs[2][7] = {
-    {2.00, 0.87, 0.33, 0.255, 0.246, 0.252, 0.375},  // R = 8
-    {0.83, 0.36, 0.248, 0.247, 0.255, 0.29, 0.60},    // R = 16
+// int16 kernel with the ungapped evidence and the gate, every height in one
+// instance), measured on MI355X with uniform workloads where every unit has
+// the shape (median over queries of 96, 300 and 1,200 residues); rows R = 8, 12, 16 by lanes per
+// group G = 1..64 (R = 12: packed int16 only).  Narrow groups pay for their
+// many distinct pass-boundary and residue streams (memory-pipeline bound),
+// 64-lane groups for cross-row shuffles.
+static const int kShapeRows[3] = {8, 12, 16};
+static const double kShapePs[3][7] = {
+    {1.98, 0.80, 0.286, 0.265, 0.283, 0.272, 0.40},   // R = 8
+    {1.27, 0.48, 0.252, 0.233, 0.245, 0.289, 0.534},  // R = 12
+    {0.88, 0.405, 0.236, 0.230, 0.237, 0.325, 0.665}, // R = 16
 };
 
 double unit_steps_cap(uint64_t batch_cells) {
@@ -346,10 +349,12 @@ bool plan_query(Plan &pl, std::vector<Unit> &pending, uint32_t qi, const st_qpro
   for (size_t s = 0; s < count;) {
     int bG = 16, bR = kind == kI32N ? 8 : 16;
     double bc = 1e300;
-    for (int ri = 0; ri < 2; ri++) {
-      const int R = 8 * (ri + 1);
+    for (int ri = 0; ri < 3; ri++) {
+      const int R = kShapeRows[ri];
       if (kind == kI32N && R != 8) continue;
-      if (fr && R != fr && kind != kI32N) continue;
+      if (R == 12 && kind != kPk16) continue;  // the int32 kernels have R = 8, 16
+      // a forced height the instance lacks (12 for the int32 kernels) is ignored
+      if (fr && R != fr && kind != kI32N && !(fr == 12 && kind != kPk16)) continue;
       for (int gi = 0; gi < 7; gi++) {
         const int G = 1 << gi;
         if (fg && G != fg) continue;
@@ -359,6 +364,8 @@ bool plan_query(Plan &pl, std::vector<Unit> &pending, uint32_t qi, const st_qpro
         const size_t e = std::min(count, s + (size_t)(64 / G * per));
         const uint64_t np = key[e - 1] >> 32;
         const uint64_t passes = std::max<uint64_t>(1, (m + GR - 1) / GR);
+        // the passes' profile rows must lie inside the profile's padded rows
+        if (passes * GR > qp->ld) continue;
         if (lim && ((lim->only_R && R != lim->only_R) || GR > (uint64_t)lim->max_GR ||
                     (passes > 1 && (uint64_t)(64 / G) * np > lim->max_half)))
           continue;

@@ -205,8 +205,8 @@ __device__ __forceinline__ void dp_unit(const Unit &u, const QueryDev *__restric
 }
 
 // Persistent DP kernel: every block (one wave) pulls units from *counter.
-// R = 8 or 16: rows per lane of every unit; R = 0: each unit's own R (8 or
-// 16), so one launch holds units of both heights.
+// R = 8 or 16: rows per lane of every unit; R = 0: each unit's own R (8, 12
+// or 16), so one launch holds units of every height.
 // END (int32 kernels without the evidence): also the end cell of the
 // optimum, first in column-major order, into qend / tend.
 template <class P, int R, bool UNG, bool END>
@@ -238,7 +238,10 @@ __global__ __launch_bounds__(64, (R == 8 && !END) ? 4 : (R == 0 ? 3 : 2)) void d
     if (ui >= nunits) break;
     const Unit u = units[ui];
     if constexpr (R == 0) {
-      if (u.R == 16)
+      if (u.R == 12)
+        dp_unit<P, 12, UNG, END>(u, queries, pairs, arena, toff, tlen, lds, lds_stride, myb, edge, passbuf_half, e0,
+                                 vopen, vext, gap_open, gap_ext, out, lane);
+      else if (u.R == 16)
         dp_unit<P, 16, UNG, END>(u, queries, pairs, arena, toff, tlen, lds, lds_stride, myb, edge, passbuf_half, e0,
                                  vopen, vext, gap_open, gap_ext, out, lane);
       else

@@ -120,7 +120,7 @@ def test_pssm_like_wide_rows(ctx):
 
 
 @pytest.mark.parametrize("G", [1, 2, 4, 8, 16, 32, 64])
-@pytest.mark.parametrize("R", [8, 16])
+@pytest.mark.parametrize("R", [8, 12, 16])
 def test_every_group_width_and_rows(ctx, monkeypatch, G, R):
     # every launch shape on multi-pass queries, with all outputs checked
     monkeypatch.setenv("ST_FORCE_G", str(G))
