@@ -258,6 +258,21 @@ def main():
         mism += got != want
     log(f"parity: {len(sample) - mism}/{len(sample)} sampled pairs bit-exact")
 
+    # the same step with the DP phase replayed from a captured HIP graph
+    batch.run(graph=True)
+    gt, gdp = [], []
+    for i in range(a.steps):
+        ts = time.perf_counter()
+        batch.run(graph=True)
+        gt.append(time.perf_counter() - ts)
+        gdp.append(batch.timing()[0])
+    gres = batch.results()
+    g_equal = bool(all((gres[f] == res[f]).all() for f in res.dtype.names))
+    graph = {"ms_per_step": round(float(np.mean(gt)) * 1e3, 3), "dp_ms": round(float(np.mean(gdp)), 3),
+             "results_equal_eager": g_equal}
+    log(f"graph replay: {graph['ms_per_step']:.1f} ms per step (dp {graph['dp_ms']:.1f} ms), "
+        f"results equal eager: {g_equal}")
+
     dp = float(np.mean(dp_ms)) / 1e3
     kinds = {0: "Pk16", 1: "I32F", 2: "I32N"}
     per_launch = []
@@ -326,7 +341,7 @@ def main():
         "queries_per_s": round(qps_rate, 1),
         "parity": {"sampled_pairs_all_outputs_vs_oracle": int(len(sample)), "mismatches": int(mism),
                    "all_pairs_swg_vs_cpu_avx2_mismatches": full_swg},
-        "roofline": roof, "cpu_baseline": cpu, "serving": serving,
+        "roofline": roof, "cpu_baseline": cpu, "graph_replay": graph, "serving": serving,
     }
     print(json.dumps(out), flush=True)
 

@@ -204,6 +204,14 @@ int st_batch_create(st_ctx *ctx, const st_db *db, const st_qprof *const *queries
                     st_batch **out);
 /* Run every kernel of the batch on the device and wait for it. */
 int st_batch_run(st_batch *b);
+/* st_batch_run with the main DP phase (clearing the clamp list, then every
+ * DP launch of the plan forked over the context's launch streams) captured
+ * into a HIP graph by the first call and replayed by every later call, one
+ * hipGraphLaunch per run.  The int32 rescoring of clamped pairs depends on
+ * how many pairs clamped and runs eagerly after the replay.  Results and
+ * st_batch_timing equal st_batch_run's; st_launch_info.ms reads 0 after it.
+ * No counterpart in the reference, whose CPU kernels have no launch cost. */
+int st_batch_run_graph(st_batch *b);
 /* Copy results (cand_off[nq] entries, in candidate order) to the host. */
 int st_batch_results(st_batch *b, st_pair_result *out);
 /* Device time of the last st_batch_run, from HIP events on the batch's

@@ -261,8 +261,10 @@ class Batch:
         self.offsets = off
         self.npairs = int(off[-1]) if nq else 0
 
-    def run(self) -> None:
-        check(lib().st_batch_run(self._h))
+    def run(self, graph: bool = False) -> None:
+        """Score the batch; `graph` replays the DP phase from a HIP graph
+        captured on the first such call."""
+        check((lib().st_batch_run_graph if graph else lib().st_batch_run)(self._h))
 
     def results(self) -> np.ndarray:
         out = np.zeros(max(self.npairs, 1), dtype=PAIR_RESULT_DT)

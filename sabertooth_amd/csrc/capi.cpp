@@ -514,7 +514,13 @@ struct st_batch {
   // per-launch start/end events of the main plan and their last durations
   std::vector<hipEvent_t> lev;
   std::vector<double> launch_ms;
+  // the main DP phase captured once (clamp-list clear + every DP launch,
+  // forked over the launch streams) and replayed by st_batch_run_graph
+  hipGraphExec_t gexec = nullptr;
+  hipEvent_t gfork = nullptr;
   ~st_batch() {
+    if (gexec) (void)hipGraphExecDestroy(gexec);
+    if (gfork) (void)hipEventDestroy(gfork);
     for (hipEvent_t &e : ev)
       if (e) (void)hipEventDestroy(e);
     for (hipEvent_t &e : join)
@@ -527,9 +533,11 @@ struct st_batch {
 namespace st {
 
 // Issue a plan's launches on the context's launch streams, after `start` on
-// the main stream, and make the main stream wait for all of them.
+// the main stream, and make the main stream wait for all of them.  `timing`
+// records per-launch events around the main plan's launches (off while the
+// launches are being captured into a graph).
 static int launch_plan(st_batch *b, const Plan &pl, const Unit *d_units, const PairDev *d_pairs, uint4 *d_buf,
-                       hipEvent_t start) {
+                       hipEvent_t start, bool timing = true) {
   st_ctx *ctx = b->ctx;
   bool used[kLaunchSlots] = {false, false, false, false};
   for (const Launch &L : pl.launches) {
@@ -561,7 +569,7 @@ static int launch_plan(st_batch *b, const Plan &pl, const Unit *d_units, const P
     a.out.qend = b->d_qend.p;
     a.out.tend = b->d_tend.p;
     a.lds_stride = L.gr_max + 2;
-    const bool timed = &pl == &b->plan && b->lev.size() == 2 * pl.launches.size();
+    const bool timed = timing && &pl == &b->plan && b->lev.size() == 2 * pl.launches.size();
     const size_t li = (size_t)(&L - pl.launches.data());
     if (timed) ST_HIP(hipEventRecord(b->lev[2 * li], s));
     ST_HIP(launch_dp(a, L.kind, L.R, b->ung, s, b->end));
@@ -682,7 +690,30 @@ int st_batch_create(st_ctx *ctx, const st_db *db, const st_qprof *const *queries
                            true, out);
 }
 
-static int batch_run_locked(st_batch *b) {
+// Capture the main DP phase into b->gexec: the clamp-list clear (a kernel;
+// graphs here take no memset nodes) and every launch of the plan, forked
+// from the main stream over the launch streams and joined back.
+static int capture_dp_graph(st_batch *b) {
+  st_ctx *ctx = b->ctx;
+  if (!b->gfork) ST_HIP(hipEventCreateWithFlags(&b->gfork, hipEventDisableTiming));
+  ST_HIP(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
+  int rc = ST_OK;
+  if (clear_u32(b->d_sat.p, 1, ctx->stream) != hipSuccess || hipEventRecord(b->gfork, ctx->stream) != hipSuccess)
+    rc = fail(ST_ERR_DEVICE, "st_batch_run_graph: capture failed");
+  if (!rc) rc = launch_plan(b, b->plan, b->d_units.p, b->d_pairs.p, b->d_passbuf.p, b->gfork, false);
+  // end the capture whatever happened, so the stream leaves capture mode
+  hipGraph_t g = nullptr;
+  const hipError_t ec = hipStreamEndCapture(ctx->stream, &g);
+  if (!rc && (ec != hipSuccess || !g)) rc = fail(ST_ERR_DEVICE, "st_batch_run_graph: hipStreamEndCapture failed");
+  if (!rc && hipGraphInstantiate(&b->gexec, g, nullptr, nullptr, 0) != hipSuccess) {
+    b->gexec = nullptr;
+    rc = fail(ST_ERR_DEVICE, "st_batch_run_graph: hipGraphInstantiate failed");
+  }
+  if (g) (void)hipGraphDestroy(g);
+  return rc;
+}
+
+static int batch_run_locked(st_batch *b, bool graph = false) {
   st_ctx *ctx = b->ctx;
   ST_HIP(hipSetDevice(ctx->device));
   for (hipEvent_t &e : b->ev)
@@ -693,10 +724,17 @@ static int batch_run_locked(st_batch *b) {
     b->lev.assign(2 * b->plan.launches.size(), nullptr);
     for (hipEvent_t &e : b->lev) ST_HIP(hipEventCreate(&e));
   }
-  ST_HIP(clear_u32(b->d_sat.p, 1, ctx->stream));
-  ST_HIP(hipEventRecord(b->ev[0], ctx->stream));
-  int rc = launch_plan(b, b->plan, b->d_units.p, b->d_pairs.p, b->d_passbuf.p, b->ev[0]);
-  if (rc) return rc;
+  int rc = ST_OK;
+  if (graph) {
+    if (!b->gexec && (rc = capture_dp_graph(b))) return rc;
+    ST_HIP(hipEventRecord(b->ev[0], ctx->stream));
+    ST_HIP(hipGraphLaunch(b->gexec, ctx->stream));
+  } else {
+    ST_HIP(clear_u32(b->d_sat.p, 1, ctx->stream));
+    ST_HIP(hipEventRecord(b->ev[0], ctx->stream));
+    rc = launch_plan(b, b->plan, b->d_units.p, b->d_pairs.p, b->d_passbuf.p, b->ev[0]);
+    if (rc) return rc;
+  }
   ST_HIP(hipEventRecord(b->ev[1], ctx->stream));
   // int16 pairs whose best reached 32767 are rescored in exact int32
   b->resc = Plan();
@@ -752,7 +790,8 @@ static int batch_run_locked(st_batch *b) {
   ST_HIP(hipEventElapsedTime(&t, b->ev[4], b->ev[5]));
   b->ms[2] = t;
   b->launch_ms.assign(b->plan.launches.size(), 0.0);
-  for (size_t i = 0; i < b->plan.launches.size(); i++) {
+  // a replayed graph records no per-launch events
+  for (size_t i = 0; !graph && i < b->plan.launches.size(); i++) {
     ST_HIP(hipEventElapsedTime(&t, b->lev[2 * i], b->lev[2 * i + 1]));
     b->launch_ms[i] = t;
   }
@@ -763,6 +802,12 @@ int st_batch_run(st_batch *b) {
   if (!b) return fail(ST_ERR_INVALID, "st_batch_run: null batch");
   std::lock_guard<std::mutex> lk(b->ctx->mu);
   return batch_run_locked(b);
+}
+
+int st_batch_run_graph(st_batch *b) {
+  if (!b) return fail(ST_ERR_INVALID, "st_batch_run_graph: null batch");
+  std::lock_guard<std::mutex> lk(b->ctx->mu);
+  return batch_run_locked(b, true);
 }
 
 int st_batch_results(st_batch *b, st_pair_result *out) {
