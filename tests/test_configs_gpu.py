@@ -1,8 +1,14 @@
-"""The benchmark configurations other than the bench line, as parity cases:
-plain all-pairs (reduced), PSSM queries from the PSSM builder (44/4, gate 30)
-and long pairs mixed with short ones (int32 scores above 32767).  Every SWG
-score is checked against the AVX2 cascade restated from the reference, and
-every output of a seeded sample against the oracle (bit-exact)."""
+"""The benchmark configurations other than the bench line, as parity cases,
+at their full sizes: plain all-pairs (32 queries x 20,000 targets), PSSM
+queries from the PSSM builder (16 profiles x 200,000 targets, 44/4, gate 30)
+and long pairs (64 queries of 2,000-35,000 x 5,000 targets of 1,000-30,000,
+9.2e13 cells), plus long pairs mixed with short ones in one launch.  Every
+SWG score of the first two is checked against the AVX2 cascade restated from
+the reference, and every output of a seeded sample against the oracle
+(bit-exact).  The long configuration, too large for a CPU check of every
+pair, is checked on sampled pairs, on the closed form of a query aligned
+with itself, and on each checked query scored alone against its rows of the
+full batch."""
 import os
 
 import numpy as np
@@ -46,8 +52,9 @@ def _check(rows, arena, offs, cands, gaps, gates, b, res, sample, seed):
         assert got == want, (q, k)
 
 
-def test_plain_allpairs_reduced(ctx):
-    w = synth.plain_allpairs(seed=1, n_queries=8, n_targets=3000)
+def test_plain_allpairs_full(ctx):
+    w = synth.plain_allpairs(seed=1)
+    assert len(w.queries) == 32 and len(w.offsets) - 1 == 20000
     rows = [O.seq_rows(q) for q in w.queries]
     gaps = [(11, 1)] * len(rows)
     gates = [w.gate] * len(rows)
@@ -55,8 +62,9 @@ def test_plain_allpairs_reduced(ctx):
     _check(rows, w.arena, w.offsets, w.cand_lists, gaps, gates, b, res, 300, 1)
 
 
-def test_pssm_queries_from_the_builder(ctx):
-    w = synth.pssm_search(seed=1, n_profiles=16, n_targets=4000)
+def test_pssm_queries_from_the_builder_full(ctx):
+    w = synth.pssm_search(seed=1)
+    assert len(w.cand_lists) == 16 and len(w.offsets) - 1 == 200000
     rows = w.meta["rows"]
     assert all(r.shape[1] == 21 and (r[:, 20] == 0).all() for r in rows)
     gaps = [(44, 4)] * len(rows)
@@ -86,3 +94,42 @@ def test_long_pairs_mixed_with_short(ctx):
     b, res = _run(ctx, rows, arena, offs, cands, gaps, gates)
     assert res["swg"].max() > 32767 and (res["kernel"] == st.KERNEL_I32).any()
     _check(rows, arena, offs, cands, gaps, gates, b, res, 12, 3)
+
+
+def test_long_pairs_full_size(ctx):
+    w = synth.long_pairs(seed=1)
+    nq, nt = len(w.queries), len(w.offsets) - 1
+    assert (nq, nt) == (64, 5000) and w.cells() > 9e13
+    # the first query is also a target: its self-alignment has a closed form
+    arena = np.concatenate([w.arena, w.queries[0]])
+    offs = np.concatenate([w.offsets, [w.offsets[-1] + len(w.queries[0])]]).astype(np.uint64)
+    rows = [O.seq_rows(q) for q in w.queries]
+    cands = [np.arange(nt, dtype=np.uint32) for _ in range(nq)]
+    cands[0] = np.arange(nt + 1, dtype=np.uint32)
+    gaps = [(11, 1)] * nq
+    gates = [w.gate] * nq
+    b, res = _run(ctx, rows, arena, offs, cands, gaps, gates)
+    off = b.offsets.astype(np.int64)
+    # every diagonal score of BLOSUM62 is positive: the main diagonal's
+    # Kadane segment is the whole query, the largest possible ungapped score
+    self_row = res[off[0] + nt]
+    diag_sum = int(rows[0][np.arange(len(w.queries[0])), w.queries[0]].sum())
+    assert int(self_row["ung"]) == diag_sum and int(self_row["best_diag"]) == 0
+    assert int(self_row["swg"]) >= diag_sum > 32767 and self_row["kernel"] == st.KERNEL_I32
+    assert bool(self_row["pass"])
+    # a random pair and the best-scoring planted pair against the oracle
+    rng = np.random.default_rng(4)
+    picks = [int(rng.integers(len(res))), int(np.argmax(np.where(np.arange(len(res)) == off[0] + nt, -1,
+                                                                 res["swg"])))]
+    for k in picks:
+        q = int(np.searchsorted(off, k, side="right") - 1)
+        tid = int(cands[q][k - off[q]])
+        t = arena[int(offs[tid]):int(offs[tid + 1])]
+        want = (O.gotoh_score(rows[q], t, 11, 1),) + O.ung_pair(rows[q], t, 11, 1, w.gate)
+        r = res[k]
+        got = (int(r["swg"]), int(r["ung"]), int(r["best_diag"]), int(r["gate_score"]), bool(r["pass"]))
+        assert got == want, (q, tid, len(rows[q]), len(t))
+    # two queries alone give the rows they have in the full batch
+    for q in (1, 37):
+        b1, r1 = _run(ctx, [rows[q]], arena, offs, [cands[q]], [(11, 1)], [w.gate])
+        assert (r1 == res[off[q]:off[q + 1]]).all(), q
