@@ -108,11 +108,6 @@ int st_sw_score_batch(st_ctx *ctx, const st_qprof *qp, const st_db *db,
                       const uint32_t *tids, uint64_t count, int32_t gap_open,
                       int32_t gap_extend, int32_t *scores, uint8_t *kernel);
 
-/* ---- ungapped evidence and gate -------------------------------------------- */
-
-/* One diagonal to score: target, diagonal (target_pos - query_pos) and the
- * query positions [lo, hi) (Job, src/prefilter.rs:1362-1369; Lane,
- * src/ungapped.rs:265-270). */
 /* A score and its end cell (0-based; query_end = target_end = -1 when the
  * score is 0). */
 typedef struct {
@@ -130,6 +125,39 @@ int st_sw_score_end_batch(st_ctx *ctx, const st_qprof *qp, const st_db *db,
                           const uint32_t *tids, uint64_t count, int32_t gap_open,
                           int32_t gap_extend, st_score_end *out);
 
+/* The span of a local alignment: score, start and end cells (0-based,
+ * inclusive; all four -1 when the score is 0; the two start cells -1 when
+ * only the end is known). */
+typedef struct {
+  int32_t score;
+  int32_t reserved;
+  int64_t query_start, query_end;
+  int64_t target_start, target_end;
+} st_span;
+
+/* The two score passes of align_canonical (src/align.rs:847-866) for many
+ * hits at once, on the device: query q's hits are the targets
+ * hit_tids[hit_off[q] .. hit_off[q + 1]), scored with (gap_open[q],
+ * gap_extend[q]).  Per hit, out[k] gets the score and its end cell (the
+ * first optimum in column-major order, as st_sw_score_end_batch), then the
+ * start from the same rule on the reversed prefixes rows[..=query_end] and
+ * target[..=target_end] (start = end - the reversed problem's end).  With a
+ * negative gap cost the reversed problem can score differently; the start
+ * then stays -1, as align_canonical then searches it from the end
+ * (src/align.rs:859-863), which st_align_span does.  Every
+ * query's reversed prefix is a window of one row-reversed copy of its
+ * profile; the reversed target prefixes are gathered on the device.  With
+ * the span known, st_align_span (sabertooth_host.h) runs only the banded
+ * traceback. */
+int st_align_spans_batch(st_ctx *ctx, const st_db *db, const st_qprof *const *queries,
+                         uint64_t nq, const uint64_t *hit_off, const uint32_t *hit_tids,
+                         const int32_t *gap_open, const int32_t *gap_extend, st_span *out);
+
+/* ---- ungapped evidence and gate -------------------------------------------- */
+
+/* One diagonal to score: target, diagonal (target_pos - query_pos) and the
+ * query positions [lo, hi) (Job, src/prefilter.rs:1362-1369; Lane,
+ * src/ungapped.rs:265-270). */
 typedef struct {
   uint32_t tid;
   int32_t diag;

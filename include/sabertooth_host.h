@@ -93,7 +93,10 @@ int st_sw_score_end_host(const int32_t *rows, uint64_t m, const uint8_t *t, uint
 /* align_canonical (src/align.rs:840-882) of rows [m][21] (qres: the query's
  * residues for identity counts, NULL for none) against t.  `known` (NULL:
  * computed here) is the forward score and end cell, e.g. from
- * st_sw_score_end_batch.  *found = 0 when no positive alignment exists.  The
+ * st_sw_score_end_batch.  *found = 0 when no positive alignment exists, and
+ * (as the reference's release build returns None, src/align.rs:859-878) when
+ * no start or no band reproduces the score, which negative gap costs or a
+ * wrong `known` can cause.  The
  * CIGAR (M, I = target residue against a query gap, D) is written
  * NUL-terminated up to cigar_cap bytes; *cigar_len receives its length.
  * st_score_end is declared in sabertooth_amd.h. */
@@ -101,6 +104,19 @@ int st_align_canonical(const int32_t *rows, const uint8_t *qres, uint64_t m, con
                        uint64_t n, int32_t gap_open, int32_t gap_extend, uint32_t band_radius,
                        const st_score_end *known, st_alignment *out, char *cigar,
                        uint64_t cigar_cap, uint64_t *cigar_len, int *found);
+/* align_canonical's traceback (src/align.rs:869-881) for a hit whose span
+ * (score, start and end cells) is known, e.g. from st_align_spans_batch:
+ * the anchored banded traceback from start to end, band doubled until the
+ * score is reproduced; a span whose start cells are -1 (end only) first
+ * searches the start from the end, as align_canonical's fallback does
+ * (src/align.rs:859-863).  Equal to st_align_canonical on the same problem.
+ * *found = 0 when span->score <= 0 or no alignment of its score joins its
+ * start and end; ST_ERR_INVALID when the span lies outside the problem.
+ * st_span is declared in sabertooth_amd.h. */
+int st_align_span(const int32_t *rows, const uint8_t *qres, uint64_t m, const uint8_t *t,
+                  uint64_t n, int32_t gap_open, int32_t gap_extend, uint32_t band_radius,
+                  const st_span *span, st_alignment *out, char *cigar, uint64_t cigar_cap,
+                  uint64_t *cigar_len, int *found);
 /* rescore_alignment (src/align.rs:1064-1135): recompute an alignment's
  * score and counts from its CIGAR; ST_ERR_INVALID on any disagreement. */
 int st_rescore_alignment(const int32_t *rows, const uint8_t *qres, uint64_t m, const uint8_t *t,
@@ -213,6 +229,19 @@ int st_search_hits(const int32_t *rows, const uint8_t *qres, uint64_t m, const u
                    double min_query_cov, double min_target_cov, uint32_t band_radius, st_hit *hits,
                    char *cigars, uint64_t cigar_stride, uint64_t cap, uint64_t *n_hits,
                    st_search_counters *counters);
+/* st_search_hits with each candidate's span from st_align_spans_batch
+ * (spans[c] for cand_tids[c]; only the candidates that pass the gapped
+ * stage are read, and their span score must equal scores[c]): every
+ * traceback is st_align_span's, so the host runs no score pass.  The hits
+ * equal st_search_hits'. */
+int st_search_hits_spans(const int32_t *rows, const uint8_t *qres, uint64_t m, const uint8_t *arena,
+                         const uint64_t *offsets, uint64_t n_targets, const uint32_t *cand_tids,
+                         const int32_t *scores, const st_span *spans, uint64_t n_cands,
+                         int32_t gap_open, int32_t gap_extend, const st_evalue_params *ev,
+                         double max_evalue, uint64_t max_hits, double min_query_cov,
+                         double min_target_cov, uint32_t band_radius, st_hit *hits, char *cigars,
+                         uint64_t cigar_stride, uint64_t cap, uint64_t *n_hits,
+                         st_search_counters *counters);
 /* Hit::to_m8 (src/search.rs:201-216) with Rust's number formatting: pident
  * {:.3}, E-value {:.2e} (exponent without '+' or padding), bits {:.1}. */
 int st_format_m8(const char *query_id, const char *target_id, const st_hit *h, char *buf,

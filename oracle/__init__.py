@@ -116,6 +116,27 @@ def gotoh_score(rows, t, open_: int, ext: int) -> int:
     return gotoh_score_end(rows, t, open_, ext)[0]
 
 
+def span(rows, t, open_: int, ext: int):
+    """(score, q_start, q_end, t_start, t_end) from align_canonical's two
+    score passes (src/align.rs:847-866): the column-major-first end cell,
+    then the same rule on the reversed prefixes rows[..=q_end] and
+    t[..=t_end]; start = end - the reversed problem's end.  All cells -1
+    when the score is 0.  With a negative gap cost the reversed problem can
+    score differently (the reference then searches the start from the end,
+    src/align.rs:859-863): the start cells are then -1.  Pinned in
+    tests/test_align.py by brute force over every start on small pairs and
+    by the reference's alignment tests."""
+    r, s = _rows(rows), _seq(t)
+    sc, ie, je = gotoh_score_end(r, s, open_, ext)
+    if sc <= 0:
+        return (0, -1, -1, -1, -1)
+    rs, a, b = gotoh_score_end(r[: ie + 1][::-1], s[: je + 1][::-1], open_, ext)
+    if rs != sc:
+        assert open_ < 0 or ext < 0, (rs, sc)
+        return (sc, -1, ie, -1, je)
+    return (sc, ie - a, ie, je - b, je)
+
+
 def gotoh_textbook(rows, t, open_: int, ext: int):
     r, s = _rows(rows), _seq(t)
     qe, te = C.c_int64(), C.c_int64()

@@ -33,6 +33,8 @@ PAIR_RESULT_DT = np.dtype([("swg", "<i4"), ("ung", "<i4"), ("best_diag", "<i4"),
 LAUNCH_INFO_DT = np.dtype([("kind", "<i4"), ("rows_per_lane", "<i4"), ("units", "<u8"), ("grid", "<u8"),
                            ("useful_cells", "<u8"), ("exec_cells", "<u8"), ("ms", "<f8")])
 SCORE_END_DT = np.dtype([("score", "<i4"), ("reserved", "<i4"), ("query_end", "<i8"), ("target_end", "<i8")])
+SPAN_DT = np.dtype([("score", "<i4"), ("reserved", "<i4"), ("query_start", "<i8"), ("query_end", "<i8"),
+                    ("target_start", "<i8"), ("target_end", "<i8")])
 SERVER_CONFIG_DT = np.dtype([("workers", "<u4"), ("max_batch_queries", "<u4"), ("state_budget_bytes", "<u8"),
                              ("persistent", "<u4"), ("arena_mib", "<u4")])
 REQUEST_DT = np.dtype([("query", "<u8"), ("gap_open", "<i4"), ("gap_extend", "<i4"), ("min_diag_score", "<i4"),
@@ -71,6 +73,7 @@ _SIGS = [
     ("st_qprof_len", C.c_uint64, [_P]),
     ("st_sw_score_batch", C.c_int, [_P, _P, _P, _P, C.c_uint64, C.c_int32, C.c_int32, _P, _P]),
     ("st_sw_score_end_batch", C.c_int, [_P, _P, _P, _P, C.c_uint64, C.c_int32, C.c_int32, _P]),
+    ("st_align_spans_batch", C.c_int, [_P, _P, _P, C.c_uint64, _P, _P, _P, _P, _P]),
     ("st_ungapped_score_jobs", C.c_int, [_P, _P, _P, _P, C.c_uint64, _P]),
     ("st_score_and_gate", C.c_int, [_P, _P, _P, _P, C.c_uint64, _P, C.c_uint64, C.c_int32, C.c_int32,
                                     C.c_int32, C.c_int, _P, _P, _P, _P]),

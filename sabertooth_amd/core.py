@@ -222,6 +222,26 @@ def score_batch(qp: QueryProfile, db: TargetDb, jobs) -> np.ndarray:
     return out
 
 
+def align_spans(ctx: Context, db: TargetDb, queries: Sequence[QueryProfile], hit_lists,
+                gaps: Sequence[GapCosts]) -> np.ndarray:
+    """Score, end and start of each hit (SPAN_DT, hits in list order): the
+    forward and reversed-prefix score passes of align_canonical
+    (src/align.rs:847-866) on the device."""
+    nq = len(queries)
+    off = np.zeros(nq + 1, dtype=np.uint64)
+    off[1:] = np.cumsum([len(h) for h in hit_lists]) if nq else []
+    tids = (np.concatenate([np.asarray(h, dtype=np.uint32) for h in hit_lists]) if nq
+            else np.zeros(0, np.uint32))
+    tids = np.ascontiguousarray(tids, dtype=np.uint32)
+    op = np.array([g.open for g in gaps], dtype=np.int32)
+    ex = np.array([g.extend for g in gaps], dtype=np.int32)
+    qh = (C.c_void_p * max(nq, 1))(*[q.handle.value for q in queries])
+    out = np.zeros(max(len(tids), 1), dtype=_lib.SPAN_DT)
+    check(lib().st_align_spans_batch(ctx.handle, db.handle, qh, nq, ptr(off), ptr(tids), ptr(op), ptr(ex),
+                                     ptr(out)))
+    return out[: len(tids)]
+
+
 def score_and_gate(qp: QueryProfile, db: TargetDb, jobs, recs, gaps: GapCosts,
                    min_diag_score: int, regions: bool = False):
     """(candidates, chains_used, job_scores) as score_and_gate produces them."""

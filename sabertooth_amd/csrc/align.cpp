@@ -281,6 +281,29 @@ int st_sw_score_end_host(const int32_t *rows, uint64_t m, const uint8_t *t, uint
   return ST_OK;
 }
 
+// Step 3 of align_canonical (src/align.rs:869-881): the anchored banded
+// traceback between a known start and end, the band doubled until the score
+// is reproduced.  When even the full width does not reproduce it there is no
+// alignment (*found stays 0), as the reference's release build returns None
+// (src/align.rs:875-878).
+static int traceback_span(const Problem &p, uint64_t is, uint64_t ie, uint64_t js, uint64_t je, int32_t score,
+                          uint32_t band_radius, st_alignment *out, char *cigar, uint64_t cigar_cap,
+                          uint64_t *cigar_len, int *found) {
+  const uint64_t rows_n = ie + 1 - is, cols_n = je + 1 - js;
+  uint64_t r = std::max<uint32_t>(band_radius, 1);
+  std::string cg;
+  for (;;) {
+    if (banded_tb(p, is, ie, js, je, r, score, out, cg)) break;
+    if (r >= rows_n + cols_n) {
+      std::memset(out, 0, sizeof *out);
+      return ST_OK;
+    }
+    r = std::min(r * 2, rows_n + cols_n);
+  }
+  *found = 1;
+  return emit_cigar(cg, cigar, cigar_cap, cigar_len);
+}
+
 int st_align_canonical(const int32_t *rows, const uint8_t *qres, uint64_t m, const uint8_t *t, uint64_t n,
                        int32_t gap_open, int32_t gap_extend, uint32_t band_radius, const st_score_end *known,
                        st_alignment *out, char *cigar, uint64_t cigar_cap, uint64_t *cigar_len, int *found) {
@@ -317,19 +340,39 @@ int st_align_canonical(const int32_t *rows, const uint8_t *qres, uint64_t m, con
     is = ie - ra;
     js = je - rb;
   } else if (!anchored_start(p, ie, je, score, &is, &js)) {
-    return fail(ST_ERR_INVALID, "st_align_canonical: the end cell does not carry the given score");
+    return ST_OK;  // anchored_start found no start: None (src/align.rs:861-862)
   }
-  // 3. banded anchored traceback, band doubled until the score is reproduced
-  const uint64_t rows_n = ie + 1 - is, cols_n = je + 1 - js;
-  uint64_t r = std::max<uint32_t>(band_radius, 1);
-  std::string cg;
-  for (;;) {
-    if (banded_tb(p, is, ie, js, je, r, score, out, cg)) break;
-    if (r >= rows_n + cols_n) return fail(ST_ERR_INVALID, "st_align_canonical: full-width traceback failed");
-    r = std::min(r * 2, rows_n + cols_n);
+  return traceback_span(p, is, ie, js, je, score, band_radius, out, cigar, cigar_cap, cigar_len, found);
+}
+
+int st_align_span(const int32_t *rows, const uint8_t *qres, uint64_t m, const uint8_t *t, uint64_t n,
+                  int32_t gap_open, int32_t gap_extend, uint32_t band_radius, const st_span *span,
+                  st_alignment *out, char *cigar, uint64_t cigar_cap, uint64_t *cigar_len, int *found) {
+  if (!out || !found || !span || (m && !rows) || (n && !t)) return fail(ST_ERR_INVALID, "st_align_span: null argument");
+  *found = 0;
+  if (cigar_len) *cigar_len = 0;
+  if (cigar && cigar_cap) cigar[0] = 0;
+  std::memset(out, 0, sizeof *out);
+  if (m == 0 || n == 0 || span->score <= 0) return ST_OK;
+  const bool no_start = span->query_start == -1 && span->target_start == -1;
+  if (span->query_end < 0 || span->target_end < 0 || (uint64_t)span->query_end >= m ||
+      (uint64_t)span->target_end >= n ||
+      (!no_start && (span->query_start < 0 || span->target_start < 0 || span->query_start > span->query_end ||
+                     span->target_start > span->target_end)))
+    return fail(ST_ERR_INVALID, "st_align_span: span outside the problem");
+  const Problem p{rows, qres, m, t, n, gap_open, gap_extend};
+  const uint64_t ie = (uint64_t)span->query_end, je = (uint64_t)span->target_end;
+  uint64_t is, js;
+  if (no_start) {
+    // the reversed pass did not reproduce the score: the anchored search
+    // from the end, as align_canonical falls back to
+    if (!anchored_start(p, ie, je, span->score, &is, &js))
+      return ST_OK;  // anchored_start found no start: None (src/align.rs:861-862)
+  } else {
+    is = (uint64_t)span->query_start;
+    js = (uint64_t)span->target_start;
   }
-  *found = 1;
-  return emit_cigar(cg, cigar, cigar_cap, cigar_len);
+  return traceback_span(p, is, ie, js, je, span->score, band_radius, out, cigar, cigar_cap, cigar_len, found);
 }
 
 int st_rescore_alignment(const int32_t *rows, const uint8_t *qres, uint64_t m, const uint8_t *t, uint64_t n,

@@ -7,6 +7,7 @@
 #include <cstdlib>
 #include <cstdio>
 #include <cstring>
+#include <memory>
 #include <mutex>
 #include <numeric>
 #include <string>
@@ -904,14 +905,28 @@ int st_sw_score_batch(st_ctx *ctx, const st_qprof *qp, const st_db *db, const ui
   return rc;
 }
 
+static int score_end_multi(st_ctx *ctx, const st_db *db, const st_qprof *const *qps, uint64_t nq,
+                           const uint64_t *off, const uint32_t *tids, const int32_t *open, const int32_t *ext,
+                           st_score_end *out, const char *who);
+
 int st_sw_score_end_batch(st_ctx *ctx, const st_qprof *qp, const st_db *db, const uint32_t *tids, uint64_t count,
                           int32_t gap_open, int32_t gap_extend, st_score_end *out) {
   if (!ctx || !qp || !db || (count && (!tids || !out)))
     return fail(ST_ERR_INVALID, "st_sw_score_end_batch: null argument");
   if (count == 0) return ST_OK;
   const uint64_t off[2] = {0, count};
+  return score_end_multi(ctx, db, &qp, 1, off, tids, &gap_open, &gap_extend, out, "st_sw_score_end_batch");
+}
+
+// Scores and end cells of a multi-query candidate list (the int32
+// end-tracking kernels), in candidate order.
+static int score_end_multi(st_ctx *ctx, const st_db *db, const st_qprof *const *qps, uint64_t nq,
+                           const uint64_t *off, const uint32_t *tids, const int32_t *open, const int32_t *ext,
+                           st_score_end *out, const char *who) {
+  const uint64_t count = nq ? off[nq] : 0;
+  if (count == 0) return ST_OK;
   st_batch *b = nullptr;
-  int rc = batch_create_impl(ctx, db, &qp, 1, off, tids, &gap_open, &gap_extend, nullptr, false, &b, true);
+  int rc = batch_create_impl(ctx, db, qps, nq, off, tids, open, ext, nullptr, false, &b, true);
   if (rc) return rc;
   rc = st_batch_run(b);
   if (!rc) {
@@ -923,7 +938,7 @@ int st_sw_score_end_batch(st_ctx *ctx, const st_qprof *qp, const st_db *db, cons
     if (e == hipSuccess) e = hipMemcpyAsync(te.data(), b->d_tend.p, count * 4, hipMemcpyDeviceToHost, ctx->stream);
     if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
     if (e != hipSuccess) {
-      rc = fail(ST_ERR_DEVICE, std::string("st_sw_score_end_batch: ") + hipGetErrorString(e));
+      rc = fail(ST_ERR_DEVICE, std::string(who) + ": " + hipGetErrorString(e));
     } else {
       for (uint64_t k = 0; k < count; k++) {
         out[k].score = po[k].swg;
@@ -935,6 +950,132 @@ int st_sw_score_end_batch(st_ctx *ctx, const st_qprof *qp, const st_db *db, cons
   }
   st_batch_destroy(b);
   return rc;
+}
+
+int st_align_spans_batch(st_ctx *ctx, const st_db *db, const st_qprof *const *queries, uint64_t nq,
+                         const uint64_t *hit_off, const uint32_t *hit_tids, const int32_t *gap_open,
+                         const int32_t *gap_extend, st_span *out) {
+  if (!ctx || !db || (nq && (!queries || !hit_off || !gap_open || !gap_extend)))
+    return fail(ST_ERR_INVALID, "st_align_spans_batch: null argument");
+  const uint64_t nh = nq ? hit_off[nq] : 0;
+  if (nh == 0) return ST_OK;
+  if (!out) return fail(ST_ERR_INVALID, "st_align_spans_batch: null output");
+  // 1. forward pass: score and the column-major-first end cell
+  std::vector<st_score_end> fw(nh);
+  int rc = score_end_multi(ctx, db, queries, nq, hit_off, hit_tids, gap_open, gap_extend, fw.data(),
+                           "st_align_spans_batch");
+  if (rc) return rc;
+  for (uint64_t k = 0; k < nh; k++) {
+    out[k].score = fw[k].score;
+    out[k].reserved = 0;
+    out[k].query_end = fw[k].query_end;
+    out[k].target_end = fw[k].target_end;
+    out[k].query_start = out[k].target_start = -1;
+  }
+  // 2. reverse pass over the hits with a positive score, one problem per
+  // (query, end row): the reversed prefix rows[..=e] is the window of the
+  // query's row-reversed profile that starts at row m - 1 - e
+  struct Rev {
+    uint32_t q, e;
+    uint64_t hit;
+  };
+  std::vector<Rev> rv;
+  for (uint64_t q = 0; q < nq; q++)
+    for (uint64_t k = hit_off[q]; k < hit_off[q + 1]; k++)
+      if (fw[k].score > 0) {
+        // the gathers below read rows [0, query_end] and residues [0, target_end]
+        if (fw[k].query_end < 0 || (uint64_t)fw[k].query_end >= queries[q]->m || fw[k].target_end < 0 ||
+            (uint64_t)fw[k].target_end >= db->hlen[hit_tids[k]])
+          return fail(ST_ERR_DEVICE, "st_align_spans_batch: end cell outside its problem");
+        rv.push_back({(uint32_t)q, (uint32_t)fw[k].query_end, k});
+      }
+  if (rv.empty()) return ST_OK;
+  std::stable_sort(rv.begin(), rv.end(), [](const Rev &a, const Rev &b) { return a.q != b.q ? a.q < b.q : a.e < b.e; });
+  const uint64_t nr = rv.size();
+  if (nr > 0xffffffffull) return fail(ST_ERR_INVALID, "st_align_spans_batch: more than 2^32 hits");
+  // row-reversed profiles of the queries with hits, and the reversed target
+  // prefixes (database layout: 16-byte aligned starts, code 21 padding)
+  std::vector<uint64_t> mir_off(nq + 1, 0);
+  std::vector<uint32_t> mir_ld(nq, 0);
+  for (const Rev &r : rv)
+    mir_ld[r.q] = (queries[r.q]->m + kMaxPassRows - 1) / kMaxPassRows * kMaxPassRows + kMaxPassRows;
+  for (uint64_t q = 0; q < nq; q++) mir_off[q + 1] = mir_off[q] + (uint64_t)kCodes * mir_ld[q];
+  std::vector<RevPrefix> pre(nr);
+  std::vector<uint64_t> doff(nr + 1, 0);
+  std::vector<uint32_t> rlen(nr);
+  for (uint64_t k = 0; k < nr; k++) {
+    const uint64_t h = rv[k].hit;
+    rlen[k] = (uint32_t)(fw[h].target_end + 1);
+    pre[k] = RevPrefix{hit_tids[h], rlen[k]};
+    doff[k + 1] = doff[k] + ((rlen[k] + 15ull) & ~15ull);
+  }
+  auto rdb = std::make_unique<st_db>();
+  rdb->ctx = ctx;
+  rdb->n_targets = nr;
+  rdb->n_res = doff[nr];
+  rdb->hlen = rlen;
+  DevBuf<int32_t> mirror;
+  DevBuf<RevPrefix> d_pre;
+  {
+    std::lock_guard<std::mutex> lk(ctx->mu);
+    ST_HIP(hipSetDevice(ctx->device));
+    if (mirror.alloc(std::max<uint64_t>(mir_off[nq], 1)) != hipSuccess ||
+        d_pre.upload(pre.data(), nr, ctx->stream) != hipSuccess ||
+        rdb->off.upload(doff.data(), nr + 1, ctx->stream) != hipSuccess ||
+        rdb->len.upload(rlen.data(), nr, ctx->stream) != hipSuccess || rdb->arena.alloc(doff[nr] + 64) != hipSuccess)
+      return fail(ST_ERR_NOMEM, "st_align_spans_batch: device allocation failed");
+    for (uint64_t q = 0; q < nq; q++)
+      if (mir_ld[q])
+        ST_HIP(launch_mirror_prof(queries[q]->prof32.p, queries[q]->ld, queries[q]->m, mirror.p + mir_off[q],
+                                  mir_ld[q], ctx->stream));
+    ST_HIP(launch_rev_prefix(db->arena.p, db->off.p, d_pre.p, (uint32_t)nr, rdb->off.p, rdb->arena.p, ctx->stream));
+    ST_HIP(hipStreamSynchronize(ctx->stream));
+  }
+  // one profile view per (query, end row): stride = the mirror's rows, rows
+  // readable from the window's start = its extent (the planner keeps every
+  // pass inside it)
+  std::vector<std::unique_ptr<st_qprof>> views;
+  std::vector<const st_qprof *> vq;
+  std::vector<uint64_t> voff{0};
+  std::vector<int32_t> vopen, vext;
+  std::vector<uint32_t> vt(nr);
+  for (uint64_t k = 0; k < nr; k++) {
+    if (k == 0 || rv[k].q != rv[k - 1].q || rv[k].e != rv[k - 1].e) {
+      const st_qprof *src = queries[rv[k].q];
+      const uint32_t o = src->m - 1 - rv[k].e;
+      auto v = std::make_unique<st_qprof>();
+      v->ctx = ctx;
+      v->m = rv[k].e + 1;
+      v->ld = mir_ld[rv[k].q] - o;
+      v->fits16 = false;
+      v->view.prof16 = nullptr;
+      v->view.prof32 = mirror.p + mir_off[rv[k].q] + o;
+      v->view.m = v->m;
+      v->view.ld = mir_ld[rv[k].q];
+      vq.push_back(v.get());
+      views.push_back(std::move(v));
+      vopen.push_back(gap_open[rv[k].q]);
+      vext.push_back(gap_extend[rv[k].q]);
+      voff.push_back(voff.back());
+    }
+    vt[k] = (uint32_t)k;
+    voff.back()++;
+  }
+  std::vector<st_score_end> rs(nr);
+  rc = score_end_multi(ctx, rdb.get(), vq.data(), vq.size(), voff.data(), vt.data(), vopen.data(), vext.data(),
+                       rs.data(), "st_align_spans_batch");
+  if (rc) return rc;
+  for (uint64_t k = 0; k < nr; k++) {
+    st_span &s = out[rv[k].hit];
+    // with gap costs >= 0 the reversed prefixes hold the optimal alignment
+    // and nothing better; with a negative cost the two passes can disagree
+    // (src/align.rs:859-863), and the start stays -1 for the anchored search
+    // of the traceback
+    if (rs[k].score != s.score) continue;
+    s.query_start = s.query_end - rs[k].query_end;
+    s.target_start = s.target_end - rs[k].target_end;
+  }
+  return ST_OK;
 }
 
 int st_ungapped_score_jobs(st_ctx *ctx, const st_qprof *qp, const st_db *db, const st_job *jobs, uint64_t njobs,

@@ -48,4 +48,17 @@ hipError_t launch_dp(const DpLaunch &a, int kind, int R, bool ung, hipStream_t s
 // n words at p set to 0 by a kernel (not a memset: safe under graph capture)
 hipError_t clear_u32(uint32_t *p, uint32_t n, hipStream_t s);
 
+// Reverse score pass inputs (span_kernels.hip).  A query's int32 profile
+// with its m rows reversed into dst ([kCodes][dst_ld], dst_ld >= m + 1024).
+hipError_t launch_mirror_prof(const int32_t *src, uint32_t src_ld, uint32_t m, int32_t *dst, uint32_t dst_ld,
+                              hipStream_t s);
+// Target tid's first len residues, reversed.
+struct RevPrefix {
+  uint32_t tid;
+  uint32_t len;
+};
+// n reversed prefixes into dst at doff[0 .. n] (database arena layout).
+hipError_t launch_rev_prefix(const uint8_t *arena, const uint64_t *toff, const RevPrefix *pre, uint32_t n,
+                             const uint64_t *doff, uint8_t *dst, hipStream_t s);
+
 }  // namespace st

@@ -77,12 +77,13 @@ int st_gapped_stage(const int32_t *scores, const uint32_t *tids, uint64_t n, int
   return ST_OK;
 }
 
-int st_search_hits(const int32_t *rows, const uint8_t *qres, uint64_t m, const uint8_t *arena,
-                   const uint64_t *offsets, uint64_t n_targets, const uint32_t *cand_tids, const int32_t *scores,
-                   uint64_t n_cands, int32_t gap_open, int32_t gap_extend, const st_evalue_params *ev,
-                   double max_evalue, uint64_t max_hits, double min_query_cov, double min_target_cov,
-                   uint32_t band_radius, st_hit *hits, char *cigars, uint64_t cigar_stride, uint64_t cap,
-                   uint64_t *n_hits, st_search_counters *counters) {
+static int search_hits_impl(const int32_t *rows, const uint8_t *qres, uint64_t m, const uint8_t *arena,
+                            const uint64_t *offsets, uint64_t n_targets, const uint32_t *cand_tids,
+                            const int32_t *scores, const st_span *spans, uint64_t n_cands, int32_t gap_open,
+                            int32_t gap_extend, const st_evalue_params *ev, double max_evalue, uint64_t max_hits,
+                            double min_query_cov, double min_target_cov, uint32_t band_radius, st_hit *hits,
+                            char *cigars, uint64_t cigar_stride, uint64_t cap, uint64_t *n_hits,
+                            st_search_counters *counters) {
   if (!ev || !n_hits || (m && !rows) || (n_cands && (!cand_tids || !scores || !arena || !offsets)) ||
       (cap && !hits))
     return fail(ST_ERR_INVALID, "st_search_hits: null argument");
@@ -110,8 +111,14 @@ int st_search_hits(const int32_t *rows, const uint8_t *qres, uint64_t m, const u
     uint64_t cl = 0;
     int found = 0;
     cnt.tracebacks++;
-    rc = st_align_canonical(rows, qres, m, t, n, gap_open, gap_extend, band_radius, nullptr, &a, cg.data(), cg.size(),
-                            &cl, &found);
+    if (spans) {
+      if (spans[c].score != scores[c]) return fail(ST_ERR_INVALID, "st_search_hits_spans: span score != score");
+      rc = st_align_span(rows, qres, m, t, n, gap_open, gap_extend, band_radius, &spans[c], &a, cg.data(), cg.size(),
+                         &cl, &found);
+    } else {
+      rc = st_align_canonical(rows, qres, m, t, n, gap_open, gap_extend, band_radius, nullptr, &a, cg.data(),
+                              cg.size(), &cl, &found);
+    }
     if (rc) return rc;
     if (!found) continue;
     if (a.raw_score != scores[c]) cnt.score_mismatches++;
@@ -165,6 +172,30 @@ int st_search_hits(const int32_t *rows, const uint8_t *qres, uint64_t m, const u
     counters->hits += cnt.hits;
   }
   return ST_OK;
+}
+
+int st_search_hits(const int32_t *rows, const uint8_t *qres, uint64_t m, const uint8_t *arena,
+                   const uint64_t *offsets, uint64_t n_targets, const uint32_t *cand_tids, const int32_t *scores,
+                   uint64_t n_cands, int32_t gap_open, int32_t gap_extend, const st_evalue_params *ev,
+                   double max_evalue, uint64_t max_hits, double min_query_cov, double min_target_cov,
+                   uint32_t band_radius, st_hit *hits, char *cigars, uint64_t cigar_stride, uint64_t cap,
+                   uint64_t *n_hits, st_search_counters *counters) {
+  return search_hits_impl(rows, qres, m, arena, offsets, n_targets, cand_tids, scores, nullptr, n_cands, gap_open,
+                          gap_extend, ev, max_evalue, max_hits, min_query_cov, min_target_cov, band_radius, hits,
+                          cigars, cigar_stride, cap, n_hits, counters);
+}
+
+int st_search_hits_spans(const int32_t *rows, const uint8_t *qres, uint64_t m, const uint8_t *arena,
+                         const uint64_t *offsets, uint64_t n_targets, const uint32_t *cand_tids,
+                         const int32_t *scores, const st_span *spans, uint64_t n_cands, int32_t gap_open,
+                         int32_t gap_extend, const st_evalue_params *ev, double max_evalue, uint64_t max_hits,
+                         double min_query_cov, double min_target_cov, uint32_t band_radius, st_hit *hits,
+                         char *cigars, uint64_t cigar_stride, uint64_t cap, uint64_t *n_hits,
+                         st_search_counters *counters) {
+  if (n_cands && !spans) return fail(ST_ERR_INVALID, "st_search_hits_spans: null spans");
+  return search_hits_impl(rows, qres, m, arena, offsets, n_targets, cand_tids, scores, spans, n_cands, gap_open,
+                          gap_extend, ev, max_evalue, max_hits, min_query_cov, min_target_cov, band_radius, hits,
+                          cigars, cigar_stride, cap, n_hits, counters);
 }
 
 int st_format_m8(const char *query_id, const char *target_id, const st_hit *h, char *buf, uint64_t cap,

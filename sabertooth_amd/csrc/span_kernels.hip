@@ -1,0 +1,58 @@
+// Inputs of the reverse score pass of align_canonical (src/align.rs:847-866),
+// which scores the reversed prefixes rows[..=qend] and target[..=tend] of
+// each hit to find its start.
+//
+//  * mirror_prof_kernel: a query's int32 profile with its rows in reverse
+//    order, so that the reversed prefix rows[..=e] of any end row e is the
+//    window starting at row m - 1 - e (no profile per hit).  Rows past m are
+//    zero, and at least 1024 of them follow the last row, as every pass of
+//    the DP kernels reads whole passes of up to 1024 rows.
+//  * rev_prefix_kernel: each hit's reversed target prefix, in the arena
+//    layout of a database (16-byte aligned starts, code 21 padding, 64 bytes
+//    of code 21 after the last target).
+#include <hip/hip_runtime.h>
+
+#include "dp_launch.h"
+
+namespace st {
+
+// one block per (code, 256-row chunk): dst[c][r] = src[c][m - 1 - r]
+__global__ __launch_bounds__(256) void mirror_prof_kernel(const int32_t *__restrict__ src, uint32_t src_ld,
+                                                          uint32_t m, int32_t *__restrict__ dst, uint32_t dst_ld) {
+  const uint32_t c = blockIdx.y;
+  const uint32_t r = blockIdx.x * 256 + threadIdx.x;
+  if (r >= dst_ld) return;
+  dst[(uint64_t)c * dst_ld + r] = r < m ? src[(uint64_t)c * src_ld + (m - 1 - r)] : 0;
+}
+
+// one block per reversed prefix k: bytes [doff[k], doff[k + 1]) (and the
+// 64-byte tail after the last one)
+__global__ __launch_bounds__(256) void rev_prefix_kernel(const uint8_t *__restrict__ arena,
+                                                         const uint64_t *__restrict__ toff,
+                                                         const RevPrefix *__restrict__ pre, uint32_t n,
+                                                         const uint64_t *__restrict__ doff,
+                                                         uint8_t *__restrict__ dst) {
+  const uint32_t k = blockIdx.x;
+  const RevPrefix p = pre[k];
+  const uint64_t o = doff[k];
+  const uint64_t end = doff[k + 1] + (k + 1 == n ? 64 : 0);
+  const uint8_t *t = arena + toff[p.tid];
+  for (uint64_t x = threadIdx.x; o + x < end; x += 256)
+    dst[o + x] = x < p.len ? t[p.len - 1 - x] : (uint8_t)kSentinel;
+}
+
+hipError_t launch_mirror_prof(const int32_t *src, uint32_t src_ld, uint32_t m, int32_t *dst, uint32_t dst_ld,
+                              hipStream_t s) {
+  hipLaunchKernelGGL(mirror_prof_kernel, dim3((dst_ld + 255) / 256, kCodes), dim3(256), 0, s, src, src_ld, m, dst,
+                     dst_ld);
+  return hipGetLastError();
+}
+
+hipError_t launch_rev_prefix(const uint8_t *arena, const uint64_t *toff, const RevPrefix *pre, uint32_t n,
+                             const uint64_t *doff, uint8_t *dst, hipStream_t s) {
+  if (n == 0) return hipSuccess;
+  hipLaunchKernelGGL(rev_prefix_kernel, dim3(n), dim3(256), 0, s, arena, toff, pre, n, doff, dst);
+  return hipGetLastError();
+}
+
+}  // namespace st

@@ -10,7 +10,7 @@ from dataclasses import dataclass
 
 import numpy as np
 
-from ._lib import check, lib, ptr
+from ._lib import SPAN_DT, check, lib, ptr
 
 GAP = 0xFF
 _P = C.c_void_p
@@ -264,6 +264,8 @@ _SIGS += [
                                        C.POINTER(C.c_int64), C.POINTER(C.c_int64)]),
     ("st_align_canonical", C.c_int, [_P, _P, C.c_uint64, _P, C.c_uint64, C.c_int32, C.c_int32, C.c_uint32, _P, _P,
                                      _P, C.c_uint64, C.POINTER(C.c_uint64), C.POINTER(C.c_int)]),
+    ("st_align_span", C.c_int, [_P, _P, C.c_uint64, _P, C.c_uint64, C.c_int32, C.c_int32, C.c_uint32, _P, _P,
+                                _P, C.c_uint64, C.POINTER(C.c_uint64), C.POINTER(C.c_int)]),
     ("st_rescore_alignment", C.c_int, [_P, _P, C.c_uint64, _P, C.c_uint64, C.c_int32, C.c_int32, _P, _P, _P]),
 ]
 EXPORTED = [s[0] for s in _SIGS]
@@ -340,6 +342,27 @@ def align_canonical(rows, qres, target, gaps, band_radius: int = DEFAULT_BAND_RA
                      int(o["gaps"]), int(o["gap_opens"]), buf.value.decode())
 
 
+def align_span(rows, qres, target, gaps, span, band_radius: int = DEFAULT_BAND_RADIUS):
+    """align_canonical's traceback from a known span (a SPAN_DT record, e.g.
+    from align_spans on the device): an Alignment or None."""
+    r, t = _rows(rows), _codes(target)
+    q = _codes(qres) if qres is not None else None
+    sp = np.zeros(1, SPAN_DT)
+    sp[0] = span
+    out = np.zeros(1, ALIGNMENT_DT)
+    cap = 2 * (len(r) + len(t)) + 64
+    buf = C.create_string_buffer(cap)
+    n, found = C.c_uint64(0), C.c_int(0)
+    check(_L().st_align_span(ptr(r), ptr(q) if q is not None else None, len(r), ptr(t), len(t), gaps.open,
+                             gaps.extend, band_radius, ptr(sp), ptr(out), buf, cap, C.byref(n), C.byref(found)))
+    if not found.value:
+        return None
+    o = out[0]
+    return Alignment(int(o["raw_score"]), int(o["query_start"]), int(o["query_end"]), int(o["target_start"]),
+                     int(o["target_end"]), int(o["aln_len"]), int(o["identities"]), int(o["mismatches"]),
+                     int(o["gaps"]), int(o["gap_opens"]), buf.value.decode())
+
+
 def rescore_alignment(rows, qres, target, gaps, aln: Alignment) -> int:
     """rescore_alignment (src/align.rs:1064-1135): the recomputed score;
     raises StError on any inconsistency."""
@@ -366,6 +389,9 @@ _SIGS += [
     ("st_search_hits", C.c_int, [_P, _P, C.c_uint64, _P, _P, C.c_uint64, _P, _P, C.c_uint64, C.c_int32, C.c_int32,
                                  _P, _D, C.c_uint64, _D, _D, C.c_uint32, _P, _P, C.c_uint64, C.c_uint64,
                                  C.POINTER(C.c_uint64), _P]),
+    ("st_search_hits_spans", C.c_int, [_P, _P, C.c_uint64, _P, _P, C.c_uint64, _P, _P, _P, C.c_uint64, C.c_int32,
+                                       C.c_int32, _P, _D, C.c_uint64, _D, _D, C.c_uint32, _P, _P, C.c_uint64,
+                                       C.c_uint64, C.POINTER(C.c_uint64), _P]),
     ("st_format_m8", C.c_int, [C.c_char_p, C.c_char_p, _P, _P, C.c_uint64, C.POINTER(C.c_uint64)]),
 ]
 EXPORTED = [s[0] for s in _SIGS]
@@ -391,8 +417,10 @@ def gapped_stage(scores, tids, min_score: int) -> np.ndarray:
 
 
 def search_hits(rows, qres, arena, offsets, cand_tids, scores, gaps, ev: EValueParams,
-                params: SearchParams = SearchParams(), counters: np.ndarray | None = None):
-    """Hits (HIT_DT array, hit_order) and their CIGARs for one query."""
+                params: SearchParams = SearchParams(), counters: np.ndarray | None = None, spans=None):
+    """Hits (HIT_DT array, hit_order) and their CIGARs for one query.  With
+    spans (SPAN_DT per candidate, from align_spans) every traceback starts
+    from its span and the host runs no score pass."""
     r = _rows(rows)
     q = _codes(qres) if qres is not None else None
     ar = np.ascontiguousarray(arena, dtype=np.uint8)
@@ -406,10 +434,18 @@ def search_hits(rows, qres, arena, offsets, cand_tids, scores, gaps, ev: EValueP
     cig = C.create_string_buffer(max(cap, 1) * stride)
     n = C.c_uint64(0)
     cnt = counters if counters is not None else np.zeros(1, COUNTERS_DT)
-    check(_L().st_search_hits(ptr(r), ptr(q) if q is not None else None, len(r), ptr(ar), ptr(of), len(of) - 1,
-                              ptr(ct), ptr(sc), len(ct), gaps.open, gaps.extend, ptr(ev._r), params.max_evalue,
-                              params.max_hits, params.min_query_cov, params.min_target_cov, params.band_radius,
-                              ptr(hits), cig, stride, cap, C.byref(n), ptr(cnt)))
+    common = (params.max_evalue, params.max_hits, params.min_query_cov, params.min_target_cov, params.band_radius,
+              ptr(hits), cig, stride, cap, C.byref(n), ptr(cnt))
+    qp = ptr(q) if q is not None else None
+    if spans is None:
+        check(_L().st_search_hits(ptr(r), qp, len(r), ptr(ar), ptr(of), len(of) - 1, ptr(ct), ptr(sc), len(ct),
+                                  gaps.open, gaps.extend, ptr(ev._r), *common))
+    else:
+        sp = np.ascontiguousarray(spans, dtype=SPAN_DT)
+        if len(sp) != len(ct):
+            raise ValueError("one span per candidate")
+        check(_L().st_search_hits_spans(ptr(r), qp, len(r), ptr(ar), ptr(of), len(of) - 1, ptr(ct), ptr(sc),
+                                        ptr(sp), len(ct), gaps.open, gaps.extend, ptr(ev._r), *common))
     k = min(cap, n.value)
     raw = cig.raw
     cigars = [raw[i * stride:(i + 1) * stride].split(b"\0", 1)[0].decode() for i in range(k)]

@@ -128,3 +128,110 @@ def test_rescore_rejects_a_wrong_cigar():
     a.cigar = "19M4I20M"
     with pytest.raises(Exception):
         H.rescore_alignment(O.seq_rows(q), q, t, GapCosts(11, 1), a)
+
+
+def _best_from(rows, t, o, e, i0, j0, ie, je):
+    """Best score of an alignment that begins by pairing (i0, j0) and ends
+    with a pair anywhere in [i0, ie] x [j0, je] (full Gotoh, no floor)."""
+    neg = -(1 << 40)
+    m, n = ie - i0 + 1, je - j0 + 1
+    H = [[neg] * n for _ in range(m)]
+    E = [[neg] * n for _ in range(m)]
+    F = [[neg] * n for _ in range(m)]
+    best = neg
+    for i in range(m):
+        for j in range(n):
+            s = int(rows[i0 + i][t[j0 + j]])
+            diag = 0 if i == 0 and j == 0 else (H[i - 1][j - 1] if i > 0 and j > 0 else neg)
+            M = diag + s if diag > neg else neg
+            E[i][j] = max(H[i][j - 1] - o, E[i][j - 1] - e) if j > 0 else neg
+            F[i][j] = max(H[i - 1][j] - o, F[i - 1][j] - e) if i > 0 else neg
+            H[i][j] = max(M, E[i][j], F[i][j])
+            best = max(best, M)
+    return best
+
+
+def test_span_start_is_the_last_optimal_start_by_brute_force():
+    # the reversed pass's column-major-first end is, in forward coordinates,
+    # the optimal start with the largest target position, then the largest
+    # query position, among alignments inside the end cell's prefix rectangle
+    rng = XorShift(0xA5A5F00D)
+    checked = 0
+    for o, e in [(11, 1), (3, 1), (2, 2), (1, 1), (4, 0)]:
+        for _ in range(40):
+            q = rng.seq(1 + rng.range(7), 3)
+            t = rng.seq(1 + rng.range(7), 3)
+            rows = O.seq_rows(q)
+            sc, qs, qe, ts, te = O.span(rows, t, o, e)
+            if sc <= 0:
+                assert (qs, qe, ts, te) == (-1, -1, -1, -1)
+                continue
+            starts = [(j0, i0) for i0 in range(qe + 1) for j0 in range(te + 1)
+                      if _best_from(rows, t, o, e, i0, j0, qe, te) == sc]
+            assert max(starts) == (ts, qs), (q, t, o, e)
+            checked += 1
+    assert checked > 100
+
+
+def test_align_span_equals_align_canonical():
+    rng = XorShift(0x51A2)
+    for o, e in [(11, 1), (6, 2), (2, 1), (-1, 1), (-2, 3)]:
+        for _ in range(60):
+            q = rng.seq(1 + rng.range(90), 20)
+            t = _mutate(rng, q, 5) if rng.range(3) > 0 else rng.seq(1 + rng.range(90), 20)
+            if len(t) == 0:
+                continue
+            rows = O.seq_rows(q)
+            sp = np.zeros(1, H.SPAN_DT)[0]
+            (sp["score"], sp["query_start"], sp["query_end"], sp["target_start"],
+             sp["target_end"]) = O.span(rows, t, o, e)
+            a = H.align_canonical(rows, q, t, GapCosts(o, e))
+            b = H.align_span(rows, q, t, GapCosts(o, e), sp)
+            assert a == b
+            if a and sp["query_start"] >= 0:
+                assert (a.query_start, a.query_end, a.target_start, a.target_end) == \
+                    (sp["query_start"], sp["query_end"], sp["target_start"], sp["target_end"])
+
+
+def test_align_span_rejects_bad_spans():
+    q, t = _enc("MKVLLACDEFGHIK"), _enc("MKVLLACDEFGHIK")
+    rows = O.seq_rows(q)
+    sp = np.zeros(1, H.SPAN_DT)[0]
+    (sp["score"], sp["query_start"], sp["query_end"], sp["target_start"],
+     sp["target_end"]) = O.span(rows, t, 11, 1)
+    assert H.align_span(rows, q, t, GapCosts(11, 1), sp).cigar == "14M"
+    for field, value in [("query_end", 14), ("target_end", 20), ("target_start", 14), ("query_start", -1)]:
+        bad = sp.copy()
+        bad[field] = value
+        with pytest.raises(Exception):
+            H.align_span(rows, q, t, GapCosts(11, 1), bad)
+    # a start that no alignment of the score reaches: no alignment, as the
+    # reference returns None when no band reproduces the score
+    bad = sp.copy()
+    bad["query_start"], bad["target_start"] = 3, 3
+    assert H.align_span(rows, q, t, GapCosts(11, 1), bad) is None
+    zero = np.zeros(1, H.SPAN_DT)[0]
+    assert H.align_span(rows, q, t, GapCosts(11, 1), zero) is None
+
+
+def test_negative_gap_costs_fall_back_like_the_reference():
+    # with a bonus for opening a gap the reversed pass can score differently:
+    # the start is then searched from the end (src/align.rs:859-863), and
+    # where nothing reproduces the score there is no alignment (None)
+    rng = XorShift(0x77)
+    n_nostart, n_none, n_found = 0, 0, 0
+    for _ in range(150):
+        q = rng.seq(1 + rng.range(30), 3)
+        t = rng.seq(1 + rng.range(30), 3)
+        rows = O.seq_rows(q)
+        sc, qs, qe, ts, te = O.span(rows, t, -1, 1)
+        sp = np.zeros(1, H.SPAN_DT)[0]
+        sp["score"], sp["query_start"], sp["query_end"], sp["target_start"], sp["target_end"] = sc, qs, qe, ts, te
+        a = H.align_canonical(rows, q, t, GapCosts(-1, 1))
+        assert a == H.align_span(rows, q, t, GapCosts(-1, 1), sp)
+        n_nostart += sc > 0 and qs == -1
+        n_none += a is None
+        if a:
+            n_found += 1
+            assert a.raw_score == sc
+    assert n_nostart > 20 and n_none > 20 and n_found > 20

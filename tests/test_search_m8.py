@@ -27,7 +27,7 @@ def _fasta(path):
     return ids, [O.encode(s) for s in seqs]
 
 
-def _m8_lines(ids, seqs, scorer):
+def _m8_lines(ids, seqs, scorer, spanner=None):
     offs = np.zeros(len(seqs) + 1, np.uint64)
     offs[1:] = np.cumsum([len(s) for s in seqs])
     arena = np.concatenate(seqs)
@@ -38,8 +38,9 @@ def _m8_lines(ids, seqs, scorer):
         rows = H.query_rows(q)
         tids = np.arange(len(seqs), dtype=np.uint32)
         scores = scorer(rows, seqs)
-        hits, cig = H.search_hits(rows, q, arena, offs, tids, scores, GapCosts(11, 1), ev, params, cnt)
-        out[ids[qi]] = [H.format_m8(ids[qi], ids[int(h["target"])], h) for h in hits]
+        spans = spanner(rows, seqs) if spanner else None
+        hits, cig = H.search_hits(rows, q, arena, offs, tids, scores, GapCosts(11, 1), ev, params, cnt, spans=spans)
+        out[ids[qi]] = [H.format_m8(ids[qi], ids[int(h["target"])], h) + "\t" + c for h, c in zip(hits, cig)]
     return out, cnt
 
 
@@ -59,11 +60,29 @@ def test_fixture_rows_are_reproduced():
     for q, rows in want.items():
         # the fixture's prefilter can only drop candidates: its rows are ours,
         # in the same order
-        mine = got[q]
+        mine = [r.rsplit("\t", 1)[0] for r in got[q]]
         pos = [mine.index(r) if r in mine else -1 for r in rows]
         assert -1 not in pos, (q, [r for r, p in zip(rows, pos) if p < 0], mine)
         assert pos == sorted(pos)
     assert cnt["score_mismatches"][0] == 0
+
+
+def _oracle_spans(rows, seqs):
+    sp = np.zeros(len(seqs), H.SPAN_DT)
+    for k, t in enumerate(seqs):
+        (sp[k]["score"], sp[k]["query_start"], sp[k]["query_end"], sp[k]["target_start"],
+         sp[k]["target_end"]) = O.span(rows, t, 11, 1)
+    return sp
+
+
+def test_hits_from_known_spans_equal_the_full_traceback():
+    # st_search_hits_spans (tracebacks from spans, no host score pass) gives
+    # the same hits, CIGARs and counters as st_search_hits
+    ids, seqs = _fasta(os.path.join(GOLD, "v1_fixture.fa"))
+    a, ca = _m8_lines(ids, seqs, _oracle_scores)
+    b, cb = _m8_lines(ids, seqs, _oracle_scores, _oracle_spans)
+    assert a == b and sum(len(v) for v in a.values()) >= 88
+    assert (ca == cb).all()
 
 
 def test_rust_number_formatting():
