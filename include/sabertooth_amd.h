@@ -87,6 +87,34 @@ int st_db_destroy(st_db *db);
 uint64_t st_db_targets(const st_db *db);
 uint64_t st_db_residues(const st_db *db);
 
+/* The target sections of a Sabertooth index file (<db>.stidx,
+ * src/index.rs:1-47): numeric residues, sequence offsets and identifiers.
+ * Every call maps the file and makes SeedIndex::open's checks on what it
+ * reads (src/index.rs:1654-1800): magic, the header's Hasher64 checksum,
+ * format version 1..2, the completion footer, the alphabet, section bounds
+ * and alignment, and the residue / offset / identifier end values; the
+ * k-mer sections (the prefilter's) are not read. */
+typedef struct {
+  uint32_t version;        /* format version */
+  uint32_t alphabet;       /* 1: ACDEFGHIKLMNPQRSTVWY + X */
+  uint64_t num_seqs;
+  uint64_t total_residues;
+  uint64_t max_seq_len;
+  uint64_t id_bytes;       /* bytes of the identifier section */
+} st_stidx_info;
+/* Header figures; verify != 0 also checks the four target sections'
+ * checksums (SeedIndex::verify, src/index.rs:1872-1880). */
+int st_stidx_info_read(const char *path, int verify, st_stidx_info *info);
+/* residues: total_residues bytes; offsets: num_seqs + 1 (either NULL). */
+int st_stidx_read_targets(const char *path, uint8_t *residues, uint64_t *offsets);
+/* Identifier bytes (id_bytes) and offsets (num_seqs + 1; identifier i is
+ * bytes[offsets[i] .. offsets[i+1]), SeedIndex::id src/index.rs:2031-2035). */
+int st_stidx_read_ids(const char *path, uint8_t *bytes, uint64_t *offsets);
+/* st_db_create on the index's residue and offset sections, read from the
+ * mapped file (the reference searches these sections in place,
+ * SeedIndex::seq src/index.rs:1975-1979). */
+int st_db_create_from_stidx(st_ctx *ctx, const char *path, st_db **out);
+
 /* ---- query profile ------------------------------------------------------- */
 typedef struct st_qprof st_qprof;
 

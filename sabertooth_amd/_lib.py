@@ -65,6 +65,7 @@ _SIGS = [
     ("st_aa2num", C.c_int, [_P]),
     ("st_seq_rows", C.c_int, [_P, C.c_uint64, _P]),
     ("st_db_create", C.c_int, [_P, _P, C.c_uint64, _P, C.c_uint64, C.POINTER(_P)]),
+    ("st_db_create_from_stidx", C.c_int, [_P, C.c_char_p, C.POINTER(_P)]),
     ("st_db_destroy", C.c_int, [_P]),
     ("st_db_targets", C.c_uint64, [_P]),
     ("st_db_residues", C.c_uint64, [_P]),

@@ -14,6 +14,7 @@ Every call runs on the GPU through the C ABI in ``include/sabertooth_amd.h``.
 from __future__ import annotations
 
 import ctypes as C
+import os
 from dataclasses import dataclass
 from typing import Iterable, Sequence
 
@@ -110,6 +111,22 @@ class TargetDb:
         check(lib().st_db_create(ctx.handle, ptr(r), len(r), ptr(o), len(o) - 1, C.byref(h)))
         self._h = h
         self.lengths = (o[1:] - o[:-1]).astype(np.int64)
+
+    @staticmethod
+    def from_stidx(ctx: Context, path: str) -> "TargetDb":
+        """The residue and offset sections of a Sabertooth index file
+        (<db>.stidx), checked as SeedIndex::open checks them."""
+        from . import host
+        info = host.stidx_info(path)
+        db = TargetDb.__new__(TargetDb)
+        db.ctx = ctx
+        h = C.c_void_p()
+        check(lib().st_db_create_from_stidx(ctx.handle, os.fsencode(path), C.byref(h)))
+        db._h = h
+        o = host.stidx_targets(path)[1]
+        db.lengths = (o[1:] - o[:-1]).astype(np.int64)
+        assert len(db.lengths) == info["num_seqs"]
+        return db
 
     @staticmethod
     def from_sequences(ctx: Context, seqs: Sequence[np.ndarray]) -> "TargetDb":

@@ -6,6 +6,7 @@ meanings follow the reference (src/profile.rs).
 from __future__ import annotations
 
 import ctypes as C
+import os
 from dataclasses import dataclass
 
 import numpy as np
@@ -251,6 +252,45 @@ def query_rows(query, comp_bias: bool = True, penalize_only: bool = False,
     check(_L().st_query_rows(ptr(q), len(q), 1 if comp_bias else 0, 1 if penalize_only else 0, w, lo, hi,
                              ptr(rows), C.byref(n)))
     return rows[: len(q)]
+
+
+# ---- Sabertooth index files (<db>.stidx, src/index.rs) ------------------------
+
+STIDX_INFO_DT = np.dtype([("version", "<u4"), ("alphabet", "<u4"), ("num_seqs", "<u8"),
+                          ("total_residues", "<u8"), ("max_seq_len", "<u8"), ("id_bytes", "<u8")])
+_SIGS += [
+    ("st_stidx_info_read", C.c_int, [C.c_char_p, C.c_int, _P]),
+    ("st_stidx_read_targets", C.c_int, [C.c_char_p, _P, _P]),
+    ("st_stidx_read_ids", C.c_int, [C.c_char_p, _P, _P]),
+]
+EXPORTED = [s[0] for s in _SIGS]
+
+
+def stidx_info(path: str, verify: bool = False):
+    """Header figures of an index file after SeedIndex::open's checks
+    (and, with verify, the target sections' checksums)."""
+    out = np.zeros(1, STIDX_INFO_DT)
+    check(_L().st_stidx_info_read(os.fsencode(path), 1 if verify else 0, ptr(out)))
+    return out[0]
+
+
+def stidx_targets(path: str):
+    """(residues u8, offsets u64) of an index file's target sections."""
+    info = stidx_info(path)
+    res = np.zeros(max(int(info["total_residues"]), 1), np.uint8)
+    off = np.zeros(int(info["num_seqs"]) + 1, np.uint64)
+    check(_L().st_stidx_read_targets(os.fsencode(path), ptr(res), ptr(off)))
+    return res[: int(info["total_residues"])], off
+
+
+def stidx_ids(path: str):
+    """Target identifiers of an index file (SeedIndex::id)."""
+    info = stidx_info(path)
+    b = np.zeros(max(int(info["id_bytes"]), 1), np.uint8)
+    off = np.zeros(int(info["num_seqs"]) + 1, np.uint64)
+    check(_L().st_stidx_read_ids(os.fsencode(path), ptr(b), ptr(off)))
+    raw = b.tobytes()
+    return [raw[int(off[i]):int(off[i + 1])].decode("utf-8", "replace") for i in range(len(off) - 1)]
 
 
 # ---- exact traceback of reported hits (src/align.rs) -------------------------
