@@ -246,6 +246,18 @@ int st_search_hits_spans(const int32_t *rows, const uint8_t *qres, uint64_t m, c
  * {:.3}, E-value {:.2e} (exponent without '+' or padding), bits {:.1}. */
 int st_format_m8(const char *query_id, const char *target_id, const st_hit *h, char *buf,
                  uint64_t cap, uint64_t *len);
+/* Hit::extra_cols (src/search.rs:219-230): raw, qlen, tlen, qcov and tcov
+ * ({:.3}, span / max(length, 1)), CIGAR; tab separated, no newline. */
+int st_format_m8_extra(const st_hit *h, const char *cigar, char *buf, uint64_t cap, uint64_t *len);
+/* A block of output lines as the reference's writer emits them
+ * (write_header / write_hits, src/main.rs:667-687): the header line when
+ * `header`, then one '\n'-terminated .m8 line per hit with the extra
+ * columns appended when `extra` (cigars[i] per hit).  *len = the block's
+ * length; up to cap - 1 bytes and a NUL are written to buf (call with
+ * buf = NULL for the length), so a caller streams blocks to its output. */
+int st_format_m8_block(const char *query_id, const char *const *target_ids, const st_hit *hits,
+                       const char *const *cigars, uint64_t n, int extra, int header, char *buf,
+                       uint64_t cap, uint64_t *len);
 
 /* ---- per-profile statistics (src/evalue.rs:714-749) ----------------------- */
 

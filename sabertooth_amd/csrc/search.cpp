@@ -217,4 +217,63 @@ int st_format_m8(const char *query_id, const char *target_id, const st_hit *h, c
   return ST_OK;
 }
 
+int st_format_m8_extra(const st_hit *h, const char *cigar, char *buf, uint64_t cap, uint64_t *len) {
+  // Hit::extra_cols (src/search.rs:219-230): raw score, query and target
+  // lengths, query and target coverage {:.3} (span / max(len, 1)), CIGAR
+  if (!h) return fail(ST_ERR_INVALID, "st_format_m8_extra: null hit");
+  const double qcov = (double)(h->q_end + 1 - h->q_start) / (double)std::max<uint64_t>(h->qlen, 1);
+  const double tcov = (double)(h->t_end + 1 - h->t_start) / (double)std::max<uint64_t>(h->tlen, 1);
+  const std::string s = std::to_string(h->raw_score) + "\t" + std::to_string(h->qlen) + "\t" +
+                        std::to_string(h->tlen) + "\t" + rust_f(qcov, 3) + "\t" + rust_f(tcov, 3) + "\t" +
+                        (cigar ? cigar : "");
+  if (len) *len = s.size();
+  if (buf && cap) {
+    const uint64_t k = std::min<uint64_t>(cap - 1, s.size());
+    std::memcpy(buf, s.data(), k);
+    buf[k] = 0;
+  }
+  return ST_OK;
+}
+
+int st_format_m8_block(const char *query_id, const char *const *target_ids, const st_hit *hits,
+                       const char *const *cigars, uint64_t n, int extra, int header, char *buf, uint64_t cap,
+                       uint64_t *len) {
+  // write_header / write_hits (src/main.rs:667-687): one '\n'-terminated
+  // line per hit, the extra columns after a tab when asked for
+  if (!len || (n && (!query_id || !target_ids || !hits)) || (extra && n && !cigars))
+    return fail(ST_ERR_INVALID, "st_format_m8_block: null argument");
+  std::string out;
+  if (header) {
+    out += "query\ttarget\tpident\talnlen\tmismatch\tgapopen\tqstart\tqend\ttstart\ttend\tevalue\tbits";
+    if (extra) out += "\traw\tqlen\ttlen\tqcov\ttcov\tcigar";
+    out += "\n";
+  }
+  std::vector<char> line;
+  for (uint64_t i = 0; i < n; i++) {
+    if (!target_ids[i]) return fail(ST_ERR_INVALID, "st_format_m8_block: null target id");
+    uint64_t l = 0;
+    int rc = st_format_m8(query_id, target_ids[i], &hits[i], nullptr, 0, &l);
+    if (rc) return rc;
+    line.resize(l + 1);
+    st_format_m8(query_id, target_ids[i], &hits[i], line.data(), line.size(), &l);
+    out.append(line.data(), l);
+    if (extra) {
+      rc = st_format_m8_extra(&hits[i], cigars[i], nullptr, 0, &l);
+      if (rc) return rc;
+      line.resize(l + 1);
+      st_format_m8_extra(&hits[i], cigars[i], line.data(), line.size(), &l);
+      out += "\t";
+      out.append(line.data(), l);
+    }
+    out += "\n";
+  }
+  *len = out.size();
+  if (buf && cap) {
+    const uint64_t k = std::min<uint64_t>(cap - 1, out.size());
+    std::memcpy(buf, out.data(), k);
+    buf[k] = 0;
+  }
+  return ST_OK;
+}
+
 }  // extern "C"

@@ -433,6 +433,9 @@ _SIGS += [
                                        C.c_int32, _P, _D, C.c_uint64, _D, _D, C.c_uint32, _P, _P, C.c_uint64,
                                        C.c_uint64, C.POINTER(C.c_uint64), _P]),
     ("st_format_m8", C.c_int, [C.c_char_p, C.c_char_p, _P, _P, C.c_uint64, C.POINTER(C.c_uint64)]),
+    ("st_format_m8_extra", C.c_int, [_P, C.c_char_p, _P, C.c_uint64, C.POINTER(C.c_uint64)]),
+    ("st_format_m8_block", C.c_int, [C.c_char_p, _P, _P, _P, C.c_uint64, C.c_int, C.c_int, _P, C.c_uint64,
+                                     C.POINTER(C.c_uint64)]),
 ]
 EXPORTED = [s[0] for s in _SIGS]
 
@@ -490,6 +493,30 @@ def search_hits(rows, qres, arena, offsets, cand_tids, scores, gaps, ev: EValueP
     raw = cig.raw
     cigars = [raw[i * stride:(i + 1) * stride].split(b"\0", 1)[0].decode() for i in range(k)]
     return hits[:k].copy(), cigars
+
+
+def format_m8_extra(hit, cigar: str) -> str:
+    """Hit::extra_cols: raw, qlen, tlen, qcov, tcov, CIGAR."""
+    h = np.zeros(1, HIT_DT)
+    h[0] = hit
+    buf = C.create_string_buffer(len(cigar) + 128)
+    n = C.c_uint64(0)
+    check(_L().st_format_m8_extra(ptr(h), cigar.encode(), buf, len(buf), C.byref(n)))
+    return buf.value.decode()
+
+
+def format_m8_block(query_id: str, target_ids, hits, cigars=None, extra: bool = False,
+                    header: bool = False) -> str:
+    """The lines the reference's writer emits for one query's hits."""
+    h = np.ascontiguousarray(hits, dtype=HIT_DT)
+    tid = (C.c_char_p * max(len(h), 1))(*[t.encode() for t in target_ids])
+    cg = (C.c_char_p * max(len(h), 1))(*[c.encode() for c in cigars]) if cigars is not None else None
+    n = C.c_uint64(0)
+    args = (query_id.encode(), tid, ptr(h), cg, len(h), 1 if extra else 0, 1 if header else 0)
+    check(_L().st_format_m8_block(*args, None, 0, C.byref(n)))
+    buf = C.create_string_buffer(n.value + 1)
+    check(_L().st_format_m8_block(*args, buf, len(buf), C.byref(n)))
+    return buf.raw[: n.value].decode()
 
 
 def format_m8(query_id: str, target_id: str, hit) -> str:

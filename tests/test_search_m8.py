@@ -104,3 +104,33 @@ def test_gapped_stage_filter_and_order():
     order = H.gapped_stage(scores, tids, 40)
     assert [int(tids[i]) for i in order] == [8, 0, 1, 3]
     assert len(H.gapped_stage(scores, tids, -5)) == 6  # s > 0 also required
+
+
+def test_extra_columns_and_output_blocks():
+    h = np.zeros(1, H.HIT_DT)[0]
+    h["raw_score"], h["qlen"], h["tlen"] = 57, 3, 16
+    h["q_start"], h["q_end"], h["t_start"], h["t_end"] = 0, 1, 3, 3
+    # 2/3 -> 0.667; 1/16 = 0.0625 exactly: Rust rounds the tie to even, 0.062
+    assert H.format_m8_extra(h, "2M") == "57\t3\t16\t0.667\t0.062\t2M"
+    h["qlen"], h["q_end"] = 0, 0
+    assert H.format_m8_extra(h, "") == "57\t0\t16\t1.000\t0.062\t"
+    # the writer's blocks: header, then one line per hit
+    ids, seqs = _fasta(os.path.join(GOLD, "v1_fixture.fa"))
+    got, _ = _m8_lines(ids, seqs, _oracle_scores)
+    q = ids[0]
+    offs = np.zeros(len(seqs) + 1, np.uint64)
+    offs[1:] = np.cumsum([len(x) for x in seqs])
+    rows = H.query_rows(seqs[0])
+    ev = H.EValueParams.sequence(11, 1, True, int(offs[-1]), len(seqs))
+    hits, cig = H.search_hits(rows, seqs[0], np.concatenate(seqs), offs, np.arange(len(seqs), dtype=np.uint32),
+                              _oracle_scores(rows, seqs), GapCosts(11, 1), ev, H.SearchParams(max_evalue=10.0))
+    tids = [ids[int(x["target"])] for x in hits]
+    plain = H.format_m8_block(q, tids, hits, header=True)
+    lines = plain.split("\n")
+    assert lines[0] == H.M8_HEADER and lines[-1] == "" and len(lines) == len(hits) + 2
+    assert lines[1:-1] == [r.rsplit("\t", 1)[0] for r in got[q]]
+    ext = H.format_m8_block(q, tids, hits, cig, extra=True, header=True).split("\n")
+    assert ext[0] == H.M8_HEADER + "\traw\tqlen\ttlen\tqcov\ttcov\tcigar"
+    for a, b, hh, c in zip(ext[1:-1], lines[1:-1], hits, cig):
+        assert a == b + "\t" + H.format_m8_extra(hh, c) and a.endswith("\t" + c)
+    assert H.format_m8_block(q, [], hits[:0]) == ""
