@@ -15,6 +15,7 @@ from __future__ import annotations
 
 import ctypes as C
 import os
+import time
 from dataclasses import dataclass
 from typing import Iterable, Sequence
 
@@ -406,7 +407,10 @@ class Server:
 
     def run_trace(self, requests, arrival_ms):
         """Open-loop replay: requests[k] = (qp, cands, gaps, gate) submitted at
-        arrival_ms[k]; returns (list of result arrays, latencies in ms)."""
+        arrival_ms[k]; returns (list of result arrays, latencies in ms).
+        ``last_trace_ms`` is the server's wall time for the trace: from the
+        first submission to the last completion (the request records and
+        output arrays are built before it starts)."""
         n = len(requests)
         recs = np.zeros(max(n, 1), dtype=_lib.REQUEST_DT)
         keep = []
@@ -425,7 +429,10 @@ class Server:
             recs[f][:n] = v
         arr = np.ascontiguousarray(arrival_ms, dtype=np.float64)
         lat = np.zeros(max(n, 1), dtype=np.float64)
-        check(lib().st_server_run_trace(self._h, ptr(recs), ptr(arr), n, ptr(lat)))
+        t0 = time.perf_counter()
+        rc = lib().st_server_run_trace(self._h, ptr(recs), ptr(arr), n, ptr(lat))
+        self.last_trace_ms = 1e3 * (time.perf_counter() - t0)
+        check(rc)
         return [out[: len(c)].copy() for c, out in keep], lat[:n].copy()
 
     def stats(self) -> dict:

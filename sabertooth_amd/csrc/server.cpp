@@ -201,7 +201,7 @@ struct Flight {
 // The persistent serving engine: work queue, pools and the kernel's launches.
 struct Persist {
   static constexpr uint64_t kRingUnits = 1ull << 20;
-  static constexpr uint64_t kResultRows = 1ull << 22;
+  static constexpr uint64_t kResultRows = 1ull << 23;
   static constexpr uint32_t kReqSlots = 1u << 16;
   static constexpr uint64_t kHalfQuads = 20480;  // pass-boundary half of one block
   uint64_t arena_bytes = 4ull << 30;
