@@ -201,6 +201,53 @@ typedef struct {
   uint32_t j1;
 } st_target_rec;
 
+/* ---- seed hits -> ungapped jobs (the prefilter's step before) -------------- */
+
+/* An index's posting lists resident on the device, each posting decoded to
+ * (target << 32) | position (SeedIndex::posting, src/index.rs:2294-2303),
+ * in the index's order. */
+typedef struct st_postings st_postings;
+int st_postings_create(st_ctx *ctx, const uint64_t *postings, uint64_t n, st_postings **out);
+int st_postings_destroy(st_postings *p);
+uint64_t st_postings_count(const st_postings *p);
+
+/* One seed of a query (Seed, src/prefilter.rs:704-708): query position and
+ * its posting range [cur, end). */
+typedef struct {
+  uint32_t qpos;
+  uint32_t reserved;
+  uint64_t cur;
+  uint64_t end;
+} st_seed;
+
+/* The PrefilterParams fields the seed-hit stage reads
+ * (src/prefilter.rs:263-303), and the seed pattern's span. */
+typedef struct {
+  uint32_t max_diagonals; /* diagonals per target, 1..16 (default 4) */
+  uint32_t min_hits;      /* seed hits per target (default 1) */
+  uint32_t diag_hits;     /* seed hits on the best diagonal (default 1) */
+  uint32_t diag_window;   /* residues around the seed hits, 0 = whole diagonal (default 48) */
+  uint32_t span;          /* SeedPattern::span */
+  uint32_t reserved;
+} st_hit_params;
+
+/* The seed-hit stage of prefilter_query with segment evidence (xdrop = 0):
+ * every seed's postings become hits (target, diagonal = position - qpos,
+ * qpos) (src/prefilter.rs:999-1086); hits are grouped by target and
+ * diagonal; a target with at least min_hits hits whose best diagonal has at
+ * least diag_hits gets its top max_diagonals diagonals (hits descending,
+ * diagonal ascending; top_diagonals src/prefilter.rs:1194-1225) as jobs over
+ * diag_range, narrowed to [qmin - diag_window, qmax + span + diag_window)
+ * when diag_window > 0, and one record (src/prefilter.rs:1412-1492).
+ * Targets come in ascending order, as the reference pushes them; jobs and
+ * records are what st_score_and_gate takes.  *njobs / *nrecs receive the
+ * sizes; when they exceed jobs_cap / recs_cap nothing is written and
+ * ST_ERR_CAPACITY is returned (caps of 0 query the sizes). */
+int st_seed_hits_to_jobs(st_ctx *ctx, const st_db *db, const st_postings *post, uint64_t m,
+                         const st_seed *seeds, uint64_t nseeds, const st_hit_params *params,
+                         st_job *jobs, uint64_t jobs_cap, uint64_t *njobs, st_target_rec *recs,
+                         uint64_t recs_cap, uint64_t *nrecs);
+
 /* Candidate (src/prefilter.rs:370-381). */
 typedef struct {
   uint32_t target_id;

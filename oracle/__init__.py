@@ -187,6 +187,62 @@ def score_and_gate(rows, arena, offsets, jobs, recs, open_, ext, min_diag_score,
     return out[:nc].copy(), chains.value, scores[:len(jobs)].copy()
 
 
+def seed_hits_to_jobs(m, seeds, postings, tlen, max_diagonals, min_hits, diag_hits, diag_window, span):
+    """(jobs, recs) of the seed-hit stage of prefilter_query with segment
+    evidence: hits packed as (target, diagonal + m, qpos) and sorted
+    (src/prefilter.rs:1033-1056), targets with >= min_hits hits, their
+    diagonals kept by insertion in count order (top_diagonals,
+    src/prefilter.rs:1194-1225), the best with >= diag_hits hits, then at most
+    16 jobs per target over diag_range narrowed by diag_window
+    (src/prefilter.rs:1432-1487).  seeds: (qpos, cur, end) rows; postings:
+    (target << 32) | position.  Pinned in tests/test_oracle.py by a
+    brute-force count of hits per (target, diagonal) ranked by sorting."""
+    postings = np.asarray(postings, dtype=np.uint64)
+    tid, dm, qp = [], [], []
+    for q, cur, end in seeds:
+        p = postings[int(cur):int(end)]
+        tid.append((p >> np.uint64(32)).astype(np.int64))
+        dm.append((p & np.uint64(0xFFFFFFFF)).astype(np.int64) + m - int(q))
+        qp.append(np.full(len(p), int(q), np.int64))
+    if not tid:
+        return np.zeros(0, JOB_DT), np.zeros(0, REC_DT)
+    tid, dm, qp = np.concatenate(tid), np.concatenate(dm), np.concatenate(qp)
+    order = np.lexsort((qp, dm, tid))
+    tid, dm, qp = tid[order], dm[order], qp[order]
+    jobs, recs = [], []
+    i, n = 0, len(tid)
+    while i < n:
+        j = i
+        while j < n and tid[j] == tid[i]:
+            j += 1
+        t = int(tid[i])
+        if j - i >= min_hits:
+            top = []  # (count, diag, qmin, qmax)
+            a = i
+            while a < j:
+                b = a
+                while b < j and dm[b] == dm[a]:
+                    b += 1
+                run = (b - a, int(dm[a]) - m, int(qp[a]), int(qp[b - 1]))
+                pos = next((k for k, x in enumerate(top) if run[0] > x[0]), len(top))
+                if pos < max_diagonals:
+                    top.insert(pos, run)
+                    if len(top) > max_diagonals:
+                        top.pop()
+                a = b
+            if top and top[0][0] >= diag_hits:
+                j0 = len(jobs)
+                for cnt, d, qmin, qmax in top[:16]:
+                    lo, hi = diag_range(m, int(tlen[t]), d)
+                    if diag_window > 0:
+                        lo = max(lo, max(qmin - diag_window, 0))
+                        hi = min(hi, qmax + span + diag_window)
+                    jobs.append((t, d, lo, max(hi, lo)))
+                recs.append((t, j - i, j0, len(jobs)))
+        i = j
+    return np.array(jobs, dtype=JOB_DT), np.array(recs, dtype=REC_DT)
+
+
 def size_adjusted_gate(gate: int, db_residues: int) -> int:
     return lib().or_size_adjusted_gate(gate, db_residues)
 

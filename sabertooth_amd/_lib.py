@@ -33,6 +33,9 @@ PAIR_RESULT_DT = np.dtype([("swg", "<i4"), ("ung", "<i4"), ("best_diag", "<i4"),
 LAUNCH_INFO_DT = np.dtype([("kind", "<i4"), ("rows_per_lane", "<i4"), ("units", "<u8"), ("grid", "<u8"),
                            ("useful_cells", "<u8"), ("exec_cells", "<u8"), ("ms", "<f8")])
 SCORE_END_DT = np.dtype([("score", "<i4"), ("reserved", "<i4"), ("query_end", "<i8"), ("target_end", "<i8")])
+SEED_DT = np.dtype([("qpos", "<u4"), ("reserved", "<u4"), ("cur", "<u8"), ("end", "<u8")])
+HIT_PARAMS_DT = np.dtype([("max_diagonals", "<u4"), ("min_hits", "<u4"), ("diag_hits", "<u4"),
+                          ("diag_window", "<u4"), ("span", "<u4"), ("reserved", "<u4")])
 SPAN_DT = np.dtype([("score", "<i4"), ("reserved", "<i4"), ("query_start", "<i8"), ("query_end", "<i8"),
                     ("target_start", "<i8"), ("target_end", "<i8")])
 SERVER_CONFIG_DT = np.dtype([("workers", "<u4"), ("max_batch_queries", "<u4"), ("state_budget_bytes", "<u8"),
@@ -75,6 +78,11 @@ _SIGS = [
     ("st_sw_score_batch", C.c_int, [_P, _P, _P, _P, C.c_uint64, C.c_int32, C.c_int32, _P, _P]),
     ("st_sw_score_end_batch", C.c_int, [_P, _P, _P, _P, C.c_uint64, C.c_int32, C.c_int32, _P]),
     ("st_align_spans_batch", C.c_int, [_P, _P, _P, C.c_uint64, _P, _P, _P, _P, _P]),
+    ("st_postings_create", C.c_int, [_P, _P, C.c_uint64, C.POINTER(_P)]),
+    ("st_postings_destroy", C.c_int, [_P]),
+    ("st_postings_count", C.c_uint64, [_P]),
+    ("st_seed_hits_to_jobs", C.c_int, [_P, _P, _P, C.c_uint64, _P, C.c_uint64, _P, _P, C.c_uint64,
+                                       C.POINTER(C.c_uint64), _P, C.c_uint64, C.POINTER(C.c_uint64)]),
     ("st_ungapped_score_jobs", C.c_int, [_P, _P, _P, _P, C.c_uint64, _P]),
     ("st_score_and_gate", C.c_int, [_P, _P, _P, _P, C.c_uint64, _P, C.c_uint64, C.c_int32, C.c_int32,
                                     C.c_int32, C.c_int, _P, _P, _P, _P]),

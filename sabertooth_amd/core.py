@@ -260,6 +260,55 @@ def align_spans(ctx: Context, db: TargetDb, queries: Sequence[QueryProfile], hit
     return out[: len(tids)]
 
 
+class Postings:
+    """An index's posting lists on the device: (target << 32) | position."""
+
+    def __init__(self, ctx: Context, postings):
+        p = np.ascontiguousarray(postings, dtype=np.uint64)
+        h = C.c_void_p()
+        check(lib().st_postings_create(ctx.handle, ptr(p), len(p), C.byref(h)))
+        self.ctx, self._h = ctx, h
+
+    @property
+    def handle(self):
+        return self._h
+
+    def __len__(self):
+        return int(lib().st_postings_count(self._h))
+
+    def close(self):
+        if self._h:
+            lib().st_postings_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def seed_hits_to_jobs(ctx: Context, db: TargetDb, postings: Postings, m: int, seeds, max_diagonals: int = 4,
+                      min_hits: int = 1, diag_hits: int = 1, diag_window: int = 48, span: int = 6):
+    """(jobs, recs) of the seed-hit stage of prefilter_query on the device;
+    seeds: (qpos, cur, end) rows."""
+    sd = np.zeros(max(len(seeds), 1), dtype=_lib.SEED_DT)
+    for k, (q, a, b) in enumerate(seeds):
+        sd[k]["qpos"], sd[k]["cur"], sd[k]["end"] = q, a, b
+    hp = np.zeros(1, dtype=_lib.HIT_PARAMS_DT)
+    hp["max_diagonals"], hp["min_hits"], hp["diag_hits"] = max_diagonals, min_hits, diag_hits
+    hp["diag_window"], hp["span"] = diag_window, span
+    nj, nr = C.c_uint64(0), C.c_uint64(0)
+    args = (ctx.handle, db.handle, postings.handle, m, ptr(sd), len(seeds), ptr(hp))
+    rc = lib().st_seed_hits_to_jobs(*args, None, 0, C.byref(nj), None, 0, C.byref(nr))
+    if rc not in (_lib.ST_OK, _lib.ST_ERR_CAPACITY):
+        check(rc)
+    jobs = np.zeros(max(nj.value, 1), dtype=JOB_DT)
+    recs = np.zeros(max(nr.value, 1), dtype=REC_DT)
+    check(lib().st_seed_hits_to_jobs(*args, ptr(jobs), len(jobs), C.byref(nj), ptr(recs), len(recs), C.byref(nr)))
+    return jobs[: nj.value], recs[: nr.value]
+
+
 def score_and_gate(qp: QueryProfile, db: TargetDb, jobs, recs, gaps: GapCosts,
                    min_diag_score: int, regions: bool = False):
     """(candidates, chains_used, job_scores) as score_and_gate produces them."""

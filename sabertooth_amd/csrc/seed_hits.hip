@@ -1,0 +1,354 @@
+// The seed-hit stage of prefilter_query in segment-evidence mode
+// (src/prefilter.rs:999-1086, 1097-1225, 1412-1492) on the device: the
+// posting lists of a query's seeds become hits, the hits are grouped by
+// target and diagonal, and each target's best-supported diagonals become the
+// ungapped jobs and target records that score_and_gate consumes.
+//
+// A hit packs (target, diagonal + m, query position) into one 64-bit key,
+// most significant first, as the reference does; one device radix sort of
+// the whole query's keys replaces its per-block sorts (the blocks only bound
+// host memory: every block's targets are contiguous and ascending, so the
+// concatenated output is the same).  With the full key sorted, each
+// (target, diagonal) run's smallest and largest query positions are its first
+// and last hits.  Runs and targets are found with boundary flags and scans;
+// one thread per target keeps its top diagonals (hits descending, diagonal
+// ascending, the order top_diagonals' insertion keeps) and emits its jobs.
+#include <hip/hip_runtime.h>
+#include <hipcub/hipcub.hpp>
+
+#include <algorithm>
+#include <string>
+#include <vector>
+
+#include "../../include/sabertooth_amd.h"
+#include "host_internal.h"
+
+struct st_postings {
+  st_ctx *ctx = nullptr;
+  st::DevBuf<uint64_t> post;  // (target << 32) | position
+  uint64_t n = 0;
+};
+
+namespace st {
+namespace {
+
+constexpr int kMaxJobsPerTarget = 16;  // evaluate_block_kadane takes at most 16 runs
+
+struct SeedDev {
+  uint32_t qpos, reserved;
+  uint64_t cur, end;
+};
+
+struct Layout {
+  uint32_t qbits, dbits, sh_t;
+  uint64_t m;
+};
+
+__device__ __forceinline__ uint64_t td_of(uint64_t k, uint32_t qbits) { return k >> qbits; }
+
+// One wave per seed: its postings [cur, end) -> keys at off[s] ..
+__global__ __launch_bounds__(256) void expand_kernel(const SeedDev *__restrict__ seeds, uint64_t nseeds,
+                                                     const uint64_t *__restrict__ off,
+                                                     const uint64_t *__restrict__ post, Layout lay,
+                                                     uint64_t n_targets, uint64_t *__restrict__ keys,
+                                                     uint32_t *__restrict__ bad) {
+  const uint64_t s = (uint64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+  const int lane = threadIdx.x & 63;
+  if (s >= nseeds) return;
+  const SeedDev sd = seeds[s];
+  const uint64_t o = off[s];
+  for (uint64_t k = sd.cur + lane; k < sd.end; k += 64) {
+    const uint64_t p = post[k];
+    const uint64_t tid = p >> 32, tpos = p & 0xffffffffull;
+    if (tid >= n_targets || tpos + lay.m - sd.qpos >= (1ull << lay.dbits)) {
+      *bad = 1;
+      keys[o + (k - sd.cur)] = 0;
+      continue;
+    }
+    keys[o + (k - sd.cur)] = (tid << lay.sh_t) | ((tpos + lay.m - sd.qpos) << lay.qbits) | sd.qpos;
+  }
+}
+
+// run_start[i]: key i begins a (target, diagonal) run
+__global__ void run_flags_kernel(const uint64_t *__restrict__ keys, uint64_t n, uint32_t qbits,
+                                 uint32_t *__restrict__ flag) {
+  const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  flag[i] = (i == 0 || td_of(keys[i], qbits) != td_of(keys[i - 1], qbits)) ? 1u : 0u;
+}
+
+// run r's first key index; tflag[r]: run r begins a target
+__global__ void run_first_kernel(const uint64_t *__restrict__ keys, uint64_t n, uint32_t qbits, uint32_t sh_t,
+                                 const uint32_t *__restrict__ flag, const uint32_t *__restrict__ idx,
+                                 uint32_t *__restrict__ first, uint32_t *__restrict__ tflag) {
+  const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n || !flag[i]) return;
+  const uint32_t r = idx[i];
+  first[r] = (uint32_t)i;
+  tflag[r] = (i == 0 || (keys[i] >> sh_t) != (keys[i - 1] >> sh_t)) ? 1u : 0u;
+}
+
+__global__ void target_first_kernel(uint64_t nruns, const uint32_t *__restrict__ tflag,
+                                    const uint32_t *__restrict__ tidx, uint32_t *__restrict__ tfirst) {
+  const uint64_t r = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= nruns || !tflag[r]) return;
+  tfirst[tidx[r]] = (uint32_t)r;
+}
+
+// One thread per target: its top diagonals (top_diagonals, src/prefilter.rs:
+// 1194-1225) and the gates min_hits / diag_hits (src/prefilter.rs:1445-1465).
+__global__ void target_top_kernel(const uint64_t *__restrict__ keys, uint64_t nkeys, uint64_t ntg,
+                                  uint64_t nruns, const uint32_t *__restrict__ rfirst,
+                                  const uint32_t *__restrict__ tfirst, uint32_t kmax, uint32_t min_hits,
+                                  uint32_t diag_hits, uint32_t *__restrict__ njobs,
+                                  uint32_t *__restrict__ top_runs) {
+  const uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= ntg) return;
+  const uint32_t r0 = tfirst[t];
+  const uint32_t r1 = t + 1 < ntg ? tfirst[t + 1] : (uint32_t)nruns;
+  const uint64_t h0 = rfirst[r0];
+  const uint64_t h1 = r1 < nruns ? rfirst[r1] : nkeys;
+  uint32_t nj = 0;
+  if (h1 - h0 >= min_hits) {
+    uint32_t cnt[kMaxJobsPerTarget], run[kMaxJobsPerTarget];
+    uint32_t ntop = 0;
+    for (uint32_t r = r0; r < r1; r++) {
+      const uint32_t c = (uint32_t)((r + 1 < nruns ? rfirst[r + 1] : nkeys) - rfirst[r]);
+      // the first held run with a smaller count (runs arrive by diagonal
+      // ascending: equal counts keep the earlier diagonal first)
+      uint32_t pos = ntop;
+      for (uint32_t k = 0; k < ntop; k++)
+        if (c > cnt[k]) {
+          pos = k;
+          break;
+        }
+      if (pos < kmax) {
+        const uint32_t last = ntop < kmax ? ntop : kmax - 1;
+        for (uint32_t k = last; k > pos; k--) {
+          cnt[k] = cnt[k - 1];
+          run[k] = run[k - 1];
+        }
+        cnt[pos] = c;
+        run[pos] = r;
+        if (ntop < kmax) ntop++;
+      }
+    }
+    if (ntop > 0 && cnt[0] >= diag_hits) {
+      nj = ntop;
+      for (uint32_t k = 0; k < ntop; k++) top_runs[t * kMaxJobsPerTarget + k] = run[k];
+    }
+  }
+  njobs[t] = nj;
+}
+
+// The jobs and the record of each kept target (src/prefilter.rs:1466-1487).
+__global__ void emit_kernel(const uint64_t *__restrict__ keys, uint64_t nkeys, uint64_t ntg, uint64_t nruns,
+                            const uint32_t *__restrict__ rfirst, const uint32_t *__restrict__ tfirst,
+                            const uint32_t *__restrict__ njobs, const uint32_t *__restrict__ joff,
+                            const uint32_t *__restrict__ roff, const uint32_t *__restrict__ top_runs,
+                            const uint32_t *__restrict__ tlen, Layout lay, uint32_t diag_window, uint32_t span,
+                            st_job *__restrict__ jobs, st_target_rec *__restrict__ recs) {
+  const uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= ntg || njobs[t] == 0) return;
+  const uint32_t r0 = tfirst[t];
+  const uint32_t r1 = t + 1 < ntg ? tfirst[t + 1] : (uint32_t)nruns;
+  const uint64_t h0 = rfirst[r0];
+  const uint64_t h1 = r1 < nruns ? rfirst[r1] : nkeys;
+  const uint32_t tid = (uint32_t)(keys[h0] >> lay.sh_t);
+  const int64_t n = tlen[tid], m = (int64_t)lay.m;
+  const uint64_t qmask = (1ull << lay.qbits) - 1;
+  const uint32_t j0 = joff[t];
+  for (uint32_t k = 0; k < njobs[t]; k++) {
+    const uint32_t r = top_runs[t * kMaxJobsPerTarget + k];
+    const uint64_t a = rfirst[r], b = (r + 1 < nruns ? rfirst[r + 1] : nkeys) - 1;
+    const int64_t diag = (int64_t)((keys[a] >> lay.qbits) & ((1ull << lay.dbits) - 1)) - m;
+    const int64_t qmin = (int64_t)(keys[a] & qmask), qmax = (int64_t)(keys[b] & qmask);
+    // diag_range (src/ungapped.rs:38-48): query positions whose target
+    // position d + i lies in [0, n)
+    int64_t lo = diag < 0 ? -diag : 0;
+    int64_t hi = n - diag < m ? n - diag : m;
+    if (hi <= lo) lo = hi = 0;
+    if (diag_window > 0) {
+      const int64_t wl = qmin - (int64_t)diag_window;
+      lo = lo > wl ? lo : (wl > 0 ? wl : 0);
+      const int64_t wh = qmax + (int64_t)span + (int64_t)diag_window;
+      hi = hi < wh ? hi : wh;
+    }
+    st_job j;
+    j.tid = tid;
+    j.diag = (int32_t)diag;
+    j.lo = (uint32_t)lo;
+    j.hi = (uint32_t)(hi > lo ? hi : lo);
+    jobs[j0 + k] = j;
+  }
+  st_target_rec rc;
+  rc.tid = tid;
+  rc.hits = (uint32_t)(h1 - h0);
+  rc.j0 = j0;
+  rc.j1 = j0 + njobs[t];
+  recs[roff[t]] = rc;
+}
+
+__global__ void flag_nonzero_kernel(const uint32_t *__restrict__ v, uint64_t n, uint32_t *__restrict__ f) {
+  const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) f[i] = v[i] ? 1u : 0u;
+}
+
+uint32_t bits_for(uint64_t x) { return x ? 64 - __builtin_clzll(x) : 0; }
+
+unsigned grid_of(uint64_t n, unsigned b = 256) { return (unsigned)((n + b - 1) / b); }
+
+// exclusive scan of n words into out; returns out[n-1] + in[n-1] (the total)
+hipError_t scan_total(const uint32_t *in, uint32_t *out, uint64_t n, DevBuf<uint8_t> &tmp, hipStream_t s,
+                      uint64_t *total) {
+  size_t tb = 0;
+  hipError_t e = hipcub::DeviceScan::ExclusiveSum(nullptr, tb, in, out, (int)n, s);
+  if (e != hipSuccess) return e;
+  if (tmp.n < tb && (e = tmp.alloc(tb)) != hipSuccess) return e;
+  if ((e = hipcub::DeviceScan::ExclusiveSum(tmp.p, tb, in, out, (int)n, s)) != hipSuccess) return e;
+  uint32_t a = 0, b = 0;
+  if ((e = hipMemcpyAsync(&a, out + n - 1, 4, hipMemcpyDeviceToHost, s)) != hipSuccess) return e;
+  if ((e = hipMemcpyAsync(&b, in + n - 1, 4, hipMemcpyDeviceToHost, s)) != hipSuccess) return e;
+  if ((e = hipStreamSynchronize(s)) != hipSuccess) return e;
+  *total = (uint64_t)a + b;
+  return hipSuccess;
+}
+
+}  // namespace
+}  // namespace st
+
+using namespace st;
+
+extern "C" {
+
+int st_postings_create(st_ctx *ctx, const uint64_t *postings, uint64_t n, st_postings **out) {
+  if (!ctx || !out || (n && !postings)) return fail(ST_ERR_INVALID, "st_postings_create: null argument");
+  *out = nullptr;
+  std::lock_guard<std::mutex> lk(ctx->mu);
+  ST_HIP(hipSetDevice(ctx->device));
+  auto *p = new st_postings();
+  p->ctx = ctx;
+  p->n = n;
+  const hipError_t e = n ? p->post.upload(postings, n, ctx->stream) : p->post.alloc(1);
+  if (e != hipSuccess) {
+    delete p;
+    return fail(ST_ERR_NOMEM, "st_postings_create: device allocation failed");
+  }
+  *out = p;
+  return ST_OK;
+}
+
+int st_postings_destroy(st_postings *p) {
+  delete p;
+  return ST_OK;
+}
+
+uint64_t st_postings_count(const st_postings *p) { return p ? p->n : 0; }
+
+int st_seed_hits_to_jobs(st_ctx *ctx, const st_db *db, const st_postings *post, uint64_t m,
+                         const st_seed *seeds, uint64_t nseeds, const st_hit_params *hp, st_job *jobs,
+                         uint64_t jobs_cap, uint64_t *njobs, st_target_rec *recs, uint64_t recs_cap,
+                         uint64_t *nrecs) {
+  if (!ctx || !db || !post || !hp || !njobs || !nrecs || (nseeds && !seeds))
+    return fail(ST_ERR_INVALID, "st_seed_hits_to_jobs: null argument");
+  *njobs = *nrecs = 0;
+  if (db->ctx != ctx || post->ctx != ctx) return fail(ST_ERR_INVALID, "st_seed_hits_to_jobs: handle of another context");
+  if (hp->max_diagonals < 1 || hp->max_diagonals > (uint32_t)kMaxJobsPerTarget)
+    return fail(ST_ERR_INVALID, "st_seed_hits_to_jobs: max_diagonals must be 1..16");
+  if (m == 0 || nseeds == 0 || db->n_targets == 0) return ST_OK;
+  // hits per seed -> key offsets
+  std::vector<uint64_t> off(nseeds + 1, 0);
+  for (uint64_t s = 0; s < nseeds; s++) {
+    const st_seed &sd = seeds[s];
+    if (sd.cur > sd.end || sd.end > post->n || sd.qpos >= m)
+      return fail(ST_ERR_INVALID, "st_seed_hits_to_jobs: seed outside the postings or the query");
+    off[s + 1] = off[s] + (sd.end - sd.cur);
+  }
+  const uint64_t nh = off[nseeds];
+  if (nh == 0) return ST_OK;
+  // the device sort and scans count items in int
+  if (nh > 0x7fffffffull) return fail(ST_ERR_INVALID, "st_seed_hits_to_jobs: more than 2^31 - 1 seed hits");
+  uint32_t maxlen = 0;
+  for (uint32_t l : db->hlen) maxlen = std::max(maxlen, l);
+  Layout lay;
+  lay.m = m;
+  lay.qbits = bits_for(m - 1);
+  lay.dbits = bits_for(m + maxlen);
+  lay.sh_t = lay.qbits + lay.dbits;
+  const uint32_t end_bit = lay.sh_t + bits_for(db->n_targets - 1);
+  if (end_bit > 64) return fail(ST_ERR_INVALID, "st_seed_hits_to_jobs: query, targets and database too large for a 64-bit hit key");
+  std::lock_guard<std::mutex> lk(ctx->mu);
+  ST_HIP(hipSetDevice(ctx->device));
+  hipStream_t s = ctx->stream;
+  DevBuf<SeedDev> d_seeds;
+  DevBuf<uint64_t> d_off, k0, k1;
+  DevBuf<uint32_t> flag, idx, bad;
+  DevBuf<uint8_t> tmp;
+  if (d_seeds.upload((const SeedDev *)seeds, nseeds, s) != hipSuccess || d_off.upload(off.data(), nseeds, s) != hipSuccess ||
+      k0.alloc(nh) != hipSuccess || k1.alloc(nh) != hipSuccess || flag.alloc(nh) != hipSuccess ||
+      idx.alloc(nh) != hipSuccess || bad.alloc(1) != hipSuccess)
+    return fail(ST_ERR_NOMEM, "st_seed_hits_to_jobs: device allocation failed");
+  ST_HIP(clear_u32(bad.p, 1, s));
+  hipLaunchKernelGGL(expand_kernel, dim3((unsigned)((nseeds + 3) / 4)), dim3(256), 0, s, d_seeds.p, nseeds, d_off.p,
+                     post->post.p, lay, db->n_targets, k0.p, bad.p);
+  ST_HIP(hipGetLastError());
+  uint32_t h_bad = 0;
+  ST_HIP(hipMemcpyAsync(&h_bad, bad.p, 4, hipMemcpyDeviceToHost, s));
+  ST_HIP(hipStreamSynchronize(s));
+  if (h_bad) return fail(ST_ERR_INVALID, "st_seed_hits_to_jobs: a posting names a target outside the database");
+  // sort the whole keys (the query positions included: runs come out with
+  // their smallest and largest positions first and last)
+  size_t tb = 0;
+  ST_HIP(hipcub::DeviceRadixSort::SortKeys(nullptr, tb, k0.p, k1.p, (int)nh, 0, (int)end_bit, s));
+  if (tmp.alloc(tb) != hipSuccess) return fail(ST_ERR_NOMEM, "st_seed_hits_to_jobs: device allocation failed");
+  ST_HIP(hipcub::DeviceRadixSort::SortKeys(tmp.p, tb, k0.p, k1.p, (int)nh, 0, (int)end_bit, s));
+  const uint64_t *keys = k1.p;
+  // runs
+  hipLaunchKernelGGL(run_flags_kernel, dim3(grid_of(nh)), dim3(256), 0, s, keys, nh, lay.qbits, flag.p);
+  ST_HIP(hipGetLastError());
+  uint64_t nruns = 0;
+  ST_HIP(scan_total(flag.p, idx.p, nh, tmp, s, &nruns));
+  DevBuf<uint32_t> rfirst, tflag, tidx;
+  if (rfirst.alloc(nruns) != hipSuccess || tflag.alloc(nruns) != hipSuccess || tidx.alloc(nruns) != hipSuccess)
+    return fail(ST_ERR_NOMEM, "st_seed_hits_to_jobs: device allocation failed");
+  hipLaunchKernelGGL(run_first_kernel, dim3(grid_of(nh)), dim3(256), 0, s, keys, nh, lay.qbits, lay.sh_t, flag.p,
+                     idx.p, rfirst.p, tflag.p);
+  ST_HIP(hipGetLastError());
+  // targets
+  uint64_t ntg = 0;
+  ST_HIP(scan_total(tflag.p, tidx.p, nruns, tmp, s, &ntg));
+  DevBuf<uint32_t> tfirst, nj, joff, tkeep, roff, top_runs;
+  if (tfirst.alloc(ntg) != hipSuccess || nj.alloc(ntg) != hipSuccess || joff.alloc(ntg) != hipSuccess ||
+      tkeep.alloc(ntg) != hipSuccess || roff.alloc(ntg) != hipSuccess ||
+      top_runs.alloc(ntg * kMaxJobsPerTarget) != hipSuccess)
+    return fail(ST_ERR_NOMEM, "st_seed_hits_to_jobs: device allocation failed");
+  hipLaunchKernelGGL(target_first_kernel, dim3(grid_of(nruns)), dim3(256), 0, s, nruns, tflag.p, tidx.p, tfirst.p);
+  ST_HIP(hipGetLastError());
+  hipLaunchKernelGGL(target_top_kernel, dim3(grid_of(ntg)), dim3(256), 0, s, keys, nh, ntg, nruns, rfirst.p,
+                     tfirst.p, hp->max_diagonals, hp->min_hits, hp->diag_hits, nj.p,
+                     top_runs.p);
+  ST_HIP(hipGetLastError());
+  uint64_t tot_jobs = 0, tot_recs = 0;
+  ST_HIP(scan_total(nj.p, joff.p, ntg, tmp, s, &tot_jobs));
+  hipLaunchKernelGGL(flag_nonzero_kernel, dim3(grid_of(ntg)), dim3(256), 0, s, nj.p, ntg, tkeep.p);
+  ST_HIP(hipGetLastError());
+  ST_HIP(scan_total(tkeep.p, roff.p, ntg, tmp, s, &tot_recs));
+  *njobs = tot_jobs;
+  *nrecs = tot_recs;
+  if (tot_jobs > jobs_cap || tot_recs > recs_cap || (tot_jobs && !jobs) || (tot_recs && !recs))
+    return fail(ST_ERR_CAPACITY, "st_seed_hits_to_jobs: output buffers too small (*njobs, *nrecs hold the sizes)");
+  if (tot_recs == 0) return ST_OK;
+  DevBuf<st_job> d_jobs;
+  DevBuf<st_target_rec> d_recs;
+  if (d_jobs.alloc(tot_jobs) != hipSuccess || d_recs.alloc(tot_recs) != hipSuccess)
+    return fail(ST_ERR_NOMEM, "st_seed_hits_to_jobs: device allocation failed");
+  hipLaunchKernelGGL(emit_kernel, dim3(grid_of(ntg)), dim3(256), 0, s, keys, nh, ntg, nruns, rfirst.p, tfirst.p,
+                     nj.p, joff.p, roff.p, top_runs.p, db->len.p, lay, hp->diag_window, hp->span, d_jobs.p, d_recs.p);
+  ST_HIP(hipGetLastError());
+  ST_HIP(hipMemcpyAsync(jobs, d_jobs.p, tot_jobs * sizeof(st_job), hipMemcpyDeviceToHost, s));
+  ST_HIP(hipMemcpyAsync(recs, d_recs.p, tot_recs * sizeof(st_target_rec), hipMemcpyDeviceToHost, s));
+  ST_HIP(hipStreamSynchronize(s));
+  return ST_OK;
+}
+
+}  // extern "C"

@@ -329,3 +329,84 @@ def test_background_sampler_frequencies():
     _, pb, _, _ = O.blosum62()
     f = np.bincount(x, minlength=20)[:20] / len(x)
     assert np.all(np.abs(f - pb[:20] / pb[:20].sum()) < 0.004)
+
+
+# ---- the seed-hit stage (hits -> top diagonals -> jobs) -----------------------
+
+def kmer_index(targets, k):
+    """Exact k-mer posting lists of targets: postings (target << 32 | pos)
+    sorted by (k-mer, target, position), and k-mer -> [start, end)."""
+    ents = []
+    for t, s in enumerate(targets):
+        for p in range(len(s) - k + 1):
+            ents.append((tuple(int(x) for x in s[p:p + k]), t, p))
+    ents.sort()
+    post = np.array([(t << 32) | p for _, t, p in ents], dtype=np.uint64)
+    rng = {}
+    for i, (km, _, _) in enumerate(ents):
+        a, b = rng.get(km, (i, i))
+        rng[km] = (a, i + 1)
+    return post, rng
+
+
+def query_seeds(q, k, rng_of, extra=0, n_post=0, rng=None):
+    seeds = []
+    for i in range(len(q) - k + 1):
+        km = tuple(int(x) for x in q[i:i + k])
+        if km in rng_of:
+            a, b = rng_of[km]
+            seeds.append((i, a, b))
+    for _ in range(extra):
+        a = int(rng.integers(0, n_post))
+        seeds.append((int(rng.integers(0, len(q))), a, int(min(n_post, a + rng.integers(0, 6)))))
+    return seeds
+
+
+def _brute_jobs(m, seeds, post, tlen, kmax, min_hits, diag_hits, window, span):
+    from collections import defaultdict
+    per = defaultdict(list)
+    for q, a, b in seeds:
+        for v in post[a:b]:
+            per[int(v) >> 32].append(((int(v) & 0xFFFFFFFF) - q, q))
+    jobs, recs = [], []
+    for t in sorted(per):
+        hits = per[t]
+        if len(hits) < min_hits:
+            continue
+        runs = {}
+        for d, q in hits:
+            c, lo, hi = runs.get(d, (0, q, q))
+            runs[d] = (c + 1, min(lo, q), max(hi, q))
+        ranked = sorted(runs.items(), key=lambda kv: (-kv[1][0], kv[0]))[:kmax]
+        if ranked[0][1][0] < diag_hits:
+            continue
+        j0 = len(jobs)
+        for d, (c, qmin, qmax) in ranked:
+            lo, hi = max(-d, 0), min(m, int(tlen[t]) - d)
+            if hi <= lo:
+                lo = hi = 0
+            if window > 0:
+                lo, hi = max(lo, qmin - window, 0), min(hi, qmax + span + window)
+            jobs.append((t, d, lo, max(hi, lo)))
+        recs.append((t, len(hits), j0, len(jobs)))
+    return jobs, recs
+
+
+@pytest.mark.parametrize("kmax,min_hits,diag_hits,window", [(4, 1, 1, 48), (1, 1, 1, 0), (16, 2, 2, 5),
+                                                            (3, 1, 2, 0), (2, 3, 1, 1)])
+def test_seed_hit_jobs_against_brute_force(kmax, min_hits, diag_hits, window):
+    rng = np.random.default_rng(kmax * 100 + min_hits * 10 + diag_hits)
+    targets = [rng.integers(0, 4, size=int(rng.integers(3, 90)), dtype=np.uint8) for _ in range(60)]
+    post, of = kmer_index(targets, 3)
+    tlen = np.array([len(t) for t in targets])
+    for _ in range(6):
+        q = rng.integers(0, 4, size=int(rng.integers(3, 70)), dtype=np.uint8)
+        seeds = query_seeds(q, 3, of, extra=10, n_post=len(post), rng=rng)
+        jobs, recs = O.seed_hits_to_jobs(len(q), seeds, post, tlen, kmax, min_hits, diag_hits, window, 3)
+        bj, br = _brute_jobs(len(q), seeds, post, tlen, kmax, min_hits, diag_hits, window, 3)
+        assert [tuple(int(x) for x in j) for j in jobs] == bj
+        assert [tuple(int(x) for x in r) for r in recs] == br
+        # the multiset of hits alone decides: seed order does not matter
+        perm = [seeds[i] for i in rng.permutation(len(seeds))]
+        j2, r2 = O.seed_hits_to_jobs(len(q), perm, post, tlen, kmax, min_hits, diag_hits, window, 3)
+        assert (j2 == jobs).all() and (r2 == recs).all()

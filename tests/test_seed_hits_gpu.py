@@ -1,0 +1,95 @@
+"""The prefilter's seed-hit stage on the device (hits -> top diagonals ->
+ungapped jobs and target records) vs the oracle, bit-exact, and chained into
+score_and_gate on the device."""
+import numpy as np
+import pytest
+
+import oracle as O
+import sabertooth_amd as st
+from sabertooth_amd import GapCosts
+from sabertooth_amd.core import Postings, seed_hits_to_jobs
+from test_oracle import kmer_index, query_seeds
+
+pytestmark = pytest.mark.gpu
+
+
+def _db(ctx, targets):
+    return st.TargetDb.from_sequences(ctx, targets)
+
+
+@pytest.mark.parametrize("kmax,min_hits,diag_hits,window", [(4, 1, 1, 48), (1, 1, 1, 0), (16, 2, 2, 5),
+                                                            (3, 1, 2, 0), (2, 3, 1, 1)])
+def test_jobs_match_the_oracle(ctx, kmax, min_hits, diag_hits, window):
+    rng = np.random.default_rng(40 + kmax + 3 * min_hits + 7 * diag_hits)
+    targets = [rng.integers(0, 4, size=int(rng.integers(3, 400)), dtype=np.uint8) for _ in range(300)]
+    post, of = kmer_index(targets, 3)
+    tlen = np.array([len(t) for t in targets])
+    db = _db(ctx, targets)
+    dp = Postings(ctx, post)
+    assert len(dp) == len(post)
+    for _ in range(4):
+        q = rng.integers(0, 4, size=int(rng.integers(3, 300)), dtype=np.uint8)
+        seeds = query_seeds(q, 3, of, extra=50, n_post=len(post), rng=rng)
+        gj, gr = seed_hits_to_jobs(ctx, db, dp, len(q), seeds, kmax, min_hits, diag_hits, window, 3)
+        oj, orr = O.seed_hits_to_jobs(len(q), seeds, post, tlen, kmax, min_hits, diag_hits, window, 3)
+        assert len(gr) == len(orr) and len(gj) == len(oj)
+        assert (gj == oj).all() and (gr == orr).all()
+    dp.close()
+
+
+def test_realistic_alphabet_chained_into_the_gate(ctx):
+    # 20-letter targets with planted homologs; seeds are the query's exact
+    # 4-mers; the jobs go through score_and_gate on the device and on the
+    # oracle, and the homologs come out as candidates
+    rng = np.random.default_rng(9)
+    q = rng.integers(0, 20, size=250, dtype=np.uint8)
+    targets = [rng.integers(0, 20, size=int(rng.integers(30, 600)), dtype=np.uint8) for _ in range(400)]
+    for k in range(0, 400, 40):
+        h = q.copy()
+        h[rng.random(len(h)) < 0.3] = rng.integers(0, 20)
+        targets[k] = np.concatenate([targets[k][:50], h])
+    post, of = kmer_index(targets, 4)
+    tlen = np.array([len(t) for t in targets])
+    db = _db(ctx, targets)
+    dp = Postings(ctx, post)
+    seeds = query_seeds(q, 4, of)
+    gj, gr = seed_hits_to_jobs(ctx, db, dp, len(q), seeds, 4, 1, 1, 48, 4)
+    oj, orr = O.seed_hits_to_jobs(len(q), seeds, post, tlen, 4, 1, 1, 48, 4)
+    assert (gj == oj).all() and (gr == orr).all() and len(gr) > 10
+    rows = O.seq_rows(q)
+    qp = st.QueryProfile.from_rows(ctx, rows)
+    cands, chains, js = st.score_and_gate(qp, db, gj, gr, GapCosts(11, 1), 35)
+    arena = np.concatenate(targets)
+    offs = np.zeros(len(targets) + 1, np.uint64)
+    offs[1:] = np.cumsum(tlen)
+    oc, ochains, ojs = O.score_and_gate(rows, arena, offs, oj, orr, 11, 1, 35)
+    assert (cands == oc).all() and chains == ochains and (js == ojs).all()
+    assert set(range(0, 400, 40)) <= set(int(c["target_id"]) for c in cands)
+    dp.close()
+
+
+def test_edges(ctx):
+    targets = [np.array([0, 1, 2, 3, 0, 1, 2], np.uint8), np.array([3, 3, 3], np.uint8)]
+    post, of = kmer_index(targets, 2)
+    db = _db(ctx, targets)
+    dp = Postings(ctx, post)
+    # no seeds, empty ranges
+    assert [len(x) for x in seed_hits_to_jobs(ctx, db, dp, 5, [])] == [0, 0]
+    assert [len(x) for x in seed_hits_to_jobs(ctx, db, dp, 5, [(0, 3, 3), (4, 0, 0)])] == [0, 0]
+    # one hit; a one-residue query
+    gj, gr = seed_hits_to_jobs(ctx, db, dp, 1, [(0, 0, 1)], 4, 1, 1, 0, 2)
+    oj, orr = O.seed_hits_to_jobs(1, [(0, 0, 1)], post, [7, 3], 4, 1, 1, 0, 2)
+    assert (gj == oj).all() and (gr == orr).all() and len(gr) == 1
+    # a seed outside the postings or the query is refused
+    with pytest.raises(st.StError):
+        seed_hits_to_jobs(ctx, db, dp, 5, [(0, 0, len(post) + 1)])
+    with pytest.raises(st.StError):
+        seed_hits_to_jobs(ctx, db, dp, 5, [(5, 0, 1)])
+    # a posting naming a target outside the database is refused
+    bad = Postings(ctx, np.array([(7 << 32) | 1], np.uint64))
+    with pytest.raises(st.StError):
+        seed_hits_to_jobs(ctx, db, bad, 5, [(0, 0, 1)])
+    with pytest.raises(st.StError):
+        seed_hits_to_jobs(ctx, db, dp, 5, [(0, 0, 1)], max_diagonals=17)
+    bad.close()
+    dp.close()
