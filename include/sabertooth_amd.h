@@ -208,6 +208,12 @@ typedef struct {
  * in the index's order. */
 typedef struct st_postings st_postings;
 int st_postings_create(st_ctx *ctx, const uint64_t *postings, uint64_t n, st_postings **out);
+/* The posting section of an index file (<db>.stidx, formats 1 and 2: fixed
+ * 8-byte or packed posting_width-byte entries), decoded and uploaded. */
+int st_postings_from_stidx(st_ctx *ctx, const char *path, st_postings **out);
+/* The same decoded postings into a host buffer (*n = count; written when
+ * cap >= *n). */
+int st_stidx_read_postings(const char *path, uint64_t *postings, uint64_t cap, uint64_t *n);
 int st_postings_destroy(st_postings *p);
 uint64_t st_postings_count(const st_postings *p);
 

@@ -80,6 +80,7 @@ _SIGS = [
     ("st_align_spans_batch", C.c_int, [_P, _P, _P, C.c_uint64, _P, _P, _P, _P, _P]),
     ("st_postings_create", C.c_int, [_P, _P, C.c_uint64, C.POINTER(_P)]),
     ("st_postings_destroy", C.c_int, [_P]),
+    ("st_postings_from_stidx", C.c_int, [_P, C.c_char_p, C.POINTER(_P)]),
     ("st_postings_count", C.c_uint64, [_P]),
     ("st_seed_hits_to_jobs", C.c_int, [_P, _P, _P, C.c_uint64, _P, C.c_uint64, _P, _P, C.c_uint64,
                                        C.POINTER(C.c_uint64), _P, C.c_uint64, C.POINTER(C.c_uint64)]),

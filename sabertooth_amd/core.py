@@ -269,6 +269,15 @@ class Postings:
         check(lib().st_postings_create(ctx.handle, ptr(p), len(p), C.byref(h)))
         self.ctx, self._h = ctx, h
 
+    @staticmethod
+    def from_stidx(ctx: Context, path: str) -> "Postings":
+        """The posting section of an index file, decoded on the host."""
+        p = Postings.__new__(Postings)
+        h = C.c_void_p()
+        check(lib().st_postings_from_stidx(ctx.handle, os.fsencode(path), C.byref(h)))
+        p.ctx, p._h = ctx, h
+        return p
+
     @property
     def handle(self):
         return self._h

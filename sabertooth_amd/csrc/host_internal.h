@@ -98,6 +98,13 @@ struct st_db {
   uint64_t n_targets = 0, n_res = 0;
 };
 
+// An index's posting lists on the device, decoded.
+struct st_postings {
+  st_ctx *ctx = nullptr;
+  st::DevBuf<uint64_t> post;  // (target << 32) | position
+  uint64_t n = 0;
+};
+
 struct st_qprof {
   st_ctx *ctx = nullptr;
   std::vector<int32_t> rows;

@@ -23,12 +23,6 @@
 #include "../../include/sabertooth_amd.h"
 #include "host_internal.h"
 
-struct st_postings {
-  st_ctx *ctx = nullptr;
-  st::DevBuf<uint64_t> post;  // (target << 32) | position
-  uint64_t n = 0;
-};
-
 namespace st {
 namespace {
 

@@ -15,6 +15,7 @@
 
 #include <cstring>
 #include <string>
+#include <vector>
 
 #include "../../include/sabertooth_amd.h"
 #include "host_internal.h"
@@ -26,7 +27,7 @@ using st::fail;
 constexpr uint64_t kHeader = 4096, kFooter = 16;
 constexpr int kMaxSections = 16;
 // section kinds (SectionKind, src/index.rs:221-241)
-enum : uint32_t { kResidues = 1, kSeqOffsets = 2, kIdBytes = 3, kIdOffsets = 4 };
+enum : uint32_t { kResidues = 1, kSeqOffsets = 2, kIdBytes = 3, kIdOffsets = 4, kPostings = 5 };
 
 // Hasher64 (src/index.rs:122-182): the index's integrity hash.  Words are
 // read little-endian; each is folded in by xor and a multiply-xorshift, the
@@ -85,7 +86,9 @@ struct Mapped {
   const uint8_t *base = nullptr;
   uint64_t size = 0;
   st_stidx_info info{};
-  Sec sec[5];  // by kind 1..4
+  Sec sec[6];  // by kind 1..5
+  uint64_t num_postings = 0;
+  uint32_t post_w = 8, pos_bits = 32;  // format 1: 8-byte (target << 32 | position) postings
   ~Mapped() {
     if (base) munmap((void *)base, size);
   }
@@ -129,6 +132,7 @@ int open_index(const char *fn, const char *path, bool verify, Mapped &m) {
   m.info.num_seqs = rd64(h + o);
   m.info.total_residues = rd64(h + o + 8);
   m.info.max_seq_len = rd64(h + o + 16);
+  m.num_postings = rd64(h + o + 24);
   o += 7 * 8;                       // num_seqs .. masked_residues
   o += 4 + 4 + 3 * 16 + 1024 + 8 + 32;  // source kind, pad, files, path, created, tool version
   const uint32_t ns = rd32(h + o);
@@ -138,12 +142,21 @@ int open_index(const char *fn, const char *path, bool verify, Mapped &m) {
     const uint8_t *e = h + o + 32 * (uint64_t)i;
     const uint32_t k = rd32(e);
     if (k < 1 || k > 10) return fail(ST_ERR_INVALID, who(fn, path) + "unknown section kind");
-    if (k <= kIdOffsets) {
+    if (k <= kPostings) {
       m.sec[k].off = rd64(e + 8);
       m.sec[k].len = rd64(e + 16);
       m.sec[k].hash = rd64(e + 24);
       m.sec[k].present = true;
     }
+  }
+  // format 2 appends the posting encoding after the section table
+  // (posting_width bytes, posting_pos_bits position bits)
+  if (m.info.version >= 2) {
+    const uint8_t *f2 = h + o + 32 * (uint64_t)kMaxSections;
+    m.post_w = rd32(f2 + 4);
+    m.pos_bits = rd32(f2 + 8);
+    if (m.post_w < 1 || m.post_w > 8 || m.pos_bits > 32 || m.pos_bits > 8 * m.post_w)
+      return fail(ST_ERR_INVALID, who(fn, path) + "bad posting encoding in header (corrupt index)");
   }
   // completion footer
   if (std::memcmp(m.base + m.size - kFooter, "SBTIDXOK", 8) != 0 || rd64(m.base + m.size - 8) != m.size)
@@ -208,6 +221,59 @@ int st_stidx_read_ids(const char *path, uint8_t *bytes, uint64_t *offsets) {
   if (bytes) std::memcpy(bytes, m.at(kIdBytes), m.sec[kIdBytes].len);
   if (offsets) std::memcpy(offsets, m.at(kIdOffsets), m.sec[kIdOffsets].len);
   return ST_OK;
+}
+
+}  // extern "C"
+
+namespace {
+
+// The decoded posting section (SeedIndex::posting, src/index.rs:2294-2303:
+// post_w little-endian bytes, the target above pos_bits, the position below).
+int read_postings(const char *fn, const char *path, std::vector<uint64_t> &dec, uint64_t *count) {
+  Mapped m;
+  int rc = open_index(fn, path, false, m);
+  if (rc) return rc;
+  const Sec &s = m.sec[kPostings];
+  if (!s.present || s.off < kHeader || s.len > m.size - kFooter || s.off > m.size - kFooter - s.len)
+    return fail(ST_ERR_INVALID, who(fn, path) + "posting section missing or out of bounds");
+  if (m.num_postings > s.len / m.post_w || s.len != m.num_postings * m.post_w)
+    return fail(ST_ERR_INVALID, who(fn, path) + "posting count disagrees with the section");
+  const uint64_t n = m.num_postings;
+  if (count) *count = n;
+  const uint64_t pmask = (1ull << m.pos_bits) - 1;
+  dec.resize(n);
+  const uint8_t *p = m.at(kPostings);
+  for (uint64_t k = 0; k < n; k++) {
+    uint64_t v = 0;
+    std::memcpy(&v, p + k * m.post_w, m.post_w);
+    const uint64_t tid = v >> m.pos_bits, pos = v & pmask;
+    if (tid >= m.info.num_seqs || pos > 0xffffffffull)
+      return fail(ST_ERR_INVALID, who(fn, path) + "posting outside the targets (corrupt index)");
+    dec[k] = (tid << 32) | pos;
+  }
+  return ST_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int st_stidx_read_postings(const char *path, uint64_t *postings, uint64_t cap, uint64_t *n) {
+  if (!n) return fail(ST_ERR_INVALID, "st_stidx_read_postings: null count");
+  std::vector<uint64_t> dec;
+  const int rc = read_postings("st_stidx_read_postings", path, dec, n);
+  if (rc) return rc;
+  if (postings && cap >= dec.size()) std::memcpy(postings, dec.data(), dec.size() * 8);
+  return ST_OK;
+}
+
+int st_postings_from_stidx(st_ctx *ctx, const char *path, st_postings **out) {
+  if (!ctx || !out) return fail(ST_ERR_INVALID, "st_postings_from_stidx: null argument");
+  *out = nullptr;
+  std::vector<uint64_t> dec;
+  const int rc = read_postings("st_postings_from_stidx", path, dec, nullptr);
+  if (rc) return rc;
+  return st_postings_create(ctx, dec.data(), dec.size(), out);
 }
 
 int st_db_create_from_stidx(st_ctx *ctx, const char *path, st_db **out) {

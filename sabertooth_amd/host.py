@@ -262,6 +262,7 @@ _SIGS += [
     ("st_stidx_info_read", C.c_int, [C.c_char_p, C.c_int, _P]),
     ("st_stidx_read_targets", C.c_int, [C.c_char_p, _P, _P]),
     ("st_stidx_read_ids", C.c_int, [C.c_char_p, _P, _P]),
+    ("st_stidx_read_postings", C.c_int, [C.c_char_p, _P, C.c_uint64, C.POINTER(C.c_uint64)]),
 ]
 EXPORTED = [s[0] for s in _SIGS]
 
@@ -281,6 +282,15 @@ def stidx_targets(path: str):
     off = np.zeros(int(info["num_seqs"]) + 1, np.uint64)
     check(_L().st_stidx_read_targets(os.fsencode(path), ptr(res), ptr(off)))
     return res[: int(info["total_residues"])], off
+
+
+def stidx_postings(path: str) -> np.ndarray:
+    """The index's posting section decoded to (target << 32) | position."""
+    n = C.c_uint64(0)
+    check(_L().st_stidx_read_postings(os.fsencode(path), None, 0, C.byref(n)))
+    out = np.zeros(max(n.value, 1), np.uint64)
+    check(_L().st_stidx_read_postings(os.fsencode(path), ptr(out), len(out), C.byref(n)))
+    return out[: n.value]
 
 
 def stidx_ids(path: str):

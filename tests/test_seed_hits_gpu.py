@@ -93,3 +93,32 @@ def test_edges(ctx):
         seed_hits_to_jobs(ctx, db, dp, 5, [(0, 0, 1)], max_diagonals=17)
     bad.close()
     dp.close()
+
+
+def test_the_reference_index_drives_the_stage(ctx):
+    # the fixture index's own postings on the device; each fixture query
+    # seeded with its exact 2-mers (the postings of its 2-mer groups)
+    import os
+    from sabertooth_amd import host as H
+    idx = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "v1_fixture_k2.stidx")
+    res, off = H.stidx_targets(idx)
+    post = H.stidx_postings(idx)
+    db = st.TargetDb.from_stidx(ctx, idx)
+    dp = Postings.from_stidx(ctx, idx)
+    assert len(dp) == len(post)
+    t = (post >> np.uint64(32)).astype(np.int64)
+    p = (post & np.uint64(0xFFFFFFFF)).astype(np.int64)
+    km = [int(res[int(off[a]) + b]) * 21 + int(res[int(off[a]) + b + 1]) for a, b in zip(t, p)]
+    ranges = {}
+    for i, k in enumerate(km):
+        a, _ = ranges.get(k, (i, i))
+        ranges[k] = (a, i + 1)
+    tlen = np.diff(off.astype(np.int64))
+    for qi in range(len(tlen)):
+        q = res[int(off[qi]):int(off[qi + 1])]
+        seeds = [(i, *ranges[int(q[i]) * 21 + int(q[i + 1])]) for i in range(len(q) - 1)
+                 if int(q[i]) * 21 + int(q[i + 1]) in ranges]
+        gj, gr = seed_hits_to_jobs(ctx, db, dp, len(q), seeds, 4, 1, 1, 48, 2)
+        oj, orr = O.seed_hits_to_jobs(len(q), seeds, post, tlen, 4, 1, 1, 48, 2)
+        assert (gj == oj).all() and (gr == orr).all() and qi in set(int(x) for x in gr["tid"])
+    dp.close()

@@ -91,3 +91,22 @@ def test_corrupt_copies_are_refused(tmp_path):
     with pytest.raises(Exception):
         H.stidx_info(str(tmp_path / "missing.stidx"))
 
+
+
+def test_fixture_postings_decode_to_the_two_mer_lists():
+    # the index was built with k = 2: its postings are every 2-mer start of
+    # the targets whose residues are both in the 20-letter alphabet, grouped
+    # by 2-mer, each group sorted by (target, position)
+    res, off = H.stidx_targets(IDX)
+    post = H.stidx_postings(IDX)
+    t = (post >> np.uint64(32)).astype(np.int64)
+    p = (post & np.uint64(0xFFFFFFFF)).astype(np.int64)
+    L = np.diff(off.astype(np.int64))
+    assert (p + 2 <= L[t]).all()
+    want = {(a, b) for a in range(len(L)) for b in range(int(L[a]) - 1)
+            if res[int(off[a]) + b] < 20 and res[int(off[a]) + b + 1] < 20}
+    assert {(int(a), int(b)) for a, b in zip(t, p)} == want and len(post) == len(want)
+    km = [int(res[int(off[a]) + b]) * 21 + int(res[int(off[a]) + b + 1]) for a, b in zip(t, p)]
+    groups = [km[0]] + [km[i] for i in range(1, len(km)) if km[i] != km[i - 1]]
+    assert len(groups) == len(set(groups))
+    assert all(km[i] != km[i - 1] or (t[i], p[i]) > (t[i - 1], p[i - 1]) for i in range(1, len(km)))
