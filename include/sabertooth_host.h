@@ -246,6 +246,27 @@ int st_search_hits_spans(const int32_t *rows, const uint8_t *qres, uint64_t m, c
  * {:.3}, E-value {:.2e} (exponent without '+' or padding), bits {:.1}. */
 int st_format_m8(const char *query_id, const char *target_id, const st_hit *h, char *buf,
                  uint64_t cap, uint64_t *len);
+/* A candidate trace (SABERTOOTH_TRACE_CANDIDATES, src/search.rs:358-385;
+ * lines written at src/search.rs:522-530): "query \t target \t prefilter
+ * score \t DP cells" per candidate.  Targets resolve against the database's
+ * identifiers (id_bytes / id_off as st_stidx_read_ids gives them); queries
+ * are numbered in order of first appearance and each query's candidates
+ * keep their trace order.  *n_queries and *n_cands are always set; every
+ * other output is written when non-NULL (qid_bytes when qid_cap holds them,
+ * qid_off and cand_off n_queries + 1 entries, the candidate arrays n_cands),
+ * so a first call with NULL outputs gives the sizes.  A malformed line or an
+ * unknown target is ST_ERR_INVALID naming the line. */
+int st_parse_candidate_trace(const char *text, uint64_t len, const uint8_t *id_bytes,
+                             const uint64_t *id_off, uint64_t n_targets, uint64_t *n_queries,
+                             uint64_t *n_cands, char *qid_bytes, uint64_t qid_cap, uint64_t *qid_off,
+                             uint64_t *cand_off, uint32_t *cand_tids, int32_t *diag_scores,
+                             uint64_t *cells);
+/* The same lines for one query's candidates (the trace's writer side), so a
+ * device prefilter's candidates can be compared with a traced reference run
+ * line by line. */
+int st_format_candidate_trace(const char *query_id, const char *const *target_ids,
+                              const int32_t *diag_scores, const uint64_t *cells, uint64_t n,
+                              char *buf, uint64_t cap, uint64_t *len);
 /* Hit::extra_cols (src/search.rs:219-230): raw, qlen, tlen, qcov and tcov
  * ({:.3}, span / max(length, 1)), CIGAR; tab separated, no newline. */
 int st_format_m8_extra(const st_hit *h, const char *cigar, char *buf, uint64_t cap, uint64_t *len);

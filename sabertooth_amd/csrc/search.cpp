@@ -9,10 +9,13 @@
 // hits are accepted (coverage filters skip a hit and continue), and orders
 // the hits by E-value, bit score and target ordinal (hit_order).
 #include <algorithm>
+#include <cerrno>
+#include <climits>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
 #include <string>
+#include <unordered_map>
 #include <vector>
 
 #include "host_internal.h"
@@ -212,6 +215,121 @@ int st_format_m8(const char *query_id, const char *target_id, const st_hit *h, c
   if (buf && cap) {
     const uint64_t k = std::min<uint64_t>(cap - 1, s.size());
     std::memcpy(buf, s.data(), k);
+    buf[k] = 0;
+  }
+  return ST_OK;
+}
+
+int st_parse_candidate_trace(const char *text, uint64_t len, const uint8_t *id_bytes, const uint64_t *id_off,
+                             uint64_t n_targets, uint64_t *n_queries, uint64_t *n_cands, char *qid_bytes,
+                             uint64_t qid_cap, uint64_t *qid_off, uint64_t *cand_off, uint32_t *cand_tids,
+                             int32_t *diag_scores, uint64_t *cells) {
+  // lines "query \t target \t prefilter score \t DP cells" (candidate_trace,
+  // src/search.rs:358-385, written at src/search.rs:522-530)
+  if (!n_queries || !n_cands || (len && !text) || (n_targets && (!id_bytes || !id_off)))
+    return fail(ST_ERR_INVALID, "st_parse_candidate_trace: null argument");
+  std::unordered_map<std::string, uint32_t> tid_of;
+  tid_of.reserve(n_targets * 2);
+  for (uint64_t t = 0; t < n_targets; t++)
+    tid_of.emplace(std::string((const char *)id_bytes + id_off[t], id_off[t + 1] - id_off[t]), (uint32_t)t);
+  struct Row {
+    uint32_t q, tid;
+    int32_t score;
+    uint64_t cells;
+  };
+  std::vector<Row> rows;
+  std::vector<std::string> qids;
+  std::unordered_map<std::string, uint32_t> q_of;
+  uint64_t pos = 0, line = 0;
+  while (pos < len) {
+    uint64_t e = pos;
+    while (e < len && text[e] != '\n') e++;
+    line++;
+    std::string l(text + pos, e - pos);
+    pos = e + 1;
+    if (!l.empty() && l.back() == '\r') l.pop_back();
+    if (l.empty()) continue;
+    std::string f[4];
+    size_t a = 0;
+    int nf = 0;
+    for (; nf < 4; nf++) {
+      const size_t b = l.find('\t', a);
+      if (nf == 3) {
+        if (b != std::string::npos) break;
+        f[nf] = l.substr(a);
+      } else {
+        if (b == std::string::npos) break;
+        f[nf] = l.substr(a, b - a);
+        a = b + 1;
+      }
+    }
+    if (nf != 4) return fail(ST_ERR_INVALID, "st_parse_candidate_trace: line " + std::to_string(line) + " has not 4 fields");
+    auto it = tid_of.find(f[1]);
+    if (it == tid_of.end())
+      return fail(ST_ERR_INVALID, "st_parse_candidate_trace: line " + std::to_string(line) + ": unknown target " + f[1]);
+    char *end = nullptr;
+    errno = 0;
+    const long long sc = std::strtoll(f[2].c_str(), &end, 10);
+    if (f[2].empty() || *end || errno || sc < INT32_MIN || sc > INT32_MAX)
+      return fail(ST_ERR_INVALID, "st_parse_candidate_trace: line " + std::to_string(line) + ": bad score");
+    const unsigned long long ce = std::strtoull(f[3].c_str(), &end, 10);
+    if (f[3].empty() || *end || errno || f[3][0] == '-')
+      return fail(ST_ERR_INVALID, "st_parse_candidate_trace: line " + std::to_string(line) + ": bad cell count");
+    auto qi = q_of.find(f[0]);
+    uint32_t q;
+    if (qi == q_of.end()) {
+      q = (uint32_t)qids.size();
+      q_of.emplace(f[0], q);
+      qids.push_back(f[0]);
+    } else {
+      q = qi->second;
+    }
+    rows.push_back(Row{q, it->second, (int32_t)sc, (uint64_t)ce});
+  }
+  // candidates grouped per query (first appearance), in trace order
+  std::stable_sort(rows.begin(), rows.end(), [](const Row &x, const Row &y) { return x.q < y.q; });
+  *n_queries = qids.size();
+  *n_cands = rows.size();
+  uint64_t qb = 0;
+  for (const std::string &q : qids) qb += q.size();
+  if (qid_off) {
+    qid_off[0] = 0;
+    for (size_t k = 0; k < qids.size(); k++) qid_off[k + 1] = qid_off[k] + qids[k].size();
+  }
+  if (qid_bytes && qid_cap >= qb) {
+    uint64_t o = 0;
+    for (const std::string &q : qids) {
+      std::memcpy(qid_bytes + o, q.data(), q.size());
+      o += q.size();
+    }
+  }
+  if (cand_off) {
+    std::fill(cand_off, cand_off + qids.size() + 1, 0);
+    for (const Row &r : rows) cand_off[r.q + 1]++;
+    for (size_t k = 0; k < qids.size(); k++) cand_off[k + 1] += cand_off[k];
+  }
+  for (size_t k = 0; k < rows.size(); k++) {
+    if (cand_tids) cand_tids[k] = rows[k].tid;
+    if (diag_scores) diag_scores[k] = rows[k].score;
+    if (cells) cells[k] = rows[k].cells;
+  }
+  return ST_OK;
+}
+
+int st_format_candidate_trace(const char *query_id, const char *const *target_ids, const int32_t *diag_scores,
+                              const uint64_t *cells, uint64_t n, char *buf, uint64_t cap, uint64_t *len) {
+  if (!len || (n && (!query_id || !target_ids || !diag_scores || !cells)))
+    return fail(ST_ERR_INVALID, "st_format_candidate_trace: null argument");
+  std::string out;
+  for (uint64_t k = 0; k < n; k++) {
+    if (!target_ids[k]) return fail(ST_ERR_INVALID, "st_format_candidate_trace: null target id");
+    out += std::string(query_id) + "\t" + target_ids[k] + "\t" + std::to_string(diag_scores[k]) + "\t" +
+           std::to_string(cells[k]) + "\n";
+  }
+  *len = out.size();
+  if (buf && cap) {
+    const uint64_t k = std::min<uint64_t>(cap - 1, out.size());
+    std::memcpy(buf, out.data(), k);
     buf[k] = 0;
   }
   return ST_OK;

@@ -444,6 +444,10 @@ _SIGS += [
                                        C.c_uint64, C.POINTER(C.c_uint64), _P]),
     ("st_format_m8", C.c_int, [C.c_char_p, C.c_char_p, _P, _P, C.c_uint64, C.POINTER(C.c_uint64)]),
     ("st_format_m8_extra", C.c_int, [_P, C.c_char_p, _P, C.c_uint64, C.POINTER(C.c_uint64)]),
+    ("st_parse_candidate_trace", C.c_int, [C.c_char_p, C.c_uint64, _P, _P, C.c_uint64, C.POINTER(C.c_uint64),
+                                           C.POINTER(C.c_uint64), _P, C.c_uint64, _P, _P, _P, _P, _P]),
+    ("st_format_candidate_trace", C.c_int, [C.c_char_p, _P, _P, _P, C.c_uint64, _P, C.c_uint64,
+                                            C.POINTER(C.c_uint64)]),
     ("st_format_m8_block", C.c_int, [C.c_char_p, _P, _P, _P, C.c_uint64, C.c_int, C.c_int, _P, C.c_uint64,
                                      C.POINTER(C.c_uint64)]),
 ]
@@ -503,6 +507,45 @@ def search_hits(rows, qres, arena, offsets, cand_tids, scores, gaps, ev: EValueP
     raw = cig.raw
     cigars = [raw[i * stride:(i + 1) * stride].split(b"\0", 1)[0].decode() for i in range(k)]
     return hits[:k].copy(), cigars
+
+
+def parse_candidate_trace(text: str, target_ids):
+    """A candidate trace -> (query ids, per-query (tids, diag scores, cells))
+    with targets resolved against target_ids (their order = target index)."""
+    raw = text.encode()
+    ib = "".join(target_ids).encode()
+    io = np.zeros(len(target_ids) + 1, np.uint64)
+    io[1:] = np.cumsum([len(t.encode()) for t in target_ids])
+    ibuf = np.frombuffer(ib, np.uint8) if ib else np.zeros(1, np.uint8)
+    nq, nc = C.c_uint64(0), C.c_uint64(0)
+    base = (raw, len(raw), ptr(ibuf), ptr(io), len(target_ids), C.byref(nq), C.byref(nc))
+    check(_L().st_parse_candidate_trace(*base, None, 0, None, None, None, None, None))
+    qoff = np.zeros(nq.value + 1, np.uint64)
+    check(_L().st_parse_candidate_trace(*base, None, 0, ptr(qoff), None, None, None, None))
+    qb = C.create_string_buffer(max(int(qoff[-1]), 1))
+    coff = np.zeros(nq.value + 1, np.uint64)
+    tids = np.zeros(max(nc.value, 1), np.uint32)
+    sc = np.zeros(max(nc.value, 1), np.int32)
+    ce = np.zeros(max(nc.value, 1), np.uint64)
+    check(_L().st_parse_candidate_trace(*base, qb, len(qb), ptr(qoff), ptr(coff), ptr(tids), ptr(sc), ptr(ce)))
+    qraw = qb.raw
+    qids = [qraw[int(qoff[k]):int(qoff[k + 1])].decode() for k in range(nq.value)]
+    per = [(tids[int(coff[k]):int(coff[k + 1])].copy(), sc[int(coff[k]):int(coff[k + 1])].copy(),
+            ce[int(coff[k]):int(coff[k + 1])].copy()) for k in range(nq.value)]
+    return qids, per
+
+
+def format_candidate_trace(query_id: str, target_ids, diag_scores, cells) -> str:
+    n = len(target_ids)
+    tid = (C.c_char_p * max(n, 1))(*[t.encode() for t in target_ids])
+    sc = np.ascontiguousarray(diag_scores, dtype=np.int32)
+    ce = np.ascontiguousarray(cells, dtype=np.uint64)
+    ln = C.c_uint64(0)
+    args = (query_id.encode(), tid, ptr(sc), ptr(ce), n)
+    check(_L().st_format_candidate_trace(*args, None, 0, C.byref(ln)))
+    buf = C.create_string_buffer(ln.value + 1)
+    check(_L().st_format_candidate_trace(*args, buf, len(buf), C.byref(ln)))
+    return buf.raw[: ln.value].decode()
 
 
 def format_m8_extra(hit, cigar: str) -> str:

@@ -134,3 +134,24 @@ def test_extra_columns_and_output_blocks():
     for a, b, hh, c in zip(ext[1:-1], lines[1:-1], hits, cig):
         assert a == b + "\t" + H.format_m8_extra(hh, c) and a.endswith("\t" + c)
     assert H.format_m8_block(q, [], hits[:0]) == ""
+
+
+def test_candidate_trace_round_trip_and_errors():
+    ids = ["seq0", "seq1", "tgt|2", "x y"]
+    # the writer's line format (src/search.rs:526): query, target, score, cells
+    text = (H.format_candidate_trace("qA", ["seq1", "x y"], [41, -3], [1200, 7])
+            + H.format_candidate_trace("qB", ["tgt|2"], [0], [0])
+            + "qA\tseq0\t55\t18446744073709551615\n")
+    assert text.split("\n")[0] == "qA\tseq1\t41\t1200"
+    qids, per = H.parse_candidate_trace(text, ids)
+    assert qids == ["qA", "qB"]
+    assert per[0][0].tolist() == [1, 3, 0] and per[0][1].tolist() == [41, -3, 55]
+    assert per[0][2].tolist() == [1200, 7, 2 ** 64 - 1]
+    assert per[1][0].tolist() == [2] and per[1][1].tolist() == [0]
+    # blank lines and CRLF are tolerated; the empty trace has no queries
+    assert H.parse_candidate_trace("\r\nqA\tseq0\t1\t2\r\n\n", ids)[0] == ["qA"]
+    assert H.parse_candidate_trace("", ids) == ([], [])
+    for bad in ["qA\tseq0\t1\n", "qA\tseq9\t1\t2\n", "qA\tseq0\tx\t2\n", "qA\tseq0\t1\t-2\n",
+                "qA\tseq0\t1\t2\t3\n", "qA\tseq0\t99999999999\t2\n"]:
+        with pytest.raises(Exception):
+            H.parse_candidate_trace(bad, ids)
