@@ -142,6 +142,42 @@ def _serve_mode(st, ctx, db, reqs, cells, ref_of, n_iso, label, **cfg):
     }
 
 
+def spans_leg(O, st, ctx, db, qps, rows, w, res, offs, per_query=32, sample=120):
+    """The two score passes of align_canonical on the device for each query's
+    best-scoring pairs of the step (the hits a search would trace back):
+    forward score and end cell, then the reversed prefixes for the start.
+    Wall time of the call (planning, uploads, both passes, read-back); a
+    seeded sample of spans against the oracle."""
+    hit_lists = []
+    for qi in range(len(qps)):
+        sw = res["swg"][offs[qi]:offs[qi + 1]]
+        top = np.argsort(-sw, kind="stable")[:per_query]
+        hit_lists.append(np.asarray(w.cand_lists[qi], np.uint32)[top[sw[top] > 0]])
+    gaps = [st.GapCosts(w.gap_open, w.gap_extend)] * len(qps)
+    st.align_spans(ctx, db, qps[:8], hit_lists[:8], gaps[:8])
+    ts = time.perf_counter()
+    sp = st.align_spans(ctx, db, qps, hit_lists, gaps)
+    t = time.perf_counter() - ts
+    tl = w.target_lengths
+    fwd = sum(len(rows[qi]) * int(tl[hit_lists[qi]].sum()) for qi in range(len(qps)))
+    pos = sp["score"] > 0
+    rev = int(((sp["query_end"][pos] + 1) * (sp["target_end"][pos] + 1)).sum())
+    qof = np.repeat(np.arange(len(qps)), [len(h) for h in hit_lists])
+    tof = np.concatenate(hit_lists)
+    rng = np.random.default_rng(11)
+    mism = 0
+    pick = rng.choice(len(sp), size=min(sample, len(sp)), replace=False)
+    for k in pick:
+        want = O.span(rows[qof[k]], w.target(int(tof[k])), w.gap_open, w.gap_extend)
+        got = tuple(int(sp[k][f]) for f in ("score", "query_start", "query_end", "target_start", "target_end"))
+        mism += got != want
+    out = {"hits": int(len(sp)), "ms": round(1e3 * t, 3), "forward_cells": int(fwd), "reverse_cells": rev,
+           "gcups": round((fwd + rev) / t / 1e9, 1), "sampled_vs_oracle": int(len(pick)), "mismatches": int(mism)}
+    log(f"alignment spans: {out['hits']} hits in {out['ms']:.1f} ms, {out['gcups']} GCUPS over both passes, "
+        f"{len(pick) - mism}/{len(pick)} sampled spans bit-exact")
+    return out
+
+
 def serving_leg(a, O, st, synth):
     """Served vs plain batch on the serving configuration: a resident
     265 M-residue database, 90% plain / 10% profile requests with 3,000
@@ -275,6 +311,8 @@ def main():
     log(f"graph replay: {graph['ms_per_step']:.1f} ms per step (dp {graph['dp_ms']:.1f} ms), "
         f"results equal eager: {g_equal}")
 
+    spans = spans_leg(O, st, ctx, db, qps, rows, w, res, offs)
+
     dp = float(np.mean(dp_ms)) / 1e3
     kinds = {0: "Pk16", 1: "I32F", 2: "I32N"}
     per_launch = []
@@ -349,7 +387,8 @@ def main():
         "queries_per_s": round(qps_rate, 1),
         "parity": {"sampled_pairs_all_outputs_vs_oracle": int(len(sample)), "mismatches": int(mism),
                    "all_pairs_swg_vs_cpu_avx2_mismatches": full_swg},
-        "roofline": roof, "cpu_baseline": cpu, "graph_replay": graph, "serving": serving,
+        "roofline": roof, "cpu_baseline": cpu, "graph_replay": graph, "alignment_spans": spans,
+        "serving": serving,
     }
     print(json.dumps(out), flush=True)
 
