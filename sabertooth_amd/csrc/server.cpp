@@ -83,11 +83,10 @@ struct Req {
 // DP state a request holds on the device while it runs: per pair the raw
 // outputs, diagonal keys, result row and pair slots; per target column the
 // pass-boundary row (H, F, U, B words shared by two pairs, two halves).
-uint64_t state_bytes_of(const Req &r, const st_db *db) {
-  uint64_t cols = 0;
-  for (uint32_t t : r.cands) cols += db->hlen[t];
-  return (uint64_t)r.cands.size() * (sizeof(PairOut) + 32 + sizeof(st_pair_result) + 2 * sizeof(PairDev) + 4) +
-         cols * 16;
+// Device DP state of a request of n candidates with `cols` target residues
+// in all: per-pair outputs and slots, and the pass-boundary rows.
+uint64_t state_bytes_of(uint64_t n, uint64_t cols) {
+  return n * (sizeof(PairOut) + 32 + sizeof(st_pair_result) + 2 * sizeof(PairDev) + 4) + cols * 16;
 }
 
 }  // namespace
@@ -960,11 +959,16 @@ int st_server_submit(st_server *s, const st_request *req, uint64_t *ticket) {
   r->gate = req->min_diag_score;
   r->out = req->out;
   r->cands.assign(req->cand_tids, req->cand_tids + req->n_cands);
+  // one pass over the candidates: validation and the target residues
+  uint64_t cols = 0;
+  const uint32_t *hl = s->db->hlen.data();
+  const uint64_t nt = s->db->n_targets;
   for (uint32_t t : r->cands) {
-    if (t >= s->db->n_targets) return fail(ST_ERR_INVALID, "st_server_submit: target id out of range");
-    r->cells += (uint64_t)r->qp->m * s->db->hlen[t];
+    if (t >= nt) return fail(ST_ERR_INVALID, "st_server_submit: target id out of range");
+    cols += hl[t];
   }
-  r->state_bytes = state_bytes_of(*r, s->db);
+  r->cells = (uint64_t)r->qp->m * cols;
+  r->state_bytes = state_bytes_of(r->cands.size(), cols);
   {
     std::lock_guard<std::mutex> lk(s->mu);
     if (s->closing) return fail(ST_ERR_CLOSED, "st_server_submit: server is closing");
