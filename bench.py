@@ -48,6 +48,7 @@ def parse():
     ap.add_argument("--parity-pairs", type=int, default=1500)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-serving", action="store_true", help="skip the serving leg")
+    ap.add_argument("--no-prefilter", action="store_true", help="skip the seed-hit stage leg")
     ap.add_argument("--serving-requests", type=int, default=2000)
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a plain device copy of 256 MiB (read 256 MiB, write 256 MiB) three "
@@ -175,6 +176,74 @@ def spans_leg(O, st, ctx, db, qps, rows, w, res, offs, per_query=32, sample=120)
            "gcups": round((fwd + rev) / t / 1e9, 1), "sampled_vs_oracle": int(len(pick)), "mismatches": int(mism)}
     log(f"alignment spans: {out['hits']} hits in {out['ms']:.1f} ms, {out['gcups']} GCUPS over both passes, "
         f"{len(pick) - mism}/{len(pick)} sampled spans bit-exact")
+    return out
+
+
+def prefilter_leg(O, st, ctx, db, w, n_queries=200, k=4, parity_queries=2):
+    """The prefilter's seed-hit stage on the device: an exact k-mer posting
+    index of the step's database (built here on the host, sorted by k-mer,
+    target, position), each query seeded with its exact k-mers, and
+    st_seed_hits_to_jobs timed per query (wall: seed upload, expansion, one
+    radix sort, runs, top diagonals, read-back of jobs and records).
+    Algorithmic bytes per hit: its 8-byte posting read once."""
+    from sabertooth_amd.core import Postings, seed_hits_to_jobs
+    t0 = time.time()
+    ar = w.arena.astype(np.int64)
+    offs = w.offsets.astype(np.int64)
+    n = len(ar) - k + 1
+    codes = np.zeros(n, np.int64)
+    ok = np.ones(n, bool)
+    for j in range(k):
+        codes = codes * 20 + ar[j:j + n]
+        ok &= ar[j:j + n] < 20
+    tid = np.searchsorted(offs, np.arange(n), side="right") - 1
+    ok &= np.arange(n) + k <= offs[tid + 1]
+    idx = np.nonzero(ok)[0]
+    order = np.argsort(codes[idx], kind="stable")
+    pc = codes[idx][order]
+    pos = idx[order]
+    tp = tid[pos]
+    post = (tp.astype(np.uint64) << np.uint64(32)) | (pos - offs[tp]).astype(np.uint64)
+    del codes, ok, tid, idx, order, pos, tp
+    dp = Postings(ctx, post)
+    log(f"prefilter: {k}-mer index of {len(post)} postings built in {time.time() - t0:.1f}s")
+    tl = w.target_lengths
+    seeds_of = []
+    for q in w.queries[:n_queries]:
+        q = q.astype(np.int64)
+        m = len(q) - k + 1
+        if m <= 0:
+            seeds_of.append([])
+            continue
+        c = np.zeros(m, np.int64)
+        for j in range(k):
+            c = c * 20 + q[j:j + m]
+        a = np.searchsorted(pc, c, side="left")
+        b = np.searchsorted(pc, c, side="right")
+        seeds_of.append([(i, int(a[i]), int(b[i])) for i in range(m) if b[i] > a[i]])
+    hits = [sum(e - s for _, s, e in sd) for sd in seeds_of]
+    seed_hits_to_jobs(ctx, db, dp, len(w.queries[0]), seeds_of[0], span=k)
+    ts = time.perf_counter()
+    nj = nr = 0
+    for qi, sd in enumerate(seeds_of):
+        j, r = seed_hits_to_jobs(ctx, db, dp, len(w.queries[qi]), sd, span=k)
+        nj += len(j)
+        nr += len(r)
+    t = time.perf_counter() - ts
+    mism = 0
+    for qi in range(parity_queries):
+        gj, gr = seed_hits_to_jobs(ctx, db, dp, len(w.queries[qi]), seeds_of[qi], span=k)
+        oj, orr = O.seed_hits_to_jobs(len(w.queries[qi]), seeds_of[qi], post, tl, 4, 1, 1, 48, k)
+        mism += int(not (len(gj) == len(oj) and len(gr) == len(orr) and (gj == oj).all() and (gr == orr).all()))
+    dp.close()
+    tot = int(sum(hits))
+    out = {"queries": n_queries, "k": k, "postings": int(len(post)), "hits": tot, "jobs": nj, "targets": nr,
+           "ms": round(1e3 * t, 3), "hits_per_s": round(tot / t, 1),
+           "hbm": {"algorithmic_bytes_per_hit": 8, "achieved_GBps": round(8 * tot / t / 1e9, 1),
+                   "frac_of_8TBps": round(8 * tot / t / 8e12, 4)},
+           "parity_queries_vs_oracle": parity_queries, "mismatched_queries": mism}
+    log(f"prefilter: {n_queries} queries, {tot} seed hits -> {nj} jobs / {nr} targets in {1e3 * t:.1f} ms "
+        f"({tot / t / 1e9:.2f} G hits/s); {parity_queries - mism}/{parity_queries} queries equal the oracle")
     return out
 
 
@@ -312,6 +381,7 @@ def main():
         f"results equal eager: {g_equal}")
 
     spans = spans_leg(O, st, ctx, db, qps, rows, w, res, offs)
+    prefilter = None if a.no_prefilter else prefilter_leg(O, st, ctx, db, w)
 
     dp = float(np.mean(dp_ms)) / 1e3
     kinds = {0: "Pk16", 1: "I32F", 2: "I32N"}
@@ -388,6 +458,7 @@ def main():
         "parity": {"sampled_pairs_all_outputs_vs_oracle": int(len(sample)), "mismatches": int(mism),
                    "all_pairs_swg_vs_cpu_avx2_mismatches": full_swg},
         "roofline": roof, "cpu_baseline": cpu, "graph_replay": graph, "alignment_spans": spans,
+        "prefilter_seed_hits": prefilter,
         "serving": serving,
     }
     print(json.dumps(out), flush=True)

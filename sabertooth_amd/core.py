@@ -309,11 +309,11 @@ def seed_hits_to_jobs(ctx: Context, db: TargetDb, postings: Postings, m: int, se
     hp["diag_window"], hp["span"] = diag_window, span
     nj, nr = C.c_uint64(0), C.c_uint64(0)
     args = (ctx.handle, db.handle, postings.handle, m, ptr(sd), len(seeds), ptr(hp))
-    rc = lib().st_seed_hits_to_jobs(*args, None, 0, C.byref(nj), None, 0, C.byref(nr))
-    if rc not in (_lib.ST_OK, _lib.ST_ERR_CAPACITY):
-        check(rc)
-    jobs = np.zeros(max(nj.value, 1), dtype=JOB_DT)
-    recs = np.zeros(max(nr.value, 1), dtype=REC_DT)
+    # every job and every record comes from at least one distinct hit: the
+    # hit count bounds both, so one call fills them
+    nh = int(sum(int(b) - int(a) for _, a, b in seeds)) if len(seeds) else 0
+    jobs = np.zeros(max(nh, 1), dtype=JOB_DT)
+    recs = np.zeros(max(nh, 1), dtype=REC_DT)
     check(lib().st_seed_hits_to_jobs(*args, ptr(jobs), len(jobs), C.byref(nj), ptr(recs), len(recs), C.byref(nr)))
     return jobs[: nj.value], recs[: nr.value]
 

@@ -98,11 +98,14 @@ struct st_db {
   uint64_t n_targets = 0, n_res = 0;
 };
 
-// An index's posting lists on the device, decoded.
+// An index's posting lists on the device, decoded, and the grow-only device
+// buffers of the seed-hit stage run against them (calls on one context are
+// serialised, so one set serves every call).
 struct st_postings {
   st_ctx *ctx = nullptr;
   st::DevBuf<uint64_t> post;  // (target << 32) | position
   uint64_t n = 0;
+  mutable st::DevBuf<uint8_t> scratch[19];
 };
 
 struct st_qprof {

@@ -190,6 +190,14 @@ __global__ void flag_nonzero_kernel(const uint32_t *__restrict__ v, uint64_t n, 
 
 uint32_t bits_for(uint64_t x) { return x ? 64 - __builtin_clzll(x) : 0; }
 
+// n elements of T in a grow-only byte buffer (nullptr when it cannot grow)
+template <class T>
+T *grow(DevBuf<uint8_t> &b, uint64_t n) {
+  const uint64_t bytes = std::max<uint64_t>(n, 1) * sizeof(T);
+  if (b.n < bytes && b.alloc(bytes) != hipSuccess) return nullptr;
+  return (T *)b.p;
+}
+
 unsigned grid_of(uint64_t n, unsigned b = 256) { return (unsigned)((n + b - 1) / b); }
 
 // exclusive scan of n words into out; returns out[n-1] + in[n-1] (the total)
@@ -274,73 +282,73 @@ int st_seed_hits_to_jobs(st_ctx *ctx, const st_db *db, const st_postings *post, 
   std::lock_guard<std::mutex> lk(ctx->mu);
   ST_HIP(hipSetDevice(ctx->device));
   hipStream_t s = ctx->stream;
-  DevBuf<SeedDev> d_seeds;
-  DevBuf<uint64_t> d_off, k0, k1;
-  DevBuf<uint32_t> flag, idx, bad;
-  DevBuf<uint8_t> tmp;
-  if (d_seeds.upload((const SeedDev *)seeds, nseeds, s) != hipSuccess || d_off.upload(off.data(), nseeds, s) != hipSuccess ||
-      k0.alloc(nh) != hipSuccess || k1.alloc(nh) != hipSuccess || flag.alloc(nh) != hipSuccess ||
-      idx.alloc(nh) != hipSuccess || bad.alloc(1) != hipSuccess)
+  DevBuf<uint8_t> *sc = post->scratch;
+  DevBuf<uint8_t> &tmp = sc[18];
+  SeedDev *d_seeds = grow<SeedDev>(sc[0], nseeds);
+  uint64_t *d_off = grow<uint64_t>(sc[1], nseeds);
+  uint64_t *k0 = grow<uint64_t>(sc[2], nh), *k1 = grow<uint64_t>(sc[3], nh);
+  uint32_t *flag = grow<uint32_t>(sc[4], nh), *idx = grow<uint32_t>(sc[5], nh), *bad = grow<uint32_t>(sc[6], 1);
+  if (!d_seeds || !d_off || !k0 || !k1 || !flag || !idx || !bad)
     return fail(ST_ERR_NOMEM, "st_seed_hits_to_jobs: device allocation failed");
-  ST_HIP(clear_u32(bad.p, 1, s));
-  hipLaunchKernelGGL(expand_kernel, dim3((unsigned)((nseeds + 3) / 4)), dim3(256), 0, s, d_seeds.p, nseeds, d_off.p,
-                     post->post.p, lay, db->n_targets, k0.p, bad.p);
+  ST_HIP(hipMemcpyAsync(d_seeds, seeds, nseeds * sizeof(SeedDev), hipMemcpyHostToDevice, s));
+  ST_HIP(hipMemcpyAsync(d_off, off.data(), nseeds * sizeof(uint64_t), hipMemcpyHostToDevice, s));
+  ST_HIP(clear_u32(bad, 1, s));
+  hipLaunchKernelGGL(expand_kernel, dim3((unsigned)((nseeds + 3) / 4)), dim3(256), 0, s, d_seeds, nseeds, d_off,
+                     post->post.p, lay, db->n_targets, k0, bad);
   ST_HIP(hipGetLastError());
   uint32_t h_bad = 0;
-  ST_HIP(hipMemcpyAsync(&h_bad, bad.p, 4, hipMemcpyDeviceToHost, s));
+  ST_HIP(hipMemcpyAsync(&h_bad, bad, 4, hipMemcpyDeviceToHost, s));
   ST_HIP(hipStreamSynchronize(s));
   if (h_bad) return fail(ST_ERR_INVALID, "st_seed_hits_to_jobs: a posting names a target outside the database");
   // sort the whole keys (the query positions included: runs come out with
   // their smallest and largest positions first and last)
   size_t tb = 0;
-  ST_HIP(hipcub::DeviceRadixSort::SortKeys(nullptr, tb, k0.p, k1.p, (int)nh, 0, (int)end_bit, s));
-  if (tmp.alloc(tb) != hipSuccess) return fail(ST_ERR_NOMEM, "st_seed_hits_to_jobs: device allocation failed");
-  ST_HIP(hipcub::DeviceRadixSort::SortKeys(tmp.p, tb, k0.p, k1.p, (int)nh, 0, (int)end_bit, s));
-  const uint64_t *keys = k1.p;
+  ST_HIP(hipcub::DeviceRadixSort::SortKeys(nullptr, tb, k0, k1, (int)nh, 0, (int)end_bit, s));
+  if (!grow<uint8_t>(tmp, tb)) return fail(ST_ERR_NOMEM, "st_seed_hits_to_jobs: device allocation failed");
+  ST_HIP(hipcub::DeviceRadixSort::SortKeys(tmp.p, tb, k0, k1, (int)nh, 0, (int)end_bit, s));
+  const uint64_t *keys = k1;
   // runs
-  hipLaunchKernelGGL(run_flags_kernel, dim3(grid_of(nh)), dim3(256), 0, s, keys, nh, lay.qbits, flag.p);
+  hipLaunchKernelGGL(run_flags_kernel, dim3(grid_of(nh)), dim3(256), 0, s, keys, nh, lay.qbits, flag);
   ST_HIP(hipGetLastError());
   uint64_t nruns = 0;
-  ST_HIP(scan_total(flag.p, idx.p, nh, tmp, s, &nruns));
-  DevBuf<uint32_t> rfirst, tflag, tidx;
-  if (rfirst.alloc(nruns) != hipSuccess || tflag.alloc(nruns) != hipSuccess || tidx.alloc(nruns) != hipSuccess)
-    return fail(ST_ERR_NOMEM, "st_seed_hits_to_jobs: device allocation failed");
-  hipLaunchKernelGGL(run_first_kernel, dim3(grid_of(nh)), dim3(256), 0, s, keys, nh, lay.qbits, lay.sh_t, flag.p,
-                     idx.p, rfirst.p, tflag.p);
+  ST_HIP(scan_total(flag, idx, nh, tmp, s, &nruns));
+  uint32_t *rfirst = grow<uint32_t>(sc[7], nruns), *tflag = grow<uint32_t>(sc[8], nruns),
+           *tidx = grow<uint32_t>(sc[9], nruns);
+  if (!rfirst || !tflag || !tidx) return fail(ST_ERR_NOMEM, "st_seed_hits_to_jobs: device allocation failed");
+  hipLaunchKernelGGL(run_first_kernel, dim3(grid_of(nh)), dim3(256), 0, s, keys, nh, lay.qbits, lay.sh_t, flag,
+                     idx, rfirst, tflag);
   ST_HIP(hipGetLastError());
   // targets
   uint64_t ntg = 0;
-  ST_HIP(scan_total(tflag.p, tidx.p, nruns, tmp, s, &ntg));
-  DevBuf<uint32_t> tfirst, nj, joff, tkeep, roff, top_runs;
-  if (tfirst.alloc(ntg) != hipSuccess || nj.alloc(ntg) != hipSuccess || joff.alloc(ntg) != hipSuccess ||
-      tkeep.alloc(ntg) != hipSuccess || roff.alloc(ntg) != hipSuccess ||
-      top_runs.alloc(ntg * kMaxJobsPerTarget) != hipSuccess)
+  ST_HIP(scan_total(tflag, tidx, nruns, tmp, s, &ntg));
+  uint32_t *tfirst = grow<uint32_t>(sc[10], ntg), *nj = grow<uint32_t>(sc[11], ntg),
+           *joff = grow<uint32_t>(sc[12], ntg), *tkeep = grow<uint32_t>(sc[13], ntg),
+           *roff = grow<uint32_t>(sc[14], ntg), *top_runs = grow<uint32_t>(sc[15], ntg * kMaxJobsPerTarget);
+  if (!tfirst || !nj || !joff || !tkeep || !roff || !top_runs)
     return fail(ST_ERR_NOMEM, "st_seed_hits_to_jobs: device allocation failed");
-  hipLaunchKernelGGL(target_first_kernel, dim3(grid_of(nruns)), dim3(256), 0, s, nruns, tflag.p, tidx.p, tfirst.p);
+  hipLaunchKernelGGL(target_first_kernel, dim3(grid_of(nruns)), dim3(256), 0, s, nruns, tflag, tidx, tfirst);
   ST_HIP(hipGetLastError());
-  hipLaunchKernelGGL(target_top_kernel, dim3(grid_of(ntg)), dim3(256), 0, s, keys, nh, ntg, nruns, rfirst.p,
-                     tfirst.p, hp->max_diagonals, hp->min_hits, hp->diag_hits, nj.p,
-                     top_runs.p);
+  hipLaunchKernelGGL(target_top_kernel, dim3(grid_of(ntg)), dim3(256), 0, s, keys, nh, ntg, nruns, rfirst, tfirst,
+                     hp->max_diagonals, hp->min_hits, hp->diag_hits, nj, top_runs);
   ST_HIP(hipGetLastError());
   uint64_t tot_jobs = 0, tot_recs = 0;
-  ST_HIP(scan_total(nj.p, joff.p, ntg, tmp, s, &tot_jobs));
-  hipLaunchKernelGGL(flag_nonzero_kernel, dim3(grid_of(ntg)), dim3(256), 0, s, nj.p, ntg, tkeep.p);
+  ST_HIP(scan_total(nj, joff, ntg, tmp, s, &tot_jobs));
+  hipLaunchKernelGGL(flag_nonzero_kernel, dim3(grid_of(ntg)), dim3(256), 0, s, nj, ntg, tkeep);
   ST_HIP(hipGetLastError());
-  ST_HIP(scan_total(tkeep.p, roff.p, ntg, tmp, s, &tot_recs));
+  ST_HIP(scan_total(tkeep, roff, ntg, tmp, s, &tot_recs));
   *njobs = tot_jobs;
   *nrecs = tot_recs;
   if (tot_jobs > jobs_cap || tot_recs > recs_cap || (tot_jobs && !jobs) || (tot_recs && !recs))
     return fail(ST_ERR_CAPACITY, "st_seed_hits_to_jobs: output buffers too small (*njobs, *nrecs hold the sizes)");
   if (tot_recs == 0) return ST_OK;
-  DevBuf<st_job> d_jobs;
-  DevBuf<st_target_rec> d_recs;
-  if (d_jobs.alloc(tot_jobs) != hipSuccess || d_recs.alloc(tot_recs) != hipSuccess)
-    return fail(ST_ERR_NOMEM, "st_seed_hits_to_jobs: device allocation failed");
-  hipLaunchKernelGGL(emit_kernel, dim3(grid_of(ntg)), dim3(256), 0, s, keys, nh, ntg, nruns, rfirst.p, tfirst.p,
-                     nj.p, joff.p, roff.p, top_runs.p, db->len.p, lay, hp->diag_window, hp->span, d_jobs.p, d_recs.p);
+  st_job *d_jobs = grow<st_job>(sc[16], tot_jobs);
+  st_target_rec *d_recs = grow<st_target_rec>(sc[17], tot_recs);
+  if (!d_jobs || !d_recs) return fail(ST_ERR_NOMEM, "st_seed_hits_to_jobs: device allocation failed");
+  hipLaunchKernelGGL(emit_kernel, dim3(grid_of(ntg)), dim3(256), 0, s, keys, nh, ntg, nruns, rfirst, tfirst, nj,
+                     joff, roff, top_runs, db->len.p, lay, hp->diag_window, hp->span, d_jobs, d_recs);
   ST_HIP(hipGetLastError());
-  ST_HIP(hipMemcpyAsync(jobs, d_jobs.p, tot_jobs * sizeof(st_job), hipMemcpyDeviceToHost, s));
-  ST_HIP(hipMemcpyAsync(recs, d_recs.p, tot_recs * sizeof(st_target_rec), hipMemcpyDeviceToHost, s));
+  ST_HIP(hipMemcpyAsync(jobs, d_jobs, tot_jobs * sizeof(st_job), hipMemcpyDeviceToHost, s));
+  ST_HIP(hipMemcpyAsync(recs, d_recs, tot_recs * sizeof(st_target_rec), hipMemcpyDeviceToHost, s));
   ST_HIP(hipStreamSynchronize(s));
   return ST_OK;
 }
