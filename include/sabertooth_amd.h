@@ -254,6 +254,18 @@ int st_seed_hits_to_jobs(st_ctx *ctx, const st_db *db, const st_postings *post, 
                          st_job *jobs, uint64_t jobs_cap, uint64_t *njobs, st_target_rec *recs,
                          uint64_t recs_cap, uint64_t *nrecs);
 
+/* st_seed_hits_to_jobs for nq queries in one call (one device sort of all
+ * their hits): query q has m[q] rows and the seeds seeds[seed_off[q] ..
+ * seed_off[q + 1]).  Jobs and records are concatenated in query order;
+ * job_off and rec_off (nq + 1 entries each) delimit query q's slices, and a
+ * record's j0 / j1 index its own query's slice, so each slice equals the
+ * single-query call's output. */
+int st_seed_hits_to_jobs_batch(st_ctx *ctx, const st_db *db, const st_postings *post, uint64_t nq,
+                               const uint64_t *m, const uint64_t *seed_off, const st_seed *seeds,
+                               const st_hit_params *params, st_job *jobs, uint64_t jobs_cap,
+                               uint64_t *njobs, st_target_rec *recs, uint64_t recs_cap,
+                               uint64_t *nrecs, uint64_t *job_off, uint64_t *rec_off);
+
 /* Candidate (src/prefilter.rs:370-381). */
 typedef struct {
   uint32_t target_id;

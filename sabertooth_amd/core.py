@@ -318,6 +318,33 @@ def seed_hits_to_jobs(ctx: Context, db: TargetDb, postings: Postings, m: int, se
     return jobs[: nj.value], recs[: nr.value]
 
 
+def seed_hits_to_jobs_batch(ctx: Context, db: TargetDb, postings: Postings, ms, seed_lists, max_diagonals: int = 4,
+                            min_hits: int = 1, diag_hits: int = 1, diag_window: int = 48, span: int = 6):
+    """[(jobs, recs) per query] from one device call over every query."""
+    nq = len(ms)
+    so = np.zeros(nq + 1, np.uint64)
+    so[1:] = np.cumsum([len(x) for x in seed_lists]) if nq else []
+    sd = np.zeros(max(int(so[-1]), 1), dtype=_lib.SEED_DT)
+    if int(so[-1]):
+        allv = np.array([t for x in seed_lists for t in x], dtype=np.uint64).reshape(-1, 3)
+        sd["qpos"][: len(allv)], sd["cur"][: len(allv)], sd["end"][: len(allv)] = allv[:, 0], allv[:, 1], allv[:, 2]
+    nh = int((sd["end"].astype(np.int64) - sd["cur"].astype(np.int64)).sum())
+    hp = np.zeros(1, dtype=_lib.HIT_PARAMS_DT)
+    hp["max_diagonals"], hp["min_hits"], hp["diag_hits"] = max_diagonals, min_hits, diag_hits
+    hp["diag_window"], hp["span"] = diag_window, span
+    m = np.ascontiguousarray(ms, dtype=np.uint64)
+    # bounds, filled only up to the counts the call returns
+    jobs = np.empty(max(nh, 1), dtype=JOB_DT)
+    recs = np.empty(max(nh, 1), dtype=REC_DT)
+    jo = np.zeros(nq + 1, np.uint64)
+    ro = np.zeros(nq + 1, np.uint64)
+    nj, nr = C.c_uint64(0), C.c_uint64(0)
+    check(lib().st_seed_hits_to_jobs_batch(ctx.handle, db.handle, postings.handle, nq, ptr(m), ptr(so), ptr(sd),
+                                           ptr(hp), ptr(jobs), len(jobs), C.byref(nj), ptr(recs), len(recs),
+                                           C.byref(nr), ptr(jo), ptr(ro)))
+    return [(jobs[int(jo[q]):int(jo[q + 1])], recs[int(ro[q]):int(ro[q + 1])]) for q in range(nq)]
+
+
 def score_and_gate(qp: QueryProfile, db: TargetDb, jobs, recs, gaps: GapCosts,
                    min_diag_score: int, regions: bool = False):
     """(candidates, chains_used, job_scores) as score_and_gate produces them."""

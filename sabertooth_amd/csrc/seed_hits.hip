@@ -33,9 +33,11 @@ struct SeedDev {
   uint64_t cur, end;
 };
 
+// Key fields, most significant first: query (several queries per call),
+// target, diagonal + M (M = the longest query), query position.
 struct Layout {
-  uint32_t qbits, dbits, sh_t;
-  uint64_t m;
+  uint32_t qbits, dbits, sh_t, tbits;
+  uint64_t m;  // M
 };
 
 __device__ __forceinline__ uint64_t td_of(uint64_t k, uint32_t qbits) { return k >> qbits; }
@@ -51,6 +53,7 @@ __global__ __launch_bounds__(256) void expand_kernel(const SeedDev *__restrict__
   if (s >= nseeds) return;
   const SeedDev sd = seeds[s];
   const uint64_t o = off[s];
+  const uint64_t qhi = (uint64_t)sd.reserved << (lay.sh_t + lay.tbits);  // the seed's query
   for (uint64_t k = sd.cur + lane; k < sd.end; k += 64) {
     const uint64_t p = post[k];
     const uint64_t tid = p >> 32, tpos = p & 0xffffffffull;
@@ -59,7 +62,7 @@ __global__ __launch_bounds__(256) void expand_kernel(const SeedDev *__restrict__
       keys[o + (k - sd.cur)] = 0;
       continue;
     }
-    keys[o + (k - sd.cur)] = (tid << lay.sh_t) | ((tpos + lay.m - sd.qpos) << lay.qbits) | sd.qpos;
+    keys[o + (k - sd.cur)] = qhi | (tid << lay.sh_t) | ((tpos + lay.m - sd.qpos) << lay.qbits) | sd.qpos;
   }
 }
 
@@ -140,22 +143,24 @@ __global__ void emit_kernel(const uint64_t *__restrict__ keys, uint64_t nkeys, u
                             const uint32_t *__restrict__ rfirst, const uint32_t *__restrict__ tfirst,
                             const uint32_t *__restrict__ njobs, const uint32_t *__restrict__ joff,
                             const uint32_t *__restrict__ roff, const uint32_t *__restrict__ top_runs,
-                            const uint32_t *__restrict__ tlen, Layout lay, uint32_t diag_window, uint32_t span,
-                            st_job *__restrict__ jobs, st_target_rec *__restrict__ recs) {
+                            const uint32_t *__restrict__ tlen, const uint64_t *__restrict__ m_of, Layout lay,
+                            uint32_t diag_window, uint32_t span, st_job *__restrict__ jobs,
+                            st_target_rec *__restrict__ recs, uint32_t *__restrict__ rec_q) {
   const uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= ntg || njobs[t] == 0) return;
   const uint32_t r0 = tfirst[t];
   const uint32_t r1 = t + 1 < ntg ? tfirst[t + 1] : (uint32_t)nruns;
   const uint64_t h0 = rfirst[r0];
   const uint64_t h1 = r1 < nruns ? rfirst[r1] : nkeys;
-  const uint32_t tid = (uint32_t)(keys[h0] >> lay.sh_t);
-  const int64_t n = tlen[tid], m = (int64_t)lay.m;
+  const uint32_t tid = (uint32_t)((keys[h0] >> lay.sh_t) & ((1ull << lay.tbits) - 1));
+  const uint32_t q = (uint32_t)(lay.sh_t + lay.tbits >= 64 ? 0 : keys[h0] >> (lay.sh_t + lay.tbits));
+  const int64_t n = tlen[tid], m = (int64_t)m_of[q], M = (int64_t)lay.m;
   const uint64_t qmask = (1ull << lay.qbits) - 1;
   const uint32_t j0 = joff[t];
   for (uint32_t k = 0; k < njobs[t]; k++) {
     const uint32_t r = top_runs[t * kMaxJobsPerTarget + k];
     const uint64_t a = rfirst[r], b = (r + 1 < nruns ? rfirst[r + 1] : nkeys) - 1;
-    const int64_t diag = (int64_t)((keys[a] >> lay.qbits) & ((1ull << lay.dbits) - 1)) - m;
+    const int64_t diag = (int64_t)((keys[a] >> lay.qbits) & ((1ull << lay.dbits) - 1)) - M;
     const int64_t qmin = (int64_t)(keys[a] & qmask), qmax = (int64_t)(keys[b] & qmask);
     // diag_range (src/ungapped.rs:38-48): query positions whose target
     // position d + i lies in [0, n)
@@ -181,6 +186,7 @@ __global__ void emit_kernel(const uint64_t *__restrict__ keys, uint64_t nkeys, u
   rc.j0 = j0;
   rc.j1 = j0 + njobs[t];
   recs[roff[t]] = rc;
+  rec_q[roff[t]] = q;
 }
 
 __global__ void flag_nonzero_kernel(const uint32_t *__restrict__ v, uint64_t n, uint32_t *__restrict__ f) {
@@ -247,51 +253,73 @@ int st_postings_destroy(st_postings *p) {
 
 uint64_t st_postings_count(const st_postings *p) { return p ? p->n : 0; }
 
-int st_seed_hits_to_jobs(st_ctx *ctx, const st_db *db, const st_postings *post, uint64_t m,
-                         const st_seed *seeds, uint64_t nseeds, const st_hit_params *hp, st_job *jobs,
-                         uint64_t jobs_cap, uint64_t *njobs, st_target_rec *recs, uint64_t recs_cap,
-                         uint64_t *nrecs) {
-  if (!ctx || !db || !post || !hp || !njobs || !nrecs || (nseeds && !seeds))
-    return fail(ST_ERR_INVALID, "st_seed_hits_to_jobs: null argument");
+// The stage over nq queries at once: query q's seeds are
+// seeds[seed_off[q] .. seed_off[q+1]) against its m[q] rows.  Outputs are
+// concatenated in query order; job_off / rec_off (nq + 1) delimit each
+// query's jobs and records, and a record's j0 / j1 index its query's jobs.
+static int seed_hits_impl(const char *fn, st_ctx *ctx, const st_db *db, const st_postings *post, uint64_t nq,
+                          const uint64_t *m, const uint64_t *seed_off, const st_seed *seeds,
+                          const st_hit_params *hp, st_job *jobs, uint64_t jobs_cap, uint64_t *njobs,
+                          st_target_rec *recs, uint64_t recs_cap, uint64_t *nrecs, uint64_t *job_off,
+                          uint64_t *rec_off) {
+  const std::string f(fn);
+  if (!ctx || !db || !post || !hp || !njobs || !nrecs || (nq && (!m || !seed_off)))
+    return fail(ST_ERR_INVALID, f + ": null argument");
   *njobs = *nrecs = 0;
-  if (db->ctx != ctx || post->ctx != ctx) return fail(ST_ERR_INVALID, "st_seed_hits_to_jobs: handle of another context");
+  if (job_off) std::fill(job_off, job_off + nq + 1, 0);
+  if (rec_off) std::fill(rec_off, rec_off + nq + 1, 0);
+  if (db->ctx != ctx || post->ctx != ctx) return fail(ST_ERR_INVALID, f + ": handle of another context");
   if (hp->max_diagonals < 1 || hp->max_diagonals > (uint32_t)kMaxJobsPerTarget)
-    return fail(ST_ERR_INVALID, "st_seed_hits_to_jobs: max_diagonals must be 1..16");
-  if (m == 0 || nseeds == 0 || db->n_targets == 0) return ST_OK;
-  // hits per seed -> key offsets
+    return fail(ST_ERR_INVALID, f + ": max_diagonals must be 1..16");
+  if (nq == 0 || db->n_targets == 0) return ST_OK;
+  if (nq > 0xffffffffull) return fail(ST_ERR_INVALID, f + ": more than 2^32 queries");
+  const uint64_t nseeds = seed_off[nq];
+  if (seed_off[0] != 0) return fail(ST_ERR_INVALID, f + ": seed_off[0] must be 0");
+  if (nseeds && !seeds) return fail(ST_ERR_INVALID, f + ": null seeds");
+  // hits per seed -> key offsets; each seed learns its query
+  std::vector<SeedDev> sv(nseeds);
   std::vector<uint64_t> off(nseeds + 1, 0);
-  for (uint64_t s = 0; s < nseeds; s++) {
-    const st_seed &sd = seeds[s];
-    if (sd.cur > sd.end || sd.end > post->n || sd.qpos >= m)
-      return fail(ST_ERR_INVALID, "st_seed_hits_to_jobs: seed outside the postings or the query");
-    off[s + 1] = off[s] + (sd.end - sd.cur);
+  uint64_t M = 0;
+  for (uint64_t q = 0; q < nq; q++) {
+    if (seed_off[q + 1] < seed_off[q]) return fail(ST_ERR_INVALID, f + ": seed_off must not decrease");
+    if (seed_off[q + 1] > seed_off[q]) M = std::max(M, m[q]);
+    for (uint64_t k = seed_off[q]; k < seed_off[q + 1]; k++) {
+      const st_seed &sd = seeds[k];
+      if (sd.cur > sd.end || sd.end > post->n || sd.qpos >= m[q])
+        return fail(ST_ERR_INVALID, f + ": seed outside the postings or the query");
+      sv[k] = SeedDev{sd.qpos, (uint32_t)q, sd.cur, sd.end};
+      off[k + 1] = off[k] + (sd.end - sd.cur);
+    }
   }
   const uint64_t nh = off[nseeds];
   if (nh == 0) return ST_OK;
   // the device sort and scans count items in int
-  if (nh > 0x7fffffffull) return fail(ST_ERR_INVALID, "st_seed_hits_to_jobs: more than 2^31 - 1 seed hits");
+  if (nh > 0x7fffffffull) return fail(ST_ERR_INVALID, f + ": more than 2^31 - 1 seed hits");
   uint32_t maxlen = 0;
   for (uint32_t l : db->hlen) maxlen = std::max(maxlen, l);
   Layout lay;
-  lay.m = m;
-  lay.qbits = bits_for(m - 1);
-  lay.dbits = bits_for(m + maxlen);
+  lay.m = M;
+  lay.qbits = bits_for(M - 1);
+  lay.dbits = bits_for(M + maxlen);
   lay.sh_t = lay.qbits + lay.dbits;
-  const uint32_t end_bit = lay.sh_t + bits_for(db->n_targets - 1);
-  if (end_bit > 64) return fail(ST_ERR_INVALID, "st_seed_hits_to_jobs: query, targets and database too large for a 64-bit hit key");
+  lay.tbits = bits_for(db->n_targets - 1);
+  const uint32_t end_bit = lay.sh_t + lay.tbits + bits_for(nq - 1);
+  if (end_bit > 64) return fail(ST_ERR_INVALID, f + ": queries, targets and database too large for a 64-bit hit key");
   std::lock_guard<std::mutex> lk(ctx->mu);
   ST_HIP(hipSetDevice(ctx->device));
   hipStream_t s = ctx->stream;
   DevBuf<uint8_t> *sc = post->scratch;
   DevBuf<uint8_t> &tmp = sc[18];
   SeedDev *d_seeds = grow<SeedDev>(sc[0], nseeds);
-  uint64_t *d_off = grow<uint64_t>(sc[1], nseeds);
+  uint64_t *d_off = grow<uint64_t>(sc[1], nseeds + nq);
   uint64_t *k0 = grow<uint64_t>(sc[2], nh), *k1 = grow<uint64_t>(sc[3], nh);
   uint32_t *flag = grow<uint32_t>(sc[4], nh), *idx = grow<uint32_t>(sc[5], nh), *bad = grow<uint32_t>(sc[6], 1);
   if (!d_seeds || !d_off || !k0 || !k1 || !flag || !idx || !bad)
-    return fail(ST_ERR_NOMEM, "st_seed_hits_to_jobs: device allocation failed");
-  ST_HIP(hipMemcpyAsync(d_seeds, seeds, nseeds * sizeof(SeedDev), hipMemcpyHostToDevice, s));
+    return fail(ST_ERR_NOMEM, f + ": device allocation failed");
+  uint64_t *d_m = d_off + nseeds;  // the queries' lengths after the key offsets
+  ST_HIP(hipMemcpyAsync(d_seeds, sv.data(), nseeds * sizeof(SeedDev), hipMemcpyHostToDevice, s));
   ST_HIP(hipMemcpyAsync(d_off, off.data(), nseeds * sizeof(uint64_t), hipMemcpyHostToDevice, s));
+  ST_HIP(hipMemcpyAsync(d_m, m, nq * sizeof(uint64_t), hipMemcpyHostToDevice, s));
   ST_HIP(clear_u32(bad, 1, s));
   hipLaunchKernelGGL(expand_kernel, dim3((unsigned)((nseeds + 3) / 4)), dim3(256), 0, s, d_seeds, nseeds, d_off,
                      post->post.p, lay, db->n_targets, k0, bad);
@@ -299,12 +327,12 @@ int st_seed_hits_to_jobs(st_ctx *ctx, const st_db *db, const st_postings *post, 
   uint32_t h_bad = 0;
   ST_HIP(hipMemcpyAsync(&h_bad, bad, 4, hipMemcpyDeviceToHost, s));
   ST_HIP(hipStreamSynchronize(s));
-  if (h_bad) return fail(ST_ERR_INVALID, "st_seed_hits_to_jobs: a posting names a target outside the database");
+  if (h_bad) return fail(ST_ERR_INVALID, f + ": a posting names a target outside the database");
   // sort the whole keys (the query positions included: runs come out with
   // their smallest and largest positions first and last)
   size_t tb = 0;
   ST_HIP(hipcub::DeviceRadixSort::SortKeys(nullptr, tb, k0, k1, (int)nh, 0, (int)end_bit, s));
-  if (!grow<uint8_t>(tmp, tb)) return fail(ST_ERR_NOMEM, "st_seed_hits_to_jobs: device allocation failed");
+  if (!grow<uint8_t>(tmp, tb)) return fail(ST_ERR_NOMEM, f + ": device allocation failed");
   ST_HIP(hipcub::DeviceRadixSort::SortKeys(tmp.p, tb, k0, k1, (int)nh, 0, (int)end_bit, s));
   const uint64_t *keys = k1;
   // runs
@@ -314,18 +342,17 @@ int st_seed_hits_to_jobs(st_ctx *ctx, const st_db *db, const st_postings *post, 
   ST_HIP(scan_total(flag, idx, nh, tmp, s, &nruns));
   uint32_t *rfirst = grow<uint32_t>(sc[7], nruns), *tflag = grow<uint32_t>(sc[8], nruns),
            *tidx = grow<uint32_t>(sc[9], nruns);
-  if (!rfirst || !tflag || !tidx) return fail(ST_ERR_NOMEM, "st_seed_hits_to_jobs: device allocation failed");
+  if (!rfirst || !tflag || !tidx) return fail(ST_ERR_NOMEM, f + ": device allocation failed");
   hipLaunchKernelGGL(run_first_kernel, dim3(grid_of(nh)), dim3(256), 0, s, keys, nh, lay.qbits, lay.sh_t, flag,
                      idx, rfirst, tflag);
   ST_HIP(hipGetLastError());
-  // targets
+  // (query, target) groups
   uint64_t ntg = 0;
   ST_HIP(scan_total(tflag, tidx, nruns, tmp, s, &ntg));
   uint32_t *tfirst = grow<uint32_t>(sc[10], ntg), *nj = grow<uint32_t>(sc[11], ntg),
            *joff = grow<uint32_t>(sc[12], ntg), *tkeep = grow<uint32_t>(sc[13], ntg),
            *roff = grow<uint32_t>(sc[14], ntg), *top_runs = grow<uint32_t>(sc[15], ntg * kMaxJobsPerTarget);
-  if (!tfirst || !nj || !joff || !tkeep || !roff || !top_runs)
-    return fail(ST_ERR_NOMEM, "st_seed_hits_to_jobs: device allocation failed");
+  if (!tfirst || !nj || !joff || !tkeep || !roff || !top_runs) return fail(ST_ERR_NOMEM, f + ": device allocation failed");
   hipLaunchKernelGGL(target_first_kernel, dim3(grid_of(nruns)), dim3(256), 0, s, nruns, tflag, tidx, tfirst);
   ST_HIP(hipGetLastError());
   hipLaunchKernelGGL(target_top_kernel, dim3(grid_of(ntg)), dim3(256), 0, s, keys, nh, ntg, nruns, rfirst, tfirst,
@@ -339,18 +366,61 @@ int st_seed_hits_to_jobs(st_ctx *ctx, const st_db *db, const st_postings *post, 
   *njobs = tot_jobs;
   *nrecs = tot_recs;
   if (tot_jobs > jobs_cap || tot_recs > recs_cap || (tot_jobs && !jobs) || (tot_recs && !recs))
-    return fail(ST_ERR_CAPACITY, "st_seed_hits_to_jobs: output buffers too small (*njobs, *nrecs hold the sizes)");
+    return fail(ST_ERR_CAPACITY, f + ": output buffers too small (*njobs, *nrecs hold the sizes)");
   if (tot_recs == 0) return ST_OK;
   st_job *d_jobs = grow<st_job>(sc[16], tot_jobs);
   st_target_rec *d_recs = grow<st_target_rec>(sc[17], tot_recs);
-  if (!d_jobs || !d_recs) return fail(ST_ERR_NOMEM, "st_seed_hits_to_jobs: device allocation failed");
+  uint32_t *d_rq = grow<uint32_t>(sc[4], tot_recs);  // the run flags are no longer needed
+  if (!d_jobs || !d_recs || !d_rq) return fail(ST_ERR_NOMEM, f + ": device allocation failed");
   hipLaunchKernelGGL(emit_kernel, dim3(grid_of(ntg)), dim3(256), 0, s, keys, nh, ntg, nruns, rfirst, tfirst, nj,
-                     joff, roff, top_runs, db->len.p, lay, hp->diag_window, hp->span, d_jobs, d_recs);
+                     joff, roff, top_runs, db->len.p, d_m, lay, hp->diag_window, hp->span, d_jobs, d_recs, d_rq);
   ST_HIP(hipGetLastError());
+  std::vector<uint32_t> rq(tot_recs);
   ST_HIP(hipMemcpyAsync(jobs, d_jobs, tot_jobs * sizeof(st_job), hipMemcpyDeviceToHost, s));
   ST_HIP(hipMemcpyAsync(recs, d_recs, tot_recs * sizeof(st_target_rec), hipMemcpyDeviceToHost, s));
+  ST_HIP(hipMemcpyAsync(rq.data(), d_rq, tot_recs * sizeof(uint32_t), hipMemcpyDeviceToHost, s));
   ST_HIP(hipStreamSynchronize(s));
+  // per-query slices; each record's jobs indexed within its query's slice
+  std::vector<uint64_t> ro(nq + 1, 0), jo(nq + 1, 0);
+  for (uint64_t k = 0; k < tot_recs; k++) {
+    ro[rq[k] + 1]++;
+    jo[rq[k] + 1] += recs[k].j1 - recs[k].j0;
+  }
+  for (uint64_t q = 0; q < nq; q++) {
+    ro[q + 1] += ro[q];
+    jo[q + 1] += jo[q];
+  }
+  for (uint64_t k = 0; k < tot_recs; k++) {
+    recs[k].j0 -= (uint32_t)jo[rq[k]];
+    recs[k].j1 -= (uint32_t)jo[rq[k]];
+  }
+  if (job_off) std::copy(jo.begin(), jo.end(), job_off);
+  if (rec_off) std::copy(ro.begin(), ro.end(), rec_off);
   return ST_OK;
+}
+
+int st_seed_hits_to_jobs(st_ctx *ctx, const st_db *db, const st_postings *post, uint64_t m,
+                         const st_seed *seeds, uint64_t nseeds, const st_hit_params *hp, st_job *jobs,
+                         uint64_t jobs_cap, uint64_t *njobs, st_target_rec *recs, uint64_t recs_cap,
+                         uint64_t *nrecs) {
+  if (m == 0) {
+    if (!njobs || !nrecs) return fail(ST_ERR_INVALID, "st_seed_hits_to_jobs: null argument");
+    *njobs = *nrecs = 0;
+    return ST_OK;
+  }
+  const uint64_t so[2] = {0, nseeds};
+  return seed_hits_impl("st_seed_hits_to_jobs", ctx, db, post, 1, &m, so, seeds, hp, jobs, jobs_cap, njobs, recs,
+                        recs_cap, nrecs, nullptr, nullptr);
+}
+
+int st_seed_hits_to_jobs_batch(st_ctx *ctx, const st_db *db, const st_postings *post, uint64_t nq,
+                               const uint64_t *m, const uint64_t *seed_off, const st_seed *seeds,
+                               const st_hit_params *hp, st_job *jobs, uint64_t jobs_cap, uint64_t *njobs,
+                               st_target_rec *recs, uint64_t recs_cap, uint64_t *nrecs, uint64_t *job_off,
+                               uint64_t *rec_off) {
+  if (nq && (!job_off || !rec_off)) return fail(ST_ERR_INVALID, "st_seed_hits_to_jobs_batch: null offsets");
+  return seed_hits_impl("st_seed_hits_to_jobs_batch", ctx, db, post, nq, m, seed_off, seeds, hp, jobs, jobs_cap,
+                        njobs, recs, recs_cap, nrecs, job_off, rec_off);
 }
 
 }  // extern "C"

@@ -122,3 +122,26 @@ def test_the_reference_index_drives_the_stage(ctx):
         oj, orr = O.seed_hits_to_jobs(len(q), seeds, post, tlen, 4, 1, 1, 48, 2)
         assert (gj == oj).all() and (gr == orr).all() and qi in set(int(x) for x in gr["tid"])
     dp.close()
+
+
+def test_many_queries_in_one_call(ctx):
+    from sabertooth_amd.core import seed_hits_to_jobs_batch
+    rng = np.random.default_rng(77)
+    targets = [rng.integers(0, 4, size=int(rng.integers(3, 300)), dtype=np.uint8) for _ in range(200)]
+    post, of = kmer_index(targets, 3)
+    tlen = np.array([len(t) for t in targets])
+    db = _db(ctx, targets)
+    dp = Postings(ctx, post)
+    queries = [rng.integers(0, 4, size=int(L), dtype=np.uint8) for L in (5, 1, 200, 3, 77, 150, 2)]
+    seeds = [query_seeds(q, 3, of, extra=5, n_post=len(post), rng=rng) if len(q) >= 3 else [] for q in queries]
+    seeds[3] = []  # a query without seeds
+    for kmax, mh, dh, win in [(4, 1, 1, 48), (16, 2, 2, 0)]:
+        out = seed_hits_to_jobs_batch(ctx, db, dp, [len(q) for q in queries], seeds, kmax, mh, dh, win, 3)
+        assert len(out) == len(queries)
+        for qi, (gj, gr) in enumerate(out):
+            oj, orr = O.seed_hits_to_jobs(len(queries[qi]), seeds[qi], post, tlen, kmax, mh, dh, win, 3)
+            assert (gj == oj).all() and (gr == orr).all(), qi
+            sj, sr = seed_hits_to_jobs(ctx, db, dp, len(queries[qi]), seeds[qi], kmax, mh, dh, win, 3)
+            assert (sj == gj).all() and (sr == gr).all(), qi
+    assert seed_hits_to_jobs_batch(ctx, db, dp, [], []) == []
+    dp.close()
