@@ -325,6 +325,15 @@ int st_batch_create(st_ctx *ctx, const st_db *db, const st_qprof *const *queries
                     st_batch **out);
 /* Run every kernel of the batch on the device and wait for it. */
 int st_batch_run(st_batch *b);
+/* The planner's launch-shape cost table (device picoseconds per executed
+ * cell for rows per lane 8, 12, 16 by lanes per group 1..64) is compiled in,
+ * or read from the file the environment variable ST_SHAPE_TABLE names, as
+ * `python -m sabertooth_amd.autotune` measures and writes it (21 numbers,
+ * '#' comments; a path is read once per process).  ST_OK when the variable
+ * is unset or names a valid table;
+ * otherwise ST_ERR_INVALID with the reason, which st_batch_create and
+ * st_server_create then return too.  No counterpart in the reference. */
+int st_shape_table_status(void);
 /* st_batch_run with the main DP phase (clearing the clamp list, then every
  * DP launch of the plan forked over the context's launch streams) captured
  * into a HIP graph by the first call and replayed by every later call, one

@@ -94,6 +94,7 @@ _SIGS = [
     ("st_batch_create", C.c_int, [_P, _P, _P, C.c_uint64, _P, _P, _P, _P, _P, C.POINTER(_P)]),
     ("st_batch_run", C.c_int, [_P]),
     ("st_batch_run_graph", C.c_int, [_P]),
+    ("st_shape_table_status", C.c_int, []),
     ("st_batch_results", C.c_int, [_P, _P]),
     ("st_batch_timing", C.c_int, [_P, _P, _P]),
     ("st_batch_stats", C.c_int, [_P, _P, _P, _P]),

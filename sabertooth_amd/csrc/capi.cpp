@@ -3,6 +3,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cctype>
 #include <cmath>
 #include <cstdlib>
 #include <cstdio>
@@ -303,6 +304,90 @@ static const double kShapePs[3][7] = {
     {0.88, 0.405, 0.236, 0.230, 0.237, 0.325, 0.665}, // R = 16
 };
 
+// The cost table in use: the compiled one, or the 21 values (rows R = 8,
+// 12, 16; columns G = 1, 2, 4, ..., 64; '#' starts a comment) of the file
+// that ST_SHAPE_TABLE names, as `python -m sabertooth_amd.autotune` writes
+// it.  A file is read once per path; a malformed one is reported by
+// shape_table_check and never used.
+namespace {
+struct ShapeTableState {
+  std::mutex mu;
+  std::string path;
+  bool ok = false;
+  std::string err;
+  double ps[3][7];
+};
+ShapeTableState &shape_state() {
+  static ShapeTableState s;
+  return s;
+}
+// loads (or reuses) the file of `path`; called with the state's lock held
+void load_shape_table(ShapeTableState &st, const std::string &path) {
+  st.path = path;
+  st.ok = false;
+  FILE *f = std::fopen(path.c_str(), "r");
+  if (!f) {
+    st.err = "ST_SHAPE_TABLE: cannot read " + path;
+    return;
+  }
+  std::string text;
+  char buf[4096];
+  size_t n;
+  while ((n = std::fread(buf, 1, sizeof buf, f)) > 0) text.append(buf, n);
+  std::fclose(f);
+  std::vector<double> v;
+  size_t i = 0;
+  while (i < text.size()) {
+    if (text[i] == '#') {
+      while (i < text.size() && text[i] != '\n') i++;
+      continue;
+    }
+    if (std::isspace((unsigned char)text[i]) || text[i] == ',') {
+      i++;
+      continue;
+    }
+    char *end = nullptr;
+    const double x = std::strtod(text.c_str() + i, &end);
+    if (end == text.c_str() + i || !std::isfinite(x) || x <= 0.0) {
+      st.err = "ST_SHAPE_TABLE: " + path + ": not a positive number at byte " + std::to_string(i);
+      return;
+    }
+    v.push_back(x);
+    i = (size_t)(end - text.c_str());
+  }
+  if (v.size() != 21) {
+    st.err = "ST_SHAPE_TABLE: " + path + ": expected 21 values (3 heights x 7 group widths), found " +
+             std::to_string(v.size());
+    return;
+  }
+  for (int r = 0; r < 3; r++)
+    for (int g = 0; g < 7; g++) st.ps[r][g] = v[(size_t)r * 7 + g];
+  st.ok = true;
+}
+}  // namespace
+
+// ST_OK, or ST_ERR_INVALID when ST_SHAPE_TABLE names a file that is not a
+// valid table.
+int shape_table_check() {
+  const char *e = std::getenv("ST_SHAPE_TABLE");
+  if (!e || !*e) return ST_OK;
+  ShapeTableState &st = shape_state();
+  std::lock_guard<std::mutex> lk(st.mu);
+  if (st.path != e) load_shape_table(st, e);
+  return st.ok ? ST_OK : fail(ST_ERR_INVALID, st.err);
+}
+
+// the table the planner uses now
+static void shape_table(double (&ps)[3][7]) {
+  std::memcpy(ps, kShapePs, sizeof ps);
+  const char *e = std::getenv("ST_SHAPE_TABLE");
+  if (!e || !*e) return;
+  ShapeTableState &st = shape_state();
+  std::lock_guard<std::mutex> lk(st.mu);
+  if (st.path != e) load_shape_table(st, e);
+  if (st.ok) std::memcpy(ps, st.ps, sizeof ps);
+}
+
 double unit_steps_cap(uint64_t batch_cells) {
   // half the mean work of a resident wave (256 CUs x 8 waves, 2048 cells per
   // packed step), within [300, 12000] steps: a lone query spreads over many
@@ -347,6 +432,8 @@ bool plan_query(Plan &pl, std::vector<Unit> &pending, uint32_t qi, const st_qpro
   const int per = kind == kPk16 ? 2 : 1;
   const uint64_t m = qp->m;
   const int fr = forced_rows(), fg = forced_groups();
+  double shape_ps[3][7];
+  shape_table(shape_ps);
   for (size_t s = 0; s < count;) {
     int bG = 16, bR = kind == kI32N ? 8 : 16;
     double bc = 1e300;
@@ -372,7 +459,7 @@ bool plan_query(Plan &pl, std::vector<Unit> &pending, uint32_t qi, const st_qpro
           continue;
         const double exec = 64.0 * R * per * passes * (np + G - 1);
         const double useful = std::max(1.0, (double)m * (double)(csum[e] - csum[s]));
-        double c = exec * kShapePs[ri][gi] / useful;
+        double c = exec * shape_ps[ri][gi] / useful;
         const double steps = (double)passes * (np + G - 1) * R / 16.0;
         if (steps > cap) c *= steps / cap;
         if (c < bc) {
@@ -596,6 +683,7 @@ static int batch_create_impl(st_ctx *ctx, const st_db *db, const st_qprof *const
     return fail(ST_ERR_INVALID, "st_batch_create: null argument");
   *out = nullptr;
   if (db->ctx != ctx) return fail(ST_ERR_INVALID, "st_batch_create: database of another context");
+  if (int rc = shape_table_check()) return rc;
   const uint64_t np = nq ? cand_off[nq] : 0;
   if (nq && cand_off[0] != 0) return fail(ST_ERR_INVALID, "st_batch_create: cand_off[0] must be 0");
   if (np && !cand_tids) return fail(ST_ERR_INVALID, "st_batch_create: null candidate list");
@@ -798,6 +886,8 @@ static int batch_run_locked(st_batch *b, bool graph = false) {
   }
   return ST_OK;
 }
+
+int st_shape_table_status(void) { return shape_table_check(); }
 
 int st_batch_run(st_batch *b) {
   if (!b) return fail(ST_ERR_INVALID, "st_batch_run: null batch");

@@ -128,6 +128,10 @@ enum Kind { kPk16 = 0, kI32F = 1, kI32N = 2 };
 
 Kind kind_for(const st_qprof *qp, int32_t open, int32_t ext);
 
+// ST_OK, or ST_ERR_INVALID (with the message) when ST_SHAPE_TABLE names a
+// malformed launch-shape cost table.
+int shape_table_check();
+
 // Streams (and unit counters) for concurrent DP launches.
 constexpr int kLaunchSlots = 4;
 

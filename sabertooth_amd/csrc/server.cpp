@@ -892,6 +892,7 @@ int st_server_create(st_ctx *ctx, const st_db *db, const st_server_config *cfg, 
   if (!ctx || !db || !out) return fail(ST_ERR_INVALID, "st_server_create: null argument");
   if (db->ctx != ctx) return fail(ST_ERR_INVALID, "st_server_create: database of another context");
   *out = nullptr;
+  if (int rc = shape_table_check()) return rc;
   auto *s = new st_server();
   s->ctx = ctx;
   s->db = db;
