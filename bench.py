@@ -179,14 +179,17 @@ def spans_leg(O, st, ctx, db, qps, rows, w, res, offs, per_query=32, sample=120)
     return out
 
 
-def prefilter_leg(O, st, ctx, db, w, n_queries=200, k=4, parity_queries=2):
+def prefilter_leg(O, st, ctx, db, w, qps, gate, n_queries=200, k=4, parity_queries=2):
     """The prefilter's seed-hit stage on the device: an exact k-mer posting
     index of the step's database (built here on the host, sorted by k-mer,
     target, position), each query seeded with its exact k-mers, and
     st_seed_hits_to_jobs timed per query (wall: seed upload, expansion, one
-    radix sort, runs, top diagonals, read-back of jobs and records).
+    radix sort, runs, top diagonals, read-back of jobs and records), then
+    every query in one call, then seeds -> candidates in one call (the jobs
+    scored and gated on the device, only candidates read back).
     Algorithmic bytes per hit: its 8-byte posting read once."""
-    from sabertooth_amd.core import Postings, seed_hits_to_jobs, seed_hits_to_jobs_batch
+    from sabertooth_amd.core import (Postings, seed_hits_to_candidates_batch, seed_hits_to_jobs,
+                                     seed_hits_to_jobs_batch, _seed_array)
     t0 = time.time()
     ar = w.arena.astype(np.int64)
     offs = w.offsets.astype(np.int64)
@@ -237,21 +240,38 @@ def prefilter_leg(O, st, ctx, db, w, n_queries=200, k=4, parity_queries=2):
     outb = seed_hits_to_jobs_batch(ctx, db, dp, ms, seeds_of, span=k)
     t = time.perf_counter() - ts
     assert sum(len(j) for j, _ in outb) == nj and sum(len(r) for _, r in outb) == nr
-    mism = 0
+    del outb
+    gaps = [st.GapCosts(w.gap_open, w.gap_extend)] * n_queries
+    gates = [gate] * n_queries
+    sa = _seed_array(seeds_of)
+    seed_hits_to_candidates_batch(db, dp, qps[:2], None, gaps[:2], gates[:2], span=k,
+                                  seed_arrays=(sa[0][:3] - sa[0][0], sa[1]))
+    ts = time.perf_counter()
+    cands, chains = seed_hits_to_candidates_batch(db, dp, qps[:n_queries], None, gaps, gates, span=k, seed_arrays=sa)
+    tc = time.perf_counter() - ts
+    nc = sum(len(c) for c in cands)
+    mism = cmism = 0
     for qi in range(parity_queries):
         gj, gr = seed_hits_to_jobs(ctx, db, dp, len(w.queries[qi]), seeds_of[qi], span=k)
         oj, orr = O.seed_hits_to_jobs(len(w.queries[qi]), seeds_of[qi], post, tl, 4, 1, 1, 48, k)
         mism += int(not (len(gj) == len(oj) and len(gr) == len(orr) and (gj == oj).all() and (gr == orr).all()))
+        c1, _, _ = st.score_and_gate(qps[qi], db, gj, gr, gaps[qi], gate)
+        cmism += int(not (len(c1) == len(cands[qi]) and (c1 == cands[qi]).all()))
     dp.close()
     tot = int(sum(hits))
     out = {"queries": n_queries, "k": k, "postings": int(len(post)), "hits": tot, "jobs": nj, "targets": nr,
            "ms": round(1e3 * t, 3), "hits_per_s": round(tot / t, 1), "one_query_per_call_ms": round(1e3 * t1, 3),
            "hbm": {"algorithmic_bytes_per_hit": 8, "achieved_GBps": round(8 * tot / t / 1e9, 1),
                    "frac_of_8TBps": round(8 * tot / t / 8e12, 4)},
-           "parity_queries_vs_oracle": parity_queries, "mismatched_queries": mism}
+           "parity_queries_vs_oracle": parity_queries, "mismatched_queries": mism,
+           "to_candidates": {"ms": round(1e3 * tc, 3), "hits_per_s": round(tot / tc, 1), "candidates": nc,
+                             "chains_used": int(chains), "gate": int(gate),
+                             "mismatched_queries_vs_score_and_gate": cmism}}
     log(f"prefilter: {n_queries} queries, {tot} seed hits -> {nj} jobs / {nr} targets in {1e3 * t:.1f} ms in one "
         f"call ({tot / t / 1e9:.2f} G hits/s; {1e3 * t1:.1f} ms one query per call); "
         f"{parity_queries - mism}/{parity_queries} queries equal the oracle")
+    log(f"prefilter: seeds -> {nc} candidates in one call: {1e3 * tc:.1f} ms ({tot / tc / 1e9:.2f} G hits/s); "
+        f"{parity_queries - cmism}/{parity_queries} queries equal score_and_gate")
     return out
 
 
@@ -389,7 +409,7 @@ def main():
         f"results equal eager: {g_equal}")
 
     spans = spans_leg(O, st, ctx, db, qps, rows, w, res, offs)
-    prefilter = None if a.no_prefilter else prefilter_leg(O, st, ctx, db, w)
+    prefilter = None if a.no_prefilter else prefilter_leg(O, st, ctx, db, w, qps, w.gate)
 
     dp = float(np.mean(dp_ms)) / 1e3
     kinds = {0: "Pk16", 1: "I32F", 2: "I32N"}

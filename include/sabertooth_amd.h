@@ -294,6 +294,24 @@ int st_score_and_gate(st_ctx *ctx, const st_qprof *qp, const st_db *db,
                       uint64_t *n_out, uint64_t *chains_used,
                       int32_t *job_scores);
 
+/* The seed-hit stage and score_and_gate chained on the device for nq
+ * queries in one call (evaluate_block_kadane's jobs scored and gated
+ * without leaving the device, src/prefilter.rs:1412-1492, 1498-1660): query
+ * q (handle queries[q]) has the seeds seeds[seed_off[q] .. seed_off[q+1])
+ * and is gated with (gap_open[q], gap_extend[q], min_diag_score[q]).  The
+ * candidates that pass, in each query's target order (score_and_gate's
+ * push order), go to out; cand_off (nq + 1) delimits each query's.  Equal
+ * to st_seed_hits_to_jobs_batch followed by st_score_and_gate per query.
+ * *n_out = the count; ST_ERR_CAPACITY when it exceeds cap (nothing written
+ * but *n_out and chains_used); chains_used (nullable) is accumulated. */
+int st_seed_hits_to_candidates_batch(st_ctx *ctx, const st_db *db, const st_postings *post,
+                                     uint64_t nq, const st_qprof *const *queries,
+                                     const uint64_t *seed_off, const st_seed *seeds,
+                                     const st_hit_params *params, const int32_t *gap_open,
+                                     const int32_t *gap_extend, const int32_t *min_diag_score,
+                                     int regions, st_candidate *out, uint64_t cap, uint64_t *n_out,
+                                     uint64_t *cand_off, uint64_t *chains_used);
+
 /* size_adjusted_gate (src/prefilter.rs:246-257). */
 int32_t st_size_adjusted_gate(int32_t gate, uint64_t db_residues);
 

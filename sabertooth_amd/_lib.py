@@ -87,6 +87,9 @@ _SIGS = [
     ("st_seed_hits_to_jobs_batch", C.c_int, [_P, _P, _P, C.c_uint64, _P, _P, _P, _P, _P, C.c_uint64,
                                              C.POINTER(C.c_uint64), _P, C.c_uint64, C.POINTER(C.c_uint64), _P,
                                              _P]),
+    ("st_seed_hits_to_candidates_batch", C.c_int, [_P, _P, _P, C.c_uint64, _P, _P, _P, _P, _P, _P, _P, C.c_int,
+                                                   _P, C.c_uint64, C.POINTER(C.c_uint64), _P,
+                                                   C.POINTER(C.c_uint64)]),
     ("st_ungapped_score_jobs", C.c_int, [_P, _P, _P, _P, C.c_uint64, _P]),
     ("st_score_and_gate", C.c_int, [_P, _P, _P, _P, C.c_uint64, _P, C.c_uint64, C.c_int32, C.c_int32,
                                     C.c_int32, C.c_int, _P, _P, _P, _P]),

@@ -322,12 +322,7 @@ def seed_hits_to_jobs_batch(ctx: Context, db: TargetDb, postings: Postings, ms, 
                             min_hits: int = 1, diag_hits: int = 1, diag_window: int = 48, span: int = 6):
     """[(jobs, recs) per query] from one device call over every query."""
     nq = len(ms)
-    so = np.zeros(nq + 1, np.uint64)
-    so[1:] = np.cumsum([len(x) for x in seed_lists]) if nq else []
-    sd = np.zeros(max(int(so[-1]), 1), dtype=_lib.SEED_DT)
-    if int(so[-1]):
-        allv = np.array([t for x in seed_lists for t in x], dtype=np.uint64).reshape(-1, 3)
-        sd["qpos"][: len(allv)], sd["cur"][: len(allv)], sd["end"][: len(allv)] = allv[:, 0], allv[:, 1], allv[:, 2]
+    so, sd = _seed_array(seed_lists)
     nh = int((sd["end"].astype(np.int64) - sd["cur"].astype(np.int64)).sum())
     hp = np.zeros(1, dtype=_lib.HIT_PARAMS_DT)
     hp["max_diagonals"], hp["min_hits"], hp["diag_hits"] = max_diagonals, min_hits, diag_hits
@@ -343,6 +338,53 @@ def seed_hits_to_jobs_batch(ctx: Context, db: TargetDb, postings: Postings, ms, 
                                            ptr(hp), ptr(jobs), len(jobs), C.byref(nj), ptr(recs), len(recs),
                                            C.byref(nr), ptr(jo), ptr(ro)))
     return [(jobs[int(jo[q]):int(jo[q + 1])], recs[int(ro[q]):int(ro[q + 1])]) for q in range(nq)]
+
+
+def _seed_array(seed_lists):
+    so = np.zeros(len(seed_lists) + 1, np.uint64)
+    so[1:] = np.cumsum([len(x) for x in seed_lists]) if len(seed_lists) else []
+    sd = np.zeros(max(int(so[-1]), 1), dtype=_lib.SEED_DT)
+    if int(so[-1]):
+        allv = np.array([t for x in seed_lists for t in x], dtype=np.uint64).reshape(-1, 3)
+        sd["qpos"][: len(allv)], sd["cur"][: len(allv)], sd["end"][: len(allv)] = allv[:, 0], allv[:, 1], allv[:, 2]
+    return so, sd
+
+
+def seed_hits_to_candidates_batch(db: TargetDb, postings: Postings, queries: Sequence[QueryProfile], seed_lists,
+                                  gaps: Sequence[GapCosts], min_diag_scores, regions: bool = False,
+                                  max_diagonals: int = 4, min_hits: int = 1, diag_hits: int = 1,
+                                  diag_window: int = 48, span: int = 6, seed_arrays=None):
+    """([candidates per query], chains_used): the seed-hit stage and
+    score_and_gate in one device call; seed_arrays = (seed_off, SEED_DT
+    rows) replaces seed_lists."""
+    nq = len(queries)
+    so, sd = seed_arrays if seed_arrays is not None else _seed_array(seed_lists)
+    so = np.ascontiguousarray(so, dtype=np.uint64)
+    sd = np.ascontiguousarray(sd, dtype=_lib.SEED_DT)
+    hp = np.zeros(1, dtype=_lib.HIT_PARAMS_DT)
+    hp["max_diagonals"], hp["min_hits"], hp["diag_hits"] = max_diagonals, min_hits, diag_hits
+    hp["diag_window"], hp["span"] = diag_window, span
+    qh = (C.c_void_p * max(nq, 1))(*[q.handle for q in queries])
+    op = np.array([g.open for g in gaps], np.int32)
+    ex = np.array([g.extend for g in gaps], np.int32)
+    gt = np.ascontiguousarray(min_diag_scores, dtype=np.int32)
+    co = np.zeros(nq + 1, np.uint64)
+    n, chains = C.c_uint64(0), C.c_uint64(0)
+    ctx = queries[0].ctx if nq else None
+    if nq == 0:
+        return [], 0
+    cap = 1 << 16
+    while True:
+        out = np.empty(cap, dtype=CAND_DT)
+        rc = lib().st_seed_hits_to_candidates_batch(ctx.handle, db.handle, postings.handle, nq, qh, ptr(so),
+                                                    ptr(sd), ptr(hp), ptr(op), ptr(ex), ptr(gt),
+                                                    1 if regions else 0, ptr(out), cap, C.byref(n), ptr(co),
+                                                    C.byref(chains))
+        if rc == _lib.ST_ERR_CAPACITY and n.value > cap:
+            cap, chains = int(n.value), C.c_uint64(0)
+            continue
+        check(rc)
+        return [out[int(co[q]):int(co[q + 1])] for q in range(nq)], chains.value
 
 
 def score_and_gate(qp: QueryProfile, db: TargetDb, jobs, recs, gaps: GapCosts,

@@ -189,6 +189,42 @@ __global__ void emit_kernel(const uint64_t *__restrict__ keys, uint64_t nkeys, u
   rec_q[roff[t]] = q;
 }
 
+// jq[j]: the query of job j (its record's query)
+__global__ void job_query_kernel(const st_target_rec *__restrict__ recs, const uint32_t *__restrict__ rq, uint64_t n,
+                                 uint32_t *__restrict__ jq) {
+  const uint64_t r = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= n) return;
+  for (uint32_t j = recs[r].j0; j < recs[r].j1; j++) jq[j] = rq[r];
+}
+
+// records with jobs: pass flag and chain flag (score_and_gate's push and its
+// chains_used count, src/prefilter.rs:1577-1660)
+__global__ void cand_flags_kernel(const GateRes *__restrict__ g, const st_target_rec *__restrict__ recs, uint64_t n,
+                                  uint32_t *__restrict__ pass, uint32_t *__restrict__ chain) {
+  const uint64_t r = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= n) return;
+  const bool jobs = recs[r].j1 > recs[r].j0;
+  pass[r] = jobs && g[r].pass ? 1u : 0u;
+  chain[r] = jobs && g[r].chain ? 1u : 0u;
+}
+
+__global__ void cand_emit_kernel(const GateRes *__restrict__ g, const st_target_rec *__restrict__ recs,
+                                 const uint32_t *__restrict__ rq, const uint32_t *__restrict__ pass,
+                                 const uint32_t *__restrict__ idx, uint64_t n, st_candidate *__restrict__ out,
+                                 uint32_t *__restrict__ cq) {
+  const uint64_t r = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= n || !pass[r]) return;
+  st_candidate c;
+  c.target_id = recs[r].tid;
+  c.hits = recs[r].hits;
+  c.best_diag = g[r].best_diag;
+  c.diag_score = g[r].score;
+  c.t_lo = g[r].t_lo;
+  c.t_hi = g[r].t_hi;
+  out[idx[r]] = c;
+  cq[idx[r]] = rq[r];
+}
+
 __global__ void flag_nonzero_kernel(const uint32_t *__restrict__ v, uint64_t n, uint32_t *__restrict__ f) {
   const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) f[i] = v[i] ? 1u : 0u;
@@ -253,29 +289,37 @@ int st_postings_destroy(st_postings *p) {
 
 uint64_t st_postings_count(const st_postings *p) { return p ? p->n : 0; }
 
-// The stage over nq queries at once: query q's seeds are
-// seeds[seed_off[q] .. seed_off[q+1]) against its m[q] rows.  Outputs are
-// concatenated in query order; job_off / rec_off (nq + 1) delimit each
-// query's jobs and records, and a record's j0 / j1 index its query's jobs.
-static int seed_hits_impl(const char *fn, st_ctx *ctx, const st_db *db, const st_postings *post, uint64_t nq,
-                          const uint64_t *m, const uint64_t *seed_off, const st_seed *seeds,
-                          const st_hit_params *hp, st_job *jobs, uint64_t jobs_cap, uint64_t *njobs,
-                          st_target_rec *recs, uint64_t recs_cap, uint64_t *nrecs, uint64_t *job_off,
-                          uint64_t *rec_off) {
-  const std::string f(fn);
-  if (!ctx || !db || !post || !hp || !njobs || !nrecs || (nq && (!m || !seed_off)))
-    return fail(ST_ERR_INVALID, f + ": null argument");
-  *njobs = *nrecs = 0;
-  if (job_off) std::fill(job_off, job_off + nq + 1, 0);
-  if (rec_off) std::fill(rec_off, rec_off + nq + 1, 0);
+// The seed-hit stage over nq queries, left on the device: query q's seeds
+// are seeds[seed_off[q] .. seed_off[q+1]) against its m[q] rows.  Jobs,
+// records (j0 / j1 index all jobs) and each record's query stay in the
+// postings' scratch, in (query, target) order.
+struct SeedStage {
+  uint64_t njobs = 0, nrecs = 0;
+  st_job *jobs = nullptr;
+  st_target_rec *recs = nullptr;
+  uint32_t *rq = nullptr;
+};
+
+static int seed_stage_check(const std::string &f, st_ctx *ctx, const st_db *db, const st_postings *post, uint64_t nq,
+                            const uint64_t *m, const uint64_t *seed_off, const st_seed *seeds,
+                            const st_hit_params *hp) {
+  if (!ctx || !db || !post || !hp || (nq && (!m || !seed_off))) return fail(ST_ERR_INVALID, f + ": null argument");
   if (db->ctx != ctx || post->ctx != ctx) return fail(ST_ERR_INVALID, f + ": handle of another context");
   if (hp->max_diagonals < 1 || hp->max_diagonals > (uint32_t)kMaxJobsPerTarget)
     return fail(ST_ERR_INVALID, f + ": max_diagonals must be 1..16");
-  if (nq == 0 || db->n_targets == 0) return ST_OK;
   if (nq > 0xffffffffull) return fail(ST_ERR_INVALID, f + ": more than 2^32 queries");
+  if (nq && seed_off[0] != 0) return fail(ST_ERR_INVALID, f + ": seed_off[0] must be 0");
+  if (nq && seed_off[nq] && !seeds) return fail(ST_ERR_INVALID, f + ": null seeds");
+  return ST_OK;
+}
+
+// ctx->mu held by the caller; arguments checked by seed_stage_check
+static int seed_stage(const std::string &f, st_ctx *ctx, const st_db *db, const st_postings *post, uint64_t nq,
+                      const uint64_t *m, const uint64_t *seed_off, const st_seed *seeds, const st_hit_params *hp,
+                      SeedStage &out) {
+  out = SeedStage();
+  if (nq == 0 || db->n_targets == 0) return ST_OK;
   const uint64_t nseeds = seed_off[nq];
-  if (seed_off[0] != 0) return fail(ST_ERR_INVALID, f + ": seed_off[0] must be 0");
-  if (nseeds && !seeds) return fail(ST_ERR_INVALID, f + ": null seeds");
   // hits per seed -> key offsets; each seed learns its query
   std::vector<SeedDev> sv(nseeds);
   std::vector<uint64_t> off(nseeds + 1, 0);
@@ -305,7 +349,6 @@ static int seed_hits_impl(const char *fn, st_ctx *ctx, const st_db *db, const st
   lay.tbits = bits_for(db->n_targets - 1);
   const uint32_t end_bit = lay.sh_t + lay.tbits + bits_for(nq - 1);
   if (end_bit > 64) return fail(ST_ERR_INVALID, f + ": queries, targets and database too large for a 64-bit hit key");
-  std::lock_guard<std::mutex> lk(ctx->mu);
   ST_HIP(hipSetDevice(ctx->device));
   hipStream_t s = ctx->stream;
   DevBuf<uint8_t> *sc = post->scratch;
@@ -363,26 +406,53 @@ static int seed_hits_impl(const char *fn, st_ctx *ctx, const st_db *db, const st
   hipLaunchKernelGGL(flag_nonzero_kernel, dim3(grid_of(ntg)), dim3(256), 0, s, nj, ntg, tkeep);
   ST_HIP(hipGetLastError());
   ST_HIP(scan_total(tkeep, roff, ntg, tmp, s, &tot_recs));
-  *njobs = tot_jobs;
-  *nrecs = tot_recs;
-  if (tot_jobs > jobs_cap || tot_recs > recs_cap || (tot_jobs && !jobs) || (tot_recs && !recs))
-    return fail(ST_ERR_CAPACITY, f + ": output buffers too small (*njobs, *nrecs hold the sizes)");
   if (tot_recs == 0) return ST_OK;
   st_job *d_jobs = grow<st_job>(sc[16], tot_jobs);
   st_target_rec *d_recs = grow<st_target_rec>(sc[17], tot_recs);
-  uint32_t *d_rq = grow<uint32_t>(sc[4], tot_recs);  // the run flags are no longer needed
+  uint32_t *d_rq = grow<uint32_t>(sc[19], tot_recs);
   if (!d_jobs || !d_recs || !d_rq) return fail(ST_ERR_NOMEM, f + ": device allocation failed");
   hipLaunchKernelGGL(emit_kernel, dim3(grid_of(ntg)), dim3(256), 0, s, keys, nh, ntg, nruns, rfirst, tfirst, nj,
                      joff, roff, top_runs, db->len.p, d_m, lay, hp->diag_window, hp->span, d_jobs, d_recs, d_rq);
   ST_HIP(hipGetLastError());
-  std::vector<uint32_t> rq(tot_recs);
-  ST_HIP(hipMemcpyAsync(jobs, d_jobs, tot_jobs * sizeof(st_job), hipMemcpyDeviceToHost, s));
-  ST_HIP(hipMemcpyAsync(recs, d_recs, tot_recs * sizeof(st_target_rec), hipMemcpyDeviceToHost, s));
-  ST_HIP(hipMemcpyAsync(rq.data(), d_rq, tot_recs * sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+  out.njobs = tot_jobs;
+  out.nrecs = tot_recs;
+  out.jobs = d_jobs;
+  out.recs = d_recs;
+  out.rq = d_rq;
+  return ST_OK;
+}
+
+// The stage's jobs and records copied out in query order; job_off /
+// rec_off (nq + 1) delimit each query's slices, and a record's j0 / j1
+// index its query's jobs.
+static int seed_hits_impl(const char *fn, st_ctx *ctx, const st_db *db, const st_postings *post, uint64_t nq,
+                          const uint64_t *m, const uint64_t *seed_off, const st_seed *seeds,
+                          const st_hit_params *hp, st_job *jobs, uint64_t jobs_cap, uint64_t *njobs,
+                          st_target_rec *recs, uint64_t recs_cap, uint64_t *nrecs, uint64_t *job_off,
+                          uint64_t *rec_off) {
+  const std::string f(fn);
+  if (!njobs || !nrecs) return fail(ST_ERR_INVALID, f + ": null argument");
+  *njobs = *nrecs = 0;
+  if (job_off) std::fill(job_off, job_off + nq + 1, 0);
+  if (rec_off) std::fill(rec_off, rec_off + nq + 1, 0);
+  if (int rc = seed_stage_check(f, ctx, db, post, nq, m, seed_off, seeds, hp)) return rc;
+  std::lock_guard<std::mutex> lk(ctx->mu);
+  SeedStage st;
+  if (int rc = seed_stage(f, ctx, db, post, nq, m, seed_off, seeds, hp, st)) return rc;
+  *njobs = st.njobs;
+  *nrecs = st.nrecs;
+  if (st.njobs > jobs_cap || st.nrecs > recs_cap || (st.njobs && !jobs) || (st.nrecs && !recs))
+    return fail(ST_ERR_CAPACITY, f + ": output buffers too small (*njobs, *nrecs hold the sizes)");
+  if (st.nrecs == 0) return ST_OK;
+  hipStream_t s = ctx->stream;
+  std::vector<uint32_t> rq(st.nrecs);
+  ST_HIP(hipMemcpyAsync(jobs, st.jobs, st.njobs * sizeof(st_job), hipMemcpyDeviceToHost, s));
+  ST_HIP(hipMemcpyAsync(recs, st.recs, st.nrecs * sizeof(st_target_rec), hipMemcpyDeviceToHost, s));
+  ST_HIP(hipMemcpyAsync(rq.data(), st.rq, st.nrecs * sizeof(uint32_t), hipMemcpyDeviceToHost, s));
   ST_HIP(hipStreamSynchronize(s));
   // per-query slices; each record's jobs indexed within its query's slice
   std::vector<uint64_t> ro(nq + 1, 0), jo(nq + 1, 0);
-  for (uint64_t k = 0; k < tot_recs; k++) {
+  for (uint64_t k = 0; k < st.nrecs; k++) {
     ro[rq[k] + 1]++;
     jo[rq[k] + 1] += recs[k].j1 - recs[k].j0;
   }
@@ -390,12 +460,84 @@ static int seed_hits_impl(const char *fn, st_ctx *ctx, const st_db *db, const st
     ro[q + 1] += ro[q];
     jo[q + 1] += jo[q];
   }
-  for (uint64_t k = 0; k < tot_recs; k++) {
+  for (uint64_t k = 0; k < st.nrecs; k++) {
     recs[k].j0 -= (uint32_t)jo[rq[k]];
     recs[k].j1 -= (uint32_t)jo[rq[k]];
   }
   if (job_off) std::copy(jo.begin(), jo.end(), job_off);
   if (rec_off) std::copy(ro.begin(), ro.end(), rec_off);
+  return ST_OK;
+}
+
+int st_seed_hits_to_candidates_batch(st_ctx *ctx, const st_db *db, const st_postings *post, uint64_t nq,
+                                     const st_qprof *const *queries, const uint64_t *seed_off,
+                                     const st_seed *seeds, const st_hit_params *hp, const int32_t *gap_open,
+                                     const int32_t *gap_extend, const int32_t *min_diag_score, int regions,
+                                     st_candidate *out, uint64_t cap, uint64_t *n_out, uint64_t *cand_off,
+                                     uint64_t *chains_used) {
+  const std::string f("st_seed_hits_to_candidates_batch");
+  if (!n_out || (nq && (!queries || !gap_open || !gap_extend || !min_diag_score || !cand_off)))
+    return fail(ST_ERR_INVALID, f + ": null argument");
+  *n_out = 0;
+  if (cand_off) std::fill(cand_off, cand_off + nq + 1, 0);
+  std::vector<uint64_t> m(nq);
+  std::vector<QueryDev> qv(nq);
+  for (uint64_t q = 0; q < nq; q++) {
+    if (!queries[q] || queries[q]->ctx != ctx) return fail(ST_ERR_INVALID, f + ": bad query handle");
+    m[q] = queries[q]->m;
+    qv[q] = queries[q]->view;
+  }
+  if (int rc = seed_stage_check(f, ctx, db, post, nq, m.data(), seed_off, seeds, hp)) return rc;
+  std::lock_guard<std::mutex> lk(ctx->mu);
+  SeedStage st;
+  if (int rc = seed_stage(f, ctx, db, post, nq, m.data(), seed_off, seeds, hp, st)) return rc;
+  if (st.nrecs == 0) return ST_OK;
+  hipStream_t s = ctx->stream;
+  DevBuf<uint8_t> *sc = post->scratch;
+  uint32_t *jq = grow<uint32_t>(sc[20], st.njobs);
+  int32_t *scores = grow<int32_t>(sc[21], st.njobs);
+  GateRes *gres = grow<GateRes>(sc[22], st.nrecs);
+  uint32_t *pass = grow<uint32_t>(sc[23], st.nrecs), *pidx = grow<uint32_t>(sc[24], st.nrecs);
+  uint32_t *chain = grow<uint32_t>(sc[25], st.nrecs), *cidx = grow<uint32_t>(sc[26], st.nrecs);
+  st_candidate *d_c = grow<st_candidate>(sc[27], st.nrecs);
+  uint32_t *d_cq = grow<uint32_t>(sc[28], st.nrecs);
+  QueryDev *d_q = grow<QueryDev>(sc[29], nq);
+  DevBuf<int32_t> prm;
+  if (!jq || !scores || !gres || !pass || !pidx || !chain || !cidx || !d_c || !d_cq || !d_q ||
+      prm.alloc(3 * nq) != hipSuccess)
+    return fail(ST_ERR_NOMEM, f + ": device allocation failed");
+  ST_HIP(hipMemcpyAsync(d_q, qv.data(), nq * sizeof(QueryDev), hipMemcpyHostToDevice, s));
+  ST_HIP(hipMemcpyAsync(prm.p, gap_open, nq * 4, hipMemcpyHostToDevice, s));
+  ST_HIP(hipMemcpyAsync(prm.p + nq, gap_extend, nq * 4, hipMemcpyHostToDevice, s));
+  ST_HIP(hipMemcpyAsync(prm.p + 2 * nq, min_diag_score, nq * 4, hipMemcpyHostToDevice, s));
+  // score every job against its query, then gate every record
+  hipLaunchKernelGGL(job_query_kernel, dim3(grid_of(st.nrecs)), dim3(256), 0, s, st.recs, st.rq, st.nrecs, jq);
+  ST_HIP(hipGetLastError());
+  ST_HIP(launch_ung_jobs_multi((const JobDev *)st.jobs, st.njobs, jq, d_q, db->arena.p, db->off.p, scores, s));
+  ST_HIP(launch_gate_recs_multi((const RecDev *)st.recs, st.nrecs, (const JobDev *)st.jobs, scores, st.rq, d_q,
+                                db->arena.p, db->off.p, prm.p, prm.p + nq, prm.p + 2 * nq, regions, gres, s));
+  // the passing records, compacted in (query, target) order
+  hipLaunchKernelGGL(cand_flags_kernel, dim3(grid_of(st.nrecs)), dim3(256), 0, s, gres, st.recs, st.nrecs, pass,
+                     chain);
+  ST_HIP(hipGetLastError());
+  uint64_t nc = 0, nchain = 0;
+  DevBuf<uint8_t> &tmp = sc[18];
+  ST_HIP(scan_total(pass, pidx, st.nrecs, tmp, s, &nc));
+  ST_HIP(scan_total(chain, cidx, st.nrecs, tmp, s, &nchain));
+  if (chains_used) *chains_used += nchain;
+  *n_out = nc;
+  if (nc > cap || (nc && !out))
+    return fail(ST_ERR_CAPACITY, f + ": output buffer too small (*n_out holds the size)");
+  if (nc == 0) return ST_OK;
+  hipLaunchKernelGGL(cand_emit_kernel, dim3(grid_of(st.nrecs)), dim3(256), 0, s, gres, st.recs, st.rq, pass, pidx,
+                     st.nrecs, d_c, d_cq);
+  ST_HIP(hipGetLastError());
+  std::vector<uint32_t> cq(nc);
+  ST_HIP(hipMemcpyAsync(out, d_c, nc * sizeof(st_candidate), hipMemcpyDeviceToHost, s));
+  ST_HIP(hipMemcpyAsync(cq.data(), d_cq, nc * 4, hipMemcpyDeviceToHost, s));
+  ST_HIP(hipStreamSynchronize(s));
+  for (uint32_t q : cq) cand_off[q + 1]++;
+  for (uint64_t q = 0; q < nq; q++) cand_off[q + 1] += cand_off[q];
   return ST_OK;
 }
 

@@ -145,3 +145,50 @@ def test_many_queries_in_one_call(ctx):
             assert (sj == gj).all() and (sr == gr).all(), qi
     assert seed_hits_to_jobs_batch(ctx, db, dp, [], []) == []
     dp.close()
+
+
+def test_seeds_to_candidates_on_the_device(ctx):
+    # several queries (planted homologs, per-query gaps and gates, one query
+    # without seeds): the device chain equals the stage's jobs pushed
+    # through score_and_gate per query, on the device and on the oracle
+    from sabertooth_amd.core import seed_hits_to_candidates_batch
+    rng = np.random.default_rng(21)
+    qs = [rng.integers(0, 20, size=int(L), dtype=np.uint8) for L in (250, 40, 600, 120, 9)]
+    targets = [rng.integers(0, 20, size=int(rng.integers(30, 700)), dtype=np.uint8) for _ in range(500)]
+    for k in range(0, 500, 25):
+        h = qs[k % len(qs)].copy()
+        h[rng.random(len(h)) < 0.3] = rng.integers(0, 20)
+        targets[k] = np.concatenate([targets[k][:40], h])
+    post, of = kmer_index(targets, 4)
+    tlen = np.array([len(t) for t in targets])
+    arena = np.concatenate(targets)
+    offs = np.zeros(len(targets) + 1, np.uint64)
+    offs[1:] = np.cumsum(tlen)
+    db = _db(ctx, targets)
+    dp = Postings(ctx, post)
+    seeds = [query_seeds(q, 4, of) for q in qs]
+    seeds[3] = []
+    rows = [O.seq_rows(q) for q in qs]
+    qps = [st.QueryProfile.from_rows(ctx, r) for r in rows]
+    gaps = [GapCosts(11, 1), GapCosts(8, 2), GapCosts(11, 1), GapCosts(5, 1), GapCosts(11, 1)]
+    gates = [35, 30, 45, 20, 10]
+    for regions, params in [(False, (4, 1, 1, 48)), (True, (16, 2, 1, 0))]:
+        cands, chains = seed_hits_to_candidates_batch(db, dp, qps, seeds, gaps, gates, regions, *params, span=4)
+        total = 0
+        for qi in range(len(qs)):
+            gj, gr = seed_hits_to_jobs(ctx, db, dp, len(qs[qi]), seeds[qi], *params, 4)
+            c1, ch1, _ = st.score_and_gate(qps[qi], db, gj, gr, gaps[qi], gates[qi], regions)
+            oc, och, _ = O.score_and_gate(rows[qi], arena, offs, gj, gr, gaps[qi].open, gaps[qi].extend, gates[qi],
+                                          regions)
+            assert len(cands[qi]) == len(c1) == len(oc), qi
+            assert (cands[qi] == c1).all() and (c1 == oc).all(), qi
+            assert ch1 == och
+            total += ch1
+        assert chains == total
+        assert sum(len(c) for c in cands) > 15
+    # a query whose seeds find nothing, alone
+    cands, chains = seed_hits_to_candidates_batch(db, dp, [qps[3]], [[]], [gaps[3]], [20])
+    assert len(cands[0]) == 0 and chains == 0
+    for q in qps:
+        q.close()
+    dp.close()
