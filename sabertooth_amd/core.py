@@ -370,21 +370,16 @@ def seed_hits_to_candidates_batch(db: TargetDb, postings: Postings, queries: Seq
     gt = np.ascontiguousarray(min_diag_scores, dtype=np.int32)
     co = np.zeros(nq + 1, np.uint64)
     n, chains = C.c_uint64(0), C.c_uint64(0)
-    ctx = queries[0].ctx if nq else None
     if nq == 0:
         return [], 0
-    cap = 1 << 16
-    while True:
-        out = np.empty(cap, dtype=CAND_DT)
-        rc = lib().st_seed_hits_to_candidates_batch(ctx.handle, db.handle, postings.handle, nq, qh, ptr(so),
-                                                    ptr(sd), ptr(hp), ptr(op), ptr(ex), ptr(gt),
-                                                    1 if regions else 0, ptr(out), cap, C.byref(n), ptr(co),
-                                                    C.byref(chains))
-        if rc == _lib.ST_ERR_CAPACITY and n.value > cap:
-            cap, chains = int(n.value), C.c_uint64(0)
-            continue
-        check(rc)
-        return [out[int(co[q]):int(co[q + 1])] for q in range(nq)], chains.value
+    # a candidate is a record, and every record has a distinct seed hit: the
+    # hit count bounds the output (np.empty leaves untouched pages unbacked)
+    nh = int((sd["end"][: int(so[-1])].astype(np.int64) - sd["cur"][: int(so[-1])].astype(np.int64)).sum())
+    out = np.empty(max(nh, 1), dtype=CAND_DT)
+    check(lib().st_seed_hits_to_candidates_batch(queries[0].ctx.handle, db.handle, postings.handle, nq, qh, ptr(so),
+                                                 ptr(sd), ptr(hp), ptr(op), ptr(ex), ptr(gt), 1 if regions else 0,
+                                                 ptr(out), len(out), C.byref(n), ptr(co), C.byref(chains)))
+    return [out[int(co[q]):int(co[q + 1])] for q in range(nq)], chains.value
 
 
 def score_and_gate(qp: QueryProfile, db: TargetDb, jobs, recs, gaps: GapCosts,

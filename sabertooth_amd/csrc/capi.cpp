@@ -1191,7 +1191,7 @@ int st_ungapped_score_jobs(st_ctx *ctx, const st_qprof *qp, const st_db *db, con
   DevBuf<int32_t> ds;
   if (dj.upload((const JobDev *)jobs, njobs, ctx->stream) != hipSuccess || ds.alloc(njobs) != hipSuccess)
     return fail(ST_ERR_NOMEM, "st_ungapped_score_jobs: device allocation failed");
-  ST_HIP(launch_ung_jobs(dj.p, njobs, qp->qd.p, db->arena.p, db->off.p, ds.p, ctx->stream));
+  ST_HIP(launch_ung_jobs(dj.p, njobs, qp->qd.p, qp->view.prof16 ? qp->m : 0, db->arena.p, db->off.p, ds.p, ctx->stream));
   ST_HIP(hipMemcpyAsync(scores, ds.p, njobs * 4, hipMemcpyDeviceToHost, ctx->stream));
   ST_HIP(hipStreamSynchronize(ctx->stream));
   return ST_OK;
@@ -1222,7 +1222,7 @@ int st_score_and_gate(st_ctx *ctx, const st_qprof *qp, const st_db *db, const st
       ds.upload(sc.data(), njobs, ctx->stream) != hipSuccess ||
       dr.upload((const RecDev *)recs, nrecs, ctx->stream) != hipSuccess || dg.alloc(nrecs) != hipSuccess)
     return fail(ST_ERR_NOMEM, "st_score_and_gate: device allocation failed");
-  ST_HIP(launch_gate_recs(dr.p, nrecs, dj.p, ds.p, qp->qd.p, db->arena.p, db->off.p, gap_open, gap_extend,
+  ST_HIP(launch_gate_recs(dr.p, nrecs, dj.p, ds.p, qp->qd.p, qp->view.prof16 ? qp->m : 0, db->arena.p, db->off.p, gap_open, gap_extend,
                           min_diag_score, regions, dg.p, ctx->stream));
   std::vector<GateRes> res(nrecs);
   ST_HIP(hipMemcpyAsync(res.data(), dg.p, nrecs * sizeof(GateRes), hipMemcpyDeviceToHost, ctx->stream));

@@ -8,8 +8,13 @@
 //  * gate_recs_kernel: score_and_gate's per-target chain and gate
 //    (src/prefilter.rs:1577-1657) over already scored jobs.
 //
-// kadane_segment scans read 16 residues per load and look scores up in the
-// query's int32 profile.
+// Jobs and records come sorted by query.  A block takes a run of them; when
+// the run belongs to one query whose profile has an int16 copy of at most
+// kStageRows rows, the block first copies that profile into LDS and every
+// score lookup (kadane_score, kadane_segment) reads LDS.  Lanes of a wave
+// look up unrelated (code, row) cells, so from global memory each lookup is
+// a gather of its own cache line; from LDS it is a banked read.  Other runs
+// read the int32 profile in global memory; both hold the same values.
 #include <hip/hip_runtime.h>
 
 #include "gate_core.h"
@@ -25,10 +30,43 @@ struct RecDev {
   uint32_t tid, hits, j0, j1;
 };
 
-// kadane_score of one job against profile (prof, ld); target codes >= 21
-// were mapped to 21 at upload
-__device__ __forceinline__ int32_t ung_job(const JobDev jb, const int32_t *__restrict__ prof, uint32_t ld,
-                                           const uint8_t *__restrict__ arena, const uint64_t *__restrict__ toff) {
+// Rows of a query profile a block can stage in LDS: 22 int16 rows of up to
+// 1,112 entries (48,928 bytes), three blocks per compute unit.
+constexpr int kStageRows = 1112;
+
+struct ProfLds {
+  const int16_t *p;
+  int S;
+  __device__ int32_t operator()(uint32_t code, int q) const { return p[code * S + q]; }
+};
+
+// Copy rows [0, S) of every code of qd's int16 profile into lds (S = m
+// rounded up to 8; rows >= m are zero in the source).  Returns false, with
+// nothing copied, when the query has no int16 profile or more than cap rows.
+// Every thread of the block computes the same answer.
+__device__ __forceinline__ bool stage_prof(const QueryDev qd, int16_t *lds, int cap, int &S) {
+  if (!qd.prof16 || qd.m == 0 || (int)qd.m > cap) return false;
+  S = (int)((qd.m + 7) & ~7u);
+  const int vpr = S / 8;  // 16-byte vectors per code row
+  for (int k = threadIdx.x; k < kCodes * vpr; k += blockDim.x) {
+    const int c = k / vpr, v = k - c * vpr;
+    *(uint4 *)(lds + c * S + v * 8) = *(const uint4 *)(qd.prof16 + (size_t)c * qd.ld + v * 8);
+  }
+  return true;
+}
+
+// Whether a block of the run [base, last] stages its query's profile: the
+// run belongs to one query with an int16 profile of at most cap rows.
+__device__ __forceinline__ bool stages(const uint32_t *qidx, const QueryDev *queries, uint64_t base, uint64_t last,
+                                       int cap) {
+  const uint32_t qa = qidx ? qidx[base] : 0, qb = qidx ? qidx[last] : 0;
+  return qa == qb && queries[qa].prof16 && queries[qa].m > 0 && (int)queries[qa].m <= cap;
+}
+
+// kadane_score of one job; target codes >= 21 were mapped to 21 at upload
+template <class Acc>
+__device__ __forceinline__ int32_t ung_job(const JobDev jb, const Acc &acc, const uint8_t *__restrict__ arena,
+                                           const uint64_t *__restrict__ toff) {
   const int lo = (int)jb.lo, hi = (int)(jb.hi > jb.lo ? jb.hi : jb.lo);
   const uint8_t *t = arena + toff[jb.tid];
   int32_t best = 0, cur = 0;
@@ -45,7 +83,7 @@ __device__ __forceinline__ int32_t ung_job(const JobDev jb, const int32_t *__res
 #pragma unroll
       for (int b = 0; b < 16; b++) {
         const int qq = q + b - skip;
-        s[b] = (b >= skip && qq < hi) ? prof[(uint64_t)((w[b >> 2] >> ((b & 3) * 8)) & 0xffu) * ld + qq] : 0;
+        s[b] = (b >= skip && qq < hi) ? acc((w[b >> 2] >> ((b & 3) * 8)) & 0xffu, qq) : 0;
       }
 #pragma unroll
       for (int b = 0; b < 16; b++) {
@@ -61,32 +99,41 @@ __device__ __forceinline__ int32_t ung_job(const JobDev jb, const int32_t *__res
   return best;
 }
 
-__global__ __launch_bounds__(256) void ung_jobs_kernel(const JobDev *__restrict__ jobs, uint64_t njobs,
-                                                       const QueryDev *__restrict__ qd,
-                                                       const uint8_t *__restrict__ arena,
-                                                       const uint64_t *__restrict__ toff,
-                                                       int32_t *__restrict__ out) {
-  const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= njobs) return;
-  out[i] = ung_job(jobs[i], qd->prof32, qd->ld, arena, toff);
-}
+constexpr int kJobsPerLane = 4;
+constexpr int kRecsPerLane = 2;
 
-// the same for jobs of several queries: job i belongs to query jq[i]
-__global__ __launch_bounds__(256) void ung_jobs_multi_kernel(const JobDev *__restrict__ jobs, uint64_t njobs,
-                                                             const uint32_t *__restrict__ jq,
-                                                             const QueryDev *__restrict__ queries,
-                                                             const uint8_t *__restrict__ arena,
-                                                             const uint64_t *__restrict__ toff,
-                                                             int32_t *__restrict__ out) {
-  const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= njobs) return;
-  const QueryDev q = queries[jq[i]];
-  out[i] = ung_job(jobs[i], q.prof32, q.ld, arena, toff);
+// Jobs [256 * kJobsPerLane * blockIdx.x, ...) of the list; job i belongs to
+// query jq[i] of `queries` (jq null: every job to queries[0]).
+__global__ __launch_bounds__(256) void ung_jobs_kernel(const JobDev *__restrict__ jobs, uint64_t njobs,
+                                                       const uint32_t *__restrict__ jq,
+                                                       const QueryDev *__restrict__ queries,
+                                                       const uint8_t *__restrict__ arena,
+                                                       const uint64_t *__restrict__ toff, int32_t *__restrict__ out,
+                                                       int cap) {
+  extern __shared__ __align__(16) int16_t lds[];
+  const uint64_t base = (uint64_t)blockIdx.x * (256 * kJobsPerLane);
+  if (base >= njobs) return;
+  const uint64_t last = (base + 256 * kJobsPerLane < njobs ? base + 256 * kJobsPerLane : njobs) - 1;
+  const QueryDev qd = queries[jq ? jq[base] : 0];
+  int S = 0;
+  const bool staged = stages(jq, queries, base, last, cap) && stage_prof(qd, lds, cap, S);
+  __syncthreads();
+  for (int k = 0; k < kJobsPerLane; k++) {
+    const uint64_t i = base + (uint64_t)k * 256 + threadIdx.x;
+    if (i >= njobs) break;
+    if (staged) {
+      out[i] = ung_job(jobs[i], ProfLds{lds, S}, arena, toff);
+    } else {
+      const QueryDev q = queries[jq ? jq[i] : 0];
+      out[i] = ung_job(jobs[i], ProfGlobal{q.prof32, q.ld}, arena, toff);
+    }
+  }
 }
 
 // score_and_gate's chain and gate of one target record (up to 16 jobs)
+template <class Acc>
 __device__ __forceinline__ GateRes gate_rec(const RecDev rc, const JobDev *__restrict__ jobs,
-                                            const int32_t *__restrict__ scores, const int32_t *prof, uint32_t ld,
+                                            const int32_t *__restrict__ scores, const Acc &acc,
                                             const uint8_t *__restrict__ arena, const uint64_t *__restrict__ toff,
                                             int32_t open, int32_t ext, int32_t min_diag_score, int regions) {
   int ns = (int)(rc.j1 - rc.j0);
@@ -101,74 +148,114 @@ __device__ __forceinline__ GateRes gate_rec(const RecDev rc, const JobDev *__res
   }
   const uint8_t *t = arena + toff[rc.tid];
   if (ns == 0) return GateRes{0, 0, 0, 0, 0, 0, 0, 0, 0};
-  return gate_eval(ProfGlobal{prof, ld}, t, sc, dg, lo, hi, ns, open, ext, min_diag_score, regions != 0);
+  return gate_eval(acc, t, sc, dg, lo, hi, ns, open, ext, min_diag_score, regions != 0);
 }
 
-__global__ __launch_bounds__(128) void gate_recs_kernel(const RecDev *__restrict__ recs, uint64_t nrecs,
+// Gap costs and gate: per query (arrays indexed by the record's query) or,
+// with null arrays, one setting for every record.
+struct GateParams {
+  const int32_t *open, *ext, *gate;
+  int32_t open1, ext1, gate1;
+};
+
+// Records [256 * kRecsPerLane * blockIdx.x, ...); record i belongs to query
+// rq[i] (rq null: every record to queries[0]).  The chain rule's code is
+// large, so each lookup path is its own kernel: kLds = true gates the blocks
+// that stage their profile, kLds = false the others; a block that is not
+// its kernel's returns at once.
+template <bool kLds>
+__global__ __launch_bounds__(256) void gate_recs_kernel(const RecDev *__restrict__ recs, uint64_t nrecs,
                                                         const JobDev *__restrict__ jobs,
                                                         const int32_t *__restrict__ scores,
-                                                        const QueryDev *__restrict__ qd,
+                                                        const uint32_t *__restrict__ rq,
+                                                        const QueryDev *__restrict__ queries,
                                                         const uint8_t *__restrict__ arena,
-                                                        const uint64_t *__restrict__ toff,
-                                                        int32_t open, int32_t ext, int32_t min_diag_score,
-                                                        int regions, GateRes *__restrict__ out) {
-  const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= nrecs) return;
-  out[i] = gate_rec(recs[i], jobs, scores, qd->prof32, qd->ld, arena, toff, open, ext, min_diag_score, regions);
+                                                        const uint64_t *__restrict__ toff, GateParams gp,
+                                                        int regions, GateRes *__restrict__ out, int cap) {
+  extern __shared__ __align__(16) int16_t lds[];
+  const uint64_t base = (uint64_t)blockIdx.x * (256 * kRecsPerLane);
+  if (base >= nrecs) return;
+  const uint64_t last = (base + 256 * kRecsPerLane < nrecs ? base + 256 * kRecsPerLane : nrecs) - 1;
+  if (stages(rq, queries, base, last, cap) != kLds) return;
+  int S = 0;
+  if (kLds) {
+    stage_prof(queries[rq ? rq[base] : 0], lds, cap, S);
+    __syncthreads();
+  }
+  for (int k = 0; k < kRecsPerLane; k++) {
+    const uint64_t i = base + (uint64_t)k * 256 + threadIdx.x;
+    if (i >= nrecs) break;
+    const uint32_t q = rq ? rq[i] : 0;
+    const int32_t open = gp.open ? gp.open[q] : gp.open1, ext = gp.ext ? gp.ext[q] : gp.ext1,
+                  gate = gp.gate ? gp.gate[q] : gp.gate1;
+    if (kLds) {
+      out[i] = gate_rec(recs[i], jobs, scores, ProfLds{lds, S}, arena, toff, open, ext, gate, regions);
+    } else {
+      const QueryDev qq = queries[q];
+      out[i] = gate_rec(recs[i], jobs, scores, ProfGlobal{qq.prof32, qq.ld}, arena, toff, open, ext, gate, regions);
+    }
+  }
 }
 
-// the same for records of several queries: record i belongs to query rq[i],
-// with that query's gap costs and gate
-__global__ __launch_bounds__(128) void gate_recs_multi_kernel(
-    const RecDev *__restrict__ recs, uint64_t nrecs, const JobDev *__restrict__ jobs,
-    const int32_t *__restrict__ scores, const uint32_t *__restrict__ rq, const QueryDev *__restrict__ queries,
-    const uint8_t *__restrict__ arena, const uint64_t *__restrict__ toff, const int32_t *__restrict__ open,
-    const int32_t *__restrict__ ext, const int32_t *__restrict__ gate, int regions, GateRes *__restrict__ out) {
-  const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= nrecs) return;
-  const uint32_t q = rq[i];
-  const QueryDev qd = queries[q];
-  out[i] = gate_rec(recs[i], jobs, scores, qd.prof32, qd.ld, arena, toff, open[q], ext[q], gate[q], regions);
+// Staged rows for a launch whose longest query with an int16 profile has
+// stage_m rows (0: none), and the dynamic LDS that takes.
+static int stage_cap(uint32_t stage_m) {
+  const uint32_t r = (stage_m + 7) & ~7u;
+  return (int)(r < (uint32_t)kStageRows ? r : (uint32_t)kStageRows);
 }
 
-hipError_t launch_ung_jobs(const JobDev *jobs, uint64_t njobs, const QueryDev *qd, const uint8_t *arena,
-                           const uint64_t *toff, int32_t *out, hipStream_t s) {
+hipError_t launch_ung_jobs_multi(const JobDev *jobs, uint64_t njobs, const uint32_t *jq, const QueryDev *queries,
+                                 uint32_t stage_m, const uint8_t *arena, const uint64_t *toff, int32_t *out,
+                                 hipStream_t s) {
   if (njobs == 0) return hipSuccess;
-  const uint64_t blocks = (njobs + 255) / 256;
-  hipLaunchKernelGGL(ung_jobs_kernel, dim3((uint32_t)blocks), dim3(256), 0, s, jobs, njobs, qd, arena, toff, out);
+  const int cap = stage_cap(stage_m);
+  const uint64_t blocks = (njobs + 256 * kJobsPerLane - 1) / (256 * kJobsPerLane);
+  hipLaunchKernelGGL(ung_jobs_kernel, dim3((uint32_t)blocks), dim3(256), (size_t)kCodes * cap * 2, s, jobs, njobs,
+                     jq, queries, arena, toff, out, cap);
   return hipGetLastError();
+}
+
+hipError_t launch_ung_jobs(const JobDev *jobs, uint64_t njobs, const QueryDev *qd, uint32_t stage_m,
+                           const uint8_t *arena, const uint64_t *toff, int32_t *out, hipStream_t s) {
+  return launch_ung_jobs_multi(jobs, njobs, nullptr, qd, stage_m, arena, toff, out, s);
+}
+
+// kLds and global gate launches over the same blocks; `lds_only` when every
+// block stages (one query whose profile a block holds).
+static hipError_t gate_launches(const RecDev *recs, uint64_t nrecs, const JobDev *jobs, const int32_t *scores,
+                                const uint32_t *rq, const QueryDev *queries, uint32_t stage_m, bool lds_only,
+                                const uint8_t *arena, const uint64_t *toff, GateParams gp, int regions,
+                                GateRes *out, hipStream_t s) {
+  if (nrecs == 0) return hipSuccess;
+  const int cap = stage_cap(stage_m);
+  const dim3 grid((uint32_t)((nrecs + 256 * kRecsPerLane - 1) / (256 * kRecsPerLane)));
+  if (cap > 0) {
+    hipLaunchKernelGGL(gate_recs_kernel<true>, grid, dim3(256), (size_t)kCodes * cap * 2, s, recs, nrecs, jobs,
+                       scores, rq, queries, arena, toff, gp, regions, out, cap);
+    if (hipError_t e = hipGetLastError(); e != hipSuccess) return e;
+  }
+  if (!lds_only || cap == 0) {
+    hipLaunchKernelGGL(gate_recs_kernel<false>, grid, dim3(256), 0, s, recs, nrecs, jobs, scores, rq, queries, arena,
+                       toff, gp, regions, out, cap);
+    return hipGetLastError();
+  }
+  return hipSuccess;
 }
 
 hipError_t launch_gate_recs(const RecDev *recs, uint64_t nrecs, const JobDev *jobs, const int32_t *scores,
-                            const QueryDev *qd, const uint8_t *arena, const uint64_t *toff, int32_t open,
-                            int32_t ext, int32_t gate, int regions, GateRes *out, hipStream_t s) {
-  if (nrecs == 0) return hipSuccess;
-  const uint64_t blocks = (nrecs + 127) / 128;
-  hipLaunchKernelGGL(gate_recs_kernel, dim3((uint32_t)blocks), dim3(128), 0, s, recs, nrecs, jobs, scores, qd,
-                     arena, toff, open, ext, gate, regions, out);
-  return hipGetLastError();
-}
-
-}  // namespace st
-
-namespace st {
-
-hipError_t launch_ung_jobs_multi(const JobDev *jobs, uint64_t njobs, const uint32_t *jq, const QueryDev *queries,
-                                 const uint8_t *arena, const uint64_t *toff, int32_t *out, hipStream_t s) {
-  if (njobs == 0) return hipSuccess;
-  hipLaunchKernelGGL(ung_jobs_multi_kernel, dim3((uint32_t)((njobs + 255) / 256)), dim3(256), 0, s, jobs, njobs, jq,
-                     queries, arena, toff, out);
-  return hipGetLastError();
+                            const QueryDev *qd, uint32_t stage_m, const uint8_t *arena, const uint64_t *toff,
+                            int32_t open, int32_t ext, int32_t gate, int regions, GateRes *out, hipStream_t s) {
+  return gate_launches(recs, nrecs, jobs, scores, nullptr, qd, stage_m, stage_m > 0 && stage_m <= (uint32_t)kStageRows,
+                       arena, toff, GateParams{nullptr, nullptr, nullptr, open, ext, gate}, regions, out, s);
 }
 
 hipError_t launch_gate_recs_multi(const RecDev *recs, uint64_t nrecs, const JobDev *jobs, const int32_t *scores,
-                                  const uint32_t *rq, const QueryDev *queries, const uint8_t *arena,
-                                  const uint64_t *toff, const int32_t *open, const int32_t *ext,
-                                  const int32_t *gate, int regions, GateRes *out, hipStream_t s) {
-  if (nrecs == 0) return hipSuccess;
-  hipLaunchKernelGGL(gate_recs_multi_kernel, dim3((uint32_t)((nrecs + 127) / 128)), dim3(128), 0, s, recs, nrecs,
-                     jobs, scores, rq, queries, arena, toff, open, ext, gate, regions, out);
-  return hipGetLastError();
+                                  const uint32_t *rq, const QueryDev *queries, uint32_t stage_m,
+                                  const uint8_t *arena, const uint64_t *toff, const int32_t *open,
+                                  const int32_t *ext, const int32_t *gate, int regions, GateRes *out,
+                                  hipStream_t s) {
+  return gate_launches(recs, nrecs, jobs, scores, rq, queries, stage_m, false, arena, toff,
+                       GateParams{open, ext, gate, 0, 0, 0}, regions, out, s);
 }
 
 }  // namespace st

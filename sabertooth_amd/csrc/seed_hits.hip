@@ -482,10 +482,12 @@ int st_seed_hits_to_candidates_batch(st_ctx *ctx, const st_db *db, const st_post
   if (cand_off) std::fill(cand_off, cand_off + nq + 1, 0);
   std::vector<uint64_t> m(nq);
   std::vector<QueryDev> qv(nq);
+  uint32_t stage_m = 0;  // the longest query with an int16 profile
   for (uint64_t q = 0; q < nq; q++) {
     if (!queries[q] || queries[q]->ctx != ctx) return fail(ST_ERR_INVALID, f + ": bad query handle");
     m[q] = queries[q]->m;
     qv[q] = queries[q]->view;
+    if (qv[q].prof16) stage_m = std::max(stage_m, qv[q].m);
   }
   if (int rc = seed_stage_check(f, ctx, db, post, nq, m.data(), seed_off, seeds, hp)) return rc;
   std::lock_guard<std::mutex> lk(ctx->mu);
@@ -502,20 +504,21 @@ int st_seed_hits_to_candidates_batch(st_ctx *ctx, const st_db *db, const st_post
   st_candidate *d_c = grow<st_candidate>(sc[27], st.nrecs);
   uint32_t *d_cq = grow<uint32_t>(sc[28], st.nrecs);
   QueryDev *d_q = grow<QueryDev>(sc[29], nq);
-  DevBuf<int32_t> prm;
-  if (!jq || !scores || !gres || !pass || !pidx || !chain || !cidx || !d_c || !d_cq || !d_q ||
-      prm.alloc(3 * nq) != hipSuccess)
+  int32_t *prm = grow<int32_t>(sc[30], 3 * nq);  // open, extend, gate per query
+  if (!jq || !scores || !gres || !pass || !pidx || !chain || !cidx || !d_c || !d_cq || !d_q || !prm)
     return fail(ST_ERR_NOMEM, f + ": device allocation failed");
+  std::vector<int32_t> hprm(3 * nq);
+  std::copy(gap_open, gap_open + nq, hprm.begin());
+  std::copy(gap_extend, gap_extend + nq, hprm.begin() + nq);
+  std::copy(min_diag_score, min_diag_score + nq, hprm.begin() + 2 * nq);
   ST_HIP(hipMemcpyAsync(d_q, qv.data(), nq * sizeof(QueryDev), hipMemcpyHostToDevice, s));
-  ST_HIP(hipMemcpyAsync(prm.p, gap_open, nq * 4, hipMemcpyHostToDevice, s));
-  ST_HIP(hipMemcpyAsync(prm.p + nq, gap_extend, nq * 4, hipMemcpyHostToDevice, s));
-  ST_HIP(hipMemcpyAsync(prm.p + 2 * nq, min_diag_score, nq * 4, hipMemcpyHostToDevice, s));
+  ST_HIP(hipMemcpyAsync(prm, hprm.data(), 3 * nq * 4, hipMemcpyHostToDevice, s));
   // score every job against its query, then gate every record
   hipLaunchKernelGGL(job_query_kernel, dim3(grid_of(st.nrecs)), dim3(256), 0, s, st.recs, st.rq, st.nrecs, jq);
   ST_HIP(hipGetLastError());
-  ST_HIP(launch_ung_jobs_multi((const JobDev *)st.jobs, st.njobs, jq, d_q, db->arena.p, db->off.p, scores, s));
-  ST_HIP(launch_gate_recs_multi((const RecDev *)st.recs, st.nrecs, (const JobDev *)st.jobs, scores, st.rq, d_q,
-                                db->arena.p, db->off.p, prm.p, prm.p + nq, prm.p + 2 * nq, regions, gres, s));
+  ST_HIP(launch_ung_jobs_multi((const JobDev *)st.jobs, st.njobs, jq, d_q, stage_m, db->arena.p, db->off.p, scores, s));
+  ST_HIP(launch_gate_recs_multi((const RecDev *)st.recs, st.nrecs, (const JobDev *)st.jobs, scores, st.rq, d_q, stage_m,
+                                db->arena.p, db->off.p, prm, prm + nq, prm + 2 * nq, regions, gres, s));
   // the passing records, compacted in (query, target) order
   hipLaunchKernelGGL(cand_flags_kernel, dim3(grid_of(st.nrecs)), dim3(256), 0, s, gres, st.recs, st.nrecs, pass,
                      chain);

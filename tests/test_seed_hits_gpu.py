@@ -192,3 +192,47 @@ def test_seeds_to_candidates_on_the_device(ctx):
     for q in qps:
         q.close()
     dp.close()
+
+
+def test_candidates_profile_in_lds_and_in_global_memory(ctx):
+    # dense 4-letter hits give thousands of jobs per query, so blocks hold
+    # one query's run: queries of 300 and 200 rows read their profile from
+    # LDS; one of 1,300 rows (longer than a block stages) and one whose rows
+    # exceed int16 read the int32 profile in global memory
+    from sabertooth_amd.core import seed_hits_to_candidates_batch
+    rng = np.random.default_rng(5)
+    targets = [rng.integers(0, 4, size=int(rng.integers(50, 400)), dtype=np.uint8) for _ in range(1500)]
+    post, of = kmer_index(targets, 4)
+    tlen = np.array([len(t) for t in targets])
+    arena = np.concatenate(targets)
+    offs = np.zeros(len(targets) + 1, np.uint64)
+    offs[1:] = np.cumsum(tlen)
+    db = _db(ctx, targets)
+    dp = Postings(ctx, post)
+    qs = [rng.integers(0, 4, size=L, dtype=np.uint8) for L in (300, 1300, 250, 200)]
+    rows = [O.seq_rows(q) for q in qs]
+    rows[2] = np.array(rows[2], dtype=np.int32)
+    rows[2][::7, 1] = 40000  # column R: beyond int16
+    rows[2][3::11, 2] = -40000
+    qps = [st.QueryProfile.from_rows(ctx, r) for r in rows]
+    seeds = [query_seeds(q, 4, of) for q in qs]
+    gaps = [GapCosts(11, 1), GapCosts(11, 1), GapCosts(9, 2), GapCosts(11, 1)]
+    gates = [110, 140, 400000, 100]
+    for regions in (False, True):
+        cands, chains = seed_hits_to_candidates_batch(db, dp, qps, seeds, gaps, gates, regions, span=4)
+        total = 0
+        for qi in range(len(qs)):
+            gj, gr = seed_hits_to_jobs(ctx, db, dp, len(qs[qi]), seeds[qi], 4, 1, 1, 48, 4)
+            assert len(gj) > 2048
+            c1, ch1, js = st.score_and_gate(qps[qi], db, gj, gr, gaps[qi], gates[qi], regions)
+            oc, och, ojs = O.score_and_gate(rows[qi], arena, offs, gj, gr, gaps[qi].open, gaps[qi].extend,
+                                            gates[qi], regions)
+            assert (js == ojs).all(), qi
+            assert len(cands[qi]) == len(c1) == len(oc) and 0 < len(oc) < len(gr), qi
+            assert (cands[qi] == c1).all() and (c1 == oc).all(), qi
+            assert ch1 == och
+            total += ch1
+        assert chains == total
+    for q in qps:
+        q.close()
+    dp.close()
