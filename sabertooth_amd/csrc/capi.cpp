@@ -1191,7 +1191,8 @@ int st_ungapped_score_jobs(st_ctx *ctx, const st_qprof *qp, const st_db *db, con
   DevBuf<int32_t> ds;
   if (dj.upload((const JobDev *)jobs, njobs, ctx->stream) != hipSuccess || ds.alloc(njobs) != hipSuccess)
     return fail(ST_ERR_NOMEM, "st_ungapped_score_jobs: device allocation failed");
-  ST_HIP(launch_ung_jobs(dj.p, njobs, qp->qd.p, qp->view.prof16 ? qp->m : 0, db->arena.p, db->off.p, ds.p, ctx->stream));
+  ST_HIP(launch_ung_jobs(dj.p, njobs, qp->qd.p, qp->view.prof16 ? qp->m : 0, db->arena.p, db->off.p, ds.p, nullptr,
+                         ctx->stream));
   ST_HIP(hipMemcpyAsync(scores, ds.p, njobs * 4, hipMemcpyDeviceToHost, ctx->stream));
   ST_HIP(hipStreamSynchronize(ctx->stream));
   return ST_OK;
@@ -1207,25 +1208,40 @@ int st_score_and_gate(st_ctx *ctx, const st_qprof *qp, const st_db *db, const st
   for (uint64_t r = 0; r < nrecs; r++)
     if (recs[r].j1 < recs[r].j0 || recs[r].j1 > njobs || recs[r].tid >= db->n_targets)
       return fail(ST_ERR_INVALID, "st_score_and_gate: record out of range");
-  std::vector<int32_t> sc(njobs);
-  int rc = st_ungapped_score_jobs(ctx, qp, db, jobs, njobs, sc.data());
-  if (rc) return rc;
-  if (job_scores && njobs) std::memcpy(job_scores, sc.data(), njobs * 4);
-  if (nrecs == 0) return ST_OK;
+  // jobs must lie inside their diagonals (st_ungapped_score_jobs's checks)
+  for (uint64_t k = 0; k < njobs; k++) {
+    const st_job &j = jobs[k];
+    if (j.tid >= db->n_targets) return fail(ST_ERR_INVALID, "st_score_and_gate: target id out of range");
+    if (j.hi > j.lo) {
+      const int64_t n = db->hlen[j.tid];
+      const int64_t a = std::max<int64_t>(-(int64_t)j.diag, 0);
+      const int64_t e = std::min<int64_t>((int64_t)qp->m, n - j.diag);
+      if ((int64_t)j.lo < a || (int64_t)j.hi > e)
+        return fail(ST_ERR_INVALID, "st_score_and_gate: job window outside its diagonal");
+    }
+  }
+  if (njobs == 0 && nrecs == 0) return ST_OK;
   std::lock_guard<std::mutex> lk(ctx->mu);
   ST_HIP(hipSetDevice(ctx->device));
   DevBuf<JobDev> dj;
   DevBuf<int32_t> ds;
+  DevBuf<int2> dseg;
   DevBuf<RecDev> dr;
   DevBuf<GateRes> dg;
-  if (dj.upload((const JobDev *)jobs, njobs, ctx->stream) != hipSuccess ||
-      ds.upload(sc.data(), njobs, ctx->stream) != hipSuccess ||
-      dr.upload((const RecDev *)recs, nrecs, ctx->stream) != hipSuccess || dg.alloc(nrecs) != hipSuccess)
+  if (dj.upload((const JobDev *)jobs, njobs, ctx->stream) != hipSuccess || ds.alloc(njobs) != hipSuccess ||
+      dseg.alloc(njobs) != hipSuccess || dr.upload((const RecDev *)recs, nrecs, ctx->stream) != hipSuccess ||
+      dg.alloc(nrecs) != hipSuccess)
     return fail(ST_ERR_NOMEM, "st_score_and_gate: device allocation failed");
-  ST_HIP(launch_gate_recs(dr.p, nrecs, dj.p, ds.p, qp->qd.p, qp->view.prof16 ? qp->m : 0, db->arena.p, db->off.p, gap_open, gap_extend,
-                          min_diag_score, regions, dg.p, ctx->stream));
+  // scores and segment bounds in one scan, then the gate on them
+  ST_HIP(launch_ung_jobs(dj.p, njobs, qp->qd.p, qp->view.prof16 ? qp->m : 0, db->arena.p, db->off.p, ds.p, dseg.p,
+                         ctx->stream));
+  ST_HIP(launch_gate_recs(dr.p, nrecs, dj.p, ds.p, dseg.p, gap_open, gap_extend, min_diag_score, regions, dg.p,
+                          ctx->stream));
   std::vector<GateRes> res(nrecs);
-  ST_HIP(hipMemcpyAsync(res.data(), dg.p, nrecs * sizeof(GateRes), hipMemcpyDeviceToHost, ctx->stream));
+  if (job_scores && njobs)
+    ST_HIP(hipMemcpyAsync(job_scores, ds.p, njobs * 4, hipMemcpyDeviceToHost, ctx->stream));
+  if (nrecs)
+    ST_HIP(hipMemcpyAsync(res.data(), dg.p, nrecs * sizeof(GateRes), hipMemcpyDeviceToHost, ctx->stream));
   ST_HIP(hipStreamSynchronize(ctx->stream));
   uint64_t nc = 0, chains = 0;
   for (uint64_t r = 0; r < nrecs; r++) {

@@ -6,15 +6,16 @@
 //    src/ungapped.rs:53-62).  One lane per job; a lane streams its diagonal's
 //    residues with 16-byte loads (the arena is 16-byte aligned per target).
 //  * gate_recs_kernel: score_and_gate's per-target chain and gate
-//    (src/prefilter.rs:1577-1657) over already scored jobs.
+//    (src/prefilter.rs:1577-1657) over already scored jobs, on the
+//    kadane_segment bounds the job scan wrote (it reads no residues).
 //
-// Jobs and records come sorted by query.  A block takes a run of them; when
-// the run belongs to one query whose profile has an int16 copy of at most
-// kStageRows rows, the block first copies that profile into LDS and every
-// score lookup (kadane_score, kadane_segment) reads LDS.  Lanes of a wave
-// look up unrelated (code, row) cells, so from global memory each lookup is
-// a gather of its own cache line; from LDS it is a banked read.  Other runs
-// read the int32 profile in global memory; both hold the same values.
+// Jobs come sorted by query.  A block takes a run of them; when the run
+// belongs to one query whose profile has an int16 copy of at most kStageRows
+// rows, the block first copies that profile into LDS and every score lookup
+// reads LDS.  Lanes of a wave look up unrelated (code, row) cells, so from
+// global memory each lookup is a gather of its own cache line; from LDS it is
+// a banked read.  Other runs read the int32 profile in global memory; both
+// hold the same values.
 #include <hip/hip_runtime.h>
 
 #include "gate_core.h"
@@ -63,13 +64,17 @@ __device__ __forceinline__ bool stages(const uint32_t *qidx, const QueryDev *que
   return qa == qb && queries[qa].prof16 && queries[qa].m > 0 && (int)queries[qa].m <= cap;
 }
 
-// kadane_score of one job; target codes >= 21 were mapped to 21 at upload
-template <class Acc>
+// kadane_score of one job; target codes >= 21 were mapped to 21 at upload.
+// kSeg: also kadane_segment's bounds (src/ungapped.rs:69-88) in *seg.  The
+// two scans restart alike (kadane_score's max(cur, 0) + s is s when
+// cur <= 0), so one pass gives both and the same best score.
+template <bool kSeg, class Acc>
 __device__ __forceinline__ int32_t ung_job(const JobDev jb, const Acc &acc, const uint8_t *__restrict__ arena,
-                                           const uint64_t *__restrict__ toff) {
+                                           const uint64_t *__restrict__ toff, int2 *seg) {
   const int lo = (int)jb.lo, hi = (int)(jb.hi > jb.lo ? jb.hi : jb.lo);
   const uint8_t *t = arena + toff[jb.tid];
   int32_t best = 0, cur = 0;
+  int bs = 0, be = 0, cs = lo;
   int q = lo;
   if (q < hi) {
     // align the target pointer to 16 bytes and stream whole vectors
@@ -88,28 +93,44 @@ __device__ __forceinline__ int32_t ung_job(const JobDev jb, const Acc &acc, cons
 #pragma unroll
       for (int b = 0; b < 16; b++) {
         if (b >= skip && q < hi) {
-          cur = (cur > 0 ? cur : 0) + s[b];
-          best = best > cur ? best : cur;
+          if (kSeg) {
+            if (cur <= 0) {
+              cur = s[b];
+              cs = q;
+            } else {
+              cur += s[b];
+            }
+            if (cur > best) {
+              best = cur;
+              bs = cs;
+              be = q;
+            }
+          } else {
+            cur = (cur > 0 ? cur : 0) + s[b];
+            best = best > cur ? best : cur;
+          }
           q++;
         }
       }
       skip = 0;
     }
   }
+  if (kSeg) *seg = make_int2(bs, be);
   return best;
 }
 
 constexpr int kJobsPerLane = 4;
-constexpr int kRecsPerLane = 2;
 
 // Jobs [256 * kJobsPerLane * blockIdx.x, ...) of the list; job i belongs to
-// query jq[i] of `queries` (jq null: every job to queries[0]).
+// query jq[i] of `queries` (jq null: every job to queries[0]).  kSeg: each
+// job's kadane_segment bounds go to segs[i].
+template <bool kSeg>
 __global__ __launch_bounds__(256) void ung_jobs_kernel(const JobDev *__restrict__ jobs, uint64_t njobs,
                                                        const uint32_t *__restrict__ jq,
                                                        const QueryDev *__restrict__ queries,
                                                        const uint8_t *__restrict__ arena,
                                                        const uint64_t *__restrict__ toff, int32_t *__restrict__ out,
-                                                       int cap) {
+                                                       int2 *__restrict__ segs, int cap) {
   extern __shared__ __align__(16) int16_t lds[];
   const uint64_t base = (uint64_t)blockIdx.x * (256 * kJobsPerLane);
   if (base >= njobs) return;
@@ -122,22 +143,22 @@ __global__ __launch_bounds__(256) void ung_jobs_kernel(const JobDev *__restrict_
     const uint64_t i = base + (uint64_t)k * 256 + threadIdx.x;
     if (i >= njobs) break;
     if (staged) {
-      out[i] = ung_job(jobs[i], ProfLds{lds, S}, arena, toff);
+      out[i] = ung_job<kSeg>(jobs[i], ProfLds{lds, S}, arena, toff, segs + i);
     } else {
       const QueryDev q = queries[jq ? jq[i] : 0];
-      out[i] = ung_job(jobs[i], ProfGlobal{q.prof32, q.ld}, arena, toff);
+      out[i] = ung_job<kSeg>(jobs[i], ProfGlobal{q.prof32, q.ld}, arena, toff, segs + i);
     }
   }
 }
 
-// score_and_gate's chain and gate of one target record (up to 16 jobs)
-template <class Acc>
+// score_and_gate's chain and gate of one target record (up to 16 jobs) on
+// its jobs' scores and kadane_segment bounds
 __device__ __forceinline__ GateRes gate_rec(const RecDev rc, const JobDev *__restrict__ jobs,
-                                            const int32_t *__restrict__ scores, const Acc &acc,
-                                            const uint8_t *__restrict__ arena, const uint64_t *__restrict__ toff,
+                                            const int32_t *__restrict__ scores, const int2 *__restrict__ segs,
                                             int32_t open, int32_t ext, int32_t min_diag_score, int regions) {
   int ns = (int)(rc.j1 - rc.j0);
   if (ns > 16) ns = 16;
+  if (ns == 0) return GateRes{0, 0, 0, 0, 0, 0, 0, 0, 0};
   int32_t sc[16], dg[16], lo[16], hi[16];
   for (int k = 0; k < ns; k++) {
     const JobDev jb = jobs[rc.j0 + k];
@@ -146,9 +167,9 @@ __device__ __forceinline__ GateRes gate_rec(const RecDev rc, const JobDev *__res
     lo[k] = (int32_t)jb.lo;
     hi[k] = (int32_t)(jb.hi > jb.lo ? jb.hi : jb.lo);
   }
-  const uint8_t *t = arena + toff[rc.tid];
-  if (ns == 0) return GateRes{0, 0, 0, 0, 0, 0, 0, 0, 0};
-  return gate_eval(acc, t, sc, dg, lo, hi, ns, open, ext, min_diag_score, regions != 0);
+  // the lookups are never made: every segment comes from segs
+  return gate_eval(ProfGlobal{nullptr, 0}, nullptr, sc, dg, lo, hi, ns, open, ext, min_diag_score, regions != 0,
+                   segs + rc.j0);
 }
 
 // Gap costs and gate: per query (arrays indexed by the record's query) or,
@@ -158,43 +179,19 @@ struct GateParams {
   int32_t open1, ext1, gate1;
 };
 
-// Records [256 * kRecsPerLane * blockIdx.x, ...); record i belongs to query
-// rq[i] (rq null: every record to queries[0]).  The chain rule's code is
-// large, so each lookup path is its own kernel: kLds = true gates the blocks
-// that stage their profile, kLds = false the others; a block that is not
-// its kernel's returns at once.
-template <bool kLds>
+// One lane per record; record i belongs to query rq[i] (rq null: query 0).
 __global__ __launch_bounds__(256) void gate_recs_kernel(const RecDev *__restrict__ recs, uint64_t nrecs,
                                                         const JobDev *__restrict__ jobs,
                                                         const int32_t *__restrict__ scores,
-                                                        const uint32_t *__restrict__ rq,
-                                                        const QueryDev *__restrict__ queries,
-                                                        const uint8_t *__restrict__ arena,
-                                                        const uint64_t *__restrict__ toff, GateParams gp,
-                                                        int regions, GateRes *__restrict__ out, int cap) {
-  extern __shared__ __align__(16) int16_t lds[];
-  const uint64_t base = (uint64_t)blockIdx.x * (256 * kRecsPerLane);
-  if (base >= nrecs) return;
-  const uint64_t last = (base + 256 * kRecsPerLane < nrecs ? base + 256 * kRecsPerLane : nrecs) - 1;
-  if (stages(rq, queries, base, last, cap) != kLds) return;
-  int S = 0;
-  if (kLds) {
-    stage_prof(queries[rq ? rq[base] : 0], lds, cap, S);
-    __syncthreads();
-  }
-  for (int k = 0; k < kRecsPerLane; k++) {
-    const uint64_t i = base + (uint64_t)k * 256 + threadIdx.x;
-    if (i >= nrecs) break;
-    const uint32_t q = rq ? rq[i] : 0;
-    const int32_t open = gp.open ? gp.open[q] : gp.open1, ext = gp.ext ? gp.ext[q] : gp.ext1,
-                  gate = gp.gate ? gp.gate[q] : gp.gate1;
-    if (kLds) {
-      out[i] = gate_rec(recs[i], jobs, scores, ProfLds{lds, S}, arena, toff, open, ext, gate, regions);
-    } else {
-      const QueryDev qq = queries[q];
-      out[i] = gate_rec(recs[i], jobs, scores, ProfGlobal{qq.prof32, qq.ld}, arena, toff, open, ext, gate, regions);
-    }
-  }
+                                                        const int2 *__restrict__ segs,
+                                                        const uint32_t *__restrict__ rq, GateParams gp,
+                                                        int regions, GateRes *__restrict__ out) {
+  const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nrecs) return;
+  const uint32_t q = rq ? rq[i] : 0;
+  const int32_t open = gp.open ? gp.open[q] : gp.open1, ext = gp.ext ? gp.ext[q] : gp.ext1,
+                gate = gp.gate ? gp.gate[q] : gp.gate1;
+  out[i] = gate_rec(recs[i], jobs, scores, segs, open, ext, gate, regions);
 }
 
 // Staged rows for a launch whose longest query with an int16 profile has
@@ -206,56 +203,39 @@ static int stage_cap(uint32_t stage_m) {
 
 hipError_t launch_ung_jobs_multi(const JobDev *jobs, uint64_t njobs, const uint32_t *jq, const QueryDev *queries,
                                  uint32_t stage_m, const uint8_t *arena, const uint64_t *toff, int32_t *out,
-                                 hipStream_t s) {
+                                 int2 *segs, hipStream_t s) {
   if (njobs == 0) return hipSuccess;
   const int cap = stage_cap(stage_m);
   const uint64_t blocks = (njobs + 256 * kJobsPerLane - 1) / (256 * kJobsPerLane);
-  hipLaunchKernelGGL(ung_jobs_kernel, dim3((uint32_t)blocks), dim3(256), (size_t)kCodes * cap * 2, s, jobs, njobs,
-                     jq, queries, arena, toff, out, cap);
+  if (segs)
+    hipLaunchKernelGGL(ung_jobs_kernel<true>, dim3((uint32_t)blocks), dim3(256), (size_t)kCodes * cap * 2, s, jobs,
+                       njobs, jq, queries, arena, toff, out, segs, cap);
+  else
+    hipLaunchKernelGGL(ung_jobs_kernel<false>, dim3((uint32_t)blocks), dim3(256), (size_t)kCodes * cap * 2, s, jobs,
+                       njobs, jq, queries, arena, toff, out, segs, cap);
   return hipGetLastError();
 }
 
 hipError_t launch_ung_jobs(const JobDev *jobs, uint64_t njobs, const QueryDev *qd, uint32_t stage_m,
-                           const uint8_t *arena, const uint64_t *toff, int32_t *out, hipStream_t s) {
-  return launch_ung_jobs_multi(jobs, njobs, nullptr, qd, stage_m, arena, toff, out, s);
-}
-
-// kLds and global gate launches over the same blocks; `lds_only` when every
-// block stages (one query whose profile a block holds).
-static hipError_t gate_launches(const RecDev *recs, uint64_t nrecs, const JobDev *jobs, const int32_t *scores,
-                                const uint32_t *rq, const QueryDev *queries, uint32_t stage_m, bool lds_only,
-                                const uint8_t *arena, const uint64_t *toff, GateParams gp, int regions,
-                                GateRes *out, hipStream_t s) {
-  if (nrecs == 0) return hipSuccess;
-  const int cap = stage_cap(stage_m);
-  const dim3 grid((uint32_t)((nrecs + 256 * kRecsPerLane - 1) / (256 * kRecsPerLane)));
-  if (cap > 0) {
-    hipLaunchKernelGGL(gate_recs_kernel<true>, grid, dim3(256), (size_t)kCodes * cap * 2, s, recs, nrecs, jobs,
-                       scores, rq, queries, arena, toff, gp, regions, out, cap);
-    if (hipError_t e = hipGetLastError(); e != hipSuccess) return e;
-  }
-  if (!lds_only || cap == 0) {
-    hipLaunchKernelGGL(gate_recs_kernel<false>, grid, dim3(256), 0, s, recs, nrecs, jobs, scores, rq, queries, arena,
-                       toff, gp, regions, out, cap);
-    return hipGetLastError();
-  }
-  return hipSuccess;
-}
-
-hipError_t launch_gate_recs(const RecDev *recs, uint64_t nrecs, const JobDev *jobs, const int32_t *scores,
-                            const QueryDev *qd, uint32_t stage_m, const uint8_t *arena, const uint64_t *toff,
-                            int32_t open, int32_t ext, int32_t gate, int regions, GateRes *out, hipStream_t s) {
-  return gate_launches(recs, nrecs, jobs, scores, nullptr, qd, stage_m, stage_m > 0 && stage_m <= (uint32_t)kStageRows,
-                       arena, toff, GateParams{nullptr, nullptr, nullptr, open, ext, gate}, regions, out, s);
+                           const uint8_t *arena, const uint64_t *toff, int32_t *out, int2 *segs, hipStream_t s) {
+  return launch_ung_jobs_multi(jobs, njobs, nullptr, qd, stage_m, arena, toff, out, segs, s);
 }
 
 hipError_t launch_gate_recs_multi(const RecDev *recs, uint64_t nrecs, const JobDev *jobs, const int32_t *scores,
-                                  const uint32_t *rq, const QueryDev *queries, uint32_t stage_m,
-                                  const uint8_t *arena, const uint64_t *toff, const int32_t *open,
-                                  const int32_t *ext, const int32_t *gate, int regions, GateRes *out,
-                                  hipStream_t s) {
-  return gate_launches(recs, nrecs, jobs, scores, rq, queries, stage_m, false, arena, toff,
-                       GateParams{open, ext, gate, 0, 0, 0}, regions, out, s);
+                                  const int2 *segs, const uint32_t *rq, const int32_t *open, const int32_t *ext,
+                                  const int32_t *gate, int regions, GateRes *out, hipStream_t s, int32_t open1,
+                                  int32_t ext1, int32_t gate1) {
+  if (nrecs == 0) return hipSuccess;
+  hipLaunchKernelGGL(gate_recs_kernel, dim3((uint32_t)((nrecs + 255) / 256)), dim3(256), 0, s, recs, nrecs, jobs,
+                     scores, segs, rq, GateParams{open, ext, gate, open1, ext1, gate1}, regions, out);
+  return hipGetLastError();
+}
+
+hipError_t launch_gate_recs(const RecDev *recs, uint64_t nrecs, const JobDev *jobs, const int32_t *scores,
+                            const int2 *segs, int32_t open, int32_t ext, int32_t gate, int regions, GateRes *out,
+                            hipStream_t s) {
+  return launch_gate_recs_multi(recs, nrecs, jobs, scores, segs, nullptr, nullptr, nullptr, nullptr, regions, out, s,
+                                open, ext, gate);
 }
 
 }  // namespace st

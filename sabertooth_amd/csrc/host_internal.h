@@ -31,24 +31,25 @@ struct RecDev {
   uint32_t tid, hits, j0, j1;
 };
 // stage_m: the longest query among the call's queries that has an int16
-// profile (0 when none); runs of such queries are scored from LDS.
+// profile (0 when none); runs of such queries are scored from LDS.  segs
+// (nullable): each job's kadane_segment bounds, which the gate needs.
 hipError_t launch_ung_jobs(const JobDev *jobs, uint64_t njobs, const QueryDev *qd, uint32_t stage_m,
-                           const uint8_t *arena, const uint64_t *toff, int32_t *out, hipStream_t s);
+                           const uint8_t *arena, const uint64_t *toff, int32_t *out, int2 *segs,
+                           hipStream_t s);
 hipError_t launch_gate_recs(const RecDev *recs, uint64_t nrecs, const JobDev *jobs,
-                            const int32_t *scores, const QueryDev *qd, uint32_t stage_m,
-                            const uint8_t *arena, const uint64_t *toff, int32_t open, int32_t ext,
+                            const int32_t *scores, const int2 *segs, int32_t open, int32_t ext,
                             int32_t gate, int regions, GateRes *out, hipStream_t s);
 // The same over jobs and records of several queries, sorted by query: job i
 // of query jq[i], record i of query rq[i] with that query's gap costs and
-// gate.
+// gate (open, ext, gate indexed by query; null: open1, ext1, gate1).
 hipError_t launch_ung_jobs_multi(const JobDev *jobs, uint64_t njobs, const uint32_t *jq,
                                  const QueryDev *queries, uint32_t stage_m, const uint8_t *arena,
-                                 const uint64_t *toff, int32_t *out, hipStream_t s);
+                                 const uint64_t *toff, int32_t *out, int2 *segs, hipStream_t s);
 hipError_t launch_gate_recs_multi(const RecDev *recs, uint64_t nrecs, const JobDev *jobs,
-                                  const int32_t *scores, const uint32_t *rq,
-                                  const QueryDev *queries, uint32_t stage_m, const uint8_t *arena,
-                                  const uint64_t *toff, const int32_t *open, const int32_t *ext,
-                                  const int32_t *gate, int regions, GateRes *out, hipStream_t s);
+                                  const int32_t *scores, const int2 *segs, const uint32_t *rq,
+                                  const int32_t *open, const int32_t *ext, const int32_t *gate,
+                                  int regions, GateRes *out, hipStream_t s, int32_t open1 = 0,
+                                  int32_t ext1 = 0, int32_t gate1 = 0);
 
 // ---- errors -------------------------------------------------------------------
 // Record msg as this thread's last error (st_last_error) and return code.
@@ -117,7 +118,7 @@ struct st_postings {
   st_ctx *ctx = nullptr;
   st::DevBuf<uint64_t> post;  // (target << 32) | position
   uint64_t n = 0;
-  mutable st::DevBuf<uint8_t> scratch[31];
+  mutable st::DevBuf<uint8_t> scratch[32];
 };
 
 struct st_qprof {

@@ -505,7 +505,8 @@ int st_seed_hits_to_candidates_batch(st_ctx *ctx, const st_db *db, const st_post
   uint32_t *d_cq = grow<uint32_t>(sc[28], st.nrecs);
   QueryDev *d_q = grow<QueryDev>(sc[29], nq);
   int32_t *prm = grow<int32_t>(sc[30], 3 * nq);  // open, extend, gate per query
-  if (!jq || !scores || !gres || !pass || !pidx || !chain || !cidx || !d_c || !d_cq || !d_q || !prm)
+  int2 *segs = grow<int2>(sc[31], st.njobs);
+  if (!jq || !scores || !gres || !pass || !pidx || !chain || !cidx || !d_c || !d_cq || !d_q || !prm || !segs)
     return fail(ST_ERR_NOMEM, f + ": device allocation failed");
   std::vector<int32_t> hprm(3 * nq);
   std::copy(gap_open, gap_open + nq, hprm.begin());
@@ -516,9 +517,10 @@ int st_seed_hits_to_candidates_batch(st_ctx *ctx, const st_db *db, const st_post
   // score every job against its query, then gate every record
   hipLaunchKernelGGL(job_query_kernel, dim3(grid_of(st.nrecs)), dim3(256), 0, s, st.recs, st.rq, st.nrecs, jq);
   ST_HIP(hipGetLastError());
-  ST_HIP(launch_ung_jobs_multi((const JobDev *)st.jobs, st.njobs, jq, d_q, stage_m, db->arena.p, db->off.p, scores, s));
-  ST_HIP(launch_gate_recs_multi((const RecDev *)st.recs, st.nrecs, (const JobDev *)st.jobs, scores, st.rq, d_q, stage_m,
-                                db->arena.p, db->off.p, prm, prm + nq, prm + 2 * nq, regions, gres, s));
+  ST_HIP(launch_ung_jobs_multi((const JobDev *)st.jobs, st.njobs, jq, d_q, stage_m, db->arena.p, db->off.p, scores,
+                               segs, s));
+  ST_HIP(launch_gate_recs_multi((const RecDev *)st.recs, st.nrecs, (const JobDev *)st.jobs, scores, segs, st.rq, prm,
+                                prm + nq, prm + 2 * nq, regions, gres, s));
   // the passing records, compacted in (query, target) order
   hipLaunchKernelGGL(cand_flags_kernel, dim3(grid_of(st.nrecs)), dim3(256), 0, s, gres, st.recs, st.nrecs, pass,
                      chain);
