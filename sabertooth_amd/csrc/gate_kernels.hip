@@ -18,6 +18,8 @@
 // hold the same values.
 #include <hip/hip_runtime.h>
 
+#include <cstdlib>
+
 #include "gate_core.h"
 
 namespace st {
@@ -81,8 +83,11 @@ __device__ __forceinline__ int32_t ung_job(const JobDev jb, const Acc &acc, cons
     const int64_t p0 = (int64_t)q + jb.diag;
     const int64_t pa = p0 & ~(int64_t)15;
     int skip = (int)(p0 - pa);
+    uint4 v = *(const uint4 *)(t + pa);
     for (int64_t pos = pa; q < hi; pos += 16) {
-      const uint4 v = *(const uint4 *)(t + pos);
+      // the next vector's load is issued before this one's lookups; it is
+      // made only when the scan reaches that vector
+      const uint4 vn = q + 16 - skip < hi ? *(const uint4 *)(t + pos + 16) : v;
       const uint32_t w[4] = {v.x, v.y, v.z, v.w};
       int32_t s[16];
 #pragma unroll
@@ -90,6 +95,7 @@ __device__ __forceinline__ int32_t ung_job(const JobDev jb, const Acc &acc, cons
         const int qq = q + b - skip;
         s[b] = (b >= skip && qq < hi) ? acc((w[b >> 2] >> ((b & 3) * 8)) & 0xffu, qq) : 0;
       }
+      v = vn;
 #pragma unroll
       for (int b = 0; b < 16; b++) {
         if (b >= skip && q < hi) {
@@ -196,9 +202,20 @@ __global__ __launch_bounds__(256) void gate_recs_kernel(const RecDev *__restrict
 
 // Staged rows for a launch whose longest query with an int16 profile has
 // stage_m rows (0: none), and the dynamic LDS that takes.
+// ST_STAGE_ROWS (8 .. kStageRows, read once) lowers the limit: less LDS per
+// block, more blocks per compute unit.
+static uint32_t stage_rows_limit() {
+  static const uint32_t lim = [] {
+    const char *e = std::getenv("ST_STAGE_ROWS");
+    const long v = e ? std::strtol(e, nullptr, 10) : kStageRows;
+    return (uint32_t)(v >= 8 && v <= kStageRows ? v & ~7L : kStageRows);
+  }();
+  return lim;
+}
+
 static int stage_cap(uint32_t stage_m) {
-  const uint32_t r = (stage_m + 7) & ~7u;
-  return (int)(r < (uint32_t)kStageRows ? r : (uint32_t)kStageRows);
+  const uint32_t r = (stage_m + 7) & ~7u, lim = stage_rows_limit();
+  return (int)(r < lim ? r : lim);
 }
 
 hipError_t launch_ung_jobs_multi(const JobDev *jobs, uint64_t njobs, const uint32_t *jq, const QueryDev *queries,
