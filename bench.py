@@ -251,6 +251,17 @@ def prefilter_leg(O, st, ctx, db, w, qps, gate, n_queries=200, k=4, parity_queri
     cands, chains = seed_hits_to_candidates_batch(db, dp, qps[:n_queries], None, gaps, gates, span=k, seed_arrays=sa)
     tc = time.perf_counter() - ts
     nc = sum(len(c) for c in cands)
+    # the same, one query per call (prefilter_query's granularity)
+    sa1 = [_seed_array([sd]) for sd in seeds_of]
+    seed_hits_to_candidates_batch(db, dp, qps[:1], None, gaps[:1], gates[:1], span=k, seed_arrays=sa1[0])
+    ts = time.perf_counter()
+    n1 = 0
+    for qi in range(n_queries):
+        c1, _ = seed_hits_to_candidates_batch(db, dp, [qps[qi]], None, gaps[:1], gates[:1], span=k,
+                                              seed_arrays=sa1[qi])
+        n1 += len(c1[0])
+    t1c = time.perf_counter() - ts
+    assert n1 == nc
     mism = cmism = 0
     for qi in range(parity_queries):
         gj, gr = seed_hits_to_jobs(ctx, db, dp, len(w.queries[qi]), seeds_of[qi], span=k)
@@ -266,12 +277,14 @@ def prefilter_leg(O, st, ctx, db, w, qps, gate, n_queries=200, k=4, parity_queri
                    "frac_of_8TBps": round(8 * tot / t / 8e12, 4)},
            "parity_queries_vs_oracle": parity_queries, "mismatched_queries": mism,
            "to_candidates": {"ms": round(1e3 * tc, 3), "hits_per_s": round(tot / tc, 1), "candidates": nc,
+                             "one_query_per_call_ms": round(1e3 * t1c, 3),
                              "chains_used": int(chains), "gate": int(gate),
                              "mismatched_queries_vs_score_and_gate": cmism}}
     log(f"prefilter: {n_queries} queries, {tot} seed hits -> {nj} jobs / {nr} targets in {1e3 * t:.1f} ms in one "
         f"call ({tot / t / 1e9:.2f} G hits/s; {1e3 * t1:.1f} ms one query per call); "
         f"{parity_queries - mism}/{parity_queries} queries equal the oracle")
-    log(f"prefilter: seeds -> {nc} candidates in one call: {1e3 * tc:.1f} ms ({tot / tc / 1e9:.2f} G hits/s); "
+    log(f"prefilter: seeds -> {nc} candidates in one call: {1e3 * tc:.1f} ms ({tot / tc / 1e9:.2f} G hits/s; "
+        f"{1e3 * t1c:.1f} ms one query per call); "
         f"{parity_queries - cmism}/{parity_queries} queries equal score_and_gate")
     return out
 

@@ -8,9 +8,10 @@
 // most significant first, as the reference does; one device radix sort of
 // the whole query's keys replaces its per-block sorts (the blocks only bound
 // host memory: every block's targets are contiguous and ascending, so the
-// concatenated output is the same).  With the full key sorted, each
-// (target, diagonal) run's smallest and largest query positions are its first
-// and last hits.  Runs and targets are found with boundary flags and scans;
+// concatenated output is the same).  The sort covers the key above the query
+// position, so each (target, diagonal) run is contiguous with its positions
+// unordered; the emit pass scans a chosen run for its smallest and largest
+// position.  Runs and targets are found with boundary flags and scans;
 // one thread per target keeps its top diagonals (hits descending, diagonal
 // ascending, the order top_diagonals' insertion keeps) and emits its jobs.
 #include <hip/hip_runtime.h>
@@ -161,7 +162,13 @@ __global__ void emit_kernel(const uint64_t *__restrict__ keys, uint64_t nkeys, u
     const uint32_t r = top_runs[t * kMaxJobsPerTarget + k];
     const uint64_t a = rfirst[r], b = (r + 1 < nruns ? rfirst[r + 1] : nkeys) - 1;
     const int64_t diag = (int64_t)((keys[a] >> lay.qbits) & ((1ull << lay.dbits) - 1)) - M;
-    const int64_t qmin = (int64_t)(keys[a] & qmask), qmax = (int64_t)(keys[b] & qmask);
+    // the sort leaves positions inside a run unordered: scan the run
+    int64_t qmin = (int64_t)(keys[a] & qmask), qmax = qmin;
+    for (uint64_t h = a + 1; h <= b; h++) {
+      const int64_t p = (int64_t)(keys[h] & qmask);
+      qmin = p < qmin ? p : qmin;
+      qmax = p > qmax ? p : qmax;
+    }
     // diag_range (src/ungapped.rs:38-48): query positions whose target
     // position d + i lies in [0, n)
     int64_t lo = diag < 0 ? -diag : 0;
@@ -371,12 +378,14 @@ static int seed_stage(const std::string &f, st_ctx *ctx, const st_db *db, const 
   ST_HIP(hipMemcpyAsync(&h_bad, bad, 4, hipMemcpyDeviceToHost, s));
   ST_HIP(hipStreamSynchronize(s));
   if (h_bad) return fail(ST_ERR_INVALID, f + ": a posting names a target outside the database");
-  // sort the whole keys (the query positions included: runs come out with
-  // their smallest and largest positions first and last)
+  // sort on (query, target, diagonal) only: runs come out contiguous, and
+  // the query positions below them need no radix passes (emit takes each
+  // chosen run's smallest and largest position by scanning it)
   size_t tb = 0;
-  ST_HIP(hipcub::DeviceRadixSort::SortKeys(nullptr, tb, k0, k1, (int)nh, 0, (int)end_bit, s));
+  const int begin_bit = (int)lay.qbits;
+  ST_HIP(hipcub::DeviceRadixSort::SortKeys(nullptr, tb, k0, k1, (int)nh, begin_bit, (int)end_bit, s));
   if (!grow<uint8_t>(tmp, tb)) return fail(ST_ERR_NOMEM, f + ": device allocation failed");
-  ST_HIP(hipcub::DeviceRadixSort::SortKeys(tmp.p, tb, k0, k1, (int)nh, 0, (int)end_bit, s));
+  ST_HIP(hipcub::DeviceRadixSort::SortKeys(tmp.p, tb, k0, k1, (int)nh, begin_bit, (int)end_bit, s));
   const uint64_t *keys = k1;
   // runs
   hipLaunchKernelGGL(run_flags_kernel, dim3(grid_of(nh)), dim3(256), 0, s, keys, nh, lay.qbits, flag);
