@@ -126,27 +126,28 @@ __device__ __forceinline__ int32_t ung_job(const JobDev jb, const Acc &acc, cons
 }
 
 constexpr int kJobsPerLane = 4;
+constexpr int kJobBlock = 512;  // threads per job-scan block unless ST_JOB_BLOCK says otherwise
 
-// Jobs [256 * kJobsPerLane * blockIdx.x, ...) of the list; job i belongs to
+// Jobs [kThreads * kJobsPerLane * blockIdx.x, ...) of the list; job i belongs to
 // query jq[i] of `queries` (jq null: every job to queries[0]).  kSeg: each
 // job's kadane_segment bounds go to segs[i].
-template <bool kSeg>
-__global__ __launch_bounds__(256) void ung_jobs_kernel(const JobDev *__restrict__ jobs, uint64_t njobs,
+template <bool kSeg, int kThreads>
+__global__ __launch_bounds__(kThreads) void ung_jobs_kernel(const JobDev *__restrict__ jobs, uint64_t njobs,
                                                        const uint32_t *__restrict__ jq,
                                                        const QueryDev *__restrict__ queries,
                                                        const uint8_t *__restrict__ arena,
                                                        const uint64_t *__restrict__ toff, int32_t *__restrict__ out,
                                                        int2 *__restrict__ segs, int cap) {
   extern __shared__ __align__(16) int16_t lds[];
-  const uint64_t base = (uint64_t)blockIdx.x * (256 * kJobsPerLane);
+  const uint64_t base = (uint64_t)blockIdx.x * (kThreads * kJobsPerLane);
   if (base >= njobs) return;
-  const uint64_t last = (base + 256 * kJobsPerLane < njobs ? base + 256 * kJobsPerLane : njobs) - 1;
+  const uint64_t last = (base + kThreads * kJobsPerLane < njobs ? base + kThreads * kJobsPerLane : njobs) - 1;
   const QueryDev qd = queries[jq ? jq[base] : 0];
   int S = 0;
   const bool staged = stages(jq, queries, base, last, cap) && stage_prof(qd, lds, cap, S);
   __syncthreads();
   for (int k = 0; k < kJobsPerLane; k++) {
-    const uint64_t i = base + (uint64_t)k * 256 + threadIdx.x;
+    const uint64_t i = base + (uint64_t)k * kThreads + threadIdx.x;
     if (i >= njobs) break;
     if (staged) {
       out[i] = ung_job<kSeg>(jobs[i], ProfLds{lds, S}, arena, toff, segs + i);
@@ -218,18 +219,41 @@ static int stage_cap(uint32_t stage_m) {
   return (int)(r < lim ? r : lim);
 }
 
+// ST_JOB_BLOCK (256, 512 or 1024 threads, read once): the job scan's block.
+// A block's staged profile serves all its waves, so larger blocks put more
+// waves on a compute unit for the same LDS.
+static int job_block() {
+  static const int b = [] {
+    const char *e = std::getenv("ST_JOB_BLOCK");
+    const long v = e ? std::strtol(e, nullptr, 10) : kJobBlock;
+    return (int)(v == 256 || v == 512 || v == 1024 ? v : kJobBlock);
+  }();
+  return b;
+}
+
+template <int kThreads>
+static void launch_ung(const JobDev *jobs, uint64_t njobs, const uint32_t *jq, const QueryDev *queries, int cap,
+                       const uint8_t *arena, const uint64_t *toff, int32_t *out, int2 *segs, hipStream_t s) {
+  const dim3 grid((uint32_t)((njobs + kThreads * kJobsPerLane - 1) / (kThreads * kJobsPerLane)));
+  const size_t lds = (size_t)kCodes * cap * 2;
+  if (segs)
+    hipLaunchKernelGGL((ung_jobs_kernel<true, kThreads>), grid, dim3(kThreads), lds, s, jobs, njobs, jq, queries,
+                       arena, toff, out, segs, cap);
+  else
+    hipLaunchKernelGGL((ung_jobs_kernel<false, kThreads>), grid, dim3(kThreads), lds, s, jobs, njobs, jq, queries,
+                       arena, toff, out, segs, cap);
+}
+
 hipError_t launch_ung_jobs_multi(const JobDev *jobs, uint64_t njobs, const uint32_t *jq, const QueryDev *queries,
                                  uint32_t stage_m, const uint8_t *arena, const uint64_t *toff, int32_t *out,
                                  int2 *segs, hipStream_t s) {
   if (njobs == 0) return hipSuccess;
   const int cap = stage_cap(stage_m);
-  const uint64_t blocks = (njobs + 256 * kJobsPerLane - 1) / (256 * kJobsPerLane);
-  if (segs)
-    hipLaunchKernelGGL(ung_jobs_kernel<true>, dim3((uint32_t)blocks), dim3(256), (size_t)kCodes * cap * 2, s, jobs,
-                       njobs, jq, queries, arena, toff, out, segs, cap);
-  else
-    hipLaunchKernelGGL(ung_jobs_kernel<false>, dim3((uint32_t)blocks), dim3(256), (size_t)kCodes * cap * 2, s, jobs,
-                       njobs, jq, queries, arena, toff, out, segs, cap);
+  switch (job_block()) {
+    case 1024: launch_ung<1024>(jobs, njobs, jq, queries, cap, arena, toff, out, segs, s); break;
+    case 512: launch_ung<512>(jobs, njobs, jq, queries, cap, arena, toff, out, segs, s); break;
+    default: launch_ung<256>(jobs, njobs, jq, queries, cap, arena, toff, out, segs, s); break;
+  }
   return hipGetLastError();
 }
 
