@@ -192,13 +192,20 @@ __global__ __launch_bounds__(256) void gate_recs_kernel(const RecDev *__restrict
                                                         const int32_t *__restrict__ scores,
                                                         const int2 *__restrict__ segs,
                                                         const uint32_t *__restrict__ rq, GateParams gp,
-                                                        int regions, GateRes *__restrict__ out) {
+                                                        int regions, GateRes *__restrict__ out,
+                                                        uint32_t *__restrict__ pass, uint32_t *__restrict__ chain) {
   const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= nrecs) return;
   const uint32_t q = rq ? rq[i] : 0;
   const int32_t open = gp.open ? gp.open[q] : gp.open1, ext = gp.ext ? gp.ext[q] : gp.ext1,
                 gate = gp.gate ? gp.gate[q] : gp.gate1;
-  out[i] = gate_rec(recs[i], jobs, scores, segs, open, ext, gate, regions);
+  const GateRes r = gate_rec(recs[i], jobs, scores, segs, open, ext, gate, regions);
+  out[i] = r;
+  // a record without jobs neither passes nor chains (score_and_gate skips it)
+  if (pass) {
+    pass[i] = r.pass;
+    chain[i] = r.chain;
+  }
 }
 
 // Staged rows for a launch whose longest query with an int16 profile has
@@ -265,10 +272,10 @@ hipError_t launch_ung_jobs(const JobDev *jobs, uint64_t njobs, const QueryDev *q
 hipError_t launch_gate_recs_multi(const RecDev *recs, uint64_t nrecs, const JobDev *jobs, const int32_t *scores,
                                   const int2 *segs, const uint32_t *rq, const int32_t *open, const int32_t *ext,
                                   const int32_t *gate, int regions, GateRes *out, hipStream_t s, int32_t open1,
-                                  int32_t ext1, int32_t gate1) {
+                                  int32_t ext1, int32_t gate1, uint32_t *pass, uint32_t *chain) {
   if (nrecs == 0) return hipSuccess;
   hipLaunchKernelGGL(gate_recs_kernel, dim3((uint32_t)((nrecs + 255) / 256)), dim3(256), 0, s, recs, nrecs, jobs,
-                     scores, segs, rq, GateParams{open, ext, gate, open1, ext1, gate1}, regions, out);
+                     scores, segs, rq, GateParams{open, ext, gate, open1, ext1, gate1}, regions, out, pass, chain);
   return hipGetLastError();
 }
 
@@ -276,7 +283,7 @@ hipError_t launch_gate_recs(const RecDev *recs, uint64_t nrecs, const JobDev *jo
                             const int2 *segs, int32_t open, int32_t ext, int32_t gate, int regions, GateRes *out,
                             hipStream_t s) {
   return launch_gate_recs_multi(recs, nrecs, jobs, scores, segs, nullptr, nullptr, nullptr, nullptr, regions, out, s,
-                                open, ext, gate);
+                                open, ext, gate, nullptr, nullptr);
 }
 
 }  // namespace st

@@ -41,7 +41,8 @@ hipError_t launch_gate_recs(const RecDev *recs, uint64_t nrecs, const JobDev *jo
                             int32_t gate, int regions, GateRes *out, hipStream_t s);
 // The same over jobs and records of several queries, sorted by query: job i
 // of query jq[i], record i of query rq[i] with that query's gap costs and
-// gate (open, ext, gate indexed by query; null: open1, ext1, gate1).
+// gate (open, ext, gate indexed by query; null: open1, ext1, gate1).  pass
+// and chain (nullable): each record's flags as 0/1 words, for a compaction.
 hipError_t launch_ung_jobs_multi(const JobDev *jobs, uint64_t njobs, const uint32_t *jq,
                                  const QueryDev *queries, uint32_t stage_m, const uint8_t *arena,
                                  const uint64_t *toff, int32_t *out, int2 *segs, hipStream_t s);
@@ -49,7 +50,8 @@ hipError_t launch_gate_recs_multi(const RecDev *recs, uint64_t nrecs, const JobD
                                   const int32_t *scores, const int2 *segs, const uint32_t *rq,
                                   const int32_t *open, const int32_t *ext, const int32_t *gate,
                                   int regions, GateRes *out, hipStream_t s, int32_t open1 = 0,
-                                  int32_t ext1 = 0, int32_t gate1 = 0);
+                                  int32_t ext1 = 0, int32_t gate1 = 0, uint32_t *pass = nullptr,
+                                  uint32_t *chain = nullptr);
 
 // ---- errors -------------------------------------------------------------------
 // Record msg as this thread's last error (st_last_error) and return code.

@@ -146,7 +146,8 @@ __global__ void emit_kernel(const uint64_t *__restrict__ keys, uint64_t nkeys, u
                             const uint32_t *__restrict__ roff, const uint32_t *__restrict__ top_runs,
                             const uint32_t *__restrict__ tlen, const uint64_t *__restrict__ m_of, Layout lay,
                             uint32_t diag_window, uint32_t span, st_job *__restrict__ jobs,
-                            st_target_rec *__restrict__ recs, uint32_t *__restrict__ rec_q) {
+                            st_target_rec *__restrict__ recs, uint32_t *__restrict__ rec_q,
+                            uint32_t *__restrict__ job_q) {
   const uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= ntg || njobs[t] == 0) return;
   const uint32_t r0 = tfirst[t];
@@ -186,6 +187,7 @@ __global__ void emit_kernel(const uint64_t *__restrict__ keys, uint64_t nkeys, u
     j.lo = (uint32_t)lo;
     j.hi = (uint32_t)(hi > lo ? hi : lo);
     jobs[j0 + k] = j;
+    if (job_q) job_q[j0 + k] = q;
   }
   st_target_rec rc;
   rc.tid = tid;
@@ -194,25 +196,6 @@ __global__ void emit_kernel(const uint64_t *__restrict__ keys, uint64_t nkeys, u
   rc.j1 = j0 + njobs[t];
   recs[roff[t]] = rc;
   rec_q[roff[t]] = q;
-}
-
-// jq[j]: the query of job j (its record's query)
-__global__ void job_query_kernel(const st_target_rec *__restrict__ recs, const uint32_t *__restrict__ rq, uint64_t n,
-                                 uint32_t *__restrict__ jq) {
-  const uint64_t r = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (r >= n) return;
-  for (uint32_t j = recs[r].j0; j < recs[r].j1; j++) jq[j] = rq[r];
-}
-
-// records with jobs: pass flag and chain flag (score_and_gate's push and its
-// chains_used count, src/prefilter.rs:1577-1660)
-__global__ void cand_flags_kernel(const GateRes *__restrict__ g, const st_target_rec *__restrict__ recs, uint64_t n,
-                                  uint32_t *__restrict__ pass, uint32_t *__restrict__ chain) {
-  const uint64_t r = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (r >= n) return;
-  const bool jobs = recs[r].j1 > recs[r].j0;
-  pass[r] = jobs && g[r].pass ? 1u : 0u;
-  chain[r] = jobs && g[r].chain ? 1u : 0u;
 }
 
 __global__ void cand_emit_kernel(const GateRes *__restrict__ g, const st_target_rec *__restrict__ recs,
@@ -305,6 +288,7 @@ struct SeedStage {
   st_job *jobs = nullptr;
   st_target_rec *recs = nullptr;
   uint32_t *rq = nullptr;
+  uint32_t *jq = nullptr;  // each job's query (when asked for)
 };
 
 static int seed_stage_check(const std::string &f, st_ctx *ctx, const st_db *db, const st_postings *post, uint64_t nq,
@@ -323,7 +307,7 @@ static int seed_stage_check(const std::string &f, st_ctx *ctx, const st_db *db, 
 // ctx->mu held by the caller; arguments checked by seed_stage_check
 static int seed_stage(const std::string &f, st_ctx *ctx, const st_db *db, const st_postings *post, uint64_t nq,
                       const uint64_t *m, const uint64_t *seed_off, const st_seed *seeds, const st_hit_params *hp,
-                      SeedStage &out) {
+                      SeedStage &out, bool want_jq = false) {
   out = SeedStage();
   if (nq == 0 || db->n_targets == 0) return ST_OK;
   const uint64_t nseeds = seed_off[nq];
@@ -419,15 +403,17 @@ static int seed_stage(const std::string &f, st_ctx *ctx, const st_db *db, const 
   st_job *d_jobs = grow<st_job>(sc[16], tot_jobs);
   st_target_rec *d_recs = grow<st_target_rec>(sc[17], tot_recs);
   uint32_t *d_rq = grow<uint32_t>(sc[19], tot_recs);
-  if (!d_jobs || !d_recs || !d_rq) return fail(ST_ERR_NOMEM, f + ": device allocation failed");
+  uint32_t *d_jq = want_jq ? grow<uint32_t>(sc[20], tot_jobs) : nullptr;
+  if (!d_jobs || !d_recs || !d_rq || (want_jq && !d_jq)) return fail(ST_ERR_NOMEM, f + ": device allocation failed");
   hipLaunchKernelGGL(emit_kernel, dim3(grid_of(ntg)), dim3(256), 0, s, keys, nh, ntg, nruns, rfirst, tfirst, nj,
-                     joff, roff, top_runs, db->len.p, d_m, lay, hp->diag_window, hp->span, d_jobs, d_recs, d_rq);
+                     joff, roff, top_runs, db->len.p, d_m, lay, hp->diag_window, hp->span, d_jobs, d_recs, d_rq, d_jq);
   ST_HIP(hipGetLastError());
   out.njobs = tot_jobs;
   out.nrecs = tot_recs;
   out.jobs = d_jobs;
   out.recs = d_recs;
   out.rq = d_rq;
+  out.jq = d_jq;
   return ST_OK;
 }
 
@@ -501,11 +487,11 @@ int st_seed_hits_to_candidates_batch(st_ctx *ctx, const st_db *db, const st_post
   if (int rc = seed_stage_check(f, ctx, db, post, nq, m.data(), seed_off, seeds, hp)) return rc;
   std::lock_guard<std::mutex> lk(ctx->mu);
   SeedStage st;
-  if (int rc = seed_stage(f, ctx, db, post, nq, m.data(), seed_off, seeds, hp, st)) return rc;
+  if (int rc = seed_stage(f, ctx, db, post, nq, m.data(), seed_off, seeds, hp, st, true)) return rc;
   if (st.nrecs == 0) return ST_OK;
   hipStream_t s = ctx->stream;
   DevBuf<uint8_t> *sc = post->scratch;
-  uint32_t *jq = grow<uint32_t>(sc[20], st.njobs);
+  const uint32_t *jq = st.jq;
   int32_t *scores = grow<int32_t>(sc[21], st.njobs);
   GateRes *gres = grow<GateRes>(sc[22], st.nrecs);
   uint32_t *pass = grow<uint32_t>(sc[23], st.nrecs), *pidx = grow<uint32_t>(sc[24], st.nrecs);
@@ -523,17 +509,13 @@ int st_seed_hits_to_candidates_batch(st_ctx *ctx, const st_db *db, const st_post
   std::copy(min_diag_score, min_diag_score + nq, hprm.begin() + 2 * nq);
   ST_HIP(hipMemcpyAsync(d_q, qv.data(), nq * sizeof(QueryDev), hipMemcpyHostToDevice, s));
   ST_HIP(hipMemcpyAsync(prm, hprm.data(), 3 * nq * 4, hipMemcpyHostToDevice, s));
-  // score every job against its query, then gate every record
-  hipLaunchKernelGGL(job_query_kernel, dim3(grid_of(st.nrecs)), dim3(256), 0, s, st.recs, st.rq, st.nrecs, jq);
-  ST_HIP(hipGetLastError());
+  // score every job against its query, then gate every record; the gate
+  // writes the pass and chain flags the compaction scans
   ST_HIP(launch_ung_jobs_multi((const JobDev *)st.jobs, st.njobs, jq, d_q, stage_m, db->arena.p, db->off.p, scores,
                                segs, s));
   ST_HIP(launch_gate_recs_multi((const RecDev *)st.recs, st.nrecs, (const JobDev *)st.jobs, scores, segs, st.rq, prm,
-                                prm + nq, prm + 2 * nq, regions, gres, s));
+                                prm + nq, prm + 2 * nq, regions, gres, s, 0, 0, 0, pass, chain));
   // the passing records, compacted in (query, target) order
-  hipLaunchKernelGGL(cand_flags_kernel, dim3(grid_of(st.nrecs)), dim3(256), 0, s, gres, st.recs, st.nrecs, pass,
-                     chain);
-  ST_HIP(hipGetLastError());
   uint64_t nc = 0, nchain = 0;
   DevBuf<uint8_t> &tmp = sc[18];
   ST_HIP(scan_total(pass, pidx, st.nrecs, tmp, s, &nc));
