@@ -194,6 +194,7 @@ int st_db_create(st_ctx *ctx, const uint8_t *residues, uint64_t n_residues, cons
   db->n_targets = n_targets;
   db->n_res = n_residues;
   db->hlen = len;
+  for (uint32_t l : len) db->max_len = std::max(db->max_len, l);
   if (db->arena.upload(host.data(), host.size(), ctx->stream) != hipSuccess ||
       db->off.upload(doff.data(), doff.size(), ctx->stream) != hipSuccess ||
       db->len.upload(len.data(), len.size(), ctx->stream) != hipSuccess) {
@@ -1104,6 +1105,7 @@ int st_align_spans_batch(st_ctx *ctx, const st_db *db, const st_qprof *const *qu
   rdb->n_targets = nr;
   rdb->n_res = doff[nr];
   rdb->hlen = rlen;
+  for (uint32_t l : rlen) rdb->max_len = std::max(rdb->max_len, l);
   DevBuf<int32_t> mirror;
   DevBuf<RevPrefix> d_pre;
   {

@@ -110,6 +110,7 @@ struct st_db {
   st::DevBuf<uint64_t> off;
   st::DevBuf<uint32_t> len;
   std::vector<uint32_t> hlen;
+  uint32_t max_len = 0;  // the longest target
   uint64_t n_targets = 0, n_res = 0;
 };
 

@@ -330,8 +330,7 @@ static int seed_stage(const std::string &f, st_ctx *ctx, const st_db *db, const 
   if (nh == 0) return ST_OK;
   // the device sort and scans count items in int
   if (nh > 0x7fffffffull) return fail(ST_ERR_INVALID, f + ": more than 2^31 - 1 seed hits");
-  uint32_t maxlen = 0;
-  for (uint32_t l : db->hlen) maxlen = std::max(maxlen, l);
+  const uint32_t maxlen = db->max_len;
   Layout lay;
   lay.m = M;
   lay.qbits = bits_for(M - 1);
