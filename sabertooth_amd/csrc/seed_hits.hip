@@ -357,10 +357,10 @@ static int seed_stage(const std::string &f, st_ctx *ctx, const st_db *db, const 
   hipLaunchKernelGGL(expand_kernel, dim3((unsigned)((nseeds + 3) / 4)), dim3(256), 0, s, d_seeds, nseeds, d_off,
                      post->post.p, lay, db->n_targets, k0, bad);
   ST_HIP(hipGetLastError());
+  // the flag is read after the run scan's synchronisation: a bad hit's key
+  // is 0, which the sort and the run passes only move
   uint32_t h_bad = 0;
   ST_HIP(hipMemcpyAsync(&h_bad, bad, 4, hipMemcpyDeviceToHost, s));
-  ST_HIP(hipStreamSynchronize(s));
-  if (h_bad) return fail(ST_ERR_INVALID, f + ": a posting names a target outside the database");
   // sort on (query, target, diagonal) only: runs come out contiguous, and
   // the query positions below them need no radix passes (emit takes each
   // chosen run's smallest and largest position by scanning it)
@@ -375,6 +375,7 @@ static int seed_stage(const std::string &f, st_ctx *ctx, const st_db *db, const 
   ST_HIP(hipGetLastError());
   uint64_t nruns = 0;
   ST_HIP(scan_total(flag, idx, nh, tmp, s, &nruns));
+  if (h_bad) return fail(ST_ERR_INVALID, f + ": a posting names a target outside the database");
   uint32_t *rfirst = grow<uint32_t>(sc[7], nruns), *tflag = grow<uint32_t>(sc[8], nruns),
            *tidx = grow<uint32_t>(sc[9], nruns);
   if (!rfirst || !tflag || !tidx) return fail(ST_ERR_NOMEM, f + ": device allocation failed");
@@ -494,13 +495,13 @@ int st_seed_hits_to_candidates_batch(st_ctx *ctx, const st_db *db, const st_post
   int32_t *scores = grow<int32_t>(sc[21], st.njobs);
   GateRes *gres = grow<GateRes>(sc[22], st.nrecs);
   uint32_t *pass = grow<uint32_t>(sc[23], st.nrecs), *pidx = grow<uint32_t>(sc[24], st.nrecs);
-  uint32_t *chain = grow<uint32_t>(sc[25], st.nrecs), *cidx = grow<uint32_t>(sc[26], st.nrecs);
+  uint32_t *chain = grow<uint32_t>(sc[25], st.nrecs), *nchain_d = grow<uint32_t>(sc[26], 1);
   st_candidate *d_c = grow<st_candidate>(sc[27], st.nrecs);
   uint32_t *d_cq = grow<uint32_t>(sc[28], st.nrecs);
   QueryDev *d_q = grow<QueryDev>(sc[29], nq);
   int32_t *prm = grow<int32_t>(sc[30], 3 * nq);  // open, extend, gate per query
   int2 *segs = grow<int2>(sc[31], st.njobs);
-  if (!jq || !scores || !gres || !pass || !pidx || !chain || !cidx || !d_c || !d_cq || !d_q || !prm || !segs)
+  if (!jq || !scores || !gres || !pass || !pidx || !chain || !nchain_d || !d_c || !d_cq || !d_q || !prm || !segs)
     return fail(ST_ERR_NOMEM, f + ": device allocation failed");
   std::vector<int32_t> hprm(3 * nq);
   std::copy(gap_open, gap_open + nq, hprm.begin());
@@ -515,10 +516,17 @@ int st_seed_hits_to_candidates_batch(st_ctx *ctx, const st_db *db, const st_post
   ST_HIP(launch_gate_recs_multi((const RecDev *)st.recs, st.nrecs, (const JobDev *)st.jobs, scores, segs, st.rq, prm,
                                 prm + nq, prm + 2 * nq, regions, gres, s, 0, 0, 0, pass, chain));
   // the passing records, compacted in (query, target) order
-  uint64_t nc = 0, nchain = 0;
+  // the chain count is a sum; its copy completes with the pass scan's
+  // synchronisation
   DevBuf<uint8_t> &tmp = sc[18];
+  size_t tb = 0;
+  ST_HIP(hipcub::DeviceReduce::Sum(nullptr, tb, chain, nchain_d, (int)st.nrecs, s));
+  if (!grow<uint8_t>(tmp, tb)) return fail(ST_ERR_NOMEM, f + ": device allocation failed");
+  ST_HIP(hipcub::DeviceReduce::Sum(tmp.p, tb, chain, nchain_d, (int)st.nrecs, s));
+  uint32_t nchain = 0;
+  ST_HIP(hipMemcpyAsync(&nchain, nchain_d, 4, hipMemcpyDeviceToHost, s));
+  uint64_t nc = 0;
   ST_HIP(scan_total(pass, pidx, st.nrecs, tmp, s, &nc));
-  ST_HIP(scan_total(chain, cidx, st.nrecs, tmp, s, &nchain));
   if (chains_used) *chains_used += nchain;
   *n_out = nc;
   if (nc > cap || (nc && !out))
