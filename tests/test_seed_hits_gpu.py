@@ -236,3 +236,40 @@ def test_candidates_profile_in_lds_and_in_global_memory(ctx):
     for q in qps:
         q.close()
     dp.close()
+
+
+def test_candidates_capacity_error_reports_the_count(ctx):
+    # a too-small output buffer: ST_ERR_CAPACITY, *n_out = the candidate
+    # count and the chain count accumulated; a buffer of that size succeeds
+    import ctypes as C
+    from sabertooth_amd import _lib
+    from sabertooth_amd.core import _seed_array, seed_hits_to_candidates_batch
+    rng = np.random.default_rng(33)
+    targets = [rng.integers(0, 4, size=int(rng.integers(50, 300)), dtype=np.uint8) for _ in range(300)]
+    post, of = kmer_index(targets, 4)
+    db = _db(ctx, targets)
+    dp = Postings(ctx, post)
+    q = rng.integers(0, 4, size=200, dtype=np.uint8)
+    qp = st.QueryProfile.from_rows(ctx, O.seq_rows(q))
+    seeds = [query_seeds(q, 4, of)]
+    want, want_chains = seed_hits_to_candidates_batch(db, dp, [qp], seeds, [GapCosts(11, 1)], [60], span=4)
+    assert 1 < len(want[0])
+    so, sd = _seed_array(seeds)
+    hp = np.zeros(1, dtype=_lib.HIT_PARAMS_DT)
+    hp["max_diagonals"], hp["min_hits"], hp["diag_hits"], hp["diag_window"], hp["span"] = 4, 1, 1, 48, 4
+    qh = (C.c_void_p * 1)(qp.handle)
+    op, ex, gt = (np.array([v], np.int32) for v in (11, 1, 60))
+    co = np.zeros(2, np.uint64)
+    for cap in (1, len(want[0])):
+        out = np.zeros(cap, dtype=_lib.CAND_DT)
+        n, chains = C.c_uint64(0), C.c_uint64(5)
+        rc = _lib.lib().st_seed_hits_to_candidates_batch(
+            ctx.handle, db.handle, dp.handle, 1, qh, _lib.ptr(so), _lib.ptr(sd), _lib.ptr(hp), _lib.ptr(op),
+            _lib.ptr(ex), _lib.ptr(gt), 0, _lib.ptr(out), cap, C.byref(n), _lib.ptr(co), C.byref(chains))
+        assert n.value == len(want[0]) and chains.value == 5 + want_chains
+        if cap < len(want[0]):
+            assert rc == _lib.ST_ERR_CAPACITY
+        else:
+            assert rc == 0 and (out == want[0]).all() and list(co) == [0, len(want[0])]
+    qp.close()
+    dp.close()
