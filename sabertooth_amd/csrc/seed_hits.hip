@@ -133,7 +133,8 @@ __global__ void target_top_kernel(const uint64_t *__restrict__ keys, uint64_t nk
     }
     if (ntop > 0 && cnt[0] >= diag_hits) {
       nj = ntop;
-      for (uint32_t k = 0; k < ntop; k++) top_runs[t * kMaxJobsPerTarget + k] = run[k];
+      // slot-major ([slot][target]): a wave's stores of one slot are contiguous
+      for (uint32_t k = 0; k < ntop; k++) top_runs[k * ntg + t] = run[k];
     }
   }
   njobs[t] = nj;
@@ -160,7 +161,7 @@ __global__ void emit_kernel(const uint64_t *__restrict__ keys, uint64_t nkeys, u
   const uint64_t qmask = (1ull << lay.qbits) - 1;
   const uint32_t j0 = joff[t];
   for (uint32_t k = 0; k < njobs[t]; k++) {
-    const uint32_t r = top_runs[t * kMaxJobsPerTarget + k];
+    const uint32_t r = top_runs[k * ntg + t];
     const uint64_t a = rfirst[r], b = (r + 1 < nruns ? rfirst[r + 1] : nkeys) - 1;
     const int64_t diag = (int64_t)((keys[a] >> lay.qbits) & ((1ull << lay.dbits) - 1)) - M;
     // the sort leaves positions inside a run unordered: scan the run
