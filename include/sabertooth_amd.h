@@ -229,7 +229,7 @@ typedef struct {
 /* The PrefilterParams fields the seed-hit stage reads
  * (src/prefilter.rs:263-303), and the seed pattern's span. */
 typedef struct {
-  uint32_t max_diagonals; /* diagonals per target, 1..16 (default 4) */
+  uint32_t max_diagonals; /* diagonals per target, >= 1 (default 4); jobs: the first 16 */
   uint32_t min_hits;      /* seed hits per target (default 1) */
   uint32_t diag_hits;     /* seed hits on the best diagonal (default 1) */
   uint32_t diag_window;   /* residues around the seed hits, 0 = whole diagonal (default 48) */
@@ -242,7 +242,8 @@ typedef struct {
  * qpos) (src/prefilter.rs:999-1086); hits are grouped by target and
  * diagonal; a target with at least min_hits hits whose best diagonal has at
  * least diag_hits gets its top max_diagonals diagonals (hits descending,
- * diagonal ascending; top_diagonals src/prefilter.rs:1194-1225) as jobs over
+ * diagonal ascending; top_diagonals src/prefilter.rs:1194-1225), at most 16
+ * of them (src/prefilter.rs:1466), as jobs over
  * diag_range, narrowed to [qmin - diag_window, qmax + span + diag_window)
  * when diag_window > 0, and one record (src/prefilter.rs:1412-1492).
  * Targets come in ascending order, as the reference pushes them; jobs and
@@ -303,7 +304,8 @@ int st_score_and_gate(st_ctx *ctx, const st_qprof *qp, const st_db *db,
  * push order), go to out; cand_off (nq + 1) delimits each query's.  Equal
  * to st_seed_hits_to_jobs_batch followed by st_score_and_gate per query.
  * *n_out = the count; ST_ERR_CAPACITY when it exceeds cap (nothing written
- * but *n_out and chains_used); chains_used (nullable) is accumulated. */
+ * but *n_out); chains_used (nullable) is accumulated by a call that succeeds,
+ * so a retry with a buffer of *n_out entries counts every chain once. */
 int st_seed_hits_to_candidates_batch(st_ctx *ctx, const st_db *db, const st_postings *post,
                                      uint64_t nq, const st_qprof *const *queries,
                                      const uint64_t *seed_off, const st_seed *seeds,

@@ -43,15 +43,23 @@ def _run(cmd):
 
 def build(verbose: bool = False) -> str:
     """Compile all sources and link the library; returns its path."""
+    from concurrent.futures import ThreadPoolExecutor
+
     objdir = os.path.join(HERE, "_build")
     os.makedirs(objdir, exist_ok=True)
-    objs = []
-    for src in SOURCES:
-        obj = os.path.join(objdir, src + ".o")
-        out = _run([HIPCC, *CFLAGS, *HOST_FP.get(src, []), "-x", "hip", "-c", os.path.join(CSRC, src), "-o", obj])
-        if verbose and out.strip():
-            print(out)
-        objs.append(obj)
+    objs = [os.path.join(objdir, src + ".o") for src in SOURCES]
+
+    def one(k):
+        src = SOURCES[k]
+        return _run([HIPCC, *CFLAGS, *HOST_FP.get(src, []), "-x", "hip", "-c", os.path.join(CSRC, src), "-o",
+                     objs[k]])
+
+    # one compiler process per source, at most 8 at a time
+    jobs = max(1, min(8, len(os.sched_getaffinity(0))))
+    with ThreadPoolExecutor(jobs) as ex:
+        for out in ex.map(one, range(len(SOURCES))):
+            if verbose and out.strip():
+                print(out)
     tmp = LIB + ".tmp"
     _run([HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", tmp, *objs])
     os.replace(tmp, LIB)

@@ -361,6 +361,12 @@ def seed_hits_to_candidates_batch(db: TargetDb, postings: Postings, queries: Seq
     so, sd = seed_arrays if seed_arrays is not None else _seed_array(seed_lists)
     so = np.ascontiguousarray(so, dtype=np.uint64)
     sd = np.ascontiguousarray(sd, dtype=_lib.SEED_DT)
+    # the C entry point reads nq gap pairs and gates and nq + 1 seed offsets
+    if len(gaps) != nq or len(min_diag_scores) != nq or len(so) != nq + 1:
+        raise ValueError(f"seed_hits_to_candidates_batch: {nq} queries but {len(gaps)} gap costs, "
+                         f"{len(min_diag_scores)} gates and {len(so) - 1} seed lists")
+    if len(sd) < int(so[-1]):
+        raise ValueError("seed_hits_to_candidates_batch: fewer seed rows than the offsets name")
     hp = np.zeros(1, dtype=_lib.HIT_PARAMS_DT)
     hp["max_diagonals"], hp["min_hits"], hp["diag_hits"] = max_diagonals, min_hits, diag_hits
     hp["diag_window"], hp["span"] = diag_window, span

@@ -7,14 +7,6 @@
 
 namespace st {
 
-struct GateRes {
-  int32_t score;      // best single or chain
-  int32_t best;       // best single segment
-  int32_t best_diag;  // diagonal of the best single segment
-  uint32_t t_lo, t_hi;
-  uint8_t pass, chain, pad0, pad1;
-};
-
 // score lookups: code 21 rows of both profiles are zero
 struct ProfGlobal {
   const int32_t *p;

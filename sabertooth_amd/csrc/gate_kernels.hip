@@ -24,15 +24,6 @@
 
 namespace st {
 
-struct JobDev {
-  uint32_t tid;
-  int32_t diag;
-  uint32_t lo, hi;
-};
-struct RecDev {
-  uint32_t tid, hits, j0, j1;
-};
-
 // Rows of a query profile a block can stage in LDS: 22 int16 rows of up to
 // 1,112 entries (48,928 bytes), three blocks per compute unit.
 constexpr int kStageRows = 1112;

@@ -4,6 +4,7 @@
 #pragma once
 #include <hip/hip_runtime.h>
 
+#include <cstddef>
 #include <cstdint>
 #include <mutex>
 #include <string>
@@ -17,19 +18,25 @@ namespace st {
 
 // ---- kernel entry points (dp_kernels.hip, gate_kernels.hip) -----------------
 
-struct GateRes {
-  int32_t score, best, best_diag;
-  uint32_t t_lo, t_hi;
-  uint8_t pass, chain, pad0, pad1;
-};
-struct JobDev {
-  uint32_t tid;
-  int32_t diag;
-  uint32_t lo, hi;
-};
-struct RecDev {
-  uint32_t tid, hits, j0, j1;
-};
+// the kernels read the caller's st_job / st_target_rec buffers as JobDev /
+// RecDev (st_device.h): the layouts must agree field by field
+static_assert(sizeof(JobDev) == sizeof(st_job) && offsetof(JobDev, tid) == offsetof(st_job, tid) &&
+                  offsetof(JobDev, diag) == offsetof(st_job, diag) && offsetof(JobDev, lo) == offsetof(st_job, lo) &&
+                  offsetof(JobDev, hi) == offsetof(st_job, hi),
+              "JobDev must have st_job's layout");
+static_assert(sizeof(RecDev) == sizeof(st_target_rec) && offsetof(RecDev, tid) == offsetof(st_target_rec, tid) &&
+                  offsetof(RecDev, hits) == offsetof(st_target_rec, hits) &&
+                  offsetof(RecDev, j0) == offsetof(st_target_rec, j0) &&
+                  offsetof(RecDev, j1) == offsetof(st_target_rec, j1),
+              "RecDev must have st_target_rec's layout");
+static_assert(sizeof(PairResultDev) == sizeof(st_pair_result) &&
+                  offsetof(PairResultDev, swg) == offsetof(st_pair_result, swg) &&
+                  offsetof(PairResultDev, ung) == offsetof(st_pair_result, ung) &&
+                  offsetof(PairResultDev, best_diag) == offsetof(st_pair_result, best_diag) &&
+                  offsetof(PairResultDev, gate_score) == offsetof(st_pair_result, gate_score) &&
+                  offsetof(PairResultDev, pass) == offsetof(st_pair_result, pass) &&
+                  offsetof(PairResultDev, kernel) == offsetof(st_pair_result, kernel),
+              "PairResultDev must have st_pair_result's layout");
 // stage_m: the longest query among the call's queries that has an int16
 // profile (0 when none); runs of such queries are scored from LDS.  segs
 // (nullable): each job's kadane_segment bounds, which the gate needs.

@@ -297,8 +297,7 @@ static int seed_stage_check(const std::string &f, st_ctx *ctx, const st_db *db, 
                             const st_hit_params *hp) {
   if (!ctx || !db || !post || !hp || (nq && (!m || !seed_off))) return fail(ST_ERR_INVALID, f + ": null argument");
   if (db->ctx != ctx || post->ctx != ctx) return fail(ST_ERR_INVALID, f + ": handle of another context");
-  if (hp->max_diagonals < 1 || hp->max_diagonals > (uint32_t)kMaxJobsPerTarget)
-    return fail(ST_ERR_INVALID, f + ": max_diagonals must be 1..16");
+  if (hp->max_diagonals < 1) return fail(ST_ERR_INVALID, f + ": max_diagonals must be at least 1");
   if (nq > 0xffffffffull) return fail(ST_ERR_INVALID, f + ": more than 2^32 queries");
   if (nq && seed_off[0] != 0) return fail(ST_ERR_INVALID, f + ": seed_off[0] must be 0");
   if (nq && seed_off[nq] && !seeds) return fail(ST_ERR_INVALID, f + ": null seeds");
@@ -386,14 +385,18 @@ static int seed_stage(const std::string &f, st_ctx *ctx, const st_db *db, const 
   // (query, target) groups
   uint64_t ntg = 0;
   ST_HIP(scan_total(tflag, tidx, nruns, tmp, s, &ntg));
+  // a target's jobs are the first 16 of its top max_diagonals diagonals
+  // (src/prefilter.rs:1466); the top list's order is total (hits descending,
+  // then arrival), so its first 16 entries are the top-16 list itself
+  const uint32_t kmax = std::min<uint32_t>(hp->max_diagonals, (uint32_t)kMaxJobsPerTarget);
   uint32_t *tfirst = grow<uint32_t>(sc[10], ntg), *nj = grow<uint32_t>(sc[11], ntg),
            *joff = grow<uint32_t>(sc[12], ntg), *tkeep = grow<uint32_t>(sc[13], ntg),
-           *roff = grow<uint32_t>(sc[14], ntg), *top_runs = grow<uint32_t>(sc[15], ntg * kMaxJobsPerTarget);
+           *roff = grow<uint32_t>(sc[14], ntg), *top_runs = grow<uint32_t>(sc[15], ntg * kmax);
   if (!tfirst || !nj || !joff || !tkeep || !roff || !top_runs) return fail(ST_ERR_NOMEM, f + ": device allocation failed");
   hipLaunchKernelGGL(target_first_kernel, dim3(grid_of(nruns)), dim3(256), 0, s, nruns, tflag, tidx, tfirst);
   ST_HIP(hipGetLastError());
   hipLaunchKernelGGL(target_top_kernel, dim3(grid_of(ntg)), dim3(256), 0, s, keys, nh, ntg, nruns, rfirst, tfirst,
-                     hp->max_diagonals, hp->min_hits, hp->diag_hits, nj, top_runs);
+                     kmax, hp->min_hits, hp->diag_hits, nj, top_runs);
   ST_HIP(hipGetLastError());
   uint64_t tot_jobs = 0, tot_recs = 0;
   ST_HIP(scan_total(nj, joff, ntg, tmp, s, &tot_jobs));
@@ -528,10 +531,12 @@ int st_seed_hits_to_candidates_batch(st_ctx *ctx, const st_db *db, const st_post
   ST_HIP(hipMemcpyAsync(&nchain, nchain_d, 4, hipMemcpyDeviceToHost, s));
   uint64_t nc = 0;
   ST_HIP(scan_total(pass, pidx, st.nrecs, tmp, s, &nc));
-  if (chains_used) *chains_used += nchain;
   *n_out = nc;
   if (nc > cap || (nc && !out))
     return fail(ST_ERR_CAPACITY, f + ": output buffer too small (*n_out holds the size)");
+  // counted only by a call that delivers its candidates: a caller that
+  // retries with a larger buffer counts each chain once
+  if (chains_used) *chains_used += nchain;
   if (nc == 0) return ST_OK;
   hipLaunchKernelGGL(cand_emit_kernel, dim3(grid_of(st.nrecs)), dim3(256), 0, s, gres, st.recs, st.rq, pass, pidx,
                      st.nrecs, d_c, d_cq);

@@ -73,6 +73,28 @@ struct ServeUnit {
 };
 static_assert(sizeof(ServeUnit) == 64, "one 64-byte descriptor per unit");
 
+// Gate outcome of one target record or pair: the best single segment, its
+// diagonal, the best single-or-chain score and its target range
+// (Candidate, src/prefilter.rs:370-381).
+struct GateRes {
+  int32_t score;      // best single or chain
+  int32_t best;       // best single segment
+  int32_t best_diag;  // diagonal of the best single segment
+  uint32_t t_lo, t_hi;
+  uint8_t pass, chain, pad0, pad1;
+};
+
+// A job and a target record as the kernels read them: the C ABI's st_job
+// and st_target_rec (layouts checked where both are visible).
+struct JobDev {
+  uint32_t tid;
+  int32_t diag;
+  uint32_t lo, hi;
+};
+struct RecDev {
+  uint32_t tid, hits, j0, j1;
+};
+
 // Top-diagonal key: best ungapped score of a diagonal in the high word, and
 // 0xFFFFFFFF - (d + m - 1) in the low word, so a larger key means a higher
 // score, then a smaller diagonal d; 0 = no diagonal.

@@ -74,3 +74,20 @@ def test_synthetic_workloads_are_deterministic():
     _, pb, _, _ = O.blosum62()
     f = np.bincount(a.arena, minlength=20)[:20] / len(a.arena)
     assert np.abs(f - pb[:20] / pb[:20].sum()).max() < 0.01
+
+
+def test_candidates_wrapper_checks_per_query_lengths():
+    # the wrapper refuses per-query lists of the wrong length before the C
+    # entry point could read past them (no device is touched)
+    from sabertooth_amd import GapCosts
+    from sabertooth_amd.core import _seed_array, seed_hits_to_candidates_batch
+    qs = [object(), object()]
+    seeds = [[(0, 0, 1)], [(1, 0, 2)]]
+    for gaps, gates, sl in [([GapCosts(11, 1)], [30, 30], seeds), ([GapCosts(11, 1)] * 2, [30], seeds),
+                            ([GapCosts(11, 1)] * 2, [30, 30], seeds[:1])]:
+        with pytest.raises(ValueError):
+            seed_hits_to_candidates_batch(None, None, qs, sl, gaps, gates)
+    so, sd = _seed_array(seeds)
+    with pytest.raises(ValueError):
+        seed_hits_to_candidates_batch(None, None, qs, None, [GapCosts(11, 1)] * 2, [30, 30],
+                                      seed_arrays=(so, sd[:1]))

@@ -8,7 +8,15 @@ the reference, and every output of a seeded sample against the oracle
 (bit-exact).  The long configuration, too large for a CPU check of every
 pair, is checked on sampled pairs, on the closed form of a query aligned
 with itself, and on each checked query scored alone against its rows of the
-full batch."""
+full batch.  The two configurations the bench also times run here at full
+size as well: the batched uneven configuration (1,000 queries x 3,000
+candidates over 68 M residues: every SWG score against the AVX2 cascade,
+1,500 sampled pairs' full outputs against the oracle, the HIP-graph replay
+equal to the eager run) and the serving configuration (a 265 M-residue
+resident database, 2,000 requests of which about 10% are profiles: every
+served row equal to the plain batch, sampled pairs of both kinds against the
+oracle).  Execution-mode parity follows the reference's own check that every
+mode gives the same hits (tests/cli_db_equivalence.rs:495-528)."""
 import os
 
 import numpy as np
@@ -133,3 +141,88 @@ def test_long_pairs_full_size(ctx):
     for q in (1, 37):
         b1, r1 = _run(ctx, [rows[q]], arena, offs, [cands[q]], [(11, 1)], [w.gate])
         assert (r1 == res[off[q]:off[q + 1]]).all(), q
+        b1.close()
+    # the two shortest queries: every one of their 5,000 SWG scores against
+    # the AVX2 cascade (about 2.5e11 cells on the host)
+    short = sorted(range(nq), key=lambda q: len(w.queries[q]))[:2]
+    for q in short:
+        pq = np.full(nt, q, np.uint32)
+        pt = np.arange(nt, dtype=np.uint32)
+        sc, _, _ = O.avx2_score_pairs(rows, pq, pt, arena, offs, 11, 1, threads=THREADS)
+        got = res["swg"][off[q]:off[q] + nt]
+        assert (sc == got).all(), (q, int((sc != got).sum()))
+    b.close()
+
+
+def test_batched_uneven_full(ctx):
+    # the bench's configuration: 3 M pairs, 1.88e11 cells, gate 41
+    w = synth.batched_uneven(seed=1)
+    assert (len(w.queries), len(w.offsets) - 1, w.pairs()) == (1000, 370000, 3000000)
+    assert 6.5e7 < int(w.offsets[-1]) < 7.1e7 and w.gate == 41
+    rows = [st.seq_rows(q) for q in w.queries]
+    gaps = [(11, 1)] * len(rows)
+    gates = [w.gate] * len(rows)
+    b, res = _run(ctx, rows, w.arena, w.offsets, w.cand_lists, gaps, gates)
+    # planted homologs pass the gate, and some pairs were rescored in int32
+    # or not, as the scores require: every score is checked below
+    assert res["pass"].sum() > 100000
+    _check(rows, w.arena, w.offsets, w.cand_lists, gaps, gates, b, res, 1500, 5)
+    # the DP phase replayed from a HIP graph: the same results, twice, with
+    # an eager run between the replays
+    b.run(graph=True)
+    assert (b.results() == res).all()
+    b.run()
+    b.run(graph=True)
+    assert (b.results() == res).all()
+    b.close()
+
+
+def test_serving_full(ctx):
+    # the serving configuration: 1.4 M targets (265 M residues) resident,
+    # 2,000 requests with 3,000 candidates each, 10% profile queries (44/4,
+    # gate 30), the rest plain (11/1, size-adjusted gate 43)
+    w = synth.serving(seed=1)
+    n = len(w.rows)
+    assert n == 2000 and len(w.offsets) - 1 == 1400000 and 2.5e8 < int(w.offsets[-1]) < 2.8e8
+    prof = np.array(w.profile)
+    assert 150 < prof.sum() < 250 and set(w.gates) == {30, 43}
+    db = st.TargetDb(ctx, w.arena, w.offsets)
+    qps = [st.QueryProfile.from_rows(ctx, r) for r in w.rows]
+    gaps = [st.GapCosts(*g) for g in w.gaps]
+    b = st.Batch(ctx, db, qps, w.cands, gaps, w.gates)
+    b.run()
+    ref = b.results()
+    off = b.offsets.astype(np.int64)
+    b.close()
+    reqs = [(qps[k], w.cands[k], gaps[k], w.gates[k]) for k in range(n)]
+    # every request through the persistent serving kernel, arriving as a
+    # Poisson stream near saturation so that requests overlap on the device
+    rng = np.random.default_rng(8)
+    arr = np.cumsum(rng.exponential(1e3 / 12000.0, size=n))
+    srv = st.Server(ctx, db, workers=3, persistent=True)
+    outs, lat = srv.run_trace(reqs, arr)
+    stats = srv.stats()
+    srv.close()
+    assert stats["published"] > 0.9 * n
+    bad = [k for k in range(n) if not (outs[k] == ref[off[k]:off[k + 1]]).all()]
+    assert not bad, bad[:10]
+    # a quarter of the requests through the micro-batch slots
+    sub = list(range(0, n, 4))
+    srv = st.Server(ctx, db, workers=3, max_batch_queries=64, persistent=False)
+    outs2, _ = srv.run_trace([reqs[k] for k in sub], np.zeros(len(sub)))
+    srv.close()
+    bad = [k for k, o in zip(sub, outs2) if not (o == ref[off[k]:off[k + 1]]).all()]
+    assert not bad, bad[:10]
+    # sampled pairs of profile and plain requests against the oracle
+    picks = list(rng.choice(np.nonzero(prof)[0], 25)) + list(rng.choice(np.nonzero(~prof)[0], 25))
+    for k in picks:
+        j = int(rng.integers(len(w.cands[k])))
+        t = w.target(int(w.cands[k][j]))
+        o, e = w.gaps[k]
+        want = (O.gotoh_score(w.rows[k], t, o, e),) + O.ung_pair(w.rows[k], t, o, e, w.gates[k])
+        r = outs[k][j]
+        got = (int(r["swg"]), int(r["ung"]), int(r["best_diag"]), int(r["gate_score"]), bool(r["pass"]))
+        assert got == want, (k, j)
+    for q in qps:
+        q.close()
+    db.close()

@@ -377,7 +377,8 @@ def _brute_jobs(m, seeds, post, tlen, kmax, min_hits, diag_hits, window, span):
         for d, q in hits:
             c, lo, hi = runs.get(d, (0, q, q))
             runs[d] = (c + 1, min(lo, q), max(hi, q))
-        ranked = sorted(runs.items(), key=lambda kv: (-kv[1][0], kv[0]))[:kmax]
+        # the kmax best; jobs for at most 16 of them (src/prefilter.rs:1466)
+        ranked = sorted(runs.items(), key=lambda kv: (-kv[1][0], kv[0]))[:min(kmax, 16)]
         if ranked[0][1][0] < diag_hits:
             continue
         j0 = len(jobs)
@@ -393,7 +394,8 @@ def _brute_jobs(m, seeds, post, tlen, kmax, min_hits, diag_hits, window, span):
 
 
 @pytest.mark.parametrize("kmax,min_hits,diag_hits,window", [(4, 1, 1, 48), (1, 1, 1, 0), (16, 2, 2, 5),
-                                                            (3, 1, 2, 0), (2, 3, 1, 1)])
+                                                            (3, 1, 2, 0), (2, 3, 1, 1), (20, 1, 1, 0),
+                                                            (40, 1, 2, 3)])
 def test_seed_hit_jobs_against_brute_force(kmax, min_hits, diag_hits, window):
     rng = np.random.default_rng(kmax * 100 + min_hits * 10 + diag_hits)
     targets = [rng.integers(0, 4, size=int(rng.integers(3, 90)), dtype=np.uint8) for _ in range(60)]
