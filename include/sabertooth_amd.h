@@ -295,6 +295,24 @@ int st_score_and_gate(st_ctx *ctx, const st_qprof *qp, const st_db *db,
                       uint64_t *n_out, uint64_t *chains_used,
                       int32_t *job_scores);
 
+/* st_score_and_gate with no device allocation: no counterpart in the
+ * reference.  For callers that cannot allocate device memory (code under HIP
+ * graph capture, tight serving loops, harnesses that forbid hipMalloc).
+ * Same arguments, checks and results as st_score_and_gate; the staged jobs
+ * and records, scores, segment bounds, gate results and candidates live in
+ * one fixed device array of the library (about 12.5 MiB per GPU, held until
+ * the process ends), so a call takes at most 131072 jobs and 131072 records
+ * (ST_ERR_INVALID beyond).  Calls on one GPU share that array and are
+ * serialised by the library; overlapping calls from several processes do not
+ * share it. */
+int st_score_and_gate_noalloc(st_ctx *ctx, const st_qprof *qp, const st_db *db,
+                              const st_job *jobs, uint64_t njobs,
+                              const st_target_rec *recs, uint64_t nrecs,
+                              int32_t gap_open, int32_t gap_extend,
+                              int32_t min_diag_score, int regions, st_candidate *out,
+                              uint64_t *n_out, uint64_t *chains_used,
+                              int32_t *job_scores);
+
 /* The seed-hit stage and score_and_gate chained on the device for nq
  * queries in one call (evaluate_block_kadane's jobs scored and gated
  * without leaving the device, src/prefilter.rs:1412-1492, 1498-1660): query

@@ -141,6 +141,13 @@ int st_ctx_create(int device, st_ctx **out) {
     delete c;
     return fail(ST_ERR_NOMEM, "st_ctx_create: counter allocation failed");
   }
+  if (int rc = noalloc_init(device)) {
+    c->counter.release();
+    for (hipStream_t &s : c->slots) (void)hipStreamDestroy(s);
+    (void)hipStreamDestroy(c->stream);
+    delete c;
+    return rc;
+  }
   *out = c;
   return ST_OK;
 }
@@ -1200,28 +1207,44 @@ int st_ungapped_score_jobs(st_ctx *ctx, const st_qprof *qp, const st_db *db, con
   return ST_OK;
 }
 
-int st_score_and_gate(st_ctx *ctx, const st_qprof *qp, const st_db *db, const st_job *jobs, uint64_t njobs,
-                      const st_target_rec *recs, uint64_t nrecs, int32_t gap_open, int32_t gap_extend,
-                      int32_t min_diag_score, int regions, st_candidate *out, uint64_t *n_out,
-                      uint64_t *chains_used, int32_t *job_scores) {
+}  // extern "C"
+
+namespace st {
+
+int score_and_gate_check(const char *fn, st_ctx *ctx, const st_qprof *qp, const st_db *db, const st_job *jobs,
+                         uint64_t njobs, const st_target_rec *recs, uint64_t nrecs, const st_candidate *out,
+                         uint64_t *n_out) {
+  const std::string f(fn);
   if (!ctx || !qp || !db || !n_out || (njobs && !jobs) || (nrecs && (!recs || !out)))
-    return fail(ST_ERR_INVALID, "st_score_and_gate: null argument");
+    return fail(ST_ERR_INVALID, f + ": null argument");
+  if (qp->ctx != ctx || db->ctx != ctx) return fail(ST_ERR_INVALID, f + ": handle of another context");
   *n_out = 0;
   for (uint64_t r = 0; r < nrecs; r++)
     if (recs[r].j1 < recs[r].j0 || recs[r].j1 > njobs || recs[r].tid >= db->n_targets)
-      return fail(ST_ERR_INVALID, "st_score_and_gate: record out of range");
+      return fail(ST_ERR_INVALID, f + ": record out of range");
   // jobs must lie inside their diagonals (st_ungapped_score_jobs's checks)
   for (uint64_t k = 0; k < njobs; k++) {
     const st_job &j = jobs[k];
-    if (j.tid >= db->n_targets) return fail(ST_ERR_INVALID, "st_score_and_gate: target id out of range");
+    if (j.tid >= db->n_targets) return fail(ST_ERR_INVALID, f + ": target id out of range");
     if (j.hi > j.lo) {
       const int64_t n = db->hlen[j.tid];
       const int64_t a = std::max<int64_t>(-(int64_t)j.diag, 0);
       const int64_t e = std::min<int64_t>((int64_t)qp->m, n - j.diag);
-      if ((int64_t)j.lo < a || (int64_t)j.hi > e)
-        return fail(ST_ERR_INVALID, "st_score_and_gate: job window outside its diagonal");
+      if ((int64_t)j.lo < a || (int64_t)j.hi > e) return fail(ST_ERR_INVALID, f + ": job window outside its diagonal");
     }
   }
+  return ST_OK;
+}
+
+}  // namespace st
+
+extern "C" {
+
+int st_score_and_gate(st_ctx *ctx, const st_qprof *qp, const st_db *db, const st_job *jobs, uint64_t njobs,
+                      const st_target_rec *recs, uint64_t nrecs, int32_t gap_open, int32_t gap_extend,
+                      int32_t min_diag_score, int regions, st_candidate *out, uint64_t *n_out,
+                      uint64_t *chains_used, int32_t *job_scores) {
+  if (int rc = score_and_gate_check("st_score_and_gate", ctx, qp, db, jobs, njobs, recs, nrecs, out, n_out)) return rc;
   if (njobs == 0 && nrecs == 0) return ST_OK;
   std::lock_guard<std::mutex> lk(ctx->mu);
   ST_HIP(hipSetDevice(ctx->device));

@@ -144,6 +144,16 @@ struct st_qprof {
 
 namespace st {
 
+// st_score_and_gate's argument checks (handles, record and job bounds),
+// shared by its variants; fn names the entry point in the message.
+int score_and_gate_check(const char *fn, st_ctx *ctx, const st_qprof *qp, const st_db *db, const st_job *jobs,
+                         uint64_t njobs, const st_target_rec *recs, uint64_t nrecs, const st_candidate *out,
+                         uint64_t *n_out);
+
+// Resolves, once per device and with that device current, the address of
+// the fixed device scratch that st_score_and_gate_noalloc uses.
+int noalloc_init(int device);
+
 int dp_occupancy(int kind, int R, bool ung, size_t lds_bytes, bool end = false);
 
 // Precision class of a query under given gaps.
