@@ -49,6 +49,7 @@ def parse():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-serving", action="store_true", help="skip the serving leg")
     ap.add_argument("--no-prefilter", action="store_true", help="skip the seed-hit stage leg")
+    ap.add_argument("--no-configs", action="store_true", help="skip timing the other configurations")
     ap.add_argument("--serving-requests", type=int, default=2000)
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a plain device copy of 256 MiB (read 256 MiB, write 256 MiB) three "
@@ -66,6 +67,99 @@ def copy_calibration():
         torch.add(a, 1, out=b)  # one elementwise kernel: n bytes read, n bytes written
     torch.cuda.synchronize()
     print(json.dumps({"copy_calibration_bytes": n, "dispatches": 3, "check": int(b[-1].item())}), flush=True)
+
+
+def host_cpu():
+    """The host the CPU baselines run on: cores, model, clock."""
+    info = {"nproc": os.cpu_count(), "affinity_cores": len(os.sched_getaffinity(0))}
+    try:
+        models, mhz, flags = set(), [], ""
+        for line in open("/proc/cpuinfo"):
+            k, _, v = line.partition(":")
+            k, v = k.strip(), v.strip()
+            if k == "model name":
+                models.add(v)
+            elif k == "cpu MHz":
+                mhz.append(float(v))
+            elif k == "flags" and not flags:
+                flags = v
+        info["model"] = "; ".join(sorted(models))
+        if mhz:
+            info["mhz_now_mean"] = round(float(np.mean(mhz)), 1)
+        info["avx2"], info["avx512bw"] = "avx2" in flags.split(), "avx512bw" in flags.split()
+    except OSError:
+        pass
+    try:
+        info["mhz_max"] = int(open("/sys/devices/system/cpu/cpu0/cpufreq/cpuinfo_max_freq").read()) / 1000.0
+    except (OSError, ValueError):
+        pass
+    return info
+
+
+def _launch_fracs(batch, ms_list):
+    """Per-launch achieved ops and fractions of the matching peak (packed
+    int16: 78.6 Tops/s; int32 kernels: 39.3)."""
+    kinds = {0: "Pk16", 1: "I32F", 2: "I32N"}
+    out = []
+    for L, ms in zip(batch.launches(), ms_list):
+        peak = INT16_PACKED_PEAK_TOPS if int(L["kind"]) == 0 else INT16_PACKED_PEAK_TOPS / 2
+        ach = float(L["useful_cells"]) * OPS_PER_CELL / (ms / 1e3) / 1e12 if ms > 0 else 0.0
+        out.append({"kernel": f"dp_kernel<{kinds[int(L['kind'])]}, {int(L['rows_per_lane'])}, true>",
+                    "useful_cells": int(L["useful_cells"]), "ms": round(float(ms), 3),
+                    "achieved": round(ach, 2), "frac": round(ach / peak, 4)})
+    return out
+
+
+def configs_leg(O, st, synth):
+    """The other benchmark configurations timed once each at full size
+    (their parity is in tests/test_configs_gpu.py): plain all-pairs, PSSM
+    queries from the builder, long pairs.  Step = the batch's DP, rescoring
+    and gate with inputs resident."""
+    out = {}
+    ctx = st.Context(0)
+    specs = [("plain_allpairs", lambda: synth.plain_allpairs(seed=1), None, (11, 1), 3),
+             ("pssm_search", lambda: synth.pssm_search(seed=1), "pssm", (44, 4), 2),
+             ("long_pairs", lambda: synth.long_pairs(seed=1), None, (11, 1), 1)]
+    for name, make, kind, g, reps in specs:
+        t0 = time.time()
+        w = make()
+        rows = w.meta["rows"] if kind == "pssm" else [st.seq_rows(q) for q in w.queries]
+        gate = 30 if kind == "pssm" else w.gate
+        db = st.TargetDb(ctx, w.arena, w.offsets)
+        qps = [st.QueryProfile.from_rows(ctx, r) for r in rows]
+        b = st.Batch(ctx, db, qps, w.cand_lists, [st.GapCosts(*g)] * len(qps), [gate] * len(qps))
+        cells = b.cells()
+        exec_cells, _, _ = b.stats()
+        if reps > 1:
+            b.run()
+        ts = time.perf_counter()
+        lms = []
+        for _ in range(reps):
+            b.run()
+            lms.append(b.launches()["ms"].copy())
+        t = (time.perf_counter() - ts) / reps
+        dpms, rms, gms, nres = b.timing()
+        per = _launch_fracs(b, np.mean(np.stack(lms), axis=0))
+        dom = max(per, key=lambda x: x["useful_cells"])
+        out[name] = {"queries": len(qps), "targets": len(w.offsets) - 1, "residues": int(w.offsets[-1]),
+                     "pairs": b.npairs, "cells": cells, "gaps": f"{g[0]}/{g[1]}", "gate": int(gate),
+                     "ms": round(1e3 * t, 3), "gcups": round(cells / t / 1e9, 1),
+                     "queries_per_s": round(len(qps) / t, 1), "rescored_pairs": nres,
+                     "executed_over_useful_cells": round(exec_cells / cells, 4),
+                     "dp_phase": {"ms": round(dpms, 3),
+                                  "frac": round(cells * OPS_PER_CELL / (dpms / 1e3) / 1e12 / INT16_PACKED_PEAK_TOPS,
+                                                4)},
+                     "dominant_launch": dom, "launches": per}
+        log(f"config {name}: {b.npairs} pairs, {cells / 1e9:.1f} G cells in {1e3 * t:.1f} ms = "
+            f"{cells / t / 1e9:.0f} GCUPS; dominant {dom['kernel']} {100 * dom['frac']:.1f}% of peak "
+            f"(workload built in {time.time() - t0:.1f}s)")
+        b.close()
+        for q in qps:
+            q.close()
+        db.close()
+        del w
+    ctx.close()
+    return out
 
 
 def _pct(x, q):
@@ -186,10 +280,12 @@ def prefilter_leg(O, st, ctx, db, w, qps, gate, n_queries=200, k=4, parity_queri
     st_seed_hits_to_jobs timed per query (wall: seed upload, expansion, one
     radix sort, runs, top diagonals, read-back of jobs and records), then
     every query in one call, then seeds -> candidates in one call (the jobs
-    scored and gated on the device, only candidates read back).
-    Algorithmic bytes per hit: its 8-byte posting read once."""
+    scored and gated on the device, only candidates read back).  The job
+    scan's HBM rate comes from its own kernel time (HIP events) and its
+    algorithmic bytes; the reference's AVX2 ungapped stage runs on the same
+    jobs on the host cores beside it."""
     from sabertooth_amd.core import (Postings, seed_hits_to_candidates_batch, seed_hits_to_jobs,
-                                     seed_hits_to_jobs_batch, _seed_array)
+                                     seed_hits_to_jobs_batch, seed_stage_timing, _seed_array)
     t0 = time.time()
     ar = w.arena.astype(np.int64)
     offs = w.offsets.astype(np.int64)
@@ -242,7 +338,10 @@ def prefilter_leg(O, st, ctx, db, w, qps, gate, n_queries=200, k=4, parity_queri
     outb = seed_hits_to_jobs_batch(ctx, db, dp, ms, seeds_of, span=k)
     t = time.perf_counter() - ts
     assert sum(len(j) for j, _ in outb) == nj and sum(len(r) for _, r in outb) == nr
-    del outb
+    # the job scan's algorithmic bytes: each job's window of target residues
+    # once, plus its 16-byte job read, 4-byte score and 8-byte bounds written
+    win = int(sum(int((j["hi"].astype(np.int64) - j["lo"].astype(np.int64)).clip(0).sum()) for j, _ in outb))
+    scan_bytes = win + 28 * nj
     gaps = [st.GapCosts(w.gap_open, w.gap_extend)] * n_queries
     gates = [gate] * n_queries
     sa = _seed_array(seeds_of)
@@ -262,6 +361,26 @@ def prefilter_leg(O, st, ctx, db, w, qps, gate, n_queries=200, k=4, parity_queri
         n1 += len(c1[0])
     t1c = time.perf_counter() - ts
     assert n1 == nc
+    # the device kernels of one more all-query chained call, by HIP events
+    seed_hits_to_candidates_batch(db, dp, qps[:n_queries], None, gaps, gates, span=k, seed_arrays=sa)
+    kt = seed_stage_timing(dp)
+    scan_ms, gate_ms = kt[1], kt[2]
+    log(f"prefilter: job scan kernel {scan_ms:.3f} ms for {nj} jobs ({win} window residues): "
+        f"{scan_bytes / scan_ms / 1e6:.0f} GB/s; gate {gate_ms:.3f} ms; device window {kt[0]:.3f} ms")
+    # the reference's AVX2 ungapped stage (score_batch + kadane_avx2 + chain
+    # and gate per EV_CHUNK) on the same jobs, all host cores
+    ncores = len(os.sched_getaffinity(0))
+    rows_pf = [np.asarray(qp.rows()) for qp in qps[:n_queries]]
+    c_cands, c_chains, _, c_secs = O.avx2_score_and_gate(rows_pf, [j for j, _ in outb], [r for _, r in outb], w.arena,
+                                                         w.offsets, [w.gap_open] * n_queries,
+                                                         [w.gap_extend] * n_queries, [gate] * n_queries, False,
+                                                         threads=ncores)
+    avx_mism = sum(int(not (len(a) == len(b) and (a == b).all())) for a, b in zip(c_cands, cands))
+    avx_chain_ok = sum(c_chains) == chains
+    log(f"prefilter: cpu avx2 ungapped stage ({ncores} threads): {c_secs:.3f} s for {nj} jobs "
+        f"({nj / c_secs / 1e6:.1f} M jobs/s, {nj / c_secs / ncores / 1e6:.2f} M/s/core); "
+        f"{n_queries - avx_mism}/{n_queries} queries' candidates equal the device's, chains equal: {avx_chain_ok}")
+    del outb
     mism = cmism = 0
     for qi in range(parity_queries):
         gj, gr = seed_hits_to_jobs(ctx, db, dp, len(w.queries[qi]), seeds_of[qi], span=k)
@@ -273,13 +392,25 @@ def prefilter_leg(O, st, ctx, db, w, qps, gate, n_queries=200, k=4, parity_queri
     tot = int(sum(hits))
     out = {"queries": n_queries, "k": k, "postings": int(len(post)), "hits": tot, "jobs": nj, "targets": nr,
            "ms": round(1e3 * t, 3), "hits_per_s": round(tot / t, 1), "one_query_per_call_ms": round(1e3 * t1, 3),
-           "hbm": {"algorithmic_bytes_per_hit": 8, "achieved_GBps": round(8 * tot / t / 1e9, 1),
-                   "frac_of_8TBps": round(8 * tot / t / 8e12, 4)},
            "parity_queries_vs_oracle": parity_queries, "mismatched_queries": mism,
            "to_candidates": {"ms": round(1e3 * tc, 3), "hits_per_s": round(tot / tc, 1), "candidates": nc,
                              "one_query_per_call_ms": round(1e3 * t1c, 3),
                              "chains_used": int(chains), "gate": int(gate),
-                             "mismatched_queries_vs_score_and_gate": cmism}}
+                             "mismatched_queries_vs_score_and_gate": cmism},
+           "job_scan": {"kernel": "ung_jobs_kernel<true,512>", "jobs": nj, "window_residues": win,
+                        "algorithmic_bytes": scan_bytes, "bytes_per_job": "(hi - lo) + 16 job + 4 score + 8 bounds",
+                        "ms": round(scan_ms, 4), "achieved_GBps": round(scan_bytes / scan_ms / 1e6, 1),
+                        "frac_of_8TBps": round(scan_bytes / scan_ms / 1e6 / 8000.0, 4),
+                        "gate_ms": round(gate_ms, 4), "device_window_ms": round(kt[0], 4),
+                        "timing": "HIP events on the context stream around the kernel"},
+           "cpu_avx2_ungapped": {"seconds": round(c_secs, 4), "threads": ncores,
+                                 "jobs_per_s": round(nj / c_secs, 1),
+                                 "jobs_per_s_per_core": round(nj / c_secs / ncores, 1),
+                                 "kind": "port",
+                                 "what": "score_batch + kadane_avx2 (32-lane biased u8) + chain/gate per EV_CHUNK, "
+                                         "the same jobs and records",
+                                 "mismatched_queries_vs_device": avx_mism, "chains_equal": bool(avx_chain_ok),
+                                 "device_scan_plus_gate_speedup": round(c_secs * 1e3 / (scan_ms + gate_ms), 1)}}
     log(f"prefilter: {n_queries} queries, {tot} seed hits -> {nj} jobs / {nr} targets in {1e3 * t:.1f} ms in one "
         f"call ({tot / t / 1e9:.2f} G hits/s; {1e3 * t1:.1f} ms one query per call); "
         f"{parity_queries - mism}/{parity_queries} queries equal the oracle")
@@ -391,8 +522,11 @@ def main():
     launches = batch.launches()
     launch_ms = np.mean(np.stack(lms), axis=0)
 
-    # parity on a seeded sample of pairs (full outputs vs the oracle)
+    # parity on a seeded sample of pairs (full outputs vs the oracle); the
+    # read-back of every result row is outside the timed steps
+    ts = time.perf_counter()
     res = batch.results()
+    readback_ms = 1e3 * (time.perf_counter() - ts)
     rng = np.random.default_rng(7)
     offs = batch.offsets.astype(np.int64)
     sample = rng.choice(len(res), size=min(a.parity_pairs, len(res)), replace=False)
@@ -482,8 +616,12 @@ def main():
                "sample": f"all {len(pair_q)} pairs of the workload, SWG score only (u8->i16->scalar AVX2 cascade)",
                "per_core_gcups": round(c1 / s1 / 1e9, 2),
                "per_core_sample": f"{n1} pairs of the first {min(len(qps), 40)} queries on 1 thread",
-               "seconds": round(secs, 3)}
+               "seconds": round(secs, 3), "host": host_cpu(),
+               "gpu_over_cpu": {"all_cores": round(gcups / (cells / secs / 1e9), 1),
+                                "per_core": round(gcups / (c1 / s1 / 1e9), 1)}}
         log(f"cpu avx2 1 thread: {c1 / s1 / 1e9:.2f} GCUPS")
+
+    configs = None if a.no_configs else configs_leg(O, st, synth)
 
     serving = None
     if not a.no_serving:
@@ -497,10 +635,14 @@ def main():
                    "residues": int(w.offsets[-1]), "candidates_per_query": a.cands, "pairs": batch.npairs,
                    "cells": cells, "gaps": "11/1", "gate": w.gate},
         "queries_per_s": round(qps_rate, 1),
+        "results_readback": {"ms": round(readback_ms, 3), "bytes": int(batch.npairs) * 20,
+                             "note": "value has the results resident in HBM; reading every result row back "
+                                     "to the host takes this much more per step"},
         "parity": {"sampled_pairs_all_outputs_vs_oracle": int(len(sample)), "mismatches": int(mism),
                    "all_pairs_swg_vs_cpu_avx2_mismatches": full_swg},
         "roofline": roof, "cpu_baseline": cpu, "graph_replay": graph, "alignment_spans": spans,
         "prefilter_seed_hits": prefilter,
+        "configs": configs,
         "serving": serving,
     }
     print(json.dumps(out), flush=True)

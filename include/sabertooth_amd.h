@@ -332,6 +332,13 @@ int st_seed_hits_to_candidates_batch(st_ctx *ctx, const st_db *db, const st_post
                                      int regions, st_candidate *out, uint64_t cap, uint64_t *n_out,
                                      uint64_t *cand_off, uint64_t *chains_used);
 
+/* Device times (ms, HIP events on the context's stream) of the last
+ * st_seed_hits_to_candidates_batch call that reached the scoring step on
+ * these postings: ms[0] its device window (the seed upload to the pass
+ * scan), ms[1] the ungapped job scan, ms[2] the gate; all 0 when none did.
+ * No counterpart in the reference (a measurement hook). */
+int st_seed_stage_timing(const st_postings *post, double *ms);
+
 /* size_adjusted_gate (src/prefilter.rs:246-257). */
 int32_t st_size_adjusted_gate(int32_t gate, uint64_t db_residues);
 

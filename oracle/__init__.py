@@ -287,6 +287,9 @@ def avx2_lib() -> C.CDLL:
         L.avx2_score_pairs.restype = C.c_double
         L.avx2_score_pairs.argtypes = [C.c_uint32, P, P, C.c_uint64, P, P, P, P, P, C.c_int32, C.c_int32,
                                        C.c_int, P, P]
+        L.avx2_score_and_gate.restype = C.c_double
+        L.avx2_score_and_gate.argtypes = [C.c_uint32, P, P, P, P, P, P, P, P, P, P, P, C.c_int, C.c_int, P, P, P,
+                                          P]
         _AVX = L
     return _AVX
 
@@ -306,3 +309,36 @@ def avx2_score_pairs(rows_list, pair_q, pair_t, arena, offsets, open_, ext, thre
     secs = avx2_lib().avx2_score_pairs(len(rows_c), ptrs, _p(qlen), len(pq), _p(pq), _p(pt), _p(arena),
                                        _p(offsets), _p(lens), open_, ext, threads, _p(sc), _p(kn))
     return sc, kn, secs
+
+
+def avx2_score_and_gate(rows_list, job_lists, rec_lists, arena, offsets, opens, exts, gates, regions=False,
+                        threads=1, want_scores=False):
+    """([candidates per query], [chains per query], job scores or None,
+    seconds): the reference's AVX2 ungapped stage (32-lane biased-u8 Kadane
+    batches over EV_CHUNK chunks, then chain and gate) per query."""
+    nq = len(rows_list)
+    rows_c = [np.ascontiguousarray(r, dtype=np.int32).reshape(-1, NPROF) for r in rows_list]
+    ptrs = (C.c_void_p * max(nq, 1))(*[r.ctypes.data for r in rows_c])
+    qlen = np.array([len(r) for r in rows_c], dtype=np.uint32)
+    jo = np.zeros(nq + 1, np.uint64)
+    jo[1:] = np.cumsum([len(j) for j in job_lists])
+    ro = np.zeros(nq + 1, np.uint64)
+    ro[1:] = np.cumsum([len(r) for r in rec_lists])
+    jobs = np.ascontiguousarray(np.concatenate(job_lists) if nq else np.zeros(0, JOB_DT), dtype=JOB_DT)
+    recs = np.ascontiguousarray(np.concatenate(rec_lists) if nq else np.zeros(0, REC_DT), dtype=REC_DT)
+    arena = _seq(arena)
+    offsets = np.ascontiguousarray(offsets, dtype=np.uint64)
+    op = np.ascontiguousarray(opens, dtype=np.int32)
+    ex = np.ascontiguousarray(exts, dtype=np.int32)
+    gt = np.ascontiguousarray(gates, dtype=np.int32)
+    ncand = np.zeros(max(nq, 1), np.uint64)
+    chains = np.zeros(max(nq, 1), np.uint64)
+    cands = np.zeros(max(len(recs), 1), dtype=CAND_DT)
+    js = np.zeros(max(len(jobs), 1), np.int32) if want_scores else None
+    secs = avx2_lib().avx2_score_and_gate(nq, ptrs, _p(qlen), _p(jo), _p(jobs), _p(ro), _p(recs), _p(arena),
+                                          _p(offsets), _p(op), _p(ex), _p(gt), 1 if regions else 0, threads,
+                                          _p(ncand), _p(cands), _p(chains), _p(js) if js is not None else None)
+    if secs < 0:
+        raise ValueError("avx2_score_and_gate: records must index their query's jobs in order")
+    out = [cands[int(ro[q]):int(ro[q]) + int(ncand[q])].copy() for q in range(nq)]
+    return out, [int(c) for c in chains[:nq]], (js[: len(jobs)] if js is not None else None), secs

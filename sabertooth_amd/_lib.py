@@ -95,6 +95,7 @@ _SIGS = [
                                     C.c_int32, C.c_int, _P, _P, _P, _P]),
     ("st_score_and_gate_noalloc", C.c_int, [_P, _P, _P, _P, C.c_uint64, _P, C.c_uint64, C.c_int32, C.c_int32,
                                             C.c_int32, C.c_int, _P, _P, _P, _P]),
+    ("st_seed_stage_timing", C.c_int, [_P, _P]),
     ("st_size_adjusted_gate", C.c_int32, [C.c_int32, C.c_uint64]),
     ("st_batch_create", C.c_int, [_P, _P, _P, C.c_uint64, _P, _P, _P, _P, _P, C.POINTER(_P)]),
     ("st_batch_run", C.c_int, [_P]),

@@ -388,6 +388,14 @@ def seed_hits_to_candidates_batch(db: TargetDb, postings: Postings, queries: Seq
     return [out[int(co[q]):int(co[q + 1])] for q in range(nq)], chains.value
 
 
+def seed_stage_timing(postings: Postings):
+    """(device window, job scan, gate) in ms of the last seeds-to-candidates
+    call on these postings (HIP events)."""
+    ms = np.zeros(3, np.float64)
+    check(lib().st_seed_stage_timing(postings.handle, ptr(ms)))
+    return float(ms[0]), float(ms[1]), float(ms[2])
+
+
 def score_and_gate(qp: QueryProfile, db: TargetDb, jobs, recs, gaps: GapCosts,
                    min_diag_score: int, regions: bool = False):
     """(candidates, chains_used, job_scores) as score_and_gate produces them."""

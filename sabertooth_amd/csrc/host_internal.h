@@ -129,6 +129,14 @@ struct st_postings {
   st::DevBuf<uint64_t> post;  // (target << 32) | position
   uint64_t n = 0;
   mutable st::DevBuf<uint8_t> scratch[32];
+  // events of the last chained seeds-to-candidates call: its start, the job
+  // scan's start and end, the gate's end, its last kernel's end
+  hipEvent_t ev[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};
+  mutable bool timed = false;
+  ~st_postings() {
+    for (hipEvent_t &e : ev)
+      if (e) (void)hipEventDestroy(e);
+  }
 };
 
 struct st_qprof {

@@ -269,6 +269,11 @@ int st_postings_create(st_ctx *ctx, const uint64_t *postings, uint64_t n, st_pos
     delete p;
     return fail(ST_ERR_NOMEM, "st_postings_create: device allocation failed");
   }
+  for (hipEvent_t &ev : p->ev)
+    if (hipEventCreate(&ev) != hipSuccess) {
+      delete p;
+      return fail(ST_ERR_DEVICE, "st_postings_create: event creation failed");
+    }
   *out = p;
   return ST_OK;
 }
@@ -490,6 +495,9 @@ int st_seed_hits_to_candidates_batch(st_ctx *ctx, const st_db *db, const st_post
   }
   if (int rc = seed_stage_check(f, ctx, db, post, nq, m.data(), seed_off, seeds, hp)) return rc;
   std::lock_guard<std::mutex> lk(ctx->mu);
+  post->timed = false;
+  ST_HIP(hipSetDevice(ctx->device));
+  ST_HIP(hipEventRecord(post->ev[0], ctx->stream));
   SeedStage st;
   if (int rc = seed_stage(f, ctx, db, post, nq, m.data(), seed_off, seeds, hp, st, true)) return rc;
   if (st.nrecs == 0) return ST_OK;
@@ -515,10 +523,13 @@ int st_seed_hits_to_candidates_batch(st_ctx *ctx, const st_db *db, const st_post
   ST_HIP(hipMemcpyAsync(prm, hprm.data(), 3 * nq * 4, hipMemcpyHostToDevice, s));
   // score every job against its query, then gate every record; the gate
   // writes the pass and chain flags the compaction scans
+  ST_HIP(hipEventRecord(post->ev[1], s));
   ST_HIP(launch_ung_jobs_multi((const JobDev *)st.jobs, st.njobs, jq, d_q, stage_m, db->arena.p, db->off.p, scores,
                                segs, s));
+  ST_HIP(hipEventRecord(post->ev[2], s));
   ST_HIP(launch_gate_recs_multi((const RecDev *)st.recs, st.nrecs, (const JobDev *)st.jobs, scores, segs, st.rq, prm,
                                 prm + nq, prm + 2 * nq, regions, gres, s, 0, 0, 0, pass, chain));
+  ST_HIP(hipEventRecord(post->ev[3], s));
   // the passing records, compacted in (query, target) order
   // the chain count is a sum; its copy completes with the pass scan's
   // synchronisation
@@ -531,6 +542,8 @@ int st_seed_hits_to_candidates_batch(st_ctx *ctx, const st_db *db, const st_post
   ST_HIP(hipMemcpyAsync(&nchain, nchain_d, 4, hipMemcpyDeviceToHost, s));
   uint64_t nc = 0;
   ST_HIP(scan_total(pass, pidx, st.nrecs, tmp, s, &nc));
+  ST_HIP(hipEventRecord(post->ev[4], s));
+  post->timed = true;
   *n_out = nc;
   if (nc > cap || (nc && !out))
     return fail(ST_ERR_CAPACITY, f + ": output buffer too small (*n_out holds the size)");
@@ -547,6 +560,23 @@ int st_seed_hits_to_candidates_batch(st_ctx *ctx, const st_db *db, const st_post
   ST_HIP(hipStreamSynchronize(s));
   for (uint32_t q : cq) cand_off[q + 1]++;
   for (uint64_t q = 0; q < nq; q++) cand_off[q + 1] += cand_off[q];
+  return ST_OK;
+}
+
+int st_seed_stage_timing(const st_postings *post, double *ms) {
+  if (!post || !ms) return fail(ST_ERR_INVALID, "st_seed_stage_timing: null argument");
+  ms[0] = ms[1] = ms[2] = 0.0;
+  std::lock_guard<std::mutex> lk(post->ctx->mu);
+  if (!post->timed) return ST_OK;
+  ST_HIP(hipSetDevice(post->ctx->device));
+  ST_HIP(hipEventSynchronize(post->ev[4]));
+  float a = 0, b = 0, c = 0;
+  ST_HIP(hipEventElapsedTime(&a, post->ev[0], post->ev[4]));
+  ST_HIP(hipEventElapsedTime(&b, post->ev[1], post->ev[2]));
+  ST_HIP(hipEventElapsedTime(&c, post->ev[2], post->ev[3]));
+  ms[0] = a;
+  ms[1] = b;
+  ms[2] = c;
   return ST_OK;
 }
 
