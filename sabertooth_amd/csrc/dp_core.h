@@ -121,8 +121,12 @@ __device__ __forceinline__ uint32_t shr1u(uint32_t v) {
   return (uint32_t)__builtin_amdgcn_mov_dpp((int)v, 0x111, 0xf, 0xf, true);
 }
 
-// lane i <- lane i-1 across the whole wave (groups of 32 or 64 lanes)
-__device__ __forceinline__ uint32_t shr1w(uint32_t v) { return (uint32_t)__shfl_up((int)v, 1); }
+// lane i <- lane i-1 across the whole wave (groups of 32 or 64 lanes): the
+// DPP wave shift (wave_shr:1), a VALU move with no LDS round trip; lane 0
+// reads 0 and is overwritten by the caller (a group start)
+__device__ __forceinline__ uint32_t shr1w(uint32_t v) {
+  return (uint32_t)__builtin_amdgcn_mov_dpp((int)v, 0x138, 0xf, 0xf, true);
+}
 
 template <class P, bool WIDE>
 __device__ __forceinline__ typename P::T shr1(typename P::T v) {

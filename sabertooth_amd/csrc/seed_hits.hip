@@ -15,7 +15,9 @@
 // one thread per target keeps its top diagonals (hits descending, diagonal
 // ascending, the order top_diagonals' insertion keeps) and emits its jobs.
 #include <hip/hip_runtime.h>
-#include <hipcub/hipcub.hpp>
+#include <rocprim/device/device_radix_sort.hpp>
+#include <rocprim/device/device_reduce.hpp>
+#include <rocprim/device/device_scan.hpp>
 
 #include <algorithm>
 #include <string>
@@ -237,10 +239,11 @@ unsigned grid_of(uint64_t n, unsigned b = 256) { return (unsigned)((n + b - 1) /
 hipError_t scan_total(const uint32_t *in, uint32_t *out, uint64_t n, DevBuf<uint8_t> &tmp, hipStream_t s,
                       uint64_t *total) {
   size_t tb = 0;
-  hipError_t e = hipcub::DeviceScan::ExclusiveSum(nullptr, tb, in, out, (int)n, s);
+  hipError_t e = rocprim::exclusive_scan(nullptr, tb, in, out, 0u, (size_t)n, rocprim::plus<uint32_t>(), s);
   if (e != hipSuccess) return e;
   if (tmp.n < tb && (e = tmp.alloc(tb)) != hipSuccess) return e;
-  if ((e = hipcub::DeviceScan::ExclusiveSum(tmp.p, tb, in, out, (int)n, s)) != hipSuccess) return e;
+  if ((e = rocprim::exclusive_scan(tmp.p, tb, in, out, 0u, (size_t)n, rocprim::plus<uint32_t>(), s)) != hipSuccess)
+    return e;
   uint32_t a = 0, b = 0;
   if ((e = hipMemcpyAsync(&a, out + n - 1, 4, hipMemcpyDeviceToHost, s)) != hipSuccess) return e;
   if ((e = hipMemcpyAsync(&b, in + n - 1, 4, hipMemcpyDeviceToHost, s)) != hipSuccess) return e;
@@ -333,7 +336,7 @@ static int seed_stage(const std::string &f, st_ctx *ctx, const st_db *db, const 
   }
   const uint64_t nh = off[nseeds];
   if (nh == 0) return ST_OK;
-  // the device sort and scans count items in int
+  // the sort, scans and keys index items in 32 bits
   if (nh > 0x7fffffffull) return fail(ST_ERR_INVALID, f + ": more than 2^31 - 1 seed hits");
   const uint32_t maxlen = db->max_len;
   Layout lay;
@@ -371,9 +374,9 @@ static int seed_stage(const std::string &f, st_ctx *ctx, const st_db *db, const 
   // chosen run's smallest and largest position by scanning it)
   size_t tb = 0;
   const int begin_bit = (int)lay.qbits;
-  ST_HIP(hipcub::DeviceRadixSort::SortKeys(nullptr, tb, k0, k1, (int)nh, begin_bit, (int)end_bit, s));
+  ST_HIP(rocprim::radix_sort_keys(nullptr, tb, k0, k1, (uint32_t)nh, (unsigned)begin_bit, (unsigned)end_bit, s));
   if (!grow<uint8_t>(tmp, tb)) return fail(ST_ERR_NOMEM, f + ": device allocation failed");
-  ST_HIP(hipcub::DeviceRadixSort::SortKeys(tmp.p, tb, k0, k1, (int)nh, begin_bit, (int)end_bit, s));
+  ST_HIP(rocprim::radix_sort_keys(tmp.p, tb, k0, k1, (uint32_t)nh, (unsigned)begin_bit, (unsigned)end_bit, s));
   const uint64_t *keys = k1;
   // runs
   hipLaunchKernelGGL(run_flags_kernel, dim3(grid_of(nh)), dim3(256), 0, s, keys, nh, lay.qbits, flag);
@@ -535,9 +538,9 @@ int st_seed_hits_to_candidates_batch(st_ctx *ctx, const st_db *db, const st_post
   // synchronisation
   DevBuf<uint8_t> &tmp = sc[18];
   size_t tb = 0;
-  ST_HIP(hipcub::DeviceReduce::Sum(nullptr, tb, chain, nchain_d, (int)st.nrecs, s));
+  ST_HIP(rocprim::reduce(nullptr, tb, chain, nchain_d, 0u, (size_t)st.nrecs, rocprim::plus<uint32_t>(), s));
   if (!grow<uint8_t>(tmp, tb)) return fail(ST_ERR_NOMEM, f + ": device allocation failed");
-  ST_HIP(hipcub::DeviceReduce::Sum(tmp.p, tb, chain, nchain_d, (int)st.nrecs, s));
+  ST_HIP(rocprim::reduce(tmp.p, tb, chain, nchain_d, 0u, (size_t)st.nrecs, rocprim::plus<uint32_t>(), s));
   uint32_t nchain = 0;
   ST_HIP(hipMemcpyAsync(&nchain, nchain_d, 4, hipMemcpyDeviceToHost, s));
   uint64_t nc = 0;
