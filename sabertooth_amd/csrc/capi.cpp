@@ -439,11 +439,15 @@ bool plan_query(Plan &pl, std::vector<Unit> &pending, uint32_t qi, const st_qpro
   for (size_t k = 0; k < count; k++) csum[k + 1] = csum[k] + (key[k] >> 32);
   const int per = kind == kPk16 ? 2 : 1;
   const uint64_t m = qp->m;
-  const int fr = forced_rows(), fg = forced_groups();
+  int fr = forced_rows(), fg = forced_groups();
   double shape_ps[3][7];
   shape_table(shape_ps);
+  // the final pass of a packed int16 unit may be lower than the others (4,
+  // 8 or 12 rows per lane): the kernels with every height take it; the
+  // serving kernel (one height) and forced shapes keep it uniform
+  const bool mixed_last = kind == kPk16 && !(lim && lim->only_R) && !fr;
   for (size_t s = 0; s < count;) {
-    int bG = 16, bR = kind == kI32N ? 8 : 16;
+    int bG = 16, bR = kind == kI32N ? 8 : 16, bRL = bR;
     double bc = 1e300;
     for (int ri = 0; ri < 3; ri++) {
       const int R = kShapeRows[ri];
@@ -465,19 +469,43 @@ bool plan_query(Plan &pl, std::vector<Unit> &pending, uint32_t qi, const st_qpro
         if (lim && ((lim->only_R && R != lim->only_R) || GR > (uint64_t)lim->max_GR ||
                     (passes > 1 && (uint64_t)(64 / G) * np > lim->max_half)))
           continue;
-        const double exec = 64.0 * R * per * passes * (np + G - 1);
+        // the lowest final-pass height that covers the query's last rows
+        int RL = R;
+        if (mixed_last) {
+          const uint64_t rem = m - (passes - 1) * GR;  // 1 .. GR rows
+          for (int h : {8, 12})
+            if (h < R && (uint64_t)G * h >= rem) {
+              RL = h;
+              break;
+            }
+        }
+        // device time: full passes at this shape's cost per executed cell,
+        // the final pass at its own height's (a 4-row pass at the 8-row
+        // cost scaled by its larger share of per-step overhead, 1.24)
+        const double cols = 64.0 * per * (np + G - 1);
+        const double ps_last = RL == R ? shape_ps[ri][gi]
+                               : RL == 4 ? 1.24 * shape_ps[0][gi] : shape_ps[RL == 8 ? 0 : 1][gi];
         const double useful = std::max(1.0, (double)m * (double)(csum[e] - csum[s]));
-        double c = exec * shape_ps[ri][gi] / useful;
-        const double steps = (double)passes * (np + G - 1) * R / 16.0;
+        double c = cols * ((double)(passes - 1) * R * shape_ps[ri][gi] + (double)RL * ps_last) / useful;
+        const double steps = (double)((passes - 1) * R + RL) * (np + G - 1) / 16.0;
         if (steps > cap) c *= steps / cap;
         if (c < bc) {
           bc = c;
           bG = G;
           bR = R;
+          bRL = RL;
         }
       }
     }
-    if (bc >= 1e300) return false;  // the limits admit no shape
+    if (bc >= 1e300) {
+      // a forced shape this kind cannot take (the int32 kernels' profile
+      // slice is twice as large): plan this query's units freely
+      if (fr || fg) {
+        fr = fg = 0;
+        continue;
+      }
+      return false;  // the limits admit no shape
+    }
     const size_t e = std::min(count, s + (size_t)(64 / bG * per));
     const uint32_t GR = (uint32_t)(bG * bR);
     Unit u{};
@@ -486,6 +514,7 @@ bool plan_query(Plan &pl, std::vector<Unit> &pending, uint32_t qi, const st_qpro
     u.npairs = (uint32_t)(e - s);
     u.G = (uint32_t)bG;
     u.R = (uint32_t)bR;
+    u.r_last = (uint32_t)bRL;
     u.passes = std::max<uint32_t>(1, (uint32_t)((m + GR - 1) / GR));
     for (size_t k = s; k < e; k++) {
       const uint32_t o = (uint32_t)(key[k] & 0xffffffffu);
@@ -495,7 +524,7 @@ bool plan_query(Plan &pl, std::vector<Unit> &pending, uint32_t qi, const st_qpro
       pl.pairs.push_back(pd);
     }
     u.n_pad = (uint32_t)(key[e - 1] >> 32);
-    pl.exec_cells += (uint64_t)64 * bR * per * u.passes * (u.n_pad + bG - 1);
+    pl.exec_cells += (uint64_t)64 * per * ((u.passes - 1) * bR + bRL) * (u.n_pad + bG - 1);
     pending.push_back(u);
     s = e;
   }
@@ -516,7 +545,7 @@ static int occupancy_cached(int kind, int R, bool ung, size_t lds, bool end) {
   return n;
 }
 
-static double unit_cost(const Unit &u) { return (double)u.passes * u.R * (u.n_pad + u.G - 1); }
+static double unit_cost(const Unit &u) { return (double)((u.passes - 1) * u.R + u.r_last) * (u.n_pad + u.G - 1); }
 
 // Group pending units into launches by (kind, R, gaps, LDS class), longest
 // first, and lay out their pass-buffer regions.
@@ -743,8 +772,11 @@ static int batch_create_impl(st_ctx *ctx, const st_db *db, const st_qprof *const
     if (end && b->qkind[q] == kPk16) b->qkind[q] = kI32F;
     std::vector<uint64_t> outs(e - s);
     std::iota(outs.begin(), outs.end(), s);
-    plan_query(b->plan, pending, (uint32_t)q, b->qps[q], cand_tids + s, outs.data(), e - s, b->qkind[q], db,
-               cap);
+    if (!plan_query(b->plan, pending, (uint32_t)q, b->qps[q], cand_tids + s, outs.data(), e - s, b->qkind[q], db,
+                    cap)) {
+      delete b;
+      return fail(ST_ERR_INVALID, "st_batch_create: no launch shape fits query " + std::to_string(q));
+    }
   }
   finish_plan(b->plan, pending, b->qkind, b->open, b->ext, -1, ctx->cus, ung, end);
   // useful and executed cells of every launch (roofline accounting)
@@ -755,7 +787,7 @@ static int batch_create_impl(st_ctx *ctx, const st_db *db, const st_qprof *const
       uint64_t cols = 0;
       for (uint32_t k = 0; k < U.npairs; k++) cols += db->hlen[b->plan.pairs[U.pair0 + k].tid];
       L.useful_cells += (uint64_t)b->qps[U.query]->m * cols;
-      L.exec_cells += (uint64_t)64 * U.R * (L.kind == kPk16 ? 2 : 1) * U.passes * (U.n_pad + U.G - 1);
+      L.exec_cells += (uint64_t)64 * (L.kind == kPk16 ? 2 : 1) * ((U.passes - 1) * U.R + U.r_last) * (U.n_pad + U.G - 1);
     }
   }
   std::vector<QueryDev> qv(nq);
@@ -861,8 +893,9 @@ static int batch_run_locked(st_batch *b, bool graph = false) {
       std::vector<uint32_t> tids;
       for (uint64_t o : perq[q]) tids.push_back(b->cand_tids[o]);
       b->rescored += perq[q].size();
-      plan_query(b->resc, pending, (uint32_t)q, b->qps[q], tids.data(), perq[q].data(), tids.size(), kI32F, b->db,
-                 unit_steps_cap(0));
+      if (!plan_query(b->resc, pending, (uint32_t)q, b->qps[q], tids.data(), perq[q].data(), tids.size(), kI32F,
+                      b->db, unit_steps_cap(0)))
+        return fail(ST_ERR_INVALID, "st_batch_run: no int32 launch shape fits query " + std::to_string(q));
     }
     finish_plan(b->resc, pending, b->qkind, b->open, b->ext, (int)kI32F, ctx->cus, b->ung, b->end);
     if (!b->resc.units.empty()) {

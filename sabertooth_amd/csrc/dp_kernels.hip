@@ -50,6 +50,39 @@ namespace st {
 
 static_assert(kGateLdsBytes <= 6144, "the gate scratch must fit the evidence launches' minimum LDS");
 
+// The final pass of a unit at a lower height RL < R (4, 8 or 12 rows per
+// lane): the query's last rows without a whole pass of padding rows.
+template <class P, int R, bool UNG, bool END>
+__device__ __forceinline__ void run_last_lower(int RL, const GroupCtx<P> &c, const typename P::L *lds, int stride,
+                                               int base, bool first, const uint4 *rd, uint4 *wr, typename P::T e0,
+                                               typename P::T vopen, typename P::T vext, typename P::T &best,
+                                               typename P::K (&top)[P::kPer][4], typename P::T *edge, int lane,
+                                               int &bj, int &bi) {
+  if constexpr (R > 4) {
+    if (RL == 4) {
+      run_pass_sel<P, 4, UNG, END>(UNG, c, lds, stride, base, first, true, rd, wr, e0, vopen, vext, best, top, edge,
+                                   lane, bj, bi);
+      return;
+    }
+  }
+  if constexpr (R > 8) {
+    if (RL == 8) {
+      run_pass_sel<P, 8, UNG, END>(UNG, c, lds, stride, base, first, true, rd, wr, e0, vopen, vext, best, top, edge,
+                                   lane, bj, bi);
+      return;
+    }
+  }
+  if constexpr (R > 12) {
+    if (RL == 12) {
+      run_pass_sel<P, 12, UNG, END>(UNG, c, lds, stride, base, first, true, rd, wr, e0, vopen, vext, best, top,
+                                    edge, lane, bj, bi);
+      return;
+    }
+  }
+  run_pass_sel<P, R, UNG, END>(UNG, c, lds, stride, base, first, true, rd, wr, e0, vopen, vext, best, top, edge,
+                               lane, bj, bi);
+}
+
 // One unit: every pass of its groups, then the merged outputs.
 template <class P, int R, bool UNG, bool END>
 __device__ __forceinline__ void dp_unit(const Unit &u, const QueryDev *__restrict__ queries,
@@ -72,7 +105,9 @@ __device__ __forceinline__ void dp_unit(const Unit &u, const QueryDev *__restric
   const int GR = c.G * R;
   c.n_pad = (int)u.n_pad;
   c.m = (int)q.m;
-  c.m_pad = (int)u.passes * GR;
+  // the final pass may be lower (r_last rows per lane); packed int16 only
+  const int RL = (sizeof(L) == 2 && u.r_last != 0 && (int)u.r_last < R) ? (int)u.r_last : R;
+  c.m_pad = ((int)u.passes - 1) * GR + c.G * RL;
 
   // this group's pair slots
   uint32_t outi[2] = {0, 0};
@@ -103,27 +138,37 @@ __device__ __forceinline__ void dp_unit(const Unit &u, const QueryDev *__restric
   int bj = 0x7fffffff, bi = 0;
   for (int p = 0; p < (int)u.passes; p++) {
     const int base = p * GR;
-    // stage the profile rows [base, base + GR) of all codes into LDS
+    const bool final_pass = p == (int)u.passes - 1;
+    const int GRp = final_pass ? c.G * RL : GR;
+    // stage the profile rows [base, base + GRp) of all codes into LDS
     __syncthreads();
     if (sizeof(L) == 2) {
-      const int dw = GR / 2;
+      const int dw = GRp / 2;
       for (int idx = lane; idx < kCodes * dw; idx += 64) {
         const int cc = idx / dw, r = idx - cc * dw;
         const uint32_t v = *(const uint32_t *)(prof + (uint64_t)cc * q.ld + base + 2 * r);
         *(uint32_t *)(lds + cc * lds_stride + 2 * r) = v;
       }
     } else {
-      for (int idx = lane; idx < kCodes * GR; idx += 64) {
-        const int cc = idx / GR, r = idx - cc * GR;
+      for (int idx = lane; idx < kCodes * GRp; idx += 64) {
+        const int cc = idx / GRp, r = idx - cc * GRp;
         lds[cc * lds_stride + r] = prof[(uint64_t)cc * q.ld + base + r];
       }
     }
     __syncthreads();
     const uint4 *rd = myb + (uint64_t)((p + 1) & 1) * passbuf_half + (uint64_t)c.g * c.n_pad;
     uint4 *wr = myb + (uint64_t)(p & 1) * passbuf_half + (uint64_t)c.g * c.n_pad;
-    const bool final_pass = p == (int)u.passes - 1;
-    run_pass_sel<P, R, UNG, END>(UNG && final_pass, c, lds, lds_stride, base, p == 0, final_pass, rd, wr, e0,
-                                 vopen, vext, best, top, edge, lane, bj, bi);
+    bool lower = false;
+    if constexpr (sizeof(L) == 2) {
+      if (final_pass && RL != R) {
+        run_last_lower<P, R, UNG, END>(RL, c, lds, lds_stride, base, p == 0, rd, wr, e0, vopen, vext, best, top,
+                                       edge, lane, bj, bi);
+        lower = true;
+      }
+    }
+    if (!lower)
+      run_pass_sel<P, R, UNG, END>(UNG && final_pass, c, lds, lds_stride, base, p == 0, final_pass, rd, wr, e0,
+                                   vopen, vext, best, top, edge, lane, bj, bi);
   }
   // best and top-4 over the group's lanes
   for (int off = 1; off < c.G; off <<= 1) {

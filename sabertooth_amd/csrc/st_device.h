@@ -29,7 +29,8 @@ struct Unit {
   uint32_t n_pad;   // columns: longest target of the unit
   uint32_t passes;  // ceil(m / (G*R))
   uint32_t R;       // rows per lane (the kernel instance)
-  uint32_t pad;
+  uint32_t r_last;  // rows per lane of the final pass (4, 8, 12 or R; <= R):
+                    // the query's last rows without a whole pass of padding
 };
 
 // Device view of a query profile.

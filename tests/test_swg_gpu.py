@@ -83,6 +83,21 @@ def test_saturation_rescored_in_int32(ctx):
     assert sc[2] == 30000 and kn[2] == st.KERNEL_I16
 
 
+@pytest.mark.parametrize("force", [("64", "16"), ("32", "16"), ("64", "8")])
+def test_forced_shape_the_int32_rescoring_cannot_take(ctx, monkeypatch, force):
+    # forced tuning shapes apply to every plan; the int32 rescoring of a
+    # saturated pair cannot take 64 x 16 (its profile slice outgrows LDS),
+    # so that plan falls back to a free shape: every score stays exact
+    monkeypatch.setenv("ST_FORCE_G", force[0])
+    monkeypatch.setenv("ST_FORCE_R", force[1])
+    W = O.aa2num()[ord("W")]
+    q = np.full(3400, W, np.uint8)
+    rows = O.seq_rows(q)
+    sc, kn = _check(ctx, rows, [q, q[:1000]], GapCosts(11, 1))
+    assert sc[0] == 34000 and kn[0] == st.KERNEL_I32
+    assert sc[1] == 10000 and kn[1] == st.KERNEL_I16
+
+
 def test_huge_rows_use_int32(ctx):
     rows = np.full((3, 21), 40000, np.int32)
     sc, kn = _check(ctx, rows, [np.array([0, 1, 2], np.uint8)], GapCosts(11, 1))
