@@ -386,6 +386,16 @@ int shape_table_check() {
 }
 
 // the table the planner uses now
+static void shape_table(double (&ps)[3][7]);
+
+PlanEnv plan_env() {
+  PlanEnv e;
+  e.fr = forced_rows();
+  e.fg = forced_groups();
+  shape_table(e.ps);
+  return e;
+}
+
 static void shape_table(double (&ps)[3][7]) {
   std::memcpy(ps, kShapePs, sizeof ps);
   const char *e = std::getenv("ST_SHAPE_TABLE");
@@ -411,7 +421,8 @@ double unit_steps_cap(uint64_t batch_cells) {
 // shape's measured cost, with a penalty on units longer than `cap` steps
 // (normalised to R = 16), whose single wave would set a launch's tail.
 bool plan_query(Plan &pl, std::vector<Unit> &pending, uint32_t qi, const st_qprof *qp, const uint32_t *tids,
-                const uint64_t *outs, size_t count, Kind kind, const st_db *db, double cap, const PlanLimits *lim) {
+                const uint64_t *outs, size_t count, Kind kind, const st_db *db, double cap, const PlanLimits *lim,
+                const PlanEnv *env) {
   if (count == 0) return true;
   // stable length order: (length << 32 | index) keys; two 8-bit LSD radix
   // passes when every length is below 65536
@@ -439,9 +450,10 @@ bool plan_query(Plan &pl, std::vector<Unit> &pending, uint32_t qi, const st_qpro
   for (size_t k = 0; k < count; k++) csum[k + 1] = csum[k] + (key[k] >> 32);
   const int per = kind == kPk16 ? 2 : 1;
   const uint64_t m = qp->m;
-  int fr = forced_rows(), fg = forced_groups();
-  double shape_ps[3][7];
-  shape_table(shape_ps);
+  const PlanEnv own = env ? PlanEnv{} : plan_env();
+  const PlanEnv &E = env ? *env : own;
+  int fr = E.fr, fg = E.fg;
+  const double (&shape_ps)[3][7] = E.ps;
   // the final pass of a packed int16 unit may be lower than the others (4,
   // 8 or 12 rows per lane): the kernels with every height take it; the
   // serving kernel (one height) and forced shapes keep it uniform
@@ -761,6 +773,7 @@ static int batch_create_impl(st_ctx *ctx, const st_db *db, const st_qprof *const
   }
   std::vector<Unit> pending;
   const double cap = unit_steps_cap(cells);
+  const PlanEnv env = plan_env();
   for (uint64_t q = 0; q < nq; q++) {
     b->qkind[q] = kind_for(b->qps[q], b->open[q], b->ext[q]);
     const uint64_t s = cand_off[q], e = cand_off[q + 1];
@@ -773,7 +786,7 @@ static int batch_create_impl(st_ctx *ctx, const st_db *db, const st_qprof *const
     std::vector<uint64_t> outs(e - s);
     std::iota(outs.begin(), outs.end(), s);
     if (!plan_query(b->plan, pending, (uint32_t)q, b->qps[q], cand_tids + s, outs.data(), e - s, b->qkind[q], db,
-                    cap)) {
+                    cap, nullptr, &env)) {
       delete b;
       return fail(ST_ERR_INVALID, "st_batch_create: no launch shape fits query " + std::to_string(q));
     }
@@ -888,13 +901,14 @@ static int batch_run_locked(st_batch *b, bool graph = false) {
     std::vector<std::vector<uint64_t>> perq(b->qps.size());
     for (uint32_t k : rows) perq[b->cand_q[k]].push_back(k);
     std::vector<Unit> pending;
+    const PlanEnv env = plan_env();
     for (size_t q = 0; q < perq.size(); q++) {
       if (perq[q].empty()) continue;
       std::vector<uint32_t> tids;
       for (uint64_t o : perq[q]) tids.push_back(b->cand_tids[o]);
       b->rescored += perq[q].size();
       if (!plan_query(b->resc, pending, (uint32_t)q, b->qps[q], tids.data(), perq[q].data(), tids.size(), kI32F,
-                      b->db, unit_steps_cap(0)))
+                      b->db, unit_steps_cap(0), nullptr, &env))
         return fail(ST_ERR_INVALID, "st_batch_run: no int32 launch shape fits query " + std::to_string(q));
     }
     finish_plan(b->resc, pending, b->qkind, b->open, b->ext, (int)kI32F, ctx->cus, b->ung, b->end);
@@ -1128,8 +1142,9 @@ int st_align_spans_batch(st_ctx *ctx, const st_db *db, const st_qprof *const *qu
   // prefixes (database layout: 16-byte aligned starts, code 21 padding)
   std::vector<uint64_t> mir_off(nq + 1, 0);
   std::vector<uint32_t> mir_ld(nq, 0);
-  for (const Rev &r : rv)
-    mir_ld[r.q] = (queries[r.q]->m + kMaxPassRows - 1) / kMaxPassRows * kMaxPassRows + kMaxPassRows;
+  // a view's passes read at most its rows rounded up to a pass (<= 1024
+  // rows) from its window start m - 1 - e: m + 1024 rows cover every view
+  for (const Rev &r : rv) mir_ld[r.q] = (queries[r.q]->m + kMaxPassRows + 63) & ~63u;
   for (uint64_t q = 0; q < nq; q++) mir_off[q + 1] = mir_off[q] + (uint64_t)kCodes * mir_ld[q];
   std::vector<RevPrefix> pre(nr);
   std::vector<uint64_t> doff(nr + 1, 0);
@@ -1146,28 +1161,42 @@ int st_align_spans_batch(st_ctx *ctx, const st_db *db, const st_qprof *const *qu
   rdb->n_res = doff[nr];
   rdb->hlen = rlen;
   for (uint32_t l : rlen) rdb->max_len = std::max(rdb->max_len, l);
+  // one mirror job per query with hits; its 256-row chunks in one launch
+  std::vector<MirrorJob> mj;
+  std::vector<uint32_t> chunk_job, chunk_first;
+  for (uint64_t q = 0; q < nq; q++)
+    if (mir_ld[q]) {
+      chunk_first.push_back((uint32_t)chunk_job.size());
+      mj.push_back(MirrorJob{queries[q]->prof32.p, mir_off[q], queries[q]->ld, queries[q]->m, mir_ld[q], 0});
+      for (uint32_t c = 0; c < (mir_ld[q] + 255) / 256; c++) chunk_job.push_back((uint32_t)(mj.size() - 1));
+    }
   DevBuf<int32_t> mirror;
   DevBuf<RevPrefix> d_pre;
+  DevBuf<MirrorJob> d_mj;
+  DevBuf<uint32_t> d_cj, d_cf;
   {
     std::lock_guard<std::mutex> lk(ctx->mu);
     ST_HIP(hipSetDevice(ctx->device));
     if (mirror.alloc(std::max<uint64_t>(mir_off[nq], 1)) != hipSuccess ||
         d_pre.upload(pre.data(), nr, ctx->stream) != hipSuccess ||
+        d_mj.upload(mj.data(), mj.size(), ctx->stream) != hipSuccess ||
+        d_cj.upload(chunk_job.data(), chunk_job.size(), ctx->stream) != hipSuccess ||
+        d_cf.upload(chunk_first.data(), chunk_first.size(), ctx->stream) != hipSuccess ||
         rdb->off.upload(doff.data(), nr + 1, ctx->stream) != hipSuccess ||
         rdb->len.upload(rlen.data(), nr, ctx->stream) != hipSuccess || rdb->arena.alloc(doff[nr] + 64) != hipSuccess)
       return fail(ST_ERR_NOMEM, "st_align_spans_batch: device allocation failed");
-    for (uint64_t q = 0; q < nq; q++)
-      if (mir_ld[q])
-        ST_HIP(launch_mirror_prof(queries[q]->prof32.p, queries[q]->ld, queries[q]->m, mirror.p + mir_off[q],
-                                  mir_ld[q], ctx->stream));
+    ST_HIP(launch_mirror_prof(d_mj.p, d_cj.p, d_cf.p, (uint32_t)chunk_job.size(), mirror.p, ctx->stream));
     ST_HIP(launch_rev_prefix(db->arena.p, db->off.p, d_pre.p, (uint32_t)nr, rdb->off.p, rdb->arena.p, ctx->stream));
     ST_HIP(hipStreamSynchronize(ctx->stream));
   }
   // one profile view per (query, end row): stride = the mirror's rows, rows
   // readable from the window's start = its extent (the planner keeps every
   // pass inside it)
-  std::vector<std::unique_ptr<st_qprof>> views;
+  uint64_t nviews = 0;
+  for (uint64_t k = 0; k < nr; k++) nviews += k == 0 || rv[k].q != rv[k - 1].q || rv[k].e != rv[k - 1].e;
+  std::unique_ptr<st_qprof[]> views(new st_qprof[nviews]);
   std::vector<const st_qprof *> vq;
+  vq.reserve(nviews);
   std::vector<uint64_t> voff{0};
   std::vector<int32_t> vopen, vext;
   std::vector<uint32_t> vt(nr);
@@ -1175,7 +1204,7 @@ int st_align_spans_batch(st_ctx *ctx, const st_db *db, const st_qprof *const *qu
     if (k == 0 || rv[k].q != rv[k - 1].q || rv[k].e != rv[k - 1].e) {
       const st_qprof *src = queries[rv[k].q];
       const uint32_t o = src->m - 1 - rv[k].e;
-      auto v = std::make_unique<st_qprof>();
+      st_qprof *v = &views[vq.size()];
       v->ctx = ctx;
       v->m = rv[k].e + 1;
       v->ld = mir_ld[rv[k].q] - o;
@@ -1184,8 +1213,7 @@ int st_align_spans_batch(st_ctx *ctx, const st_db *db, const st_qprof *const *qu
       v->view.prof32 = mirror.p + mir_off[rv[k].q] + o;
       v->view.m = v->m;
       v->view.ld = mir_ld[rv[k].q];
-      vq.push_back(v.get());
-      views.push_back(std::move(v));
+      vq.push_back(v);
       vopen.push_back(gap_open[rv[k].q]);
       vext.push_back(gap_extend[rv[k].q]);
       voff.push_back(voff.back());

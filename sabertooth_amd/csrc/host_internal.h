@@ -209,11 +209,19 @@ struct PlanLimits {
   uint64_t max_half = ~0ull;   // (64 / G) * n_pad of multi-pass units
 };
 
+// The planner's environment: forced shapes (ST_FORCE_R, ST_FORCE_G) and the
+// launch-shape cost table, read once per plan of many queries.
+struct PlanEnv {
+  int fr = 0, fg = 0;
+  double ps[3][7];
+};
+PlanEnv plan_env();
+
 // Returns false (and plans nothing further) when `lim` admits no shape for
-// some unit.
+// some unit.  env (nullable): the plan's environment, read here otherwise.
 bool plan_query(Plan &pl, std::vector<Unit> &pending, uint32_t qi, const st_qprof *qp, const uint32_t *tids,
                 const uint64_t *outs, size_t count, Kind kind, const st_db *db, double cap,
-                const PlanLimits *lim = nullptr);
+                const PlanLimits *lim = nullptr, const PlanEnv *env = nullptr);
 
 // Unit step cap for a batch of `batch_cells` DP cells: short units when the
 // batch is small (latency), up to 12000 steps for large batches.

@@ -333,6 +333,7 @@ int run_micro_batch(Slot &sl, std::vector<std::shared_ptr<Req>> &reqs) {
   Plan pl;
   pl.pairs.reserve(np);
   std::vector<Unit> pending;
+  const PlanEnv env = plan_env();
   for (size_t q = 0; q < nq; q++) {
     const Req &r = *reqs[q];
     open[q] = r.open;
@@ -343,7 +344,8 @@ int run_micro_batch(Slot &sl, std::vector<std::shared_ptr<Req>> &reqs) {
     if (qkind[q] == kPk16 && (uint64_t)r.qp->m + nmax - 1 > kMaxDiagsPk16) qkind[q] = kI32F;
     std::vector<uint64_t> outs(r.cands.size());
     std::iota(outs.begin(), outs.end(), off[q]);
-    if (!plan_query(pl, pending, (uint32_t)q, r.qp, r.cands.data(), outs.data(), r.cands.size(), qkind[q], db, cap))
+    if (!plan_query(pl, pending, (uint32_t)q, r.qp, r.cands.data(), outs.data(), r.cands.size(), qkind[q], db, cap,
+                    nullptr, &env))
       return fail(ST_ERR_INVALID, "serving micro-batch: no launch shape fits a request");
   }
   finish_plan(pl, pending, qkind, open, ext, -1, srv->ctx->cus, true);
@@ -401,7 +403,7 @@ int run_micro_batch(Slot &sl, std::vector<std::shared_ptr<Req>> &reqs) {
     if (outs.empty()) continue;
     nres += outs.size();
     if (!plan_query(rs, rs_pending, (uint32_t)q, reqs[q]->qp, tids.data(), outs.data(), tids.size(), kI32F, db,
-                    unit_steps_cap(0)))
+                    unit_steps_cap(0), nullptr, &env))
       return fail(ST_ERR_INVALID, "serving micro-batch: no int32 launch shape fits a request");
   }
   if (nres) {

@@ -2,11 +2,12 @@
 // which scores the reversed prefixes rows[..=qend] and target[..=tend] of
 // each hit to find its start.
 //
-//  * mirror_prof_kernel: a query's int32 profile with its rows in reverse
+//  * mirror_prof_kernel: each query's int32 profile with its rows in reverse
 //    order, so that the reversed prefix rows[..=e] of any end row e is the
 //    window starting at row m - 1 - e (no profile per hit).  Rows past m are
 //    zero, and at least 1024 of them follow the last row, as every pass of
-//    the DP kernels reads whole passes of up to 1024 rows.
+//    the DP kernels reads whole passes of up to 1024 rows.  One launch
+//    writes every query's mirror.
 //  * rev_prefix_kernel: each hit's reversed target prefix, in the arena
 //    layout of a database (16-byte aligned starts, code 21 padding, 64 bytes
 //    of code 21 after the last target).
@@ -16,13 +17,19 @@
 
 namespace st {
 
-// one block per (code, 256-row chunk): dst[c][r] = src[c][m - 1 - r]
-__global__ __launch_bounds__(256) void mirror_prof_kernel(const int32_t *__restrict__ src, uint32_t src_ld,
-                                                          uint32_t m, int32_t *__restrict__ dst, uint32_t dst_ld) {
-  const uint32_t c = blockIdx.y;
-  const uint32_t r = blockIdx.x * 256 + threadIdx.x;
-  if (r >= dst_ld) return;
-  dst[(uint64_t)c * dst_ld + r] = r < m ? src[(uint64_t)c * src_ld + (m - 1 - r)] : 0;
+// every query's mirror in one launch: one block per (256-row chunk of some
+// query's mirror, code); chunk_job[x] names the query of chunk x and
+// chunk_first[j] its first chunk: dst[c][r] = src[c][m - 1 - r]
+__global__ __launch_bounds__(256) void mirror_prof_kernel(const MirrorJob *__restrict__ jobs,
+                                                          const uint32_t *__restrict__ chunk_job,
+                                                          const uint32_t *__restrict__ chunk_first,
+                                                          int32_t *__restrict__ dst) {
+  const uint32_t x = blockIdx.x, c = blockIdx.y;
+  const uint32_t j = chunk_job[x];
+  const MirrorJob jb = jobs[j];
+  const uint32_t r = (x - chunk_first[j]) * 256 + threadIdx.x;
+  if (r >= jb.dst_ld) return;
+  dst[jb.dst_off + (uint64_t)c * jb.dst_ld + r] = r < jb.m ? jb.src[(uint64_t)c * jb.src_ld + (jb.m - 1 - r)] : 0;
 }
 
 // one block per reversed prefix k: bytes [doff[k], doff[k + 1]) (and the
@@ -41,10 +48,10 @@ __global__ __launch_bounds__(256) void rev_prefix_kernel(const uint8_t *__restri
     dst[o + x] = x < p.len ? t[p.len - 1 - x] : (uint8_t)kSentinel;
 }
 
-hipError_t launch_mirror_prof(const int32_t *src, uint32_t src_ld, uint32_t m, int32_t *dst, uint32_t dst_ld,
-                              hipStream_t s) {
-  hipLaunchKernelGGL(mirror_prof_kernel, dim3((dst_ld + 255) / 256, kCodes), dim3(256), 0, s, src, src_ld, m, dst,
-                     dst_ld);
+hipError_t launch_mirror_prof(const MirrorJob *jobs, const uint32_t *chunk_job, const uint32_t *chunk_first,
+                              uint32_t nchunks, int32_t *dst, hipStream_t s) {
+  if (nchunks == 0) return hipSuccess;
+  hipLaunchKernelGGL(mirror_prof_kernel, dim3(nchunks, kCodes), dim3(256), 0, s, jobs, chunk_job, chunk_first, dst);
   return hipGetLastError();
 }
 
