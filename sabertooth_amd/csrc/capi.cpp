@@ -408,10 +408,12 @@ static void shape_table(double (&ps)[3][7]) {
 
 double unit_steps_cap(uint64_t batch_cells) {
   // half the mean work of a resident wave (256 CUs x 8 waves, 2048 cells per
-  // packed step), within [300, 12000] steps: a lone query spreads over many
-  // short units (latency), a large batch keeps wide units (padding)
+  // packed step), at least 300 steps: a lone query spreads over many short
+  // units (latency); in a large batch a unit is a small part of a wave's
+  // work, so no step count is a tail and the cheapest shape wins (the
+  // long-pair workload's units run 300,000 steps and more)
   const double mean = (double)batch_cells / 2048.0 / 2048.0;
-  return std::min(12000.0, std::max(300.0, mean / 2.0));
+  return std::max(300.0, mean / 2.0);
 }
 
 // Cut the sorted candidates of one query into wave units.  Each unit takes
