@@ -214,6 +214,11 @@ __device__ __forceinline__ void run_pass(const GroupCtx<P> &c, const typename P:
   }
   T hdiag = 0, udiag = 0, bdiag = 0;
   T out_h = 0, out_f = e0, out_u = 0, out_b = 0;
+  // END: this lane's first optimum of the pass (its cells in column-major
+  // order: columns ascending, rows ascending in a column), merged into the
+  // unit's after the pass
+  T pbest = 0;
+  int pbj = 0x7fffffff, pbi = 0;
   // the group's first lane streams residues: cw = columns [4u, 4u + 4),
   // nw = the next four; the boundary row of the previous pass one ahead
   uint32_t cw[2] = {kSent4, kSent4}, nw[2] = {kSent4, kSent4};
@@ -310,20 +315,16 @@ __device__ __forceinline__ void run_pass(const GroupCtx<P> &c, const typename P:
         H[0] = P::add(hdiag, sc[0]);
         // vertical / horizontal gaps, first row first
         T F = in_f;
+        // END: the column's maximum over this lane's cells of the real
+        // matrix (I32N: cells outside it count as 0, which never improves)
+        T cm = 0;
 #pragma unroll
         for (int k = 0; k < R; k++) {
           T h = P::max(P::max(H[k], E[k]), F);
           if (!P::kFloor) h = P::max(h, 0);
           H[k] = h;
           if (END) {
-            // first optimum in column-major order: a lane sees its columns in
-            // increasing j and, within a column, its rows in increasing i
-            const int row = base + l * R + k;
-            if ((P::kFloor || (j < c.len[0] && row < m)) && (h > best || (h == best && h > 0 && j < bj))) {
-              best = h;
-              bj = j;
-              bi = row;
-            }
+            cm = P::max(cm, (P::kFloor || (j < c.len[0] && base + l * R + k < m)) ? h : T(0));
           } else if (P::kFloor) {
             best = P::max(best, h);
           } else if (j < c.len[0] && base + l * R + k < m) {
@@ -332,6 +333,19 @@ __device__ __forceinline__ void run_pass(const GroupCtx<P> &c, const typename P:
           const T ho = P::subf(h, vopen);
           E[k] = P::max(P::subf(E[k], vext), ho);
           F = P::max(P::subf(F, vext), ho);
+        }
+        if (END && __builtin_expect(cm > pbest, 0)) {
+          // a strictly larger value in this column: the first optimum of
+          // the pass so far is this column's first row holding it (a lane
+          // sees its columns in increasing j, a column's rows in increasing
+          // i); a rare, divergent branch
+          int kk = R - 1;
+#pragma unroll
+          for (int k = R - 1; k >= 0; k--)
+            if (H[k] == cm && (P::kFloor || base + l * R + k < m)) kk = k;
+          pbest = cm;
+          pbj = j;
+          pbi = base + l * R + kk;
         }
         out_h = H[R - 1];
         out_f = F;
@@ -396,6 +410,16 @@ __device__ __forceinline__ void run_pass(const GroupCtx<P> &c, const typename P:
       for (int x = 0; x < kPer; x++)
         if (c.has[x] && d >= -(m - 1) && d <= c.len[x] - 1)
           insert4<K>(top[x], P::key(P::half(v, x), (uint32_t)(d + m - 1)));
+    }
+  }
+  if (END) {
+    // the pass's first optimum against the earlier passes' (score desc,
+    // column asc, row asc): a later pass holds lower rows, so it wins a tie
+    // only at a smaller column
+    if ((int)pbest > (int)best || ((int)pbest == (int)best && (int)pbest > 0 && (pbj < bj || (pbj == bj && pbi < bi)))) {
+      best = pbest;
+      bj = pbj;
+      bi = pbi;
     }
   }
 }
