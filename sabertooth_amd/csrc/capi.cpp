@@ -12,6 +12,7 @@
 #include <mutex>
 #include <numeric>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "../../include/sabertooth_amd.h"
@@ -152,8 +153,14 @@ int st_ctx_create(int device, st_ctx **out) {
   return ST_OK;
 }
 
+static void span_work_free(SpanWork *w);
+
 int st_ctx_destroy(st_ctx *ctx) {
   if (!ctx) return ST_OK;
+  if (ctx->end_batch) st_batch_destroy(ctx->end_batch);
+  ctx->end_batch = nullptr;
+  span_work_free(ctx->spans);
+  ctx->spans = nullptr;
   (void)hipSetDevice(ctx->device);
   (void)hipStreamSynchronize(ctx->stream);
   for (hipStream_t &s : ctx->slots) {
@@ -435,7 +442,12 @@ bool plan_query(Plan &pl, std::vector<Unit> &pending, uint32_t qi, const st_qpro
     lmax = std::max(lmax, l);
     key[k] = ((uint64_t)l << 32) | k;
   }
-  if (lmax < 65536) {
+  if (count <= 32) {
+    // short lists (a lone hit, a serving request's few targets): insertion
+    // sort, the same stable order
+    for (size_t k = 1; k < count; k++)
+      for (size_t x = k; x > 0 && key[x - 1] > key[x]; x--) std::swap(key[x - 1], key[x]);
+  } else if (lmax < 65536) {
     std::vector<uint64_t> tmp(count);
     for (int pass = 0; pass < 2; pass++) {
       const int sh = 32 + 8 * pass;
@@ -545,6 +557,48 @@ bool plan_query(Plan &pl, std::vector<Unit> &pending, uint32_t qi, const st_qpro
   return true;
 }
 
+// plan_query over the queries [0, nq): one(pl, pending, q) plans query q.
+// Batches of many pairs are planned on up to 8 threads, each over a
+// contiguous run of queries into a plan of its own; the runs are appended
+// in query order with their pair indices shifted, which gives the plan one
+// thread would make.  Returns the first query whose planning failed, or nq.
+template <class F>
+static uint64_t plan_queries(Plan &pl, std::vector<Unit> &pending, uint64_t nq, uint64_t npairs, F one) {
+  const unsigned hw = std::max(1u, std::thread::hardware_concurrency());
+  const uint64_t T = std::min<uint64_t>({(uint64_t)std::min(hw, 8u), nq, npairs / 2048 + 1});
+  if (T <= 1) {
+    for (uint64_t q = 0; q < nq; q++)
+      if (!one(pl, pending, q)) return q;
+    return nq;
+  }
+  std::vector<Plan> pls(T);
+  std::vector<std::vector<Unit>> pend(T);
+  std::vector<uint64_t> bad(T, nq);
+  std::vector<std::thread> th;
+  th.reserve(T);
+  for (uint64_t t = 0; t < T; t++)
+    th.emplace_back([&, t] {
+      for (uint64_t q = nq * t / T; q < nq * (t + 1) / T; q++)
+        if (!one(pls[t], pend[t], q)) {
+          bad[t] = q;
+          return;
+        }
+    });
+  for (std::thread &x : th) x.join();
+  for (uint64_t t = 0; t < T; t++)
+    if (bad[t] < nq) return bad[t];
+  for (uint64_t t = 0; t < T; t++) {
+    const uint32_t off = (uint32_t)pl.pairs.size();
+    pl.pairs.insert(pl.pairs.end(), pls[t].pairs.begin(), pls[t].pairs.end());
+    pl.exec_cells += pls[t].exec_cells;
+    for (Unit u : pend[t]) {
+      u.pair0 += off;
+      pending.push_back(u);
+    }
+  }
+  return nq;
+}
+
 // dp_occupancy memoised per (kind, R, ung, LDS bytes): the runtime query is
 // slow next to a serving micro-batch's planning.
 static int occupancy_cached(int kind, int R, bool ung, size_t lds, bool end) {
@@ -563,6 +617,22 @@ static double unit_cost(const Unit &u) { return (double)((u.passes - 1) * u.R + 
 
 // Group pending units into launches by (kind, R, gaps, LDS class), longest
 // first, and lay out their pass-buffer regions.
+// Stable LSD radix sort of (key, position) pairs by key, 8 bits a pass;
+// passes whose byte is the same in every key are skipped.
+static void radix_sort_keys(std::vector<std::pair<uint64_t, uint32_t>> &v) {
+  std::vector<std::pair<uint64_t, uint32_t>> tmp(v.size());
+  for (int sh = 0; sh < 64; sh += 8) {
+    size_t cnt[257] = {0};
+    for (const auto &x : v) cnt[((x.first >> sh) & 0xff) + 1]++;
+    bool one = false;
+    for (int d = 1; d <= 256; d++) one |= cnt[d] == v.size();
+    if (one) continue;
+    for (int d = 0; d < 256; d++) cnt[d + 1] += cnt[d];
+    for (const auto &x : v) tmp[cnt[(x.first >> sh) & 0xff]++] = x;
+    v.swap(tmp);
+  }
+}
+
 void finish_plan(Plan &pl, std::vector<Unit> &pending, const std::vector<Kind> &qkind,
                         const std::vector<int32_t> &open, const std::vector<int32_t> &ext, int force_kind,
                         int cus, bool ung, bool end) {
@@ -571,16 +641,57 @@ void finish_plan(Plan &pl, std::vector<Unit> &pending, const std::vector<Kind> &
   // packed int16 units of both heights share a launch (the kernel instance
   // with R = 0 takes each unit's own R); the int32 kernels are per R
   auto r_of = [&](const Unit &u) { return kind_of(u) == kPk16 ? 0 : (int)u.R; };
-  std::stable_sort(pending.begin(), pending.end(), [&](const Unit &a, const Unit &b) {
-    if (kind_of(a) != kind_of(b)) return kind_of(a) < kind_of(b);
-    if (r_of(a) != r_of(b)) return r_of(a) < r_of(b);
-    // wide-group launches first: their few long units must start before the
-    // main launch fills the device
-    if (big(a) != big(b)) return big(a) > big(b);
-    if (open[a.query] != open[b.query]) return open[a.query] < open[b.query];
-    if (ext[a.query] != ext[b.query]) return ext[a.query] < ext[b.query];
-    return unit_cost(a) > unit_cost(b);
-  });
+  // launch order: kind, R, wide-group launches first (their few long units
+  // must start before the main launch fills the device), gap costs, then
+  // the longest units first, ties in planning order; as one precomputed key
+  // per unit (a comparator reading the query tables costs milliseconds for
+  // tens of thousands of units)
+  // the distinct gap costs (few: usually one), ranked
+  std::vector<std::pair<int32_t, int32_t>> gaps;
+  for (const Unit &u : pending) {
+    const std::pair<int32_t, int32_t> g(open[u.query], ext[u.query]);
+    if (gaps.empty() || gaps.back() != g) gaps.push_back(g);
+  }
+  std::sort(gaps.begin(), gaps.end());
+  gaps.erase(std::unique(gaps.begin(), gaps.end()), gaps.end());
+  // class = (kind, R, width class, gap rank), ranked; key = class rank in
+  // the top 12 bits, then the cost descending (saturated at 2^52 - 1 cells
+  // x steps), ties by position
+  std::vector<uint64_t> cls(pending.size());
+  for (size_t i = 0; i < pending.size(); i++) {
+    const Unit &u = pending[i];
+    const uint64_t g = (uint64_t)(std::lower_bound(gaps.begin(), gaps.end(), std::make_pair(open[u.query], ext[u.query])) -
+                                  gaps.begin());
+    cls[i] = ((uint64_t)kind_of(u) << 56) | ((uint64_t)r_of(u) << 48) | ((uint64_t)(1 - big(u)) << 40) | g;
+  }
+  std::vector<uint64_t> ranks;
+  for (uint64_t c : cls)
+    if (ranks.empty() || ranks.back() != c) ranks.push_back(c);
+  std::sort(ranks.begin(), ranks.end());
+  ranks.erase(std::unique(ranks.begin(), ranks.end()), ranks.end());
+  std::vector<std::pair<uint64_t, uint32_t>> keys(pending.size());
+  constexpr uint64_t kCostMax = (1ull << 52) - 1;
+  if (ranks.size() <= 4096) {
+    for (size_t i = 0; i < pending.size(); i++) {
+      const uint64_t r = (uint64_t)(std::lower_bound(ranks.begin(), ranks.end(), cls[i]) - ranks.begin());
+      const uint64_t c = std::min<uint64_t>((uint64_t)unit_cost(pending[i]), kCostMax);
+      keys[i] = {(r << 52) | (kCostMax - c), (uint32_t)i};
+    }
+    radix_sort_keys(keys);
+  } else {
+    // more classes than the key holds (thousands of distinct gap costs)
+    for (size_t i = 0; i < pending.size(); i++) keys[i] = {0, (uint32_t)i};
+    std::sort(keys.begin(), keys.end(), [&](const std::pair<uint64_t, uint32_t> &x, const std::pair<uint64_t, uint32_t> &y) {
+      if (cls[x.second] != cls[y.second]) return cls[x.second] < cls[y.second];
+      const double cx = unit_cost(pending[x.second]), cy = unit_cost(pending[y.second]);
+      return cx != cy ? cx > cy : x.second < y.second;
+    });
+  }
+  {
+    std::vector<Unit> sorted(pending.size());
+    for (size_t i = 0; i < keys.size(); i++) sorted[i] = pending[keys[i].second];
+    pending.swap(sorted);
+  }
   for (size_t i = 0; i < pending.size();) {
     const Unit &u = pending[i];
     Launch L{};
@@ -631,6 +742,7 @@ struct st_batch {
   std::vector<uint32_t> cand_tids;
   std::vector<uint32_t> cand_q;
   Plan plan;
+  std::vector<QueryDev> qv;  // the queries' device views, source of d_queries
   DevBuf<QueryDev> d_queries;
   DevBuf<int32_t> d_open, d_ext, d_gate;
   DevBuf<Unit> d_units;
@@ -726,10 +838,13 @@ static int launch_plan(st_batch *b, const Plan &pl, const Unit *d_units, const P
 
 extern "C" {
 
+// Plan a batch and put its tables on the device.  reuse (nullable): an
+// earlier batch object of the same context to refill; its device buffers
+// are kept where they are large enough.
 static int batch_create_impl(st_ctx *ctx, const st_db *db, const st_qprof *const *queries, uint64_t nq,
                              const uint64_t *cand_off, const uint32_t *cand_tids, const int32_t *gap_open,
                              const int32_t *gap_extend, const int32_t *min_diag_score, bool ung,
-                             st_batch **out, bool end = false) {
+                             st_batch **out, bool end = false, st_batch *reuse = nullptr) {
   if (!ctx || !db || !out || (nq && (!queries || !cand_off || !gap_open || !gap_extend)))
     return fail(ST_ERR_INVALID, "st_batch_create: null argument");
   *out = nullptr;
@@ -745,12 +860,22 @@ static int batch_create_impl(st_ctx *ctx, const st_db *db, const st_qprof *const
   }
   for (uint64_t k = 0; k < np; k++)
     if (cand_tids[k] >= db->n_targets) return fail(ST_ERR_INVALID, "st_batch_create: target id out of range");
-  auto *b = new st_batch();
+  st_batch *b = reuse ? reuse : new st_batch();
+  auto drop = [&]() {
+    if (!reuse) delete b;
+  };
+  if (reuse && reuse->gexec) {
+    (void)hipGraphExecDestroy(reuse->gexec);
+    reuse->gexec = nullptr;
+  }
   b->ctx = ctx;
   b->db = db;
   b->ung = ung;
   b->end = end;
   b->npairs = np;
+  b->plan = Plan();
+  b->resc = Plan();
+  b->rescored = 0;
   b->qps.assign(queries, queries + nq);
   b->open.assign(gap_open, gap_open + nq);
   b->ext.assign(gap_extend, gap_extend + nq);
@@ -768,30 +893,33 @@ static int batch_create_impl(st_ctx *ctx, const st_db *db, const st_qprof *const
     }
   b->cells = cells;
   b->qkind.resize(nq);
+  for (uint64_t q = 0; q < nq; q++) {
+    b->qkind[q] = kind_for(b->qps[q], b->open[q], b->ext[q]);
+    // the packed kernel's diagonal keys hold m + n - 1 <= kMaxDiagsPk16
+    uint32_t nmax = 0;
+    for (uint64_t k = cand_off[q]; k < cand_off[q + 1]; k++) nmax = std::max(nmax, db->hlen[cand_tids[k]]);
+    if (b->qkind[q] == kPk16 && (uint64_t)b->qps[q]->m + nmax - 1 > kMaxDiagsPk16) b->qkind[q] = kI32F;
+    // the end cell is tracked by the int32 kernels
+    if (end && b->qkind[q] == kPk16) b->qkind[q] = kI32F;
+  }
   std::lock_guard<std::mutex> lk(ctx->mu);
   if (hipSetDevice(ctx->device) != hipSuccess) {
-    delete b;
+    drop();
     return fail(ST_ERR_DEVICE, "st_batch_create: hipSetDevice failed");
   }
   std::vector<Unit> pending;
   const double cap = unit_steps_cap(cells);
   const PlanEnv env = plan_env();
-  for (uint64_t q = 0; q < nq; q++) {
-    b->qkind[q] = kind_for(b->qps[q], b->open[q], b->ext[q]);
+  const uint64_t bad = plan_queries(b->plan, pending, nq, np, [&](Plan &pl, std::vector<Unit> &pend, uint64_t q) {
     const uint64_t s = cand_off[q], e = cand_off[q + 1];
-    // the packed kernel's diagonal keys hold m + n - 1 <= kMaxDiagsPk16
-    uint32_t nmax = 0;
-    for (uint64_t k = s; k < e; k++) nmax = std::max(nmax, db->hlen[cand_tids[k]]);
-    if (b->qkind[q] == kPk16 && (uint64_t)b->qps[q]->m + nmax - 1 > kMaxDiagsPk16) b->qkind[q] = kI32F;
-    // the end cell is tracked by the int32 kernels
-    if (end && b->qkind[q] == kPk16) b->qkind[q] = kI32F;
     std::vector<uint64_t> outs(e - s);
     std::iota(outs.begin(), outs.end(), s);
-    if (!plan_query(b->plan, pending, (uint32_t)q, b->qps[q], cand_tids + s, outs.data(), e - s, b->qkind[q], db,
-                    cap, nullptr, &env)) {
-      delete b;
-      return fail(ST_ERR_INVALID, "st_batch_create: no launch shape fits query " + std::to_string(q));
-    }
+    return plan_query(pl, pend, (uint32_t)q, b->qps[q], cand_tids + s, outs.data(), e - s, b->qkind[q], db, cap,
+                      nullptr, &env);
+  });
+  if (bad < nq) {
+    drop();
+    return fail(ST_ERR_INVALID, "st_batch_create: no launch shape fits query " + std::to_string(bad));
   }
   finish_plan(b->plan, pending, b->qkind, b->open, b->ext, -1, ctx->cus, ung, end);
   // useful and executed cells of every launch (roofline accounting)
@@ -805,22 +933,26 @@ static int batch_create_impl(st_ctx *ctx, const st_db *db, const st_qprof *const
       L.exec_cells += (uint64_t)64 * (L.kind == kPk16 ? 2 : 1) * ((U.passes - 1) * U.R + U.r_last) * (U.n_pad + U.G - 1);
     }
   }
-  std::vector<QueryDev> qv(nq);
-  for (uint64_t q = 0; q < nq; q++) qv[q] = b->qps[q]->view;
-  bool ok = b->d_queries.upload(qv.data(), qv.size(), ctx->stream) == hipSuccess &&
-            b->d_open.upload(b->open.data(), nq, ctx->stream) == hipSuccess &&
-            b->d_ext.upload(b->ext.data(), nq, ctx->stream) == hipSuccess &&
-            b->d_gate.upload(b->gate.data(), nq, ctx->stream) == hipSuccess &&
-            b->d_units.upload(b->plan.units.data(), b->plan.units.size(), ctx->stream) == hipSuccess &&
-            b->d_pairs.upload(b->plan.pairs.data(), b->plan.pairs.size(), ctx->stream) == hipSuccess &&
-            b->d_pout.alloc(std::max<uint64_t>(np, 1)) == hipSuccess &&
-            b->d_res.alloc(std::max<uint64_t>(np, 1) * sizeof(st_pair_result)) == hipSuccess &&
-            b->d_sat.alloc(np + 1) == hipSuccess &&
-            b->d_passbuf.alloc(std::max<uint64_t>(b->plan.buf_elems, 1)) == hipSuccess &&
-            (!end || (b->d_qend.alloc(std::max<uint64_t>(np, 1)) == hipSuccess &&
-                      b->d_tend.alloc(std::max<uint64_t>(np, 1)) == hipSuccess));
+  b->qv.resize(nq);
+  for (uint64_t q = 0; q < nq; q++) b->qv[q] = b->qps[q]->view;
+  // the tables are copied in stream order from the batch's own vectors
+  // (unchanged until the batch is planned again or destroyed, after its runs
+  // have synchronised); outputs and the pass buffer are grow-only
+  hipStream_t st = ctx->stream;
+  bool ok = b->d_queries.put(b->qv.data(), nq, st) == hipSuccess &&
+            b->d_open.put(b->open.data(), nq, st) == hipSuccess &&
+            b->d_ext.put(b->ext.data(), nq, st) == hipSuccess &&
+            b->d_gate.put(b->gate.data(), nq, st) == hipSuccess &&
+            b->d_units.put(b->plan.units.data(), b->plan.units.size(), st) == hipSuccess &&
+            b->d_pairs.put(b->plan.pairs.data(), b->plan.pairs.size(), st) == hipSuccess &&
+            b->d_pout.reserve(std::max<uint64_t>(np, 1)) == hipSuccess &&
+            (!ung || b->d_res.reserve(std::max<uint64_t>(np, 1) * sizeof(st_pair_result)) == hipSuccess) &&
+            b->d_sat.reserve(np + 1) == hipSuccess &&
+            b->d_passbuf.reserve(std::max<uint64_t>(b->plan.buf_elems, 1)) == hipSuccess &&
+            (!end || (b->d_qend.reserve(std::max<uint64_t>(np, 1)) == hipSuccess &&
+                      b->d_tend.reserve(std::max<uint64_t>(np, 1)) == hipSuccess));
   if (!ok) {
-    delete b;
+    drop();
     return fail(ST_ERR_NOMEM, "st_batch_create: device allocation failed");
   }
   *out = b;
@@ -865,6 +997,9 @@ static int batch_run_locked(st_batch *b, bool graph = false) {
   for (hipEvent_t &e : b->join)
     if (!e) ST_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
   if (b->lev.size() != 2 * b->plan.launches.size()) {
+    // a refilled batch object may have another launch count
+    for (hipEvent_t &e : b->lev)
+      if (e) (void)hipEventDestroy(e);
     b->lev.assign(2 * b->plan.launches.size(), nullptr);
     for (hipEvent_t &e : b->lev) ST_HIP(hipEventCreate(&e));
   }
@@ -1072,9 +1207,13 @@ static int score_end_multi(st_ctx *ctx, const st_db *db, const st_qprof *const *
                            st_score_end *out, const char *who) {
   const uint64_t count = nq ? off[nq] : 0;
   if (count == 0) return ST_OK;
+  // the context's end-cell batch object, refilled: its device buffers and
+  // events outlive the call
+  std::lock_guard<std::mutex> el(ctx->end_mu);
   st_batch *b = nullptr;
-  int rc = batch_create_impl(ctx, db, qps, nq, off, tids, open, ext, nullptr, false, &b, true);
+  int rc = batch_create_impl(ctx, db, qps, nq, off, tids, open, ext, nullptr, false, &b, true, ctx->end_batch);
   if (rc) return rc;
+  ctx->end_batch = b;
   rc = st_batch_run(b);
   if (!rc) {
     std::lock_guard<std::mutex> lk(ctx->mu);
@@ -1095,9 +1234,22 @@ static int score_end_multi(st_ctx *ctx, const st_db *db, const st_qprof *const *
       }
     }
   }
-  st_batch_destroy(b);
   return rc;
 }
+
+// The alignment spans' device buffers, kept by the context and grown as
+// calls need (one call at a time, under the context's spans_mu): the
+// row-reversed profiles, the reversed target prefixes as a database, and
+// the tables of the kernels that write them.
+struct SpanWork {
+  DevBuf<int32_t> mirror;
+  DevBuf<RevPrefix> pre;
+  DevBuf<MirrorJob> mj;
+  DevBuf<uint32_t> cj, cf;
+  st_db rdb;
+};
+
+static void span_work_free(SpanWork *w) { delete w; }
 
 int st_align_spans_batch(st_ctx *ctx, const st_db *db, const st_qprof *const *queries, uint64_t nq,
                          const uint64_t *hit_off, const uint32_t *hit_tids, const int32_t *gap_open,
@@ -1107,6 +1259,7 @@ int st_align_spans_batch(st_ctx *ctx, const st_db *db, const st_qprof *const *qu
   const uint64_t nh = nq ? hit_off[nq] : 0;
   if (nh == 0) return ST_OK;
   if (!out) return fail(ST_ERR_INVALID, "st_align_spans_batch: null output");
+  std::lock_guard<std::mutex> sl(ctx->spans_mu);
   // 1. forward pass: score and the column-major-first end cell
   std::vector<st_score_end> fw(nh);
   int rc = score_end_multi(ctx, db, queries, nq, hit_off, hit_tids, gap_open, gap_extend, fw.data(),
@@ -1157,11 +1310,13 @@ int st_align_spans_batch(st_ctx *ctx, const st_db *db, const st_qprof *const *qu
     pre[k] = RevPrefix{hit_tids[h], rlen[k]};
     doff[k + 1] = doff[k] + ((rlen[k] + 15ull) & ~15ull);
   }
-  auto rdb = std::make_unique<st_db>();
+  if (!ctx->spans) ctx->spans = new SpanWork();
+  st_db *rdb = &ctx->spans->rdb;
   rdb->ctx = ctx;
   rdb->n_targets = nr;
   rdb->n_res = doff[nr];
   rdb->hlen = rlen;
+  rdb->max_len = 0;
   for (uint32_t l : rlen) rdb->max_len = std::max(rdb->max_len, l);
   // one mirror job per query with hits; its 256-row chunks in one launch
   std::vector<MirrorJob> mj;
@@ -1172,25 +1327,25 @@ int st_align_spans_batch(st_ctx *ctx, const st_db *db, const st_qprof *const *qu
       mj.push_back(MirrorJob{queries[q]->prof32.p, mir_off[q], queries[q]->ld, queries[q]->m, mir_ld[q], 0});
       for (uint32_t c = 0; c < (mir_ld[q] + 255) / 256; c++) chunk_job.push_back((uint32_t)(mj.size() - 1));
     }
-  DevBuf<int32_t> mirror;
-  DevBuf<RevPrefix> d_pre;
-  DevBuf<MirrorJob> d_mj;
-  DevBuf<uint32_t> d_cj, d_cf;
+  SpanWork &w = *ctx->spans;
   {
     std::lock_guard<std::mutex> lk(ctx->mu);
     ST_HIP(hipSetDevice(ctx->device));
-    if (mirror.alloc(std::max<uint64_t>(mir_off[nq], 1)) != hipSuccess ||
-        d_pre.upload(pre.data(), nr, ctx->stream) != hipSuccess ||
-        d_mj.upload(mj.data(), mj.size(), ctx->stream) != hipSuccess ||
-        d_cj.upload(chunk_job.data(), chunk_job.size(), ctx->stream) != hipSuccess ||
-        d_cf.upload(chunk_first.data(), chunk_first.size(), ctx->stream) != hipSuccess ||
-        rdb->off.upload(doff.data(), nr + 1, ctx->stream) != hipSuccess ||
-        rdb->len.upload(rlen.data(), nr, ctx->stream) != hipSuccess || rdb->arena.alloc(doff[nr] + 64) != hipSuccess)
+    // grow-only buffers; the host tables are locals of this call, which
+    // waits for the stream below before they go
+    hipStream_t st = ctx->stream;
+    if (w.mirror.reserve(std::max<uint64_t>(mir_off[nq], 1)) != hipSuccess ||
+        w.pre.put(pre.data(), nr, st) != hipSuccess || w.mj.put(mj.data(), mj.size(), st) != hipSuccess ||
+        w.cj.put(chunk_job.data(), chunk_job.size(), st) != hipSuccess ||
+        w.cf.put(chunk_first.data(), chunk_first.size(), st) != hipSuccess ||
+        rdb->off.put(doff.data(), nr + 1, st) != hipSuccess || rdb->len.put(rlen.data(), nr, st) != hipSuccess ||
+        rdb->arena.reserve(doff[nr] + 64) != hipSuccess)
       return fail(ST_ERR_NOMEM, "st_align_spans_batch: device allocation failed");
-    ST_HIP(launch_mirror_prof(d_mj.p, d_cj.p, d_cf.p, (uint32_t)chunk_job.size(), mirror.p, ctx->stream));
-    ST_HIP(launch_rev_prefix(db->arena.p, db->off.p, d_pre.p, (uint32_t)nr, rdb->off.p, rdb->arena.p, ctx->stream));
-    ST_HIP(hipStreamSynchronize(ctx->stream));
+    ST_HIP(launch_mirror_prof(w.mj.p, w.cj.p, w.cf.p, (uint32_t)chunk_job.size(), w.mirror.p, st));
+    ST_HIP(launch_rev_prefix(db->arena.p, db->off.p, w.pre.p, (uint32_t)nr, rdb->off.p, rdb->arena.p, st));
+    ST_HIP(hipStreamSynchronize(st));
   }
+  DevBuf<int32_t> &mirror = w.mirror;
   // one profile view per (query, end row): stride = the mirror's rows, rows
   // readable from the window's start = its extent (the planner keeps every
   // pass inside it)
@@ -1224,7 +1379,7 @@ int st_align_spans_batch(st_ctx *ctx, const st_db *db, const st_qprof *const *qu
     voff.back()++;
   }
   std::vector<st_score_end> rs(nr);
-  rc = score_end_multi(ctx, rdb.get(), vq.data(), vq.size(), voff.data(), vt.data(), vopen.data(), vext.data(),
+  rc = score_end_multi(ctx, rdb, vq.data(), vq.size(), voff.data(), vt.data(), vopen.data(), vext.data(),
                        rs.data(), "st_align_spans_batch");
   if (rc) return rc;
   for (uint64_t k = 0; k < nr; k++) {

@@ -75,7 +75,8 @@ int fail(int code, const std::string &msg);
 template <class T>
 struct DevBuf {
   T *p = nullptr;
-  size_t n = 0;
+  size_t n = 0;    // elements in use
+  size_t cap = 0;  // elements allocated
   DevBuf() = default;
   DevBuf(const DevBuf &) = delete;
   DevBuf &operator=(const DevBuf &) = delete;
@@ -83,13 +84,23 @@ struct DevBuf {
   void release() {
     if (p) (void)hipFree(p);
     p = nullptr;
-    n = 0;
+    n = cap = 0;
   }
   hipError_t alloc(size_t count) {
     release();
     n = count;
     if (count == 0) return hipSuccess;
-    return hipMalloc((void **)&p, count * sizeof(T));
+    const hipError_t e = hipMalloc((void **)&p, count * sizeof(T));
+    if (e == hipSuccess) cap = count;
+    return e;
+  }
+  // grow-only: keeps the allocation when it holds count elements
+  hipError_t reserve(size_t count) {
+    if (p && count <= cap) {
+      n = count;
+      return hipSuccess;
+    }
+    return alloc(count);
   }
   hipError_t upload(const T *src, size_t count, hipStream_t s) {
     hipError_t e = alloc(count);
@@ -97,6 +108,13 @@ struct DevBuf {
     e = hipMemcpyAsync(p, src, count * sizeof(T), hipMemcpyHostToDevice, s);
     if (e != hipSuccess) return e;
     return hipStreamSynchronize(s);
+  }
+  // grow-only upload in stream order, no wait: src must stay unchanged until
+  // the stream has passed the copy
+  hipError_t put(const T *src, size_t count, hipStream_t s) {
+    hipError_t e = reserve(count);
+    if (e != hipSuccess || count == 0) return e;
+    return hipMemcpyAsync(p, src, count * sizeof(T), hipMemcpyHostToDevice, s);
   }
 };
 
@@ -109,6 +127,13 @@ struct st_ctx {
   std::mutex mu;
   int cus = 256;
   st::DevBuf<uint32_t> counter;  // one unit counter per launch slot
+  // the end-cell batches (st_sw_score_end_batch, both passes of the
+  // alignment spans) reuse one batch object and its grow-only device
+  // buffers, one call at a time under end_mu; the alignment spans' own
+  // buffers likewise under spans_mu
+  std::mutex end_mu, spans_mu;
+  st_batch *end_batch = nullptr;
+  struct SpanWork *spans = nullptr;
 };
 
 struct st_db {
