@@ -473,20 +473,20 @@ bool plan_query(Plan &pl, std::vector<Unit> &pending, uint32_t qi, const st_qpro
   // serving kernel (one height) and forced shapes keep it uniform
   const bool mixed_last = kind == kPk16 && !(lim && lim->only_R) && !fr;
   for (size_t s = 0; s < count;) {
-    int bG = 16, bR = kind == kI32N ? 8 : 16, bRL = bR;
+    int bG = 16, bR = kind_unfloored(kind) ? 8 : 16, bRL = bR;
     double bc = 1e300;
     for (int ri = 0; ri < 3; ri++) {
       const int R = kShapeRows[ri];
-      if (kind == kI32N && R != 8) continue;
+      if (kind_unfloored(kind) && R != 8) continue;
       if (R == 12 && kind != kPk16) continue;  // the int32 kernels have R = 8, 16
       // a forced height the instance lacks (12 for the int32 kernels) is ignored
-      if (fr && R != fr && kind != kI32N && !(fr == 12 && kind != kPk16)) continue;
+      if (fr && R != fr && !kind_unfloored(kind) && !(fr == 12 && kind != kPk16)) continue;
       for (int gi = 0; gi < 7; gi++) {
         const int G = 1 << gi;
         if (fg && G != fg) continue;
         const uint64_t GR = (uint64_t)G * R;
         // the pass's profile slice must fit 64 KB of LDS
-        if ((uint64_t)kCodes * (GR + 2) * (kind == kPk16 ? 2 : 4) > 65536) continue;
+        if ((uint64_t)kCodes * (GR + 2) * kind_entry_bytes(kind) > 65536) continue;
         const size_t e = std::min(count, s + (size_t)(64 / G * per));
         const uint64_t np = key[e - 1] >> 32;
         const uint64_t passes = std::max<uint64_t>(1, (m + GR - 1) / GR);
@@ -714,7 +714,7 @@ void finish_plan(Plan &pl, std::vector<Unit> &pending, const std::vector<Kind> &
       j++;
     }
     L.nunits = (uint32_t)(j - i);
-    const size_t lds = dp_lds_bytes(L.gr_max + 2, L.kind == kPk16 ? 2 : 4, ung && !end);
+    const size_t lds = dp_lds_bytes(L.gr_max + 2, kind_entry_bytes(L.kind), ung && !end);
     const int occ = std::max(1, occupancy_cached(L.kind, L.R, ung, lds, end));
     L.grid = std::max<uint32_t>(1, std::min<uint32_t>(L.nunits, (uint32_t)(occ * cus)));
     L.slot = (int)(pl.launches.size() % kLaunchSlots);
@@ -899,8 +899,12 @@ static int batch_create_impl(st_ctx *ctx, const st_db *db, const st_qprof *const
     uint32_t nmax = 0;
     for (uint64_t k = cand_off[q]; k < cand_off[q + 1]; k++) nmax = std::max(nmax, db->hlen[cand_tids[k]]);
     if (b->qkind[q] == kPk16 && (uint64_t)b->qps[q]->m + nmax - 1 > kMaxDiagsPk16) b->qkind[q] = kI32F;
-    // the end cell is tracked by the int32 kernels
-    if (end && b->qkind[q] == kPk16) b->qkind[q] = kI32F;
+    // the end cell is tracked by the int32 kernels, over the int16 profile
+    // where the query has one
+    if (end) {
+      const bool p16 = b->qps[q]->fits16 && b->qps[q]->view.prof16;
+      b->qkind[q] = kind_unfloored(b->qkind[q]) ? (p16 ? kI32N16 : kI32N) : (p16 ? kI32F16 : kI32F);
+    }
   }
   std::lock_guard<std::mutex> lk(ctx->mu);
   if (hipSetDevice(ctx->device) != hipSuccess) {
@@ -1242,7 +1246,7 @@ static int score_end_multi(st_ctx *ctx, const st_db *db, const st_qprof *const *
 // row-reversed profiles, the reversed target prefixes as a database, and
 // the tables of the kernels that write them.
 struct SpanWork {
-  DevBuf<int32_t> mirror;
+  DevBuf<uint8_t> mirror;
   DevBuf<RevPrefix> pre;
   DevBuf<MirrorJob> mj;
   DevBuf<uint32_t> cj, cf;
@@ -1293,14 +1297,19 @@ int st_align_spans_batch(st_ctx *ctx, const st_db *db, const st_qprof *const *qu
   std::stable_sort(rv.begin(), rv.end(), [](const Rev &a, const Rev &b) { return a.q != b.q ? a.q < b.q : a.e < b.e; });
   const uint64_t nr = rv.size();
   if (nr > 0xffffffffull) return fail(ST_ERR_INVALID, "st_align_spans_batch: more than 2^32 hits");
-  // row-reversed profiles of the queries with hits, and the reversed target
-  // prefixes (database layout: 16-byte aligned starts, code 21 padding)
-  std::vector<uint64_t> mir_off(nq + 1, 0);
-  std::vector<uint32_t> mir_ld(nq, 0);
+  // row-reversed profiles of the queries with hits (int16 where the query
+  // has an int16 profile: the end-cell kernels then stage half the bytes),
+  // and the reversed target prefixes (database layout: 16-byte aligned
+  // starts, code 21 padding)
+  std::vector<uint64_t> mir_off(nq + 1, 0);  // bytes
+  std::vector<uint32_t> mir_ld(nq, 0), mir_es(nq, 4);
   // a view's passes read at most its rows rounded up to a pass (<= 1024
   // rows) from its window start m - 1 - e: m + 1024 rows cover every view
   for (const Rev &r : rv) mir_ld[r.q] = (queries[r.q]->m + kMaxPassRows + 63) & ~63u;
-  for (uint64_t q = 0; q < nq; q++) mir_off[q + 1] = mir_off[q] + (uint64_t)kCodes * mir_ld[q];
+  for (uint64_t q = 0; q < nq; q++) {
+    if (queries[q]->fits16 && queries[q]->prof16.p) mir_es[q] = 2;
+    mir_off[q + 1] = mir_off[q] + (((uint64_t)kCodes * mir_ld[q] * mir_es[q] + 255) & ~255ull);
+  }
   std::vector<RevPrefix> pre(nr);
   std::vector<uint64_t> doff(nr + 1, 0);
   std::vector<uint32_t> rlen(nr);
@@ -1324,7 +1333,8 @@ int st_align_spans_batch(st_ctx *ctx, const st_db *db, const st_qprof *const *qu
   for (uint64_t q = 0; q < nq; q++)
     if (mir_ld[q]) {
       chunk_first.push_back((uint32_t)chunk_job.size());
-      mj.push_back(MirrorJob{queries[q]->prof32.p, mir_off[q], queries[q]->ld, queries[q]->m, mir_ld[q], 0});
+      const void *src = mir_es[q] == 2 ? (const void *)queries[q]->prof16.p : (const void *)queries[q]->prof32.p;
+      mj.push_back(MirrorJob{src, mir_off[q], queries[q]->ld, queries[q]->m, mir_ld[q], mir_es[q]});
       for (uint32_t c = 0; c < (mir_ld[q] + 255) / 256; c++) chunk_job.push_back((uint32_t)(mj.size() - 1));
     }
   SpanWork &w = *ctx->spans;
@@ -1345,7 +1355,7 @@ int st_align_spans_batch(st_ctx *ctx, const st_db *db, const st_qprof *const *qu
     ST_HIP(launch_rev_prefix(db->arena.p, db->off.p, w.pre.p, (uint32_t)nr, rdb->off.p, rdb->arena.p, st));
     ST_HIP(hipStreamSynchronize(st));
   }
-  DevBuf<int32_t> &mirror = w.mirror;
+  DevBuf<uint8_t> &mirror = w.mirror;
   // one profile view per (query, end row): stride = the mirror's rows, rows
   // readable from the window's start = its extent (the planner keeps every
   // pass inside it)
@@ -1365,9 +1375,10 @@ int st_align_spans_batch(st_ctx *ctx, const st_db *db, const st_qprof *const *qu
       v->ctx = ctx;
       v->m = rv[k].e + 1;
       v->ld = mir_ld[rv[k].q] - o;
-      v->fits16 = false;
-      v->view.prof16 = nullptr;
-      v->view.prof32 = mirror.p + mir_off[rv[k].q] + o;
+      uint8_t *mb = mirror.p + mir_off[rv[k].q];
+      v->fits16 = mir_es[rv[k].q] == 2;
+      v->view.prof16 = v->fits16 ? (int16_t *)mb + o : nullptr;
+      v->view.prof32 = v->fits16 ? nullptr : (int32_t *)mb + o;
       v->view.m = v->m;
       v->view.ld = mir_ld[rv[k].q];
       vq.push_back(v);

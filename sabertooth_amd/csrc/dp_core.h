@@ -106,6 +106,16 @@ struct I32N : I32F {
   __device__ static T subf(T a, T c) { return a - c; }
 };
 
+// The int32 recurrences over a profile whose values fit int16 (the query's
+// int16 profile): the LDS slice is half as large, so a 512-row pass leaves
+// room for twice the waves; each value is sign-extended as it is read.
+struct I32F16 : I32F {
+  using L = int16_t;
+};
+struct I32N16 : I32N {
+  using L = int16_t;
+};
+
 template <class K>
 __device__ __forceinline__ void insert4(K (&k)[4], K v) {
   // k[0] >= k[1] >= k[2] >= k[3]; keys are distinct
@@ -141,15 +151,23 @@ template <>
 __device__ __forceinline__ const int32_t *prof_of<I32F>(const QueryDev &q) { return q.prof32; }
 template <>
 __device__ __forceinline__ const int32_t *prof_of<I32N>(const QueryDev &q) { return q.prof32; }
+template <>
+__device__ __forceinline__ const int16_t *prof_of<I32F16>(const QueryDev &q) { return q.prof16; }
+template <>
+__device__ __forceinline__ const int16_t *prof_of<I32N16>(const QueryDev &q) { return q.prof16; }
 
 // One step's profile words: Pk16 reads R/2 dwords (row pairs) per target,
-// the int32 kernels R dwords.  Row pointers are dword aligned (even stride,
-// even R).
+// the int32 kernels R dwords (R/2 over an int16 profile).  Row pointers are
+// dword aligned (even stride, even R).
 template <class P, int R>
 __device__ __forceinline__ void load_raw(const typename P::L *lds, int stride, uint32_t code, int l,
                                          uint32_t (&raw)[R]) {
   // code * stride < 2^24: full-rate 24-bit multiplies
-  if (sizeof(typename P::L) == 2) {
+  if (sizeof(typename P::L) == 2 && P::kPer == 1) {
+    const uint32_t *ra = (const uint32_t *)(lds + (int)__umul24(code & 0xff, (uint32_t)stride) + l * R);
+#pragma unroll
+    for (int i = 0; i < R / 2; i++) raw[i] = ra[i];
+  } else if (sizeof(typename P::L) == 2) {
     const uint32_t *ra = (const uint32_t *)(lds + (int)__umul24(code & 0xff, (uint32_t)stride) + l * R);
     const uint32_t *rb = (const uint32_t *)(lds + (int)__umul24((code >> 8) & 0xff, (uint32_t)stride) + l * R);
 #pragma unroll
@@ -167,7 +185,14 @@ __device__ __forceinline__ void load_raw(const typename P::L *lds, int stride, u
 // raw words -> per-row scores (Pk16: first target low half, second high)
 template <class P, int R>
 __device__ __forceinline__ void unpack_raw(const uint32_t (&raw)[R], typename P::T (&sc)[R]) {
-  if (sizeof(typename P::L) == 2) {
+  if (sizeof(typename P::L) == 2 && P::kPer == 1) {
+    // int32 recurrences over int16 values: row 2i in the low half
+#pragma unroll
+    for (int i = 0; i < R / 2; i++) {
+      sc[2 * i] = (typename P::T)(int32_t)(int16_t)(raw[i] & 0xffffu);
+      sc[2 * i + 1] = (typename P::T)((int32_t)raw[i] >> 16);
+    }
+  } else if (sizeof(typename P::L) == 2) {
 #pragma unroll
     for (int i = 0; i < R / 2; i++) {
       sc[2 * i] = (typename P::T)__builtin_amdgcn_perm(raw[R / 2 + i], raw[i], 0x05040100u);

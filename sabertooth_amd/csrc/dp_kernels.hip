@@ -106,7 +106,7 @@ __device__ __forceinline__ void dp_unit(const Unit &u, const QueryDev *__restric
   c.n_pad = (int)u.n_pad;
   c.m = (int)q.m;
   // the final pass may be lower (r_last rows per lane); packed int16 only
-  const int RL = (sizeof(L) == 2 && u.r_last != 0 && (int)u.r_last < R) ? (int)u.r_last : R;
+  const int RL = (P::kPer == 2 && u.r_last != 0 && (int)u.r_last < R) ? (int)u.r_last : R;
   c.m_pad = ((int)u.passes - 1) * GR + c.G * RL;
 
   // this group's pair slots
@@ -142,7 +142,9 @@ __device__ __forceinline__ void dp_unit(const Unit &u, const QueryDev *__restric
     const int GRp = final_pass ? c.G * RL : GR;
     // stage the profile rows [base, base + GRp) of all codes into LDS
     __syncthreads();
-    if (sizeof(L) == 2) {
+    // int16 rows two at a time where the rows start on a dword (every
+    // query's own profile; a window of a row-reversed profile may not)
+    if (sizeof(L) == 2 && ((((uintptr_t)prof) | ((uintptr_t)q.ld * 2)) & 3) == 0) {
       const int dw = GRp / 2;
       for (int idx = lane; idx < kCodes * dw; idx += 64) {
         const int cc = idx / dw, r = idx - cc * dw;
@@ -159,7 +161,7 @@ __device__ __forceinline__ void dp_unit(const Unit &u, const QueryDev *__restric
     const uint4 *rd = myb + (uint64_t)((p + 1) & 1) * passbuf_half + (uint64_t)c.g * c.n_pad;
     uint4 *wr = myb + (uint64_t)(p & 1) * passbuf_half + (uint64_t)c.g * c.n_pad;
     bool lower = false;
-    if constexpr (sizeof(L) == 2) {
+    if constexpr (P::kPer == 2) {
       if (final_pass && RL != R) {
         run_last_lower<P, R, UNG, END>(RL, c, lds, lds_stride, base, p == 0, rd, wr, e0, vopen, vext, best, top,
                                        edge, lane, bj, bi);
@@ -314,8 +316,13 @@ int dp_occupancy(int kind, int R, bool ung, size_t lds_bytes, bool end) {
   (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, dp_kernel<PP, RR, UU, EE>, 64, lds_bytes)
   if (end) {
     if (kind == 2) ST_OCC(I32N, 8, false, true);
-    else if (R == 8) ST_OCC(I32F, 8, false, true);
+    else if (kind == 4) ST_OCC(I32N16, 8, false, true);
+    else if (kind == 3) {
+      if (R == 8) ST_OCC(I32F16, 8, false, true); else ST_OCC(I32F16, 16, false, true);
+    } else if (R == 8) ST_OCC(I32F, 8, false, true);
     else ST_OCC(I32F, 16, false, true);
+  } else if (kind >= 3) {
+    return 0;  // the int16-profile instances track end cells only
   } else if (kind == 0) {
     if (R == 0) {
       if (ung) ST_OCC(Pk16, 0, true, false); else ST_OCC(Pk16, 0, false, false);
@@ -337,15 +344,17 @@ int dp_occupancy(int kind, int R, bool ung, size_t lds_bytes, bool end) {
   return n;
 }
 
-// kind: 0 = Pk16, 1 = I32F, 2 = I32N; R: 8 or 16 (I32N: 8); end: the
-// score + end-cell instances (int32, no evidence)
 hipError_t clear_u32(uint32_t *p, uint32_t n, hipStream_t s) {
   if (n == 0) return hipSuccess;
   hipLaunchKernelGGL(zero_u32, dim3((n + 63) / 64), dim3(64), 0, s, p, n);
   return hipGetLastError();
 }
 
+// kind: 0 = Pk16, 1 = I32F, 2 = I32N, 3 = I32F16, 4 = I32N16 (the last two:
+// end-cell instances only); R: 8 or 16 (I32N, I32N16: 8); end: the score +
+// end-cell instances (int32, no evidence)
 hipError_t launch_dp(const DpLaunch &a, int kind, int R, bool ung, hipStream_t s, bool end) {
+  if (kind >= 3 && (!end || ung)) return hipErrorInvalidValue;
   hipLaunchKernelGGL(zero_u32, dim3(1), dim3(64), 0, s, a.counter, 1u);
 #define ST_DP(PP, RR, UU, EE)                                                                       \
   hipLaunchKernelGGL((dp_kernel<PP, RR, UU, EE>), dim3(a.grid), dim3(64),                            \
@@ -354,7 +363,10 @@ hipError_t launch_dp(const DpLaunch &a, int kind, int R, bool ung, hipStream_t s
                      a.passbuf_half, a.out, a.lds_stride)
   if (end) {
     if (kind == 2) ST_DP(I32N, 8, false, true);
-    else if (R == 8) ST_DP(I32F, 8, false, true);
+    else if (kind == 4) ST_DP(I32N16, 8, false, true);
+    else if (kind == 3) {
+      if (R == 8) ST_DP(I32F16, 8, false, true); else ST_DP(I32F16, 16, false, true);
+    } else if (R == 8) ST_DP(I32F, 8, false, true);
     else ST_DP(I32F, 16, false, true);
   } else if (kind == 0) {
     if (R == 0) {

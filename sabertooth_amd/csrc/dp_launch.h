@@ -48,17 +48,18 @@ hipError_t launch_dp(const DpLaunch &a, int kind, int R, bool ung, hipStream_t s
 // n words at p set to 0 by a kernel (not a memset: safe under graph capture)
 hipError_t clear_u32(uint32_t *p, uint32_t n, hipStream_t s);
 
-// Reverse score pass inputs (span_kernels.hip).  A query's int32 profile
-// with its m rows reversed into dst + dst_off ([kCodes][dst_ld], dst_ld >=
-// m + 1024); one launch for many queries: chunk_job / chunk_first map each
-// 256-row chunk of a mirror to its job.
+// Reverse score pass inputs (span_kernels.hip).  A query's profile (int16
+// or int32 entries: esize 2 or 4) with its m rows reversed into the bytes
+// dst + dst_off ([kCodes][dst_ld] entries, dst_ld >= m + 1024); one launch
+// for many queries: chunk_job / chunk_first map each 256-row chunk of a
+// mirror to its job.
 struct MirrorJob {
-  const int32_t *src;
+  const void *src;
   uint64_t dst_off;
-  uint32_t src_ld, m, dst_ld, pad;
+  uint32_t src_ld, m, dst_ld, esize;
 };
 hipError_t launch_mirror_prof(const MirrorJob *jobs, const uint32_t *chunk_job, const uint32_t *chunk_first,
-                              uint32_t nchunks, int32_t *dst, hipStream_t s);
+                              uint32_t nchunks, uint8_t *dst, hipStream_t s);
 // Target tid's first len residues, reversed.
 struct RevPrefix {
   uint32_t tid;

@@ -190,7 +190,13 @@ int noalloc_init(int device);
 int dp_occupancy(int kind, int R, bool ung, size_t lds_bytes, bool end = false);
 
 // Precision class of a query under given gaps.
-enum Kind { kPk16 = 0, kI32F = 1, kI32N = 2 };
+// kI32F16 / kI32N16: the int32 recurrences over the query's int16 profile
+// (end-cell batches of queries whose profile fits int16)
+enum Kind { kPk16 = 0, kI32F = 1, kI32N = 2, kI32F16 = 3, kI32N16 = 4 };
+// bytes per profile entry in LDS
+inline int kind_entry_bytes(int k) { return (k == kPk16 || k == kI32F16 || k == kI32N16) ? 2 : 4; }
+// the unfloored recurrences (negative gap costs): 8 rows per lane only
+inline bool kind_unfloored(int k) { return k == kI32N || k == kI32N16; }
 
 Kind kind_for(const st_qprof *qp, int32_t open, int32_t ext);
 

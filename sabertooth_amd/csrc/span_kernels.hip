@@ -2,8 +2,8 @@
 // which scores the reversed prefixes rows[..=qend] and target[..=tend] of
 // each hit to find its start.
 //
-//  * mirror_prof_kernel: each query's int32 profile with its rows in reverse
-//    order, so that the reversed prefix rows[..=e] of any end row e is the
+//  * mirror_prof_kernel: each query's profile (its int16 profile where it
+//    has one, else its int32 profile) with its rows in reverse order, so that the reversed prefix rows[..=e] of any end row e is the
 //    window starting at row m - 1 - e (no profile per hit).  Rows past m are
 //    zero, and at least 1024 of them follow the last row, as every pass of
 //    the DP kernels reads whole passes of up to 1024 rows.  One launch
@@ -23,13 +23,17 @@ namespace st {
 __global__ __launch_bounds__(256) void mirror_prof_kernel(const MirrorJob *__restrict__ jobs,
                                                           const uint32_t *__restrict__ chunk_job,
                                                           const uint32_t *__restrict__ chunk_first,
-                                                          int32_t *__restrict__ dst) {
+                                                          uint8_t *__restrict__ dst) {
   const uint32_t x = blockIdx.x, c = blockIdx.y;
   const uint32_t j = chunk_job[x];
   const MirrorJob jb = jobs[j];
   const uint32_t r = (x - chunk_first[j]) * 256 + threadIdx.x;
   if (r >= jb.dst_ld) return;
-  dst[jb.dst_off + (uint64_t)c * jb.dst_ld + r] = r < jb.m ? jb.src[(uint64_t)c * jb.src_ld + (jb.m - 1 - r)] : 0;
+  const uint64_t o = (uint64_t)c * jb.dst_ld + r, i = (uint64_t)c * jb.src_ld + (jb.m - 1 - r);
+  if (jb.esize == 2)
+    ((int16_t *)(dst + jb.dst_off))[o] = r < jb.m ? ((const int16_t *)jb.src)[i] : (int16_t)0;
+  else
+    ((int32_t *)(dst + jb.dst_off))[o] = r < jb.m ? ((const int32_t *)jb.src)[i] : 0;
 }
 
 // one block per reversed prefix k: bytes [doff[k], doff[k + 1]) (and the
@@ -49,7 +53,7 @@ __global__ __launch_bounds__(256) void rev_prefix_kernel(const uint8_t *__restri
 }
 
 hipError_t launch_mirror_prof(const MirrorJob *jobs, const uint32_t *chunk_job, const uint32_t *chunk_first,
-                              uint32_t nchunks, int32_t *dst, hipStream_t s) {
+                              uint32_t nchunks, uint8_t *dst, hipStream_t s) {
   if (nchunks == 0) return hipSuccess;
   hipLaunchKernelGGL(mirror_prof_kernel, dim3(nchunks, kCodes), dim3(256), 0, s, jobs, chunk_job, chunk_first, dst);
   return hipGetLastError();

@@ -58,3 +58,14 @@ def test_every_group_width(ctx, monkeypatch, G):
         targets = [rng.integers(0, 4, size=int(L)).astype(np.uint8) for L in rng.integers(1, 600, size=20)]
         targets.append(q.copy())
         _check(ctx, O.seq_rows(q), targets, GapCosts(11, 1))
+
+
+@pytest.mark.parametrize("o,e", [(30000, 2000), (0, 0), (-3000, 100)])
+def test_end_rule_profile_outside_int16(ctx, o, e):
+    # values beyond int16: the int32 profile is staged (the int16-profile
+    # instances take the other tests' queries)
+    rng = np.random.default_rng(31 + abs(o) + e)
+    rows = rng.integers(-20000, 40000, size=(150, 21)).astype(np.int32)
+    rows[:, 20] = 0
+    targets = [rng.integers(0, 4, size=int(L)).astype(np.uint8) for L in rng.integers(1, 300, size=25)]
+    _check(ctx, rows, targets, GapCosts(o, e))
