@@ -557,6 +557,81 @@ bool plan_query(Plan &pl, std::vector<Unit> &pending, uint32_t qi, const st_qpro
   return true;
 }
 
+// Window units (end-cell batches): hit k scores target tids[k] against the
+// profile view qps[views[k]] and writes result row outs[k]; the views are
+// windows of one source query, so they share gap costs and kind, and
+// rep_query (any of them) stands for them in the launch order.  Hits are
+// ordered by their rows' 64-row pass count, then target length, and cut
+// into units of 64 / G hits at 8 rows per lane, G per unit by the least
+// modelled device time per useful cell (as plan_query, with the step-cap
+// penalty).  Unit.query carries bit 31 to mark the kind of unit.
+bool plan_windows(Plan &pl, std::vector<Unit> &pending, uint32_t rep_query, const uint32_t *views,
+                  const uint32_t *tids, const uint64_t *outs, size_t count, const st_qprof *const *qps,
+                  const st_db *db, double cap, const PlanEnv *env) {
+  if (count == 0) return true;
+  if (rep_query >> 31) return false;
+  constexpr int R = 8;
+  std::vector<std::pair<uint64_t, uint32_t>> ord(count);
+  for (size_t k = 0; k < count; k++) {
+    const uint64_t m = qps[views[k]]->m, n = db->hlen[tids[k]];
+    ord[k] = {(((m + 63) / 64) << 32) | n, (uint32_t)k};
+  }
+  std::sort(ord.begin(), ord.end());
+  const PlanEnv own = env ? PlanEnv{} : plan_env();
+  const PlanEnv &E = env ? *env : own;
+  for (size_t s = 0; s < count;) {
+    int bG = 64;
+    double bc = 1e300;
+    for (int gi = 0; gi < 7; gi++) {
+      const int G = 1 << gi;
+      if (E.fg && G != E.fg) continue;
+      const size_t e = std::min(count, s + (size_t)(64 / G));
+      uint64_t mmax = 0, np = 0;
+      double useful = 0;
+      for (size_t x = s; x < e; x++) {
+        const uint64_t m = qps[views[ord[x].second]]->m, n = db->hlen[tids[ord[x].second]];
+        mmax = std::max(mmax, m);
+        np = std::max(np, n);
+        useful += (double)m * (double)n;
+      }
+      const uint64_t passes = std::max<uint64_t>(1, (mmax + (uint64_t)G * R - 1) / ((uint64_t)G * R));
+      const double exec = 64.0 * R * (double)passes * (double)(np + G - 1);
+      double c = exec * E.ps[0][gi] / std::max(1.0, useful);
+      const double steps = (double)(passes * R) * (double)(np + G - 1) / 16.0;
+      if (steps > cap) c *= steps / cap;
+      if (c < bc) {
+        bc = c;
+        bG = G;
+      }
+    }
+    const size_t e = std::min(count, s + (size_t)(64 / bG));
+    Unit u{};
+    u.query = rep_query | 0x80000000u;
+    u.pair0 = (uint32_t)pl.pairs.size();
+    u.npairs = (uint32_t)(e - s);
+    u.G = (uint32_t)bG;
+    u.R = R;
+    u.r_last = R;
+    uint64_t mmax = 0, np = 0;
+    for (size_t x = s; x < e; x++) {
+      const uint32_t k = ord[x].second;
+      mmax = std::max<uint64_t>(mmax, qps[views[k]]->m);
+      np = std::max<uint64_t>(np, db->hlen[tids[k]]);
+      PairDev pd;
+      pd.tid = tids[k];
+      pd.out = (uint32_t)outs[k];
+      pl.pairs.push_back(pd);
+      pl.pair_q.push_back(views[k]);
+    }
+    u.n_pad = (uint32_t)np;
+    u.passes = (uint32_t)std::max<uint64_t>(1, (mmax + (uint64_t)bG * R - 1) / ((uint64_t)bG * R));
+    pl.exec_cells += (uint64_t)64 * R * u.passes * (u.n_pad + bG - 1);
+    pending.push_back(u);
+    s = e;
+  }
+  return true;
+}
+
 // plan_query over the queries [0, nq): one(pl, pending, q) plans query q.
 // Batches of many pairs are planned on up to 8 threads, each over a
 // contiguous run of queries into a plan of its own; the runs are appended
@@ -590,6 +665,7 @@ static uint64_t plan_queries(Plan &pl, std::vector<Unit> &pending, uint64_t nq, 
   for (uint64_t t = 0; t < T; t++) {
     const uint32_t off = (uint32_t)pl.pairs.size();
     pl.pairs.insert(pl.pairs.end(), pls[t].pairs.begin(), pls[t].pairs.end());
+    pl.pair_q.insert(pl.pair_q.end(), pls[t].pair_q.begin(), pls[t].pair_q.end());
     pl.exec_cells += pls[t].exec_cells;
     for (Unit u : pend[t]) {
       u.pair0 += off;
@@ -636,8 +712,11 @@ static void radix_sort_keys(std::vector<std::pair<uint64_t, uint32_t>> &v) {
 void finish_plan(Plan &pl, std::vector<Unit> &pending, const std::vector<Kind> &qkind,
                         const std::vector<int32_t> &open, const std::vector<int32_t> &ext, int force_kind,
                         int cus, bool ung, bool end) {
-  auto kind_of = [&](const Unit &u) { return force_kind >= 0 ? force_kind : (int)qkind[u.query]; };
-  auto big = [](const Unit &u) { return u.G * u.R > 256 ? 1 : 0; };
+  // a window unit (bit 31 of query) stages 64 lanes' rows, one part per group
+  auto qi = [](const Unit &u) { return u.query & 0x7fffffffu; };
+  auto gr_of = [](const Unit &u) { return (u.query >> 31) ? 64 * u.R : u.G * u.R; };
+  auto kind_of = [&](const Unit &u) { return force_kind >= 0 ? force_kind : (int)qkind[qi(u)]; };
+  auto big = [&](const Unit &u) { return gr_of(u) > 256 ? 1 : 0; };
   // packed int16 units of both heights share a launch (the kernel instance
   // with R = 0 takes each unit's own R); the int32 kernels are per R
   auto r_of = [&](const Unit &u) { return kind_of(u) == kPk16 ? 0 : (int)u.R; };
@@ -649,7 +728,7 @@ void finish_plan(Plan &pl, std::vector<Unit> &pending, const std::vector<Kind> &
   // the distinct gap costs (few: usually one), ranked
   std::vector<std::pair<int32_t, int32_t>> gaps;
   for (const Unit &u : pending) {
-    const std::pair<int32_t, int32_t> g(open[u.query], ext[u.query]);
+    const std::pair<int32_t, int32_t> g(open[qi(u)], ext[qi(u)]);
     if (gaps.empty() || gaps.back() != g) gaps.push_back(g);
   }
   std::sort(gaps.begin(), gaps.end());
@@ -660,7 +739,7 @@ void finish_plan(Plan &pl, std::vector<Unit> &pending, const std::vector<Kind> &
   std::vector<uint64_t> cls(pending.size());
   for (size_t i = 0; i < pending.size(); i++) {
     const Unit &u = pending[i];
-    const uint64_t g = (uint64_t)(std::lower_bound(gaps.begin(), gaps.end(), std::make_pair(open[u.query], ext[u.query])) -
+    const uint64_t g = (uint64_t)(std::lower_bound(gaps.begin(), gaps.end(), std::make_pair(open[qi(u)], ext[qi(u)])) -
                                   gaps.begin());
     cls[i] = ((uint64_t)kind_of(u) << 56) | ((uint64_t)r_of(u) << 48) | ((uint64_t)(1 - big(u)) << 40) | g;
   }
@@ -697,18 +776,18 @@ void finish_plan(Plan &pl, std::vector<Unit> &pending, const std::vector<Kind> &
     Launch L{};
     L.kind = kind_of(u);
     L.R = r_of(u);
-    L.open = open[u.query];
-    L.ext = ext[u.query];
+    L.open = open[qi(u)];
+    L.ext = ext[qi(u)];
     L.unit0 = (uint32_t)pl.units.size();
     size_t j = i;
     L.gr_max = 0;
     L.half = 0;
     while (j < pending.size()) {
       const Unit &v = pending[j];
-      if (kind_of(v) != L.kind || r_of(v) != L.R || big(v) != big(u) || open[v.query] != L.open ||
-          ext[v.query] != L.ext)
+      if (kind_of(v) != L.kind || r_of(v) != L.R || big(v) != big(u) || open[qi(v)] != L.open ||
+          ext[qi(v)] != L.ext)
         break;
-      L.gr_max = std::max(L.gr_max, (int)(v.G * v.R));
+      L.gr_max = std::max(L.gr_max, (int)gr_of(v));
       if (v.passes > 1) L.half = std::max<uint64_t>(L.half, (uint64_t)(64 / v.G) * v.n_pad);
       pl.units.push_back(v);
       j++;
@@ -747,6 +826,7 @@ struct st_batch {
   DevBuf<int32_t> d_open, d_ext, d_gate;
   DevBuf<Unit> d_units;
   DevBuf<PairDev> d_pairs;
+  DevBuf<uint32_t> d_pair_q;    // window units: each pair's profile view
   DevBuf<PairOut> d_pout;       // score-only batches
   DevBuf<uint8_t> d_res;        // evidence batches: st_pair_result rows
   DevBuf<uint32_t> d_sat;       // int16 pairs at the clamp: count, then rows
@@ -797,12 +877,14 @@ static int launch_plan(st_batch *b, const Plan &pl, const Unit *d_units, const P
       ST_HIP(hipStreamWaitEvent(s, start, 0));
       used[L.slot] = true;
     }
-    DpLaunch a;
+    DpLaunch a{};
     a.units = d_units + L.unit0;
     a.nunits = L.nunits;
     a.counter = ctx->counter.p + L.slot;
     a.pairs = d_pairs;
     a.queries = b->d_queries.p;
+    // window units occur in the main plan of end-cell batches only
+    a.pair_q = &pl == &b->plan && !b->plan.pair_q.empty() ? b->d_pair_q.p : nullptr;
     a.arena = b->db->arena.p;
     a.toff = b->db->off.p;
     a.tlen = b->db->len.p;
@@ -840,11 +922,14 @@ extern "C" {
 
 // Plan a batch and put its tables on the device.  reuse (nullable): an
 // earlier batch object of the same context to refill; its device buffers
-// are kept where they are large enough.
+// are kept where they are large enough.  win_off (nullable, end-cell
+// batches): the queries are profile views, those in [win_off[s],
+// win_off[s + 1]) windows of one source s, planned as window units.
 static int batch_create_impl(st_ctx *ctx, const st_db *db, const st_qprof *const *queries, uint64_t nq,
                              const uint64_t *cand_off, const uint32_t *cand_tids, const int32_t *gap_open,
                              const int32_t *gap_extend, const int32_t *min_diag_score, bool ung,
-                             st_batch **out, bool end = false, st_batch *reuse = nullptr) {
+                             st_batch **out, bool end = false, st_batch *reuse = nullptr,
+                             const uint64_t *win_off = nullptr, uint64_t nwin = 0) {
   if (!ctx || !db || !out || (nq && (!queries || !cand_off || !gap_open || !gap_extend)))
     return fail(ST_ERR_INVALID, "st_batch_create: null argument");
   *out = nullptr;
@@ -914,16 +999,39 @@ static int batch_create_impl(st_ctx *ctx, const st_db *db, const st_qprof *const
   std::vector<Unit> pending;
   const double cap = unit_steps_cap(cells);
   const PlanEnv env = plan_env();
-  const uint64_t bad = plan_queries(b->plan, pending, nq, np, [&](Plan &pl, std::vector<Unit> &pend, uint64_t q) {
-    const uint64_t s = cand_off[q], e = cand_off[q + 1];
-    std::vector<uint64_t> outs(e - s);
-    std::iota(outs.begin(), outs.end(), s);
-    return plan_query(pl, pend, (uint32_t)q, b->qps[q], cand_tids + s, outs.data(), e - s, b->qkind[q], db, cap,
-                      nullptr, &env);
-  });
-  if (bad < nq) {
-    drop();
-    return fail(ST_ERR_INVALID, "st_batch_create: no launch shape fits query " + std::to_string(bad));
+  uint64_t bad = 0;
+  if (win_off) {
+    // one plan_windows per source over the hits of all its views
+    bad = plan_queries(b->plan, pending, nwin, np, [&](Plan &pl, std::vector<Unit> &pend, uint64_t w) {
+      const uint64_t v0 = win_off[w], v1 = win_off[w + 1];
+      if (v0 >= v1) return true;
+      std::vector<uint32_t> vs, ts;
+      std::vector<uint64_t> os;
+      for (uint64_t v = v0; v < v1; v++)
+        for (uint64_t k = cand_off[v]; k < cand_off[v + 1]; k++) {
+          vs.push_back((uint32_t)v);
+          ts.push_back(cand_tids[k]);
+          os.push_back(k);
+        }
+      return plan_windows(pl, pend, (uint32_t)v0, vs.data(), ts.data(), os.data(), vs.size(), b->qps.data(), db, cap,
+                          &env);
+    });
+    if (bad < nwin) {
+      drop();
+      return fail(ST_ERR_INVALID, "st_batch_create: no window units for source " + std::to_string(bad));
+    }
+  } else {
+    bad = plan_queries(b->plan, pending, nq, np, [&](Plan &pl, std::vector<Unit> &pend, uint64_t q) {
+      const uint64_t s = cand_off[q], e = cand_off[q + 1];
+      std::vector<uint64_t> outs(e - s);
+      std::iota(outs.begin(), outs.end(), s);
+      return plan_query(pl, pend, (uint32_t)q, b->qps[q], cand_tids + s, outs.data(), e - s, b->qkind[q], db, cap,
+                        nullptr, &env);
+    });
+    if (bad < nq) {
+      drop();
+      return fail(ST_ERR_INVALID, "st_batch_create: no launch shape fits query " + std::to_string(bad));
+    }
   }
   finish_plan(b->plan, pending, b->qkind, b->open, b->ext, -1, ctx->cus, ung, end);
   // useful and executed cells of every launch (roofline accounting)
@@ -931,9 +1039,10 @@ static int batch_create_impl(st_ctx *ctx, const st_db *db, const st_qprof *const
     L.useful_cells = L.exec_cells = 0;
     for (uint32_t u = L.unit0; u < L.unit0 + L.nunits; u++) {
       const Unit &U = b->plan.units[u];
-      uint64_t cols = 0;
-      for (uint32_t k = 0; k < U.npairs; k++) cols += db->hlen[b->plan.pairs[U.pair0 + k].tid];
-      L.useful_cells += (uint64_t)b->qps[U.query]->m * cols;
+      for (uint32_t k = 0; k < U.npairs; k++) {
+        const uint32_t v = (U.query >> 31) ? b->plan.pair_q[U.pair0 + k] : U.query;
+        L.useful_cells += (uint64_t)b->qps[v]->m * db->hlen[b->plan.pairs[U.pair0 + k].tid];
+      }
       L.exec_cells += (uint64_t)64 * (L.kind == kPk16 ? 2 : 1) * ((U.passes - 1) * U.R + U.r_last) * (U.n_pad + U.G - 1);
     }
   }
@@ -949,6 +1058,7 @@ static int batch_create_impl(st_ctx *ctx, const st_db *db, const st_qprof *const
             b->d_gate.put(b->gate.data(), nq, st) == hipSuccess &&
             b->d_units.put(b->plan.units.data(), b->plan.units.size(), st) == hipSuccess &&
             b->d_pairs.put(b->plan.pairs.data(), b->plan.pairs.size(), st) == hipSuccess &&
+            b->d_pair_q.put(b->plan.pair_q.data(), b->plan.pair_q.size(), st) == hipSuccess &&
             b->d_pout.reserve(std::max<uint64_t>(np, 1)) == hipSuccess &&
             (!ung || b->d_res.reserve(std::max<uint64_t>(np, 1) * sizeof(st_pair_result)) == hipSuccess) &&
             b->d_sat.reserve(np + 1) == hipSuccess &&
@@ -1193,7 +1303,8 @@ int st_sw_score_batch(st_ctx *ctx, const st_qprof *qp, const st_db *db, const ui
 
 static int score_end_multi(st_ctx *ctx, const st_db *db, const st_qprof *const *qps, uint64_t nq,
                            const uint64_t *off, const uint32_t *tids, const int32_t *open, const int32_t *ext,
-                           st_score_end *out, const char *who);
+                           st_score_end *out, const char *who, const uint64_t *win_off = nullptr,
+                           uint64_t nwin = 0);
 
 int st_sw_score_end_batch(st_ctx *ctx, const st_qprof *qp, const st_db *db, const uint32_t *tids, uint64_t count,
                           int32_t gap_open, int32_t gap_extend, st_score_end *out) {
@@ -1205,17 +1316,19 @@ int st_sw_score_end_batch(st_ctx *ctx, const st_qprof *qp, const st_db *db, cons
 }
 
 // Scores and end cells of a multi-query candidate list (the int32
-// end-tracking kernels), in candidate order.
+// end-tracking kernels), in candidate order.  win_off (nullable): the
+// queries are profile views grouped by source, planned as window units.
 static int score_end_multi(st_ctx *ctx, const st_db *db, const st_qprof *const *qps, uint64_t nq,
                            const uint64_t *off, const uint32_t *tids, const int32_t *open, const int32_t *ext,
-                           st_score_end *out, const char *who) {
+                           st_score_end *out, const char *who, const uint64_t *win_off, uint64_t nwin) {
   const uint64_t count = nq ? off[nq] : 0;
   if (count == 0) return ST_OK;
   // the context's end-cell batch object, refilled: its device buffers and
   // events outlive the call
   std::lock_guard<std::mutex> el(ctx->end_mu);
   st_batch *b = nullptr;
-  int rc = batch_create_impl(ctx, db, qps, nq, off, tids, open, ext, nullptr, false, &b, true, ctx->end_batch);
+  int rc = batch_create_impl(ctx, db, qps, nq, off, tids, open, ext, nullptr, false, &b, true, ctx->end_batch,
+                             win_off, nwin);
   if (rc) return rc;
   ctx->end_batch = b;
   rc = st_batch_run(b);
@@ -1389,9 +1502,23 @@ int st_align_spans_batch(st_ctx *ctx, const st_db *db, const st_qprof *const *qu
     vt[k] = (uint32_t)k;
     voff.back()++;
   }
+  // the views of one query are consecutive (rv is sorted by query): each
+  // query's views are planned together, several views to a unit
+  std::vector<uint64_t> win_off{0};
+  {
+    uint64_t v = 0;
+    for (uint64_t k = 0; k < nr; k++) {
+      const bool nv = k == 0 || rv[k].q != rv[k - 1].q || rv[k].e != rv[k - 1].e;
+      if (nv) {
+        if (k > 0 && rv[k].q != rv[k - 1].q) win_off.push_back(v);
+        v++;
+      }
+    }
+    win_off.push_back(v);
+  }
   std::vector<st_score_end> rs(nr);
   rc = score_end_multi(ctx, rdb, vq.data(), vq.size(), voff.data(), vt.data(), vopen.data(), vext.data(),
-                       rs.data(), "st_align_spans_batch");
+                       rs.data(), "st_align_spans_batch", win_off.data(), win_off.size() - 1);
   if (rc) return rc;
   for (uint64_t k = 0; k < nr; k++) {
     st_span &s = out[rv[k].hit];

@@ -251,6 +251,88 @@ __device__ __forceinline__ void dp_unit(const Unit &u, const QueryDev *__restric
   }
 }
 
+// A window unit (end-cell kernels, one pair per group): group g scores pair
+// slot g against its own profile view queries[pair_q[slot]], staged into its
+// own part of the LDS slice (rows [g * G * R, (g + 1) * G * R) of every
+// code), so one wave holds 64 / G pairs of different views (the reversed
+// prefixes of several hits).  Rows past a view's m are staged as zeros,
+// like the zero rows that follow every view of a row-reversed profile; the
+// unit runs the passes of its tallest view.
+template <class P, int R>
+__device__ __forceinline__ void dp_unit_win(const Unit &u, const QueryDev *__restrict__ queries,
+                                            const uint32_t *__restrict__ pair_q, const PairDev *__restrict__ pairs,
+                                            const uint8_t *__restrict__ arena, const uint64_t *__restrict__ toff,
+                                            const uint32_t *__restrict__ tlen, typename P::L *lds, int lds_stride,
+                                            uint4 *myb, typename P::T *edge, uint64_t passbuf_half,
+                                            typename P::T e0, typename P::T vopen, typename P::T vext,
+                                            const DpOut &o, int lane) {
+  using T = typename P::T;
+  using L = typename P::L;
+  using K = typename P::K;
+  GroupCtx<P> c;
+  c.G = (int)u.G;
+  c.l = lane & (c.G - 1);
+  c.g = lane / c.G;
+  const int GR = c.G * R;
+  c.n_pad = (int)u.n_pad;
+  const bool valid = (uint32_t)c.g < u.npairs;
+  const L *prof = nullptr;
+  uint32_t ld = 0, outi = 0;
+  c.m = 0;
+  c.len[0] = c.len[1] = 0;
+  c.tp[0] = c.tp[1] = arena;
+  if (valid) {
+    const PairDev pd = pairs[u.pair0 + c.g];
+    const QueryDev q = queries[pair_q[u.pair0 + c.g]];
+    prof = prof_of<P>(q);
+    ld = q.ld;
+    c.m = (int)q.m;
+    c.len[0] = (int)tlen[pd.tid];
+    c.tp[0] = arena + toff[pd.tid];
+    outi = pd.out;
+  }
+  c.m_pad = (int)u.passes * GR;
+  c.has[0] = valid && c.len[0] > 0 && c.m > 0;
+  c.has[1] = false;
+  K top[1][4] = {{0, 0, 0, 0}};
+  T best = 0;
+  int bj = 0x7fffffff, bi = 0;
+  L *const mine = lds + c.g * GR;
+  for (int p = 0; p < (int)u.passes; p++) {
+    const int base = p * GR;
+    // this group's rows [base, base + GR) of its view, zeros past its m
+    __syncthreads();
+    for (int idx = c.l; idx < kCodes * GR; idx += c.G) {
+      const int cc = idx / GR, r = idx - cc * GR;
+      mine[cc * lds_stride + r] = base + r < c.m ? prof[(uint64_t)cc * ld + base + r] : (L)0;
+    }
+    __syncthreads();
+    const uint4 *rd = myb + (uint64_t)((p + 1) & 1) * passbuf_half + (uint64_t)c.g * c.n_pad;
+    uint4 *wr = myb + (uint64_t)(p & 1) * passbuf_half + (uint64_t)c.g * c.n_pad;
+    const bool final_pass = p == (int)u.passes - 1;
+    run_pass_sel<P, R, false, true>(false, c, mine, lds_stride, base, p == 0, final_pass, rd, wr, e0, vopen, vext,
+                                    best, top, edge, lane, bj, bi);
+  }
+  // the group's optimum: (score desc, column asc, row asc)
+  for (int off = 1; off < c.G; off <<= 1) {
+    const T ob = (T)__shfl_xor((int)best, off);
+    const int oj = __shfl_xor(bj, off), oi = __shfl_xor(bi, off);
+    if ((int)ob > (int)best || ((int)ob == (int)best && (oj < bj || (oj == bj && oi < bi)))) {
+      bj = oj;
+      bi = oi;
+    }
+    best = P::max(best, ob);
+  }
+  if (c.l == 0 && valid) {
+    PairOut po;
+    po.swg = (int)best;
+    po.flag = 2;  // exact int32
+    o.pout[outi] = po;
+    o.qend[outi] = (int)best > 0 ? bi : -1;
+    o.tend[outi] = (int)best > 0 ? bj : -1;
+  }
+}
+
 // Persistent DP kernel: every block (one wave) pulls units from *counter.
 // R = 8 or 16: rows per lane of every unit; R = 0: each unit's own R (8, 12
 // or 16), so one launch holds units of every height.
@@ -260,7 +342,7 @@ template <class P, int R, bool UNG, bool END>
 __global__ __launch_bounds__(64, (R == 8 && !END) ? 4 : (R == 0 ? 3 : 2)) void dp_kernel(
     const Unit *__restrict__ units, uint32_t nunits, uint32_t *__restrict__ counter,
     const PairDev *__restrict__ pairs, const QueryDev *__restrict__ queries,
-    const uint8_t *__restrict__ arena, const uint64_t *__restrict__ toff,
+    const uint32_t *__restrict__ pair_q, const uint8_t *__restrict__ arena, const uint64_t *__restrict__ toff,
     const uint32_t *__restrict__ tlen, int32_t gap_open, int32_t gap_ext,
     uint4 *__restrict__ passbuf, uint64_t passbuf_half, DpOut out, int lds_stride) {
   using T = typename P::T;
@@ -284,6 +366,14 @@ __global__ __launch_bounds__(64, (R == 8 && !END) ? 4 : (R == 0 ? 3 : 2)) void d
     const uint32_t ui = __builtin_amdgcn_readfirstlane(__shfl(got, 0));
     if (ui >= nunits) break;
     const Unit u = units[ui];
+    if constexpr (END && P::kPer == 1 && R == 8) {
+      // bit 31 of the query index marks a window unit
+      if (u.query >> 31) {
+        dp_unit_win<P, R>(u, queries, pair_q, pairs, arena, toff, tlen, lds, lds_stride, myb, edge, passbuf_half, e0,
+                          vopen, vext, out, lane);
+        continue;
+      }
+    }
     if constexpr (R == 0) {
       if (u.R == 12)
         dp_unit<P, 12, UNG, END>(u, queries, pairs, arena, toff, tlen, lds, lds_stride, myb, edge, passbuf_half, e0,
@@ -359,7 +449,7 @@ hipError_t launch_dp(const DpLaunch &a, int kind, int R, bool ung, hipStream_t s
 #define ST_DP(PP, RR, UU, EE)                                                                       \
   hipLaunchKernelGGL((dp_kernel<PP, RR, UU, EE>), dim3(a.grid), dim3(64),                            \
                      dp_lds_bytes(a.lds_stride, sizeof(typename PP::L), UU), s, a.units, a.nunits,   \
-                     a.counter, a.pairs, a.queries, a.arena, a.toff, a.tlen, a.open, a.ext, a.passbuf, \
+                     a.counter, a.pairs, a.queries, a.pair_q, a.arena, a.toff, a.tlen, a.open, a.ext, a.passbuf, \
                      a.passbuf_half, a.out, a.lds_stride)
   if (end) {
     if (kind == 2) ST_DP(I32N, 8, false, true);

@@ -25,6 +25,7 @@ struct DpLaunch {
   uint32_t *counter;
   const PairDev *pairs;
   const QueryDev *queries;
+  const uint32_t *pair_q;  // window units (end-cell kernels): each pair slot's profile view
   const uint8_t *arena;
   const uint64_t *toff;
   const uint32_t *tlen;

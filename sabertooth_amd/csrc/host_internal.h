@@ -224,6 +224,7 @@ struct Launch {
 struct Plan {
   std::vector<Unit> units;
   std::vector<PairDev> pairs;
+  std::vector<uint32_t> pair_q;  // window units: each pair's profile view (parallel to pairs)
   std::vector<Launch> launches;
   uint64_t buf_elems = 0;
   uint64_t exec_cells = 0;  // cells the launches execute, padding included

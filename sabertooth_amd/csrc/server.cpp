@@ -279,7 +279,7 @@ int launch_on(Slot &sl, const Plan &pl, const QueryDev *d_q, const Unit *d_units
   for (size_t i = 0; i < nl; i++) {
     const Launch &L = pl.launches[i];
     const int k = (int)(i % kSubStreams);
-    DpLaunch a;
+    DpLaunch a{};
     a.units = d_units + L.unit0;
     a.nunits = L.nunits;
     a.counter = sl.counter.p + (fan ? k : 0);

@@ -96,6 +96,29 @@ def test_spans_profiles_in_and_outside_int16(ctx):
     _check(ctx, rows, targets, [hits] * len(rows), gaps)
 
 
+@pytest.mark.parametrize("G", [1, 4, 16, 64])
+def test_reverse_windows_every_group_width(ctx, monkeypatch, G):
+    # the reverse pass puts up to 64 / G reversed prefixes of one query in a
+    # unit, each group with its own profile window; many hits per query with
+    # different end rows, int16 and int32 profiles, floored and unfloored gaps
+    monkeypatch.setenv("ST_FORCE_G", str(G))
+    rng = np.random.default_rng(40 + G)
+    queries = [rng.integers(0, 20, size=L, dtype=np.uint8) for L in (35, 180, 620)]
+    targets = [rng.integers(0, 20, size=int(rng.integers(1, 700)), dtype=np.uint8) for _ in range(50)]
+    for q in queries:
+        for _ in range(6):
+            a = int(rng.integers(0, len(q)))
+            b = int(rng.integers(a + 1, len(q) + 1))
+            targets.append(np.concatenate([rng.integers(0, 20, size=int(rng.integers(0, 40)), dtype=np.uint8),
+                                           _mutate(rng, q[a:b], 0.75, 0.02)]))
+    wide = rng.integers(-40000, 45000, size=(140, 21)).astype(np.int32)
+    wide[:, 20] = 0
+    rows = [O.seq_rows(q) for q in queries] + [wide, O.seq_rows(queries[1])]
+    hits = np.arange(len(targets), dtype=np.uint32)
+    gaps = [GapCosts(11, 1), GapCosts(3, 1), GapCosts(11, 1), GapCosts(30000, 2000), GapCosts(-1, 1)]
+    _check(ctx, rows, targets, [hits] * len(rows), gaps)
+
+
 def test_long_spans_above_int16(ctx):
     W = O.aa2num()[ord("W")]
     rng = np.random.default_rng(23)
