@@ -1359,6 +1359,7 @@ static int score_end_multi(st_ctx *ctx, const st_db *db, const st_qprof *const *
 // row-reversed profiles, the reversed target prefixes as a database, and
 // the tables of the kernels that write them.
 struct SpanWork {
+  std::vector<st_qprof> views;  // the reversed prefixes' profile views, reused
   DevBuf<uint8_t> mirror;
   DevBuf<RevPrefix> pre;
   DevBuf<MirrorJob> mj;
@@ -1396,8 +1397,12 @@ int st_align_spans_batch(st_ctx *ctx, const st_db *db, const st_qprof *const *qu
     uint32_t q, e;
     uint64_t hit;
   };
+  // (query, end row) order: the hits come query by query, so each query's
+  // run is sorted by end row on its own (stable: hit order within a row)
   std::vector<Rev> rv;
-  for (uint64_t q = 0; q < nq; q++)
+  rv.reserve(nh);
+  for (uint64_t q = 0; q < nq; q++) {
+    const size_t r0 = rv.size();
     for (uint64_t k = hit_off[q]; k < hit_off[q + 1]; k++)
       if (fw[k].score > 0) {
         // the gathers below read rows [0, query_end] and residues [0, target_end]
@@ -1406,8 +1411,9 @@ int st_align_spans_batch(st_ctx *ctx, const st_db *db, const st_qprof *const *qu
           return fail(ST_ERR_DEVICE, "st_align_spans_batch: end cell outside its problem");
         rv.push_back({(uint32_t)q, (uint32_t)fw[k].query_end, k});
       }
+    std::stable_sort(rv.begin() + r0, rv.end(), [](const Rev &a, const Rev &b) { return a.e < b.e; });
+  }
   if (rv.empty()) return ST_OK;
-  std::stable_sort(rv.begin(), rv.end(), [](const Rev &a, const Rev &b) { return a.q != b.q ? a.q < b.q : a.e < b.e; });
   const uint64_t nr = rv.size();
   if (nr > 0xffffffffull) return fail(ST_ERR_INVALID, "st_align_spans_batch: more than 2^32 hits");
   // row-reversed profiles of the queries with hits (int16 where the query
@@ -1474,7 +1480,9 @@ int st_align_spans_batch(st_ctx *ctx, const st_db *db, const st_qprof *const *qu
   // pass inside it)
   uint64_t nviews = 0;
   for (uint64_t k = 0; k < nr; k++) nviews += k == 0 || rv[k].q != rv[k - 1].q || rv[k].e != rv[k - 1].e;
-  std::unique_ptr<st_qprof[]> views(new st_qprof[nviews]);
+  // st_qprof objects hold no allocations of their own here (views only)
+  if (w.views.size() < nviews) w.views = std::vector<st_qprof>(nviews + nviews / 4);
+  st_qprof *views = w.views.data();
   std::vector<const st_qprof *> vq;
   vq.reserve(nviews);
   std::vector<uint64_t> voff{0};
