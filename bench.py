@@ -241,18 +241,22 @@ def spans_leg(O, st, ctx, db, qps, rows, w, res, offs, per_query=32, sample=120)
     """The two score passes of align_canonical on the device for each query's
     best-scoring pairs of the step (the hits a search would trace back):
     forward score and end cell, then the reversed prefixes for the start.
-    Wall time of the call (planning, uploads, both passes, read-back); a
-    seeded sample of spans against the oracle."""
+    Wall time of the call (planning, uploads, both passes, read-back), the
+    mean of three calls after one at the same size; a seeded sample of spans
+    against the oracle."""
     hit_lists = []
     for qi in range(len(qps)):
         sw = res["swg"][offs[qi]:offs[qi + 1]]
         top = np.argsort(-sw, kind="stable")[:per_query]
         hit_lists.append(np.asarray(w.cand_lists[qi], np.uint32)[top[sw[top] > 0]])
     gaps = [st.GapCosts(w.gap_open, w.gap_extend)] * len(qps)
-    st.align_spans(ctx, db, qps[:8], hit_lists[:8], gaps[:8])
+    # one untimed call at full size (the context's span buffers grow to it),
+    # then the mean of three timed calls
+    st.align_spans(ctx, db, qps, hit_lists, gaps)
     ts = time.perf_counter()
-    sp = st.align_spans(ctx, db, qps, hit_lists, gaps)
-    t = time.perf_counter() - ts
+    for _ in range(3):
+        sp = st.align_spans(ctx, db, qps, hit_lists, gaps)
+    t = (time.perf_counter() - ts) / 3
     tl = w.target_lengths
     fwd = sum(len(rows[qi]) * int(tl[hit_lists[qi]].sum()) for qi in range(len(qps)))
     pos = sp["score"] > 0
@@ -266,10 +270,20 @@ def spans_leg(O, st, ctx, db, qps, rows, w, res, offs, per_query=32, sample=120)
         want = O.span(rows[qof[k]], w.target(int(tof[k])), w.gap_open, w.gap_extend)
         got = tuple(int(sp[k][f]) for f in ("score", "query_start", "query_end", "target_start", "target_end"))
         mism += got != want
+    li, nf = st.align_spans_launches(ctx)
+    launches = [{"pass": "forward" if i < nf else "reverse", "kind": int(r["kind"]),
+                 "rows_per_lane": int(r["rows_per_lane"]), "units": int(r["units"]), "grid": int(r["grid"]),
+                 "useful_cells": int(r["useful_cells"]), "exec_cells": int(r["exec_cells"]),
+                 "ms": round(float(r["ms"]), 3)} for i, r in enumerate(li)]
+    useful = {p: sum(x["useful_cells"] for x in launches if x["pass"] == p) for p in ("forward", "reverse")}
+    execd = {p: sum(x["exec_cells"] for x in launches if x["pass"] == p) for p in ("forward", "reverse")}
     out = {"hits": int(len(sp)), "ms": round(1e3 * t, 3), "forward_cells": int(fwd), "reverse_cells": rev,
-           "gcups": round((fwd + rev) / t / 1e9, 1), "sampled_vs_oracle": int(len(pick)), "mismatches": int(mism)}
+           "gcups": round((fwd + rev) / t / 1e9, 1), "sampled_vs_oracle": int(len(pick)), "mismatches": int(mism),
+           "executed_over_useful": {p: round(execd[p] / max(useful[p], 1), 4) for p in ("forward", "reverse")},
+           "launches": launches}
     log(f"alignment spans: {out['hits']} hits in {out['ms']:.1f} ms, {out['gcups']} GCUPS over both passes, "
-        f"{len(pick) - mism}/{len(pick)} sampled spans bit-exact")
+        f"{len(pick) - mism}/{len(pick)} sampled spans bit-exact; executed/useful cells forward "
+        f"{out['executed_over_useful']['forward']}, reverse {out['executed_over_useful']['reverse']}")
     return out
 
 

@@ -398,8 +398,9 @@ int st_batch_timing(const st_batch *b, double *ms3, uint64_t *rescored);
  * empty group slots), work units and kernel launches. */
 int st_batch_stats(const st_batch *b, uint64_t *exec_cells, uint64_t *units, uint64_t *launches);
 /* One DP kernel launch of a batch's plan: precision (0 packed int16, 1
- * int32 floored, 2 int32 unfloored), rows per lane (8 or 16; 0: each unit's
- * own, 8 or 16 — the packed int16 launches), units, grid, useful and
+ * int32 floored, 2 int32 unfloored, 3 and 4 the same over the query's int16
+ * profile: end-cell launches), rows per lane (8 or 16; 0: each unit's own,
+ * 8, 12 or 16 — the packed int16 launches), units, grid, useful and
  * executed cells, and its device time in the last st_batch_run (HIP events
  * around the launch on its stream). */
 typedef struct {
@@ -413,6 +414,12 @@ typedef struct {
 } st_launch_info;
 /* Writes up to cap launches; *n = the plan's launch count. */
 int st_batch_launches(const st_batch *b, st_launch_info *out, uint64_t cap, uint64_t *n);
+/* The DP launches of the last st_align_spans_batch call on ctx: its forward
+ * launches first (*n_forward of them), then its reverse launches; up to cap
+ * are written, *n = all of them.  A diagnostic with no counterpart in the
+ * reference, for the executed/useful cells and device time of each pass. */
+int st_align_spans_launches(st_ctx *ctx, st_launch_info *out, uint64_t cap, uint64_t *n,
+                            uint64_t *n_forward);
 /* DP cells of the batch (sum of m*n) and pairs. */
 uint64_t st_batch_cells(const st_batch *b);
 uint64_t st_batch_pairs(const st_batch *b);

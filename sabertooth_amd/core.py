@@ -260,6 +260,17 @@ def align_spans(ctx: Context, db: TargetDb, queries: Sequence[QueryProfile], hit
     return out[: len(tids)]
 
 
+def align_spans_launches(ctx: Context):
+    """(launches, n_forward): the DP launches of the last align_spans call on
+    ctx (LAUNCH_INFO_DT rows: the forward pass's first, then the reverse
+    pass's) with their executed/useful cells and device time."""
+    n, nf = C.c_uint64(0), C.c_uint64(0)
+    check(lib().st_align_spans_launches(ctx.handle, None, 0, C.byref(n), C.byref(nf)))
+    out = np.zeros(max(n.value, 1), dtype=_lib.LAUNCH_INFO_DT)
+    check(lib().st_align_spans_launches(ctx.handle, ptr(out), n.value, C.byref(n), C.byref(nf)))
+    return out[: n.value], int(nf.value)
+
+
 class Postings:
     """An index's posting lists on the device: (target << 32) | position."""
 

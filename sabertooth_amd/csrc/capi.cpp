@@ -1304,7 +1304,7 @@ int st_sw_score_batch(st_ctx *ctx, const st_qprof *qp, const st_db *db, const ui
 static int score_end_multi(st_ctx *ctx, const st_db *db, const st_qprof *const *qps, uint64_t nq,
                            const uint64_t *off, const uint32_t *tids, const int32_t *open, const int32_t *ext,
                            st_score_end *out, const char *who, const uint64_t *win_off = nullptr,
-                           uint64_t nwin = 0);
+                           uint64_t nwin = 0, std::vector<st_launch_info> *launches = nullptr);
 
 int st_sw_score_end_batch(st_ctx *ctx, const st_qprof *qp, const st_db *db, const uint32_t *tids, uint64_t count,
                           int32_t gap_open, int32_t gap_extend, st_score_end *out) {
@@ -1320,7 +1320,8 @@ int st_sw_score_end_batch(st_ctx *ctx, const st_qprof *qp, const st_db *db, cons
 // queries are profile views grouped by source, planned as window units.
 static int score_end_multi(st_ctx *ctx, const st_db *db, const st_qprof *const *qps, uint64_t nq,
                            const uint64_t *off, const uint32_t *tids, const int32_t *open, const int32_t *ext,
-                           st_score_end *out, const char *who, const uint64_t *win_off, uint64_t nwin) {
+                           st_score_end *out, const char *who, const uint64_t *win_off, uint64_t nwin,
+                           std::vector<st_launch_info> *launches) {
   const uint64_t count = nq ? off[nq] : 0;
   if (count == 0) return ST_OK;
   // the context's end-cell batch object, refilled: its device buffers and
@@ -1332,6 +1333,14 @@ static int score_end_multi(st_ctx *ctx, const st_db *db, const st_qprof *const *
   if (rc) return rc;
   ctx->end_batch = b;
   rc = st_batch_run(b);
+  if (!rc && launches) {
+    // the run's launches, while the batch object is still this call's
+    uint64_t n = 0;
+    st_batch_launches(b, nullptr, 0, &n);
+    const size_t at = launches->size();
+    launches->resize(at + n);
+    st_batch_launches(b, launches->data() + at, n, &n);
+  }
   if (!rc) {
     std::lock_guard<std::mutex> lk(ctx->mu);
     std::vector<PairOut> po(count);
@@ -1360,6 +1369,8 @@ static int score_end_multi(st_ctx *ctx, const st_db *db, const st_qprof *const *
 // the tables of the kernels that write them.
 struct SpanWork {
   std::vector<st_qprof> views;  // the reversed prefixes' profile views, reused
+  std::vector<st_launch_info> launches;  // the last call's: forward, then reverse
+  uint64_t n_forward = 0;
   DevBuf<uint8_t> mirror;
   DevBuf<RevPrefix> pre;
   DevBuf<MirrorJob> mj;
@@ -1380,9 +1391,13 @@ int st_align_spans_batch(st_ctx *ctx, const st_db *db, const st_qprof *const *qu
   std::lock_guard<std::mutex> sl(ctx->spans_mu);
   // 1. forward pass: score and the column-major-first end cell
   std::vector<st_score_end> fw(nh);
+  if (!ctx->spans) ctx->spans = new SpanWork();
+  ctx->spans->launches.clear();
+  ctx->spans->n_forward = 0;
   int rc = score_end_multi(ctx, db, queries, nq, hit_off, hit_tids, gap_open, gap_extend, fw.data(),
-                           "st_align_spans_batch");
+                           "st_align_spans_batch", nullptr, 0, &ctx->spans->launches);
   if (rc) return rc;
+  ctx->spans->n_forward = ctx->spans->launches.size();
   for (uint64_t k = 0; k < nh; k++) {
     out[k].score = fw[k].score;
     out[k].reserved = 0;
@@ -1438,7 +1453,6 @@ int st_align_spans_batch(st_ctx *ctx, const st_db *db, const st_qprof *const *qu
     pre[k] = RevPrefix{hit_tids[h], rlen[k]};
     doff[k + 1] = doff[k] + ((rlen[k] + 15ull) & ~15ull);
   }
-  if (!ctx->spans) ctx->spans = new SpanWork();
   st_db *rdb = &ctx->spans->rdb;
   rdb->ctx = ctx;
   rdb->n_targets = nr;
@@ -1526,7 +1540,7 @@ int st_align_spans_batch(st_ctx *ctx, const st_db *db, const st_qprof *const *qu
   }
   std::vector<st_score_end> rs(nr);
   rc = score_end_multi(ctx, rdb, vq.data(), vq.size(), voff.data(), vt.data(), vopen.data(), vext.data(),
-                       rs.data(), "st_align_spans_batch", win_off.data(), win_off.size() - 1);
+                       rs.data(), "st_align_spans_batch", win_off.data(), win_off.size() - 1, &w.launches);
   if (rc) return rc;
   for (uint64_t k = 0; k < nr; k++) {
     st_span &s = out[rv[k].hit];
@@ -1538,6 +1552,16 @@ int st_align_spans_batch(st_ctx *ctx, const st_db *db, const st_qprof *const *qu
     s.query_start = s.query_end - rs[k].query_end;
     s.target_start = s.target_end - rs[k].target_end;
   }
+  return ST_OK;
+}
+
+int st_align_spans_launches(st_ctx *ctx, st_launch_info *out, uint64_t cap, uint64_t *n, uint64_t *n_forward) {
+  if (!ctx || !n || (cap && !out)) return fail(ST_ERR_INVALID, "st_align_spans_launches: null argument");
+  std::lock_guard<std::mutex> sl(ctx->spans_mu);
+  const SpanWork *w = ctx->spans;
+  *n = w ? w->launches.size() : 0;
+  if (n_forward) *n_forward = w ? w->n_forward : 0;
+  for (uint64_t i = 0; w && i < std::min<uint64_t>(cap, *n); i++) out[i] = w->launches[i];
   return ST_OK;
 }
 

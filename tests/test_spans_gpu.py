@@ -119,6 +119,26 @@ def test_reverse_windows_every_group_width(ctx, monkeypatch, G):
     _check(ctx, rows, targets, [hits] * len(rows), gaps)
 
 
+def test_spans_launches_count_both_passes(ctx):
+    # the diagnostic of the last call: forward launches, then reverse ones,
+    # whose useful cells are the two passes' cells exactly
+    rng = np.random.default_rng(25)
+    queries = [rng.integers(0, 20, size=L, dtype=np.uint8) for L in (50, 300)]
+    targets = [rng.integers(0, 20, size=int(rng.integers(1, 400)), dtype=np.uint8) for _ in range(30)]
+    for q in queries:
+        targets += [_mutate(rng, q, 0.7, 0.02) for _ in range(3)]
+    rows = [O.seq_rows(q) for q in queries]
+    hits = [np.arange(len(targets), dtype=np.uint32)] * 2
+    got, db = _check(ctx, rows, targets, hits, [GapCosts(11, 1)] * 2)
+    li, nf = st.align_spans_launches(ctx)
+    assert 0 < nf < len(li)
+    fwd = sum(len(r) * len(targets[int(t)]) for r, h in zip(rows, hits) for t in h)
+    pos = got["score"] > 0
+    rev = int(((got["query_end"][pos] + 1) * (got["target_end"][pos] + 1)).sum())
+    assert int(li["useful_cells"][:nf].sum()) == fwd and int(li["useful_cells"][nf:].sum()) == rev
+    assert (li["exec_cells"] >= li["useful_cells"]).all() and (li["ms"] > 0).all()
+
+
 def test_long_spans_above_int16(ctx):
     W = O.aa2num()[ord("W")]
     rng = np.random.default_rng(23)
