@@ -399,7 +399,8 @@ int st_batch_timing(const st_batch *b, double *ms3, uint64_t *rescored);
 int st_batch_stats(const st_batch *b, uint64_t *exec_cells, uint64_t *units, uint64_t *launches);
 /* One DP kernel launch of a batch's plan: precision (0 packed int16, 1
  * int32 floored, 2 int32 unfloored, 3 and 4 the same over the query's int16
- * profile: end-cell launches), rows per lane (8 or 16; 0: each unit's own,
+ * profile, 5 and 6 over its int8 profile: end-cell launches), rows per lane
+ * (8 or 16; 0: each unit's own,
  * 8, 12 or 16 — the packed int16 launches), units, grid, useful and
  * executed cells, and its device time in the last st_batch_run (HIP events
  * around the launch on its stream). */

@@ -242,6 +242,10 @@ int st_qprof_create(st_ctx *ctx, const int32_t *rows, uint64_t m, st_qprof **out
   for (uint64_t k = 0; k < m * 21; k++)
     if (rows[k] < -32768 || rows[k] > 32767) fits = false;
   qp->fits16 = fits;
+  bool fits8 = true;
+  for (uint64_t k = 0; k < m * 21; k++)
+    if (rows[k] < -128 || rows[k] > 127) fits8 = false;
+  qp->fits8 = fits8;
   // code-major profiles [22][ld]: code 21 and rows >= m are zero
   std::vector<int32_t> p32((size_t)kCodes * qp->ld, 0);
   for (uint64_t i = 0; i < m; i++)
@@ -260,6 +264,15 @@ int st_qprof_create(st_ctx *ctx, const int32_t *rows, uint64_t m, st_qprof **out
       return fail(ST_ERR_NOMEM, "st_qprof_create: device upload failed");
     }
   }
+  if (fits8) {
+    std::vector<int8_t> p8(p32.size());
+    for (size_t k = 0; k < p32.size(); k++) p8[k] = (int8_t)p32[k];
+    if (qp->prof8.upload(p8.data(), p8.size(), ctx->stream) != hipSuccess) {
+      delete qp;
+      return fail(ST_ERR_NOMEM, "st_qprof_create: device upload failed");
+    }
+  }
+  qp->view.prof8 = qp->prof8.p;
   qp->view.prof16 = qp->prof16.p;
   qp->view.prof32 = qp->prof32.p;
   qp->view.m = qp->m;
@@ -486,7 +499,7 @@ bool plan_query(Plan &pl, std::vector<Unit> &pending, uint32_t qi, const st_qpro
         if (fg && G != fg) continue;
         const uint64_t GR = (uint64_t)G * R;
         // the pass's profile slice must fit 64 KB of LDS
-        if ((uint64_t)kCodes * (GR + 2) * kind_entry_bytes(kind) > 65536) continue;
+        if ((uint64_t)kCodes * kind_lds_stride((int)GR, kind) * kind_entry_bytes(kind) > 65536) continue;
         const size_t e = std::min(count, s + (size_t)(64 / G * per));
         const uint64_t np = key[e - 1] >> 32;
         const uint64_t passes = std::max<uint64_t>(1, (m + GR - 1) / GR);
@@ -793,7 +806,7 @@ void finish_plan(Plan &pl, std::vector<Unit> &pending, const std::vector<Kind> &
       j++;
     }
     L.nunits = (uint32_t)(j - i);
-    const size_t lds = dp_lds_bytes(L.gr_max + 2, kind_entry_bytes(L.kind), ung && !end);
+    const size_t lds = dp_lds_bytes(kind_lds_stride(L.gr_max, L.kind), kind_entry_bytes(L.kind), ung && !end);
     const int occ = std::max(1, occupancy_cached(L.kind, L.R, ung, lds, end));
     L.grid = std::max<uint32_t>(1, std::min<uint32_t>(L.nunits, (uint32_t)(occ * cus)));
     L.slot = (int)(pl.launches.size() % kLaunchSlots);
@@ -901,7 +914,7 @@ static int launch_plan(st_batch *b, const Plan &pl, const Unit *d_units, const P
     a.out.sat_cap = (uint32_t)b->npairs;
     a.out.qend = b->d_qend.p;
     a.out.tend = b->d_tend.p;
-    a.lds_stride = L.gr_max + 2;
+    a.lds_stride = kind_lds_stride(L.gr_max, L.kind);
     const bool timed = timing && &pl == &b->plan && b->lev.size() == 2 * pl.launches.size();
     const size_t li = (size_t)(&L - pl.launches.data());
     if (timed) ST_HIP(hipEventRecord(b->lev[2 * li], s));
@@ -984,11 +997,13 @@ static int batch_create_impl(st_ctx *ctx, const st_db *db, const st_qprof *const
     uint32_t nmax = 0;
     for (uint64_t k = cand_off[q]; k < cand_off[q + 1]; k++) nmax = std::max(nmax, db->hlen[cand_tids[k]]);
     if (b->qkind[q] == kPk16 && (uint64_t)b->qps[q]->m + nmax - 1 > kMaxDiagsPk16) b->qkind[q] = kI32F;
-    // the end cell is tracked by the int32 kernels, over the int16 profile
-    // where the query has one
+    // the end cell is tracked by the int32 kernels, over the int8 or int16
+    // profile where the query has one
     if (end) {
+      const bool p8 = b->qps[q]->fits8 && b->qps[q]->view.prof8;
       const bool p16 = b->qps[q]->fits16 && b->qps[q]->view.prof16;
-      b->qkind[q] = kind_unfloored(b->qkind[q]) ? (p16 ? kI32N16 : kI32N) : (p16 ? kI32F16 : kI32F);
+      b->qkind[q] = kind_unfloored(b->qkind[q]) ? (p8 ? kI32N8 : p16 ? kI32N16 : kI32N)
+                                                : (p8 ? kI32F8 : p16 ? kI32F16 : kI32F);
     }
   }
   std::lock_guard<std::mutex> lk(ctx->mu);
@@ -1441,7 +1456,10 @@ int st_align_spans_batch(st_ctx *ctx, const st_db *db, const st_qprof *const *qu
   // rows) from its window start m - 1 - e: m + 1024 rows cover every view
   for (const Rev &r : rv) mir_ld[r.q] = (queries[r.q]->m + kMaxPassRows + 63) & ~63u;
   for (uint64_t q = 0; q < nq; q++) {
-    if (queries[q]->fits16 && queries[q]->prof16.p) mir_es[q] = 2;
+    if (queries[q]->fits8 && queries[q]->prof8.p)
+      mir_es[q] = 1;
+    else if (queries[q]->fits16 && queries[q]->prof16.p)
+      mir_es[q] = 2;
     mir_off[q + 1] = mir_off[q] + (((uint64_t)kCodes * mir_ld[q] * mir_es[q] + 255) & ~255ull);
   }
   std::vector<RevPrefix> pre(nr);
@@ -1466,7 +1484,9 @@ int st_align_spans_batch(st_ctx *ctx, const st_db *db, const st_qprof *const *qu
   for (uint64_t q = 0; q < nq; q++)
     if (mir_ld[q]) {
       chunk_first.push_back((uint32_t)chunk_job.size());
-      const void *src = mir_es[q] == 2 ? (const void *)queries[q]->prof16.p : (const void *)queries[q]->prof32.p;
+      const void *src = mir_es[q] == 1   ? (const void *)queries[q]->prof8.p
+                        : mir_es[q] == 2 ? (const void *)queries[q]->prof16.p
+                                         : (const void *)queries[q]->prof32.p;
       mj.push_back(MirrorJob{src, mir_off[q], queries[q]->ld, queries[q]->m, mir_ld[q], mir_es[q]});
       for (uint32_t c = 0; c < (mir_ld[q] + 255) / 256; c++) chunk_job.push_back((uint32_t)(mj.size() - 1));
     }
@@ -1511,9 +1531,12 @@ int st_align_spans_batch(st_ctx *ctx, const st_db *db, const st_qprof *const *qu
       v->m = rv[k].e + 1;
       v->ld = mir_ld[rv[k].q] - o;
       uint8_t *mb = mirror.p + mir_off[rv[k].q];
-      v->fits16 = mir_es[rv[k].q] == 2;
-      v->view.prof16 = v->fits16 ? (int16_t *)mb + o : nullptr;
-      v->view.prof32 = v->fits16 ? nullptr : (int32_t *)mb + o;
+      const uint32_t es = mir_es[rv[k].q];
+      v->fits8 = es == 1;
+      v->fits16 = es <= 2;
+      v->view.prof8 = es == 1 ? (int8_t *)mb + o : nullptr;
+      v->view.prof16 = es == 2 ? (int16_t *)mb + o : nullptr;
+      v->view.prof32 = es == 4 ? (int32_t *)mb + o : nullptr;
       v->view.m = v->m;
       v->view.ld = mir_ld[rv[k].q];
       vq.push_back(v);

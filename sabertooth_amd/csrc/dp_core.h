@@ -115,6 +115,14 @@ struct I32F16 : I32F {
 struct I32N16 : I32N {
   using L = int16_t;
 };
+// ... and over a profile whose values fit int8 (a quarter of the int32
+// slice).
+struct I32F8 : I32F {
+  using L = int8_t;
+};
+struct I32N8 : I32N {
+  using L = int8_t;
+};
 
 template <class K>
 __device__ __forceinline__ void insert4(K (&k)[4], K v) {
@@ -155,15 +163,23 @@ template <>
 __device__ __forceinline__ const int16_t *prof_of<I32F16>(const QueryDev &q) { return q.prof16; }
 template <>
 __device__ __forceinline__ const int16_t *prof_of<I32N16>(const QueryDev &q) { return q.prof16; }
+template <>
+__device__ __forceinline__ const int8_t *prof_of<I32F8>(const QueryDev &q) { return q.prof8; }
+template <>
+__device__ __forceinline__ const int8_t *prof_of<I32N8>(const QueryDev &q) { return q.prof8; }
 
 // One step's profile words: Pk16 reads R/2 dwords (row pairs) per target,
-// the int32 kernels R dwords (R/2 over an int16 profile).  Row pointers are
-// dword aligned (even stride, even R).
+// the int32 kernels R dwords (R/2 over an int16 profile, R/4 over an int8
+// one).  Row pointers are dword aligned (the stride and R keep them so).
 template <class P, int R>
 __device__ __forceinline__ void load_raw(const typename P::L *lds, int stride, uint32_t code, int l,
                                          uint32_t (&raw)[R]) {
   // code * stride < 2^24: full-rate 24-bit multiplies
-  if (sizeof(typename P::L) == 2 && P::kPer == 1) {
+  if (sizeof(typename P::L) == 1) {
+    const uint32_t *ra = (const uint32_t *)(lds + (int)__umul24(code & 0xff, (uint32_t)stride) + l * R);
+#pragma unroll
+    for (int i = 0; i < R / 4; i++) raw[i] = ra[i];
+  } else if (sizeof(typename P::L) == 2 && P::kPer == 1) {
     const uint32_t *ra = (const uint32_t *)(lds + (int)__umul24(code & 0xff, (uint32_t)stride) + l * R);
 #pragma unroll
     for (int i = 0; i < R / 2; i++) raw[i] = ra[i];
@@ -185,7 +201,13 @@ __device__ __forceinline__ void load_raw(const typename P::L *lds, int stride, u
 // raw words -> per-row scores (Pk16: first target low half, second high)
 template <class P, int R>
 __device__ __forceinline__ void unpack_raw(const uint32_t (&raw)[R], typename P::T (&sc)[R]) {
-  if (sizeof(typename P::L) == 2 && P::kPer == 1) {
+  if (sizeof(typename P::L) == 1) {
+    // int32 recurrences over int8 values: row 4i + b in byte b
+#pragma unroll
+    for (int i = 0; i < R / 4; i++)
+#pragma unroll
+      for (int b = 0; b < 4; b++) sc[4 * i + b] = (typename P::T)((int32_t)(raw[i] << (24 - 8 * b)) >> 24);
+  } else if (sizeof(typename P::L) == 2 && P::kPer == 1) {
     // int32 recurrences over int16 values: row 2i in the low half
 #pragma unroll
     for (int i = 0; i < R / 2; i++) {

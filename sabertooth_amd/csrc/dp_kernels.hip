@@ -142,14 +142,15 @@ __device__ __forceinline__ void dp_unit(const Unit &u, const QueryDev *__restric
     const int GRp = final_pass ? c.G * RL : GR;
     // stage the profile rows [base, base + GRp) of all codes into LDS
     __syncthreads();
-    // int16 rows two at a time where the rows start on a dword (every
-    // query's own profile; a window of a row-reversed profile may not)
-    if (sizeof(L) == 2 && ((((uintptr_t)prof) | ((uintptr_t)q.ld * 2)) & 3) == 0) {
-      const int dw = GRp / 2;
+    // int16 / int8 rows a dword at a time where the rows start on a dword
+    // (every query's own profile; a window of a row-reversed profile may not)
+    constexpr int kPerDw = 4 / (int)sizeof(L);
+    if (sizeof(L) < 4 && ((((uintptr_t)prof) | ((uintptr_t)q.ld * sizeof(L))) & 3) == 0) {
+      const int dw = GRp / kPerDw;
       for (int idx = lane; idx < kCodes * dw; idx += 64) {
         const int cc = idx / dw, r = idx - cc * dw;
-        const uint32_t v = *(const uint32_t *)(prof + (uint64_t)cc * q.ld + base + 2 * r);
-        *(uint32_t *)(lds + cc * lds_stride + 2 * r) = v;
+        const uint32_t v = *(const uint32_t *)(prof + (uint64_t)cc * q.ld + base + kPerDw * r);
+        *(uint32_t *)(lds + cc * lds_stride + kPerDw * r) = v;
       }
     } else {
       for (int idx = lane; idx < kCodes * GRp; idx += 64) {
@@ -406,7 +407,10 @@ int dp_occupancy(int kind, int R, bool ung, size_t lds_bytes, bool end) {
   (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, dp_kernel<PP, RR, UU, EE>, 64, lds_bytes)
   if (end) {
     if (kind == 2) ST_OCC(I32N, 8, false, true);
-    else if (kind == 4) ST_OCC(I32N16, 8, false, true);
+    else if (kind == 6) ST_OCC(I32N8, 8, false, true);
+    else if (kind == 5) {
+      if (R == 8) ST_OCC(I32F8, 8, false, true); else ST_OCC(I32F8, 16, false, true);
+    } else if (kind == 4) ST_OCC(I32N16, 8, false, true);
     else if (kind == 3) {
       if (R == 8) ST_OCC(I32F16, 8, false, true); else ST_OCC(I32F16, 16, false, true);
     } else if (R == 8) ST_OCC(I32F, 8, false, true);
@@ -440,9 +444,10 @@ hipError_t clear_u32(uint32_t *p, uint32_t n, hipStream_t s) {
   return hipGetLastError();
 }
 
-// kind: 0 = Pk16, 1 = I32F, 2 = I32N, 3 = I32F16, 4 = I32N16 (the last two:
-// end-cell instances only); R: 8 or 16 (I32N, I32N16: 8); end: the score +
-// end-cell instances (int32, no evidence)
+// kind: 0 = Pk16, 1 = I32F, 2 = I32N, 3 = I32F16, 4 = I32N16, 5 = I32F8,
+// 6 = I32N8 (the last four: end-cell instances only); R: 8 or 16 (the
+// unfloored kinds: 8); end: the score + end-cell instances (int32, no
+// evidence)
 hipError_t launch_dp(const DpLaunch &a, int kind, int R, bool ung, hipStream_t s, bool end) {
   if (kind >= 3 && (!end || ung)) return hipErrorInvalidValue;
   hipLaunchKernelGGL(zero_u32, dim3(1), dim3(64), 0, s, a.counter, 1u);
@@ -453,7 +458,10 @@ hipError_t launch_dp(const DpLaunch &a, int kind, int R, bool ung, hipStream_t s
                      a.passbuf_half, a.out, a.lds_stride)
   if (end) {
     if (kind == 2) ST_DP(I32N, 8, false, true);
-    else if (kind == 4) ST_DP(I32N16, 8, false, true);
+    else if (kind == 6) ST_DP(I32N8, 8, false, true);
+    else if (kind == 5) {
+      if (R == 8) ST_DP(I32F8, 8, false, true); else ST_DP(I32F8, 16, false, true);
+    } else if (kind == 4) ST_DP(I32N16, 8, false, true);
     else if (kind == 3) {
       if (R == 8) ST_DP(I32F16, 8, false, true); else ST_DP(I32F16, 16, false, true);
     } else if (R == 8) ST_DP(I32F, 8, false, true);

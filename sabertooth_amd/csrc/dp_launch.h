@@ -49,8 +49,8 @@ hipError_t launch_dp(const DpLaunch &a, int kind, int R, bool ung, hipStream_t s
 // n words at p set to 0 by a kernel (not a memset: safe under graph capture)
 hipError_t clear_u32(uint32_t *p, uint32_t n, hipStream_t s);
 
-// Reverse score pass inputs (span_kernels.hip).  A query's profile (int16
-// or int32 entries: esize 2 or 4) with its m rows reversed into the bytes
+// Reverse score pass inputs (span_kernels.hip).  A query's profile (int8,
+// int16 or int32 entries: esize 1, 2 or 4) with its m rows reversed into the bytes
 // dst + dst_off ([kCodes][dst_ld] entries, dst_ld >= m + 1024); one launch
 // for many queries: chunk_job / chunk_first map each 256-row chunk of a
 // mirror to its job.

@@ -169,6 +169,8 @@ struct st_qprof {
   std::vector<int32_t> rows;
   uint32_t m = 0, ld = 0;
   bool fits16 = false;
+  bool fits8 = false;  // every value in [-128, 127]: prof8 holds them
+  st::DevBuf<int8_t> prof8;
   st::DevBuf<int16_t> prof16;
   st::DevBuf<int32_t> prof32;
   st::DevBuf<st::QueryDev> qd;  // a one-entry table for single-query calls
@@ -190,13 +192,20 @@ int noalloc_init(int device);
 int dp_occupancy(int kind, int R, bool ung, size_t lds_bytes, bool end = false);
 
 // Precision class of a query under given gaps.
-// kI32F16 / kI32N16: the int32 recurrences over the query's int16 profile
-// (end-cell batches of queries whose profile fits int16)
-enum Kind { kPk16 = 0, kI32F = 1, kI32N = 2, kI32F16 = 3, kI32N16 = 4 };
+// kI32F16 / kI32N16 and kI32F8 / kI32N8: the int32 recurrences over the
+// query's int16 or int8 profile (end-cell batches of queries whose profile
+// fits)
+enum Kind { kPk16 = 0, kI32F = 1, kI32N = 2, kI32F16 = 3, kI32N16 = 4, kI32F8 = 5, kI32N8 = 6 };
 // bytes per profile entry in LDS
-inline int kind_entry_bytes(int k) { return (k == kPk16 || k == kI32F16 || k == kI32N16) ? 2 : 4; }
+inline int kind_entry_bytes(int k) {
+  return (k == kI32F8 || k == kI32N8) ? 1 : (k == kPk16 || k == kI32F16 || k == kI32N16) ? 2 : 4;
+}
+// the LDS row stride (entries) of a launch whose passes hold at most gr_max
+// rows: a whole number of dwords plus one (profile rows start on a dword,
+// rows of consecutive codes fall in different banks)
+inline int kind_lds_stride(int gr_max, int k) { return gr_max + (kind_entry_bytes(k) == 1 ? 4 : 2); }
 // the unfloored recurrences (negative gap costs): 8 rows per lane only
-inline bool kind_unfloored(int k) { return k == kI32N || k == kI32N16; }
+inline bool kind_unfloored(int k) { return k == kI32N || k == kI32N16 || k == kI32N8; }
 
 Kind kind_for(const st_qprof *qp, int32_t open, int32_t ext);
 

@@ -179,6 +179,7 @@ __global__ __launch_bounds__(64, 3) void serve_kernel(ServeArgs a) {
     q.prof32 = u.prof32;
     q.m = u.m;
     q.ld = u.ld;
+    q.prof8 = nullptr;
 
     UnitResult<P> r;
     score_unit<P, R, true, false>(q, u.pairs, u.npairs, u.G, u.n_pad, u.passes, u.open, u.ext, a.arena, a.toff,

@@ -296,7 +296,7 @@ int launch_on(Slot &sl, const Plan &pl, const QueryDev *d_q, const Unit *d_units
     a.out = DpOut{};
     a.out.res = (PairResultDev *)d_res;  // the gate runs in the DP launches
     a.out.qgate = d_gate;
-    a.lds_stride = L.gr_max + 2;
+    a.lds_stride = kind_lds_stride(L.gr_max, L.kind);
     ST_HIP(launch_dp(a, L.kind, L.R, true, fan ? sl.sub[k] : sl.stream));
   }
   if (fan)

@@ -2,8 +2,8 @@
 // which scores the reversed prefixes rows[..=qend] and target[..=tend] of
 // each hit to find its start.
 //
-//  * mirror_prof_kernel: each query's profile (its int16 profile where it
-//    has one, else its int32 profile) with its rows in reverse order, so that the reversed prefix rows[..=e] of any end row e is the
+//  * mirror_prof_kernel: each query's profile (its int8 or int16 profile
+//    where it has one, else its int32 profile) with its rows in reverse order, so that the reversed prefix rows[..=e] of any end row e is the
 //    window starting at row m - 1 - e (no profile per hit).  Rows past m are
 //    zero, and at least 1024 of them follow the last row, as every pass of
 //    the DP kernels reads whole passes of up to 1024 rows.  One launch
@@ -30,7 +30,9 @@ __global__ __launch_bounds__(256) void mirror_prof_kernel(const MirrorJob *__res
   const uint32_t r = (x - chunk_first[j]) * 256 + threadIdx.x;
   if (r >= jb.dst_ld) return;
   const uint64_t o = (uint64_t)c * jb.dst_ld + r, i = (uint64_t)c * jb.src_ld + (jb.m - 1 - r);
-  if (jb.esize == 2)
+  if (jb.esize == 1)
+    ((int8_t *)(dst + jb.dst_off))[o] = r < jb.m ? ((const int8_t *)jb.src)[i] : (int8_t)0;
+  else if (jb.esize == 2)
     ((int16_t *)(dst + jb.dst_off))[o] = r < jb.m ? ((const int16_t *)jb.src)[i] : (int16_t)0;
   else
     ((int32_t *)(dst + jb.dst_off))[o] = r < jb.m ? ((const int32_t *)jb.src)[i] : 0;

@@ -39,6 +39,7 @@ struct QueryDev {
   const int32_t *prof32;  // [kCodes][ld]
   uint32_t m;
   uint32_t ld;            // leading dimension, multiple of kMaxPassRows
+  const int8_t *prof8;    // [kCodes][ld] where every value fits int8, else null
 };
 
 // One (query, target) pair slot of a batch.

@@ -60,6 +60,17 @@ def test_every_group_width(ctx, monkeypatch, G):
         _check(ctx, O.seq_rows(q), targets, GapCosts(11, 1))
 
 
+@pytest.mark.parametrize("o,e", [(11, 1), (0, 0), (-3, 1)])
+def test_end_rule_profile_int16_not_int8(ctx, o, e):
+    # values beyond int8: the int16 profile is staged (the int8-profile
+    # instances take the BLOSUM queries of the other tests)
+    rng = np.random.default_rng(51 + abs(o) + e)
+    rows = rng.integers(-2000, 3000, size=(130, 21)).astype(np.int32)
+    rows[:, 20] = 0
+    targets = [rng.integers(0, 4, size=int(L)).astype(np.uint8) for L in rng.integers(1, 300, size=25)]
+    _check(ctx, rows, targets, GapCosts(o * 100, e * 50))
+
+
 @pytest.mark.parametrize("o,e", [(30000, 2000), (0, 0), (-3000, 100)])
 def test_end_rule_profile_outside_int16(ctx, o, e):
     # values beyond int16: the int32 profile is staged (the int16-profile

@@ -80,9 +80,9 @@ def test_spans_with_ties_pssm_rows_and_negative_gaps(ctx):
 
 
 def test_spans_profiles_in_and_outside_int16(ctx):
-    # one call over queries whose profiles fit int16 (their row-reversed
-    # copies are int16) and queries whose values do not (int32 copies),
-    # with floored and unfloored gap costs on both
+    # one call over queries whose profiles fit int8 (their row-reversed
+    # copies are int8), int16 but not int8 (int16 copies) and neither (int32
+    # copies), with floored and unfloored gap costs on each
     rng = np.random.default_rng(24)
     queries = [rng.integers(0, 20, size=L, dtype=np.uint8) for L in (90, 400)]
     targets = [rng.integers(0, 20, size=int(rng.integers(1, 500)), dtype=np.uint8) for _ in range(40)]
@@ -90,9 +90,12 @@ def test_spans_profiles_in_and_outside_int16(ctx):
         targets += [_mutate(rng, q, 0.7, 0.02) for _ in range(3)]
     wide = rng.integers(-40000, 45000, size=(230, 21)).astype(np.int32)
     wide[:, 20] = 0
-    rows = [O.seq_rows(queries[0]), wide, O.seq_rows(queries[1]), wide[:57]]
+    mid = rng.integers(-1500, 2500, size=(170, 21)).astype(np.int32)
+    mid[:, 20] = 0
+    rows = [O.seq_rows(queries[0]), wide, O.seq_rows(queries[1]), wide[:57], mid, mid[:40]]
     hits = np.arange(len(targets), dtype=np.uint32)
-    gaps = [GapCosts(11, 1), GapCosts(30000, 2000), GapCosts(-1, 1), GapCosts(-5000, 300)]
+    gaps = [GapCosts(11, 1), GapCosts(30000, 2000), GapCosts(-1, 1), GapCosts(-5000, 300), GapCosts(1100, 100),
+            GapCosts(-200, 50)]
     _check(ctx, rows, targets, [hits] * len(rows), gaps)
 
 
