@@ -180,6 +180,27 @@ int32_t st_min_score_for(const st_evalue_params *p, double max_evalue, double sp
  * st_fit_gumbel_ml returns ST_ERR_INVALID where the reference returns None. */
 double st_gumbel_mu_given_lambda(const double *xs, uint64_t n, double lambda);
 int st_fit_gumbel_ml(const double *xs, uint64_t n, st_gumbel *out);
+
+/* CalibrationCell (src/evalue.rs:507-519): one simulated (m, n) cell. */
+typedef struct {
+  uint64_t m, n, samples;
+  double mean, sd;        /* sample mean and standard deviation (n - 1) */
+  double lambda_ml, mu_ml; /* the cell's own maximum-likelihood Gumbel */
+  double mu;              /* its location under the shared lambda, + 1/2 */
+} st_calibration_cell;
+
+/* fit_calibration (src/evalue.rs:576-668): cell c holds the scores
+ * scores[score_off[c] .. score_off[c + 1]) of (cell_m[c], cell_n[c]) pairs.
+ * A shared lambda by golden-section maximum likelihood over the cells whose
+ * shorter side is >= lambda_min_len (all cells when none is), each cell's
+ * location under it, and the length model (alpha/lambda, beta) on the
+ * reference's grid minimising the variance of ln K across cells;
+ * K = e^(mean ln K).  cells (nullable, ncells entries) receives the cells in
+ * (m, n) order, ln_k_rms (nullable) the root-mean-square spread of ln K.
+ * ST_ERR_INVALID where the reference returns None. */
+int st_fit_calibration(const uint64_t *cell_m, const uint64_t *cell_n, const uint64_t *score_off,
+                       const double *scores, uint64_t ncells, uint64_t lambda_min_len,
+                       st_gapped_params *out, st_calibration_cell *cells, double *ln_k_rms);
 /* solve_lambda (src/evalue.rs:466-498): ungapped lambda of the integer
  * BLOSUM62 (diagnostic). */
 double st_ungapped_lambda(void);
@@ -299,6 +320,28 @@ int st_calibration_targets(const uint64_t *lengths, uint64_t n_lengths, uint64_t
 int st_calibrate_profile(st_ctx *ctx, const st_qprof *qp, int32_t gap_open, int32_t gap_extend,
                          const uint64_t *lengths, uint64_t n_lengths, uint64_t samples, uint64_t seed,
                          st_gapped_params *out, st_gumbel *fit);
+
+/* ---- sequence statistics by simulation (src/evalue.rs:537-700) ------------ */
+
+/* simulate_scores (src/evalue.rs:537-565): sample i of cell cell_id draws a
+ * query of m and then a target of n residues from the BLOSUM62 background
+ * with SplitMix64(stream_seed(seed, cell_id, i)); out[i] = the exact SWG
+ * score of the pair with the given gaps, the query's rows BLOSUM62 rows
+ * plus, when comp_bias, its composition-bias correction
+ * (local_bias_correction, src/compbias.rs:32-34).  The scores are computed
+ * on the GPU, every query's profile built on the device. */
+int st_simulate_scores(st_ctx *ctx, int32_t gap_open, int32_t gap_extend, int comp_bias, uint64_t m,
+                       uint64_t n, uint64_t samples, uint64_t seed, uint64_t cell_id, double *out);
+
+/* calibrate_sequence (src/evalue.rs:673-700): one cell of `samples` pairs
+ * per length l = lengths[c] (m = n = l, cell id c) by st_simulate_scores,
+ * then st_fit_calibration with lambda_min_len 200.  The reference's
+ * SEQUENCE_TABLE came from lengths {60, 100, 150, 220, 320, 480, 700,
+ * 1000}, 20000 samples, seed 24301.  cells (nullable) receives n_lengths
+ * cells, ln_k_rms (nullable) the spread of ln K. */
+int st_calibrate_sequence(st_ctx *ctx, int32_t gap_open, int32_t gap_extend, int comp_bias,
+                          const uint64_t *lengths, uint64_t n_lengths, uint64_t samples, uint64_t seed,
+                          st_gapped_params *out, st_calibration_cell *cells, double *ln_k_rms);
 
 #ifdef __cplusplus
 }

@@ -378,3 +378,68 @@ def calibrate_profile(rows, open_, ext, lengths, samples, seed):
     lam, mu = f
     mean_len = float(sum(int(x) for x in lengths)) / len(lengths)
     return lam, math.exp(lam * (mu + 0.5)) / (len(rows) * mean_len), (lam, mu)
+
+
+# ---- calibrate_sequence's simulation and fit (src/evalue.rs:537-700) ----------
+
+def simulate_pairs(seed, cell_id, m, n, samples):
+    """simulate_scores' pairs: sample i's query (m) then target (n) from
+    SplitMix64(stream_seed(seed, cell_id, i)); two uint8 arrays
+    [samples, m] and [samples, n]."""
+    from . import _p as P, lib as L
+    qs = np.zeros((samples, m), np.uint8)
+    ts = np.zeros((samples, n), np.uint8)
+    for i in range(samples):
+        st = C.c_uint64(stream_seed(seed, cell_id, i))
+        L().or_background_sequence(C.byref(st), m, P(qs[i]))
+        L().or_background_sequence(C.byref(st), n, P(ts[i]))
+    return qs, ts
+
+
+def local_bias_correction_rows(qs):
+    """local_bias_correction (src/compbias.rs:38-70) of many queries of one
+    length at once ([samples, m] codes -> [samples, m] int32): the window
+    sums are exact integers, then the same f64 steps per position."""
+    from . import blosum62
+    sc, pb, _, _ = blosum62()
+    sc = np.asarray(sc, dtype=np.int64).reshape(21, 21)
+    exp_ = np.zeros(21)
+    for a in range(21):
+        e = 0.0
+        for b in range(20):
+            e += float(pb[b]) * float(sc[a, b])
+        exp_[a] = e
+    qs = np.asarray(qs, dtype=np.int64)
+    s, n = qs.shape
+    tot = np.zeros((s, n), np.int64)
+    for d in range(-20, 20):
+        if d == 0:
+            continue
+        lo, hi = max(0, -d), min(n, n - d)
+        if lo < hi:
+            tot[:, lo:hi] += sc[qs[:, lo:hi], qs[:, lo + d:hi + d]]
+    i = np.arange(n)
+    wl = (np.minimum(i + 20, n) - np.maximum(i - 20, 0)).astype(np.float64)
+    delta = exp_[np.minimum(qs, 20)] - tot.astype(np.float64) / wl[None, :]
+    out = np.where(delta >= 0.0, np.floor(delta + 0.5), -np.floor(-delta + 0.5))
+    return out.astype(np.int32)
+
+
+def fit_calibration(cells, lmin=200):
+    """fit_calibration over [(m, n, scores)]: ((lambda, K, alpha/lambda,
+    beta), cells [k, 8], ln K rms) or None."""
+    from . import _p as P, lib as L
+    cells = sorted(cells, key=lambda c: (c[0], c[1]))
+    nc = len(cells)
+    cm = np.array([c[0] for c in cells], np.uint64)
+    cn = np.array([c[1] for c in cells], np.uint64)
+    off = np.zeros(nc + 1, np.uint64)
+    off[1:] = np.cumsum([len(c[2]) for c in cells])
+    xs = np.ascontiguousarray(np.concatenate([np.asarray(c[2], np.float64) for c in cells]), dtype=np.float64)
+    out6 = np.zeros(6)
+    co = np.zeros((max(nc, 1), 8))
+    f = L().or_fit_calibration
+    f.argtypes = [C.c_void_p] * 4 + [C.c_size_t, C.c_uint64, C.c_void_p, C.c_void_p]
+    if f(P(cm), P(cn), P(off), P(xs), nc, lmin, P(out6), P(co)) != 0:
+        return None
+    return (out6[0], out6[1], out6[2], out6[3]), co[:nc], out6[4]

@@ -127,6 +127,15 @@ void or_background_sequence(uint64_t *state, size_t len, uint8_t *out);
 int or_pssm_from_msa(const uint8_t *msa, size_t nrows, size_t stride, size_t qlen, float pca, float pcb,
                      float score_bias, int8_t *pssm, float *prob, float *neff, uint8_t *cons);
 
+/* gumbel_mu_given_lambda, fit_gumbel_ml (src/evalue.rs:130-182) and
+ * fit_calibration (src/evalue.rs:576-668) over cells sorted by (m, n):
+ * out6 = {lambda, K, alpha/lambda, beta, ln K rms, 0}, cell_out 8 doubles
+ * per cell {m, n, samples, mean, sd, lambda_ml, mu_ml, mu}; -1 = None. */
+double or_gumbel_mu_given_lambda(const double *xs, size_t n, double lam);
+int or_fit_gumbel_ml(const double *xs, size_t n, double *lam, double *mu);
+int or_fit_calibration(const uint64_t *cm, const uint64_t *cn, const uint64_t *off, const double *xs, size_t nc,
+                       uint64_t lmin, double *out6, double *cell_out);
+
 #ifdef __cplusplus
 }
 #endif

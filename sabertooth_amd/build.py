@@ -14,7 +14,7 @@ import sys
 HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libsabertooth_amd.so")
-SOURCES = ["dp_kernels.hip", "gate_kernels.hip", "serve_kernel.hip", "span_kernels.hip", "seed_hits.hip", "noalloc.hip", "capi.cpp", "stidx.cpp", "server.cpp", "synth.cpp", "profile.cpp", "evalue.cpp", "query_rows.cpp", "align.cpp", "search.cpp", "calibrate.cpp"]
+SOURCES = ["dp_kernels.hip", "gate_kernels.hip", "serve_kernel.hip", "span_kernels.hip", "sim_kernels.hip", "seed_hits.hip", "noalloc.hip", "capi.cpp", "stidx.cpp", "server.cpp", "synth.cpp", "profile.cpp", "evalue.cpp", "query_rows.cpp", "align.cpp", "search.cpp", "calibrate.cpp"]
 # host-only floating-point code keeps the reference's separate roundings
 HOST_FP = {"profile.cpp": ["-ffp-contract=off"], "evalue.cpp": ["-ffp-contract=off"],
            "query_rows.cpp": ["-ffp-contract=off"], "search.cpp": ["-ffp-contract=off"],

@@ -1700,3 +1700,24 @@ int st_score_and_gate(st_ctx *ctx, const st_qprof *qp, const st_db *db, const st
 }
 
 }  // extern "C"
+
+namespace st {
+
+int score_multi(st_ctx *ctx, const st_db *db, const st_qprof *const *qps, uint64_t nq, const uint64_t *off,
+                const uint32_t *tids, const int32_t *open, const int32_t *ext, int32_t *scores) {
+  const uint64_t count = nq ? off[nq] : 0;
+  if (count == 0) return ST_OK;
+  st_batch *b = nullptr;
+  int rc = batch_create_impl(ctx, db, qps, nq, off, tids, open, ext, nullptr, false, &b);
+  if (rc) return rc;
+  rc = st_batch_run(b);
+  if (!rc) {
+    std::vector<st_pair_result> res(count);
+    rc = st_batch_results(b, res.data());
+    if (!rc)
+      for (uint64_t k = 0; k < count; k++) scores[k] = res[k].swg;
+  }
+  st_batch_destroy(b);
+  return rc;
+}
+}  // namespace st

@@ -70,4 +70,11 @@ struct RevPrefix {
 hipError_t launch_rev_prefix(const uint8_t *arena, const uint64_t *toff, const RevPrefix *pre, uint32_t n,
                              const uint64_t *doff, uint8_t *dst, hipStream_t s);
 
+// Profiles of simulated pairs (sim_kernels.hip): sample s's query codes
+// q[s*m .. s*m + m) and (nullable) bias give its [kCodes][ld] int16 and int32
+// profiles at p16 / p32 + s * kCodes * ld; mat is the kCodes x kCodes score
+// table (code 21's row and column zero).
+hipError_t launch_sim_prof(const uint8_t *q, const int32_t *bias, const int8_t *mat, uint32_t m, uint32_t ld,
+                           uint32_t samples, int16_t *p16, int32_t *p32, hipStream_t s);
+
 }  // namespace st

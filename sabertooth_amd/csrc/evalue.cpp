@@ -8,6 +8,8 @@
 #include <cstring>
 #include <limits>
 #include <string>
+#include <algorithm>
+#include <vector>
 
 #include "host_internal.h"
 #include "../../include/sabertooth_host.h"
@@ -200,6 +202,136 @@ int st_fit_gumbel_ml(const double *xs, uint64_t n, st_gumbel *out) {
   out->lambda = 0.5 * (lo + hi);
   out->mu = st_gumbel_mu_given_lambda(xs, n, out->lambda);
   out->n = n;
+  return ST_OK;
+}
+
+int st_fit_calibration(const uint64_t *cell_m, const uint64_t *cell_n, const uint64_t *score_off,
+                       const double *scores, uint64_t ncells, uint64_t lambda_min_len, st_gapped_params *out,
+                       st_calibration_cell *cells, double *ln_k_rms) {
+  // fit_calibration (src/evalue.rs:576-668): one lambda shared by the cells
+  // whose shorter side is at least lambda_min_len (every cell when none
+  // is), each cell's location with it (+1/2 for integer scores), then the
+  // length model (alpha/lambda, beta) on the reference's grid that makes
+  // ln K the same across cells, K = e^(mean ln K)
+  if (!out || (ncells && (!cell_m || !cell_n || !score_off || !scores)))
+    return fail(ST_ERR_INVALID, "st_fit_calibration: null argument");
+  if (ncells == 0) return fail(ST_ERR_INVALID, "st_fit_calibration: no cells");
+  if (score_off[0] != 0) return fail(ST_ERR_INVALID, "st_fit_calibration: score_off[0] must be 0");
+  for (uint64_t c = 0; c < ncells; c++)
+    if (score_off[c + 1] < score_off[c]) return fail(ST_ERR_INVALID, "st_fit_calibration: score_off decreases");
+  // cells by (m, n), ties in input order (a stable sort, as sort_by_key)
+  std::vector<uint64_t> ord(ncells);
+  for (uint64_t c = 0; c < ncells; c++) ord[c] = c;
+  std::stable_sort(ord.begin(), ord.end(), [&](uint64_t a, uint64_t b) {
+    return cell_m[a] != cell_m[b] ? cell_m[a] < cell_m[b] : cell_n[a] < cell_n[b];
+  });
+  auto xs_of = [&](uint64_t c) { return scores + score_off[c]; };
+  auto n_of = [&](uint64_t c) { return score_off[c + 1] - score_off[c]; };
+  std::vector<uint64_t> pool;
+  for (uint64_t c : ord)
+    if (std::min(cell_m[c], cell_n[c]) >= lambda_min_len) pool.push_back(c);
+  if (pool.empty()) pool = ord;
+  for (uint64_t c : ord)
+    if (n_of(c) == 0) return fail(ST_ERR_INVALID, "st_fit_calibration: a cell without scores");
+  auto loglik = [&](double lam) {
+    double ll = 0.0;
+    for (uint64_t c : pool) {
+      const double *xs = xs_of(c);
+      const double mu = st_gumbel_mu_given_lambda(xs, n_of(c), lam);
+      for (uint64_t i = 0; i < n_of(c); i++) {
+        const double z = lam * (xs[i] - mu);
+        ll += std::log(lam) - z - std::exp(-z);
+      }
+    }
+    return ll;
+  };
+  // golden-section search for the maximum over lambda in [0.05, 1.5]
+  double a = 0.05, b = 1.5;
+  const double gr = (std::sqrt(5.0) - 1.0) / 2.0;
+  double c1 = b - gr * (b - a), c2 = a + gr * (b - a);
+  double f1 = loglik(c1), f2 = loglik(c2);
+  for (int it = 0; it < 80; it++) {
+    if (f1 < f2) {
+      a = c1;
+      c1 = c2;
+      f1 = f2;
+      c2 = a + gr * (b - a);
+      f2 = loglik(c2);
+    } else {
+      b = c2;
+      c2 = c1;
+      f2 = f1;
+      c1 = b - gr * (b - a);
+      f1 = loglik(c1);
+    }
+  }
+  const double lambda = 0.5 * (a + b);
+  std::vector<st_calibration_cell> fitted(ncells);
+  for (uint64_t k = 0; k < ncells; k++) {
+    const uint64_t c = ord[k];
+    const double *xs = xs_of(c);
+    const uint64_t n = n_of(c);
+    double mean = 0.0;
+    for (uint64_t i = 0; i < n; i++) mean += xs[i];
+    mean /= (double)n;
+    double ss = 0.0;
+    for (uint64_t i = 0; i < n; i++) ss += (xs[i] - mean) * (xs[i] - mean);
+    st_gumbel free_fit;
+    if (int rc = st_fit_gumbel_ml(xs, n, &free_fit)) return rc;
+    st_calibration_cell &o = fitted[k];
+    o.m = cell_m[c];
+    o.n = cell_n[c];
+    o.samples = n;
+    o.mean = mean;
+    o.sd = std::sqrt(ss / ((double)n - 1.0));
+    o.lambda_ml = free_fit.lambda;
+    o.mu_ml = free_fit.mu;
+    o.mu = st_gumbel_mu_given_lambda(xs, n, lambda) + 0.5;
+  }
+  // the length model: alpha/lambda in 0.1 steps over [0, 20], beta in unit
+  // steps over [-80, 20] (accumulated as the reference accumulates them)
+  bool found = false;
+  double bvar = 0.0, badl = 0.0, bbeta = 0.0, bmean = 0.0;
+  std::vector<double> lnks;
+  lnks.reserve(ncells);
+  for (double adl = 0.0; adl <= 20.0 + 1e-9; adl += 0.1) {
+    for (double beta = -80.0; beta <= 20.0 + 1e-9; beta += 1.0) {
+      lnks.clear();
+      bool ok = true;
+      for (const st_calibration_cell &o : fitted) {
+        const double ell = std::fmax(adl * lambda * o.mu + beta, 0.0);
+        const double me = (double)o.m - ell, ne = (double)o.n - ell;
+        if (me < 1.0 || ne < 1.0) {
+          ok = false;
+          break;
+        }
+        lnks.push_back(lambda * o.mu - std::log(me * ne));
+      }
+      if (!ok || lnks.empty()) continue;
+      double mean = 0.0;
+      for (double v : lnks) mean += v;
+      mean /= (double)lnks.size();
+      double var = 0.0;
+      for (double v : lnks) var += (v - mean) * (v - mean);
+      var /= (double)lnks.size();
+      if (!found || var < bvar) {
+        found = true;
+        bvar = var;
+        badl = adl;
+        bbeta = beta;
+        bmean = mean;
+      }
+    }
+  }
+  if (!found) return fail(ST_ERR_INVALID, "st_fit_calibration: no length model fits the cells");
+  std::memset(out, 0, sizeof *out);
+  out->lambda = lambda;
+  out->k = std::exp(bmean);
+  out->has_length = 1;
+  out->alpha_d_lambda = std::round(badl * 10.0) / 10.0;
+  out->beta = bbeta;
+  if (cells) std::memcpy(cells, fitted.data(), ncells * sizeof *cells);
+  if (ln_k_rms) *ln_k_rms = std::sqrt(bvar);
   return ST_OK;
 }
 

@@ -191,6 +191,12 @@ int noalloc_init(int device);
 
 int dp_occupancy(int kind, int R, bool ung, size_t lds_bytes, bool end = false);
 
+// Exact SWG scores of a multi-query candidate list (score-only batch, the
+// int32 rescoring included), in candidate order: query q scores
+// tids[off[q] .. off[q + 1]) with (open[q], ext[q]).
+int score_multi(st_ctx *ctx, const st_db *db, const st_qprof *const *qps, uint64_t nq, const uint64_t *off,
+                const uint32_t *tids, const int32_t *open, const int32_t *ext, int32_t *scores);
+
 // Precision class of a query under given gaps.
 // kI32F16 / kI32N16 and kI32F8 / kI32N8: the int32 recurrences over the
 // query's int16 or int8 profile (end-cell batches of queries whose profile

@@ -120,8 +120,11 @@ _SIGS += [
     ("st_min_score_for", C.c_int32, [_P, _D, _D]),
     ("st_gumbel_mu_given_lambda", _D, [_P, C.c_uint64, _D]),
     ("st_fit_gumbel_ml", C.c_int, [_P, C.c_uint64, _P]),
+    ("st_fit_calibration", C.c_int, [_P, _P, _P, _P, C.c_uint64, C.c_uint64, _P, _P, _P]),
     ("st_ungapped_lambda", _D, []),
 ]
+CALIBRATION_CELL_DT = np.dtype([("m", "<u8"), ("n", "<u8"), ("samples", "<u8"), ("mean", "<f8"), ("sd", "<f8"),
+                                ("lambda_ml", "<f8"), ("mu_ml", "<f8"), ("mu", "<f8")])
 EXPORTED = [s[0] for s in _SIGS]
 
 
@@ -193,6 +196,32 @@ def fit_gumbel_ml(xs):
     if rc != 0:
         return None
     return float(out["lambda"][0]), float(out["mu"][0])
+
+
+def _params_tuple(g):
+    return {"lambda": float(g["lambda"][0]), "k": float(g["k"][0]),
+            "length": (float(g["alpha_d_lambda"][0]), float(g["beta"][0])) if g["has_length"][0] else None}
+
+
+def fit_calibration(cells, lambda_min_len: int = 200):
+    """fit_calibration (src/evalue.rs:576-668) over [(m, n, scores)]:
+    ({"lambda", "k", "length": (alpha/lambda, beta)}, cells record array in
+    (m, n) order, ln K rms), or None where the reference returns None."""
+    cm = np.array([c[0] for c in cells], np.uint64)
+    cn = np.array([c[1] for c in cells], np.uint64)
+    off = np.zeros(len(cells) + 1, np.uint64)
+    off[1:] = np.cumsum([len(c[2]) for c in cells])
+    xs = np.ascontiguousarray(np.concatenate([np.asarray(c[2], np.float64) for c in cells]) if cells
+                              else np.zeros(0), dtype=np.float64)
+    g = np.zeros(1, GAPPED_PARAMS_DT)
+    out = np.zeros(max(len(cells), 1), CALIBRATION_CELL_DT)
+    rms = C.c_double(0.0)
+    rc = _L().st_fit_calibration(ptr(cm), ptr(cn), ptr(off), ptr(xs), len(cells), lambda_min_len, ptr(g), ptr(out),
+                                 C.byref(rms))
+    if rc == 1:  # ST_ERR_INVALID: no fit
+        return None
+    check(rc)
+    return _params_tuple(g), out[: len(cells)], float(rms.value)
 
 
 def ungapped_lambda() -> float:
@@ -587,6 +616,10 @@ _SIGS += [
     ("st_calibration_targets", C.c_int, [_P, C.c_uint64, C.c_uint64, C.c_uint64, _P, C.c_uint64, _P]),
     ("st_calibrate_profile", C.c_int, [_P, _P, C.c_int32, C.c_int32, _P, C.c_uint64, C.c_uint64, C.c_uint64, _P,
                                        _P]),
+    ("st_simulate_scores", C.c_int, [_P, C.c_int32, C.c_int32, C.c_int, C.c_uint64, C.c_uint64, C.c_uint64,
+                                     C.c_uint64, C.c_uint64, _P]),
+    ("st_calibrate_sequence", C.c_int, [_P, C.c_int32, C.c_int32, C.c_int, _P, C.c_uint64, C.c_uint64,
+                                        C.c_uint64, _P, _P, _P]),
 ]
 EXPORTED = [s[0] for s in _SIGS]
 CALIBRATION_SEED = 0x5EED  # SearchParams::calibration_seed (src/search.rs:121)
@@ -614,3 +647,37 @@ def calibrate_profile(qp, gaps, lengths, samples: int = 2000, seed: int = CALIBR
         return None
     check(rc)
     return float(g["lambda"][0]), float(g["k"][0]), (float(fit["lambda"][0]), float(fit["mu"][0]))
+
+
+# ---- sequence statistics by simulation (src/evalue.rs:537-700) ---------------
+
+# calibrate_sequence's default lengths (DEFAULT_CALIBRATION_LENGTHS,
+# src/evalue.rs:670) and the settings SEQUENCE_TABLE was made with
+# (its provenance strings, src/evalue.rs:287, 300)
+DEFAULT_CALIBRATION_LENGTHS = (60, 100, 150, 220, 320, 480, 700, 1000)
+TABLE_SAMPLES, TABLE_SEED = 20000, 24301
+
+
+def simulate_scores(ctx, gaps, comp_bias: bool, m: int, n: int, samples: int, seed: int, cell_id: int) -> np.ndarray:
+    """simulate_scores (src/evalue.rs:537-565): exact SWG scores (float64) of
+    `samples` random background pairs, scored on the GPU."""
+    out = np.zeros(max(samples, 1), np.float64)
+    check(_L().st_simulate_scores(ctx.handle, gaps.open, gaps.extend, 1 if comp_bias else 0, m, n, samples, seed,
+                                  cell_id, ptr(out)))
+    return out[:samples]
+
+
+def calibrate_sequence(ctx, gaps, comp_bias: bool, lengths=DEFAULT_CALIBRATION_LENGTHS, samples: int = 4000,
+                       seed: int = 0x5EED):
+    """calibrate_sequence (src/evalue.rs:673-700): (params dict, cells, ln K
+    rms), or None where the reference returns None."""
+    ln = np.ascontiguousarray(lengths, dtype=np.uint64)
+    g = np.zeros(1, GAPPED_PARAMS_DT)
+    out = np.zeros(max(len(ln), 1), CALIBRATION_CELL_DT)
+    rms = C.c_double(0.0)
+    rc = _L().st_calibrate_sequence(ctx.handle, gaps.open, gaps.extend, 1 if comp_bias else 0, ptr(ln), len(ln),
+                                    samples, seed, ptr(g), ptr(out), C.byref(rms))
+    if rc == 1:  # ST_ERR_INVALID: no fit
+        return None
+    check(rc)
+    return _params_tuple(g), out[: len(ln)], float(rms.value)
