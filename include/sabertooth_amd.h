@@ -440,7 +440,11 @@ int st_batch_destroy(st_batch *b);
  * already running, scores them as one micro-batch and completes them;
  * requests that arrive meanwhile wait for the next free slot, and the slots'
  * launches overlap on the device.  Every request's results equal those of
- * st_batch_create / st_batch_run on the same query and candidates. */
+ * st_batch_create / st_batch_run on the same query and candidates.  In the
+ * persistent kernel a unit's wave raises its priority as its request ages
+ * (1, 2, 3 from 0.6, 1.2, 2 ms after publication), so older requests win the
+ * SIMDs' issue slots; the environment variable ST_SERVE_PRIO_US="a,b,c"
+ * (microseconds) sets other thresholds and "0" turns it off. */
 typedef struct st_server st_server;
 
 typedef struct {

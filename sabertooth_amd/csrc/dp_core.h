@@ -226,6 +226,26 @@ __device__ __forceinline__ void unpack_raw(const uint32_t (&raw)[R], typename P:
   }
 }
 
+// Wave priority by request age (serving): every 64 column steps the wave
+// sets its priority to the number of thresholds th[k] that the age of its
+// request (device wall clock, 100 MHz, low 32 bits) has passed, so older
+// requests' units win the SIMD's issue slots over newer ones.
+struct AgeClock {
+  uint32_t t_pub;
+  uint32_t th[3];
+};
+
+__device__ __forceinline__ void set_age_priority(const AgeClock &ac) {
+  const uint32_t age = (uint32_t)wall_clock64() - ac.t_pub;
+  const int lvl = (age >= ac.th[0]) + (age >= ac.th[1]) + (age >= ac.th[2]);
+  switch (__builtin_amdgcn_readfirstlane(lvl)) {
+    case 0: __builtin_amdgcn_s_setprio(0); break;
+    case 1: __builtin_amdgcn_s_setprio(1); break;
+    case 2: __builtin_amdgcn_s_setprio(2); break;
+    default: __builtin_amdgcn_s_setprio(3); break;
+  }
+}
+
 // Per-unit, per-group state shared by the passes.
 template <class P>
 struct GroupCtx {
@@ -238,12 +258,14 @@ struct GroupCtx {
 // One pass of G*R rows over all n_pad columns.  LAST: the pass holding the
 // matrix's last padded row (bottom-row diagonal ends are folded here).
 // WIDE: groups of 32 or 64 lanes (boundaries cross 16-lane DPP rows).
-template <class P, int R, bool UNG, bool LAST, bool END, bool WIDE>
+// AGE: the serving kernel's wave priority by request age (ac).
+template <class P, int R, bool UNG, bool LAST, bool END, bool WIDE, bool AGE = false>
 __device__ __forceinline__ void run_pass(const GroupCtx<P> &c, const typename P::L *lds, int stride, int base,
                                          bool first, bool final_pass, const uint4 *rd, uint4 *wr,
                                          typename P::T e0, typename P::T vopen, typename P::T vext,
                                          typename P::T &best, typename P::K (&top)[P::kPer][4],
-                                         typename P::T *edge, int lane, int &bj, int &bi) {
+                                         typename P::T *edge, int lane, int &bj, int &bi,
+                                         const AgeClock *ac = nullptr) {
   using T = typename P::T;
   using K = typename P::K;
   constexpr int kPer = P::kPer;
@@ -291,6 +313,9 @@ __device__ __forceinline__ void run_pass(const GroupCtx<P> &c, const typename P:
 
   const int steps = n_pad + G - 1;
   for (int t0 = 0; t0 < steps; t0 += kStepUnroll) {
+    if constexpr (AGE) {
+      if ((t0 & 63) == 0) set_age_priority(*ac);
+    }
 #pragma unroll
     for (int s = 0; s < kStepUnroll; s++) {
       const int t = t0 + s;
@@ -473,26 +498,27 @@ __device__ __forceinline__ void run_pass(const GroupCtx<P> &c, const typename P:
 
 // One pass with the instance for the group width and the pass position
 // chosen once (uniform per unit), not per step.
-template <class P, int R, bool UNG, bool END>
+template <class P, int R, bool UNG, bool END, bool AGE = false>
 __device__ __forceinline__ void run_pass_sel(bool last, const GroupCtx<P> &c, const typename P::L *lds, int stride,
                                              int base, bool first, bool final_pass, const uint4 *rd, uint4 *wr,
                                              typename P::T e0, typename P::T vopen, typename P::T vext,
                                              typename P::T &best, typename P::K (&top)[P::kPer][4],
-                                             typename P::T *edge, int lane, int &bj, int &bi) {
+                                             typename P::T *edge, int lane, int &bj, int &bi,
+                                             const AgeClock *ac = nullptr) {
   if (c.G > 16) {
     if (last)
-      run_pass<P, R, UNG, true, END, true>(c, lds, stride, base, first, true, rd, wr, e0, vopen, vext, best, top,
-                                           edge, lane, bj, bi);
+      run_pass<P, R, UNG, true, END, true, AGE>(c, lds, stride, base, first, true, rd, wr, e0, vopen, vext, best,
+                                                top, edge, lane, bj, bi, ac);
     else
-      run_pass<P, R, UNG, false, END, true>(c, lds, stride, base, first, final_pass, rd, wr, e0, vopen, vext, best,
-                                            top, edge, lane, bj, bi);
+      run_pass<P, R, UNG, false, END, true, AGE>(c, lds, stride, base, first, final_pass, rd, wr, e0, vopen, vext,
+                                                 best, top, edge, lane, bj, bi, ac);
   } else {
     if (last)
-      run_pass<P, R, UNG, true, END, false>(c, lds, stride, base, first, true, rd, wr, e0, vopen, vext, best, top,
-                                            edge, lane, bj, bi);
+      run_pass<P, R, UNG, true, END, false, AGE>(c, lds, stride, base, first, true, rd, wr, e0, vopen, vext, best,
+                                                 top, edge, lane, bj, bi, ac);
     else
-      run_pass<P, R, UNG, false, END, false>(c, lds, stride, base, first, final_pass, rd, wr, e0, vopen, vext, best,
-                                             top, edge, lane, bj, bi);
+      run_pass<P, R, UNG, false, END, false, AGE>(c, lds, stride, base, first, final_pass, rd, wr, e0, vopen, vext,
+                                                  best, top, edge, lane, bj, bi, ac);
   }
 }
 
@@ -513,12 +539,12 @@ struct UnitResult {
 // columns, `passes` passes of G*R rows.  lds holds kCodes rows of lds_stride
 // entries; myb is the block's pass-boundary region (two halves of
 // passbuf_half quads, then the right-edge parking area).
-template <class P, int R, bool UNG, bool END>
+template <class P, int R, bool UNG, bool END, bool AGE = false>
 __device__ __forceinline__ void score_unit(const QueryDev q, const PairDev *upairs, uint32_t npairs, uint32_t G,
                                            uint32_t n_pad, uint32_t passes, int32_t gap_open, int32_t gap_ext,
                                            const uint8_t *arena, const uint64_t *toff, const uint32_t *tlen,
                                            typename P::L *lds, int lds_stride, uint4 *myb, uint64_t passbuf_half,
-                                           int lane, UnitResult<P> &r) {
+                                           int lane, UnitResult<P> &r, const AgeClock *ac = nullptr) {
   using T = typename P::T;
   using L = typename P::L;
   using K = typename P::K;
@@ -584,8 +610,8 @@ __device__ __forceinline__ void score_unit(const QueryDev q, const PairDev *upai
     const uint4 *rd = myb + (uint64_t)((p + 1) & 1) * passbuf_half + (uint64_t)c.g * c.n_pad;
     uint4 *wr = myb + (uint64_t)(p & 1) * passbuf_half + (uint64_t)c.g * c.n_pad;
     const bool final_pass = p == (int)passes - 1;
-    run_pass_sel<P, R, UNG, END>(UNG && final_pass, c, lds, lds_stride, base, p == 0, final_pass, rd, wr, e0, vopen,
-                                 vext, best, r.top, edge, lane, bj, bi);
+    run_pass_sel<P, R, UNG, END, AGE>(UNG && final_pass, c, lds, lds_stride, base, p == 0, final_pass, rd, wr, e0,
+                                      vopen, vext, best, r.top, edge, lane, bj, bi, ac);
   }
   // best and top-4 over the group's lanes
   for (int off = 1; off < c.G; off <<= 1) {

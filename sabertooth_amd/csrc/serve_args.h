@@ -48,6 +48,9 @@ struct ServeArgs {
   uint4 *passbuf;
   uint64_t passbuf_half;
   uint64_t idle_ticks;     // exit after this long without work (100 MHz ticks)
+  uint64_t *clock;         // host-coherent: the device wall clock, written by the poller
+  uint32_t prio_ticks[3];  // a unit's wave priority rises to k + 1 once its request is
+                           // prio_ticks[k] ticks old (0: priorities off)
   uint32_t epoch;          // this launch's number, 1..65535
   uint64_t zero;           // 0: operand of the read-modify-write reads
   int lds_stride;

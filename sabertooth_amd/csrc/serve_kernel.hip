@@ -86,6 +86,8 @@ __device__ void serve_poller(const ServeArgs &a, int lane) {
     t = bcast64(t);
     stop = (uint32_t)__builtin_amdgcn_readfirstlane((int)stop);
     const uint64_t now = wall_clock64();
+    // the host stamps each request's units with this clock when publishing
+    if (lane == 0) __hip_atomic_store(a.clock, now, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
     if (t != last_t) {
       if (lane < kSeenCopies) rmw_write(mine, t);
       last_t = t;
@@ -182,8 +184,15 @@ __global__ __launch_bounds__(64, 3) void serve_kernel(ServeArgs a) {
     q.prof8 = nullptr;
 
     UnitResult<P> r;
-    score_unit<P, R, true, false>(q, u.pairs, u.npairs, u.G, u.n_pad, u.passes, u.open, u.ext, a.arena, a.toff,
-                                  a.tlen, lds, a.lds_stride, myb, a.passbuf_half, lane, r);
+    if (a.prio_ticks[0]) {
+      const AgeClock ac{u.t_pub, {a.prio_ticks[0], a.prio_ticks[1], a.prio_ticks[2]}};
+      score_unit<P, R, true, false, true>(q, u.pairs, u.npairs, u.G, u.n_pad, u.passes, u.open, u.ext, a.arena,
+                                          a.toff, a.tlen, lds, a.lds_stride, myb, a.passbuf_half, lane, r, &ac);
+      __builtin_amdgcn_s_setprio(0);
+    } else {
+      score_unit<P, R, true, false>(q, u.pairs, u.npairs, u.G, u.n_pad, u.passes, u.open, u.ext, a.arena, a.toff,
+                                    a.tlen, lds, a.lds_stride, myb, a.passbuf_half, lane, r);
+    }
 
     // gate of each pair slot s: lane s mod 64; its group's merged values come
     // from the group's first lane (every lane of a group holds the merge)

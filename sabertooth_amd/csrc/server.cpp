@@ -17,6 +17,8 @@
 #include <atomic>
 #include <chrono>
 #include <condition_variable>
+#include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <deque>
 #include <memory>
@@ -205,7 +207,10 @@ struct Persist {
   static constexpr uint64_t kHalfQuads = 20480;  // pass-boundary half of one block
   uint64_t arena_bytes = 4ull << 30;
   Coherent<ServeUnit> ring;
-  Coherent<uint64_t> ctl;  // [0] tail
+  Coherent<uint64_t> ctl;  // [0] tail, [kClockWord] the device wall clock
+  static constexpr int kClockWord = 8;  // its own 64-byte line
+  // wave priority by request age: 1, 2, 3 from 0.6, 1.2, 2 ms (100 MHz ticks)
+  uint32_t prio_ticks[3] = {60000, 120000, 200000};
   Coherent<uint32_t> stop;  // [0] stop, [4] malformed-unit flag
   Coherent<ServeRow> results;
   Coherent<uint32_t> remaining;
@@ -508,6 +513,8 @@ int ensure_running(st_server *srv) {
   a.passbuf = pe.passbuf.p;
   a.passbuf_half = Persist::kHalfQuads;
   a.idle_ticks = 500000;  // 5 ms of the 100 MHz wall clock
+  a.clock = pe.ctl.d + Persist::kClockWord;
+  for (int k = 0; k < 3; k++) a.prio_ticks[k] = pe.prio_ticks[k];
   a.epoch = (uint32_t)(pe.launches % 65535) + 1;
   a.zero = 0;
   // tickets restart at the first unit the previous launch did not finish
@@ -657,7 +664,8 @@ bool publish(st_server *srv, const std::shared_ptr<Req> &r, int *rc) {
     pe.inflight_cells -= r->cells;
     return true;
   }
-  // unit descriptors
+  // unit descriptors, stamped with the device clock the poller last wrote
+  const uint32_t t_pub = (uint32_t)__atomic_load_n(pe.ctl.p + Persist::kClockWord, __ATOMIC_RELAXED);
   const uint64_t u0 = pe.published;
   for (size_t i = 0; i < units.size(); i++) {
     const Unit &u = units[i];
@@ -673,7 +681,7 @@ bool publish(st_server *srv, const std::shared_ptr<Req> &r, int *rc) {
     su.passes = u.passes;
     su.open = r->open;
     su.ext = r->ext;
-    su.gate = r->gate;
+    su.t_pub = t_pub;
     su.req = slot;
     pe.ring.p[(u0 + i) & (Persist::kRingUnits - 1)] = su;
   }
@@ -842,12 +850,22 @@ void completer(st_server *srv) {
 int setup_persistent(st_server *s) {
   auto pe = std::make_unique<Persist>();
   pe->rows.cap = Persist::kResultRows;
+  // ST_SERVE_PRIO_US="a,b,c": a unit's wave priority rises to 1, 2, 3 once
+  // its request is a, b, c microseconds old; "0": priorities off
+  if (const char *e = std::getenv("ST_SERVE_PRIO_US")) {
+    unsigned v[3] = {0, 0, 0};
+    if (std::sscanf(e, "%u,%u,%u", &v[0], &v[1], &v[2]) == 3 && v[0] > 0 && v[0] <= v[1] && v[1] <= v[2] &&
+        v[2] < 40000000u)
+      for (int k = 0; k < 3; k++) pe->prio_ticks[k] = v[k] * 100u;
+    else if (std::strcmp(e, "0") == 0)
+      pe->prio_ticks[0] = pe->prio_ticks[1] = pe->prio_ticks[2] = 0;
+  }
   if (s->cfg.arena_mib) pe->arena_bytes = (uint64_t)s->cfg.arena_mib << 20;
   const size_t lds = (size_t)kCodes * pe->lds_stride * 2;
   const int occ = serve_occupancy(lds);
   // leave one block per CU to the other launches (rescoring, int32 requests)
   pe->grid = (uint32_t)std::max(1, occ - 1) * (uint32_t)s->ctx->cus;
-  bool ok = pe->ring.alloc(Persist::kRingUnits) == hipSuccess && pe->ctl.alloc(8) == hipSuccess &&
+  bool ok = pe->ring.alloc(Persist::kRingUnits) == hipSuccess && pe->ctl.alloc(2 * Persist::kClockWord) == hipSuccess &&
             pe->stop.alloc(8) == hipSuccess && pe->results.alloc(Persist::kResultRows) == hipSuccess &&
             pe->remaining.alloc(Persist::kReqSlots) == hipSuccess &&
             pe->dctl.alloc((size_t)kCtlWords * kCtlStride) == hipSuccess &&

@@ -70,7 +70,8 @@ struct ServeUnit {
   uint32_t m, ld;
   uint32_t npairs;
   uint32_t G, n_pad, passes;
-  int32_t open, ext, gate;
+  int32_t open, ext;
+  uint32_t t_pub;          // low 32 bits of the device wall clock (100 MHz) when published
   uint32_t req;            // request slot: its completion counter
 };
 static_assert(sizeof(ServeUnit) == 64, "one 64-byte descriptor per unit");

@@ -171,3 +171,22 @@ def test_bad_requests_fail_cleanly(ctx, setup):
             srv.wait(123456, timeout_ms=10)
     finally:
         srv.close()
+
+
+@pytest.mark.parametrize("prio", ["0", "1,2,3"])
+def test_age_priorities_leave_results_unchanged(ctx, setup, monkeypatch, prio):
+    # the serving kernel raises a unit's wave priority with its request's
+    # age; off, and with thresholds so small that every unit reaches
+    # priority 3, every row still equals the plain batch's
+    monkeypatch.setenv("ST_SERVE_PRIO_US", prio)
+    srv = st.Server(ctx, setup["db"], workers=2, persistent=True)
+    try:
+        rng = np.random.default_rng(31)
+        qi = rng.integers(0, len(setup["qps"]), size=40)
+        arrivals = np.cumsum(rng.exponential(0.2, size=len(qi)))
+        reqs = [(setup["qps"][q], setup["cands"][q], setup["gaps"][q], setup["gates"][q]) for q in qi]
+        outs, _ = srv.run_trace(reqs, arrivals)
+        for q, out in zip(qi, outs):
+            assert (_fields(out) == _fields(setup["ref"][q])).all(), q
+    finally:
+        srv.close()
