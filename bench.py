@@ -270,6 +270,7 @@ def spans_leg(O, st, ctx, db, qps, rows, w, res, offs, per_query=32, sample=120)
         want = O.span(rows[qof[k]], w.target(int(tof[k])), w.gap_open, w.gap_extend)
         got = tuple(int(sp[k][f]) for f in ("score", "query_start", "query_end", "target_start", "target_end"))
         mism += got != want
+    phases = st.align_spans_phases(ctx)
     li, nf = st.align_spans_launches(ctx)
     launches = [{"pass": "forward" if i < nf else "reverse", "kind": int(r["kind"]),
                  "rows_per_lane": int(r["rows_per_lane"]), "units": int(r["units"]), "grid": int(r["grid"]),
@@ -280,6 +281,7 @@ def spans_leg(O, st, ctx, db, qps, rows, w, res, offs, per_query=32, sample=120)
     out = {"hits": int(len(sp)), "ms": round(1e3 * t, 3), "forward_cells": int(fwd), "reverse_cells": rev,
            "gcups": round((fwd + rev) / t / 1e9, 1), "sampled_vs_oracle": int(len(pick)), "mismatches": int(mism),
            "executed_over_useful": {p: round(execd[p] / max(useful[p], 1), 4) for p in ("forward", "reverse")},
+           "phases_ms": {k: round(v, 3) for k, v in phases.items()},
            "launches": launches}
     log(f"alignment spans: {out['hits']} hits in {out['ms']:.1f} ms, {out['gcups']} GCUPS over both passes, "
         f"{len(pick) - mism}/{len(pick)} sampled spans bit-exact; executed/useful cells forward "

@@ -421,6 +421,15 @@ int st_batch_launches(const st_batch *b, st_launch_info *out, uint64_t cap, uint
  * reference, for the executed/useful cells and device time of each pass. */
 int st_align_spans_launches(st_ctx *ctx, st_launch_info *out, uint64_t cap, uint64_t *n,
                             uint64_t *n_forward);
+/* Host wall-clock phases of the last st_align_spans_batch call on ctx, in
+ * milliseconds: ms[0] forward planning and uploads, [1] forward device run,
+ * [2] forward read-back, [3] reverse preparation (the end rows sorted, the
+ * reversed target prefixes gathered, the profile views), [4] reverse
+ * planning and uploads, [5] reverse device run, [6] reverse read-back and
+ * the spans, [7] the whole call.  A diagnostic with no counterpart in the
+ * reference. */
+#define ST_SPAN_PHASES 8
+int st_align_spans_phases(st_ctx *ctx, double ms[ST_SPAN_PHASES]);
 /* DP cells of the batch (sum of m*n) and pairs. */
 uint64_t st_batch_cells(const st_batch *b);
 uint64_t st_batch_pairs(const st_batch *b);

@@ -106,6 +106,7 @@ _SIGS = [
     ("st_batch_stats", C.c_int, [_P, _P, _P, _P]),
     ("st_batch_launches", C.c_int, [_P, _P, C.c_uint64, C.POINTER(C.c_uint64)]),
     ("st_align_spans_launches", C.c_int, [_P, _P, C.c_uint64, C.POINTER(C.c_uint64), C.POINTER(C.c_uint64)]),
+    ("st_align_spans_phases", C.c_int, [_P, _P]),
     ("st_batch_cells", C.c_uint64, [_P]),
     ("st_batch_pairs", C.c_uint64, [_P]),
     ("st_batch_destroy", C.c_int, [_P]),

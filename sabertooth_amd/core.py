@@ -271,6 +271,17 @@ def align_spans_launches(ctx: Context):
     return out[: n.value], int(nf.value)
 
 
+SPAN_PHASES = ("forward_plan", "forward_run", "forward_readback", "reverse_prep", "reverse_plan", "reverse_run",
+               "reverse_readback", "call")
+
+
+def align_spans_phases(ctx: Context) -> dict:
+    """Host wall-clock phases (ms) of the last align_spans call on ctx."""
+    ms = (C.c_double * len(SPAN_PHASES))()
+    check(lib().st_align_spans_phases(ctx.handle, ms))
+    return {k: float(ms[i]) for i, k in enumerate(SPAN_PHASES)}
+
+
 class Postings:
     """An index's posting lists on the device: (target << 32) | position."""
 

@@ -4,7 +4,11 @@
 
 #include <algorithm>
 #include <cctype>
+#include <chrono>
 #include <cmath>
+#include <condition_variable>
+#include <deque>
+#include <functional>
 #include <cstdlib>
 #include <cstdio>
 #include <cstring>
@@ -645,15 +649,96 @@ bool plan_windows(Plan &pl, std::vector<Unit> &pending, uint32_t rep_query, cons
   return true;
 }
 
+// Host worker threads shared by every planning call of the process, created
+// once (up to 15, one fewer than the cores, max 16): run(n, f) calls f(t)
+// for t in [0, n) on the workers and the calling thread and returns when all
+// n have finished.  Concurrent callers queue their tasks in one FIFO; a
+// caller runs queued tasks itself while it waits, so it never waits on
+// workers that are busy elsewhere without making progress.
+class WorkPool {
+ public:
+  static WorkPool &get() {
+    static WorkPool *p = new WorkPool();  // never destroyed: workers outlive static teardown
+    return *p;
+  }
+  unsigned width() const { return (unsigned)workers_.size() + 1; }
+  template <class F>
+  void run(unsigned n, F f) {
+    struct Group {
+      std::mutex mu;
+      std::condition_variable cv;
+      unsigned left;
+    };
+    auto g = std::make_shared<Group>();
+    g->left = n;
+    auto finish = [g] {
+      std::lock_guard<std::mutex> lk(g->mu);
+      if (--g->left == 0) g->cv.notify_all();
+    };
+    {
+      std::lock_guard<std::mutex> lk(mu_);
+      for (unsigned t = 1; t < n; t++)
+        tasks_.push_back([f, t, finish]() mutable {
+          f(t);
+          finish();
+        });
+    }
+    cv_.notify_all();
+    f(0);
+    finish();
+    for (;;) {
+      std::function<void()> task;
+      {
+        std::lock_guard<std::mutex> lk(mu_);
+        if (!tasks_.empty()) {
+          task = std::move(tasks_.front());
+          tasks_.pop_front();
+        }
+      }
+      if (task) {
+        task();
+        continue;
+      }
+      std::unique_lock<std::mutex> lk(g->mu);
+      if (g->left == 0) return;
+      g->cv.wait_for(lk, std::chrono::microseconds(200));
+      if (g->left == 0) return;
+    }
+  }
+
+ private:
+  WorkPool() {
+    const unsigned hw = std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
+    for (unsigned k = 0; k + 1 < hw; k++)
+      workers_.emplace_back([this] {
+        for (;;) {
+          std::function<void()> task;
+          {
+            std::unique_lock<std::mutex> lk(mu_);
+            cv_.wait(lk, [this] { return !tasks_.empty(); });
+            task = std::move(tasks_.front());
+            tasks_.pop_front();
+          }
+          task();
+        }
+      });
+    for (auto &w : workers_) w.detach();
+  }
+  std::mutex mu_;
+  std::condition_variable cv_;
+  std::deque<std::function<void()>> tasks_;
+  std::vector<std::thread> workers_;
+};
+
 // plan_query over the queries [0, nq): one(pl, pending, q) plans query q.
-// Batches of many pairs are planned on up to 8 threads, each over a
-// contiguous run of queries into a plan of its own; the runs are appended
-// in query order with their pair indices shifted, which gives the plan one
-// thread would make.  Returns the first query whose planning failed, or nq.
+// Batches of many pairs are planned on the shared worker threads (one run
+// per 2,048 pairs, at most one per core), each over a contiguous run of
+// queries into a plan of its own; the runs are appended in query order with
+// their pair indices shifted, which gives the plan one thread would make.
+// Returns the first query whose planning failed, or nq.
 template <class F>
 static uint64_t plan_queries(Plan &pl, std::vector<Unit> &pending, uint64_t nq, uint64_t npairs, F one) {
-  const unsigned hw = std::max(1u, std::thread::hardware_concurrency());
-  const uint64_t T = std::min<uint64_t>({(uint64_t)std::min(hw, 8u), nq, npairs / 2048 + 1});
+  const uint64_t T = std::min<uint64_t>({(uint64_t)WorkPool::get().width(), nq, npairs / 2048 + 1});
   if (T <= 1) {
     for (uint64_t q = 0; q < nq; q++)
       if (!one(pl, pending, q)) return q;
@@ -662,17 +747,13 @@ static uint64_t plan_queries(Plan &pl, std::vector<Unit> &pending, uint64_t nq, 
   std::vector<Plan> pls(T);
   std::vector<std::vector<Unit>> pend(T);
   std::vector<uint64_t> bad(T, nq);
-  std::vector<std::thread> th;
-  th.reserve(T);
-  for (uint64_t t = 0; t < T; t++)
-    th.emplace_back([&, t] {
-      for (uint64_t q = nq * t / T; q < nq * (t + 1) / T; q++)
-        if (!one(pls[t], pend[t], q)) {
-          bad[t] = q;
-          return;
-        }
-    });
-  for (std::thread &x : th) x.join();
+  WorkPool::get().run((unsigned)T, [&](unsigned t) {
+    for (uint64_t q = nq * t / T; q < nq * (t + 1) / T; q++)
+      if (!one(pls[t], pend[t], q)) {
+        bad[t] = q;
+        return;
+      }
+  });
   for (uint64_t t = 0; t < T; t++)
     if (bad[t] < nq) return bad[t];
   for (uint64_t t = 0; t < T; t++) {
@@ -991,19 +1072,25 @@ static int batch_create_impl(st_ctx *ctx, const st_db *db, const st_qprof *const
     }
   b->cells = cells;
   b->qkind.resize(nq);
+  // ST_END_PK16=0 keeps every end-cell batch on the int32 kernels
+  const char *e16 = std::getenv("ST_END_PK16");
+  const bool end_pk16 = !(e16 && e16[0] == '0');
   for (uint64_t q = 0; q < nq; q++) {
     b->qkind[q] = kind_for(b->qps[q], b->open[q], b->ext[q]);
     // the packed kernel's diagonal keys hold m + n - 1 <= kMaxDiagsPk16
     uint32_t nmax = 0;
     for (uint64_t k = cand_off[q]; k < cand_off[q + 1]; k++) nmax = std::max(nmax, db->hlen[cand_tids[k]]);
     if (b->qkind[q] == kPk16 && (uint64_t)b->qps[q]->m + nmax - 1 > kMaxDiagsPk16) b->qkind[q] = kI32F;
-    // the end cell is tracked by the int32 kernels, over the int8 or int16
-    // profile where the query has one
+    // the end cell: the packed int16 kernel (two pairs per group, an int32
+    // rescoring at the clamp) where the query has its own int16 profile and
+    // the gaps allow it; the window units of profile views, and the rest,
+    // on the int32 kernels over the int8 or int16 profile where there is one
     if (end) {
       const bool p8 = b->qps[q]->fits8 && b->qps[q]->view.prof8;
       const bool p16 = b->qps[q]->fits16 && b->qps[q]->view.prof16;
-      b->qkind[q] = kind_unfloored(b->qkind[q]) ? (p8 ? kI32N8 : p16 ? kI32N16 : kI32N)
-                                                : (p8 ? kI32F8 : p16 ? kI32F16 : kI32F);
+      if (!(b->qkind[q] == kPk16 && p16 && !win_off && end_pk16))
+        b->qkind[q] = kind_unfloored(b->qkind[q]) ? (p8 ? kI32N8 : p16 ? kI32N16 : kI32N)
+                                                  : (p8 ? kI32F8 : p16 ? kI32F16 : kI32F);
     }
   }
   std::lock_guard<std::mutex> lk(ctx->mu);
@@ -1319,7 +1406,13 @@ int st_sw_score_batch(st_ctx *ctx, const st_qprof *qp, const st_db *db, const ui
 static int score_end_multi(st_ctx *ctx, const st_db *db, const st_qprof *const *qps, uint64_t nq,
                            const uint64_t *off, const uint32_t *tids, const int32_t *open, const int32_t *ext,
                            st_score_end *out, const char *who, const uint64_t *win_off = nullptr,
-                           uint64_t nwin = 0, std::vector<st_launch_info> *launches = nullptr);
+                           uint64_t nwin = 0, std::vector<st_launch_info> *launches = nullptr,
+                           double *phase_ms = nullptr);
+
+// host wall clock in milliseconds
+static double wall_ms() {
+  return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now().time_since_epoch()).count();
+}
 
 int st_sw_score_end_batch(st_ctx *ctx, const st_qprof *qp, const st_db *db, const uint32_t *tids, uint64_t count,
                           int32_t gap_open, int32_t gap_extend, st_score_end *out) {
@@ -1336,18 +1429,21 @@ int st_sw_score_end_batch(st_ctx *ctx, const st_qprof *qp, const st_db *db, cons
 static int score_end_multi(st_ctx *ctx, const st_db *db, const st_qprof *const *qps, uint64_t nq,
                            const uint64_t *off, const uint32_t *tids, const int32_t *open, const int32_t *ext,
                            st_score_end *out, const char *who, const uint64_t *win_off, uint64_t nwin,
-                           std::vector<st_launch_info> *launches) {
+                           std::vector<st_launch_info> *launches, double *phase_ms) {
   const uint64_t count = nq ? off[nq] : 0;
   if (count == 0) return ST_OK;
   // the context's end-cell batch object, refilled: its device buffers and
   // events outlive the call
   std::lock_guard<std::mutex> el(ctx->end_mu);
+  const double t0 = wall_ms();
   st_batch *b = nullptr;
   int rc = batch_create_impl(ctx, db, qps, nq, off, tids, open, ext, nullptr, false, &b, true, ctx->end_batch,
                              win_off, nwin);
   if (rc) return rc;
   ctx->end_batch = b;
+  const double t1 = wall_ms();
   rc = st_batch_run(b);
+  const double t2 = wall_ms();
   if (!rc && launches) {
     // the run's launches, while the batch object is still this call's
     uint64_t n = 0;
@@ -1375,6 +1471,11 @@ static int score_end_multi(st_ctx *ctx, const st_db *db, const st_qprof *const *
       }
     }
   }
+  if (phase_ms) {
+    phase_ms[0] = t1 - t0;
+    phase_ms[1] = t2 - t1;
+    phase_ms[2] = wall_ms() - t2;
+  }
   return rc;
 }
 
@@ -1386,6 +1487,7 @@ struct SpanWork {
   std::vector<st_qprof> views;  // the reversed prefixes' profile views, reused
   std::vector<st_launch_info> launches;  // the last call's: forward, then reverse
   uint64_t n_forward = 0;
+  double phase_ms[ST_SPAN_PHASES] = {};  // the last call's host phases
   DevBuf<uint8_t> mirror;
   DevBuf<RevPrefix> pre;
   DevBuf<MirrorJob> mj;
@@ -1404,15 +1506,64 @@ int st_align_spans_batch(st_ctx *ctx, const st_db *db, const st_qprof *const *qu
   if (nh == 0) return ST_OK;
   if (!out) return fail(ST_ERR_INVALID, "st_align_spans_batch: null output");
   std::lock_guard<std::mutex> sl(ctx->spans_mu);
+  const double t_call = wall_ms();
+  if (!ctx->spans) ctx->spans = new SpanWork();
+  SpanWork &w = *ctx->spans;
+  w.launches.clear();
+  w.n_forward = 0;
+  for (double &x : w.phase_ms) x = 0.0;
+  // 0. row-reversed profiles of the queries with hits (int8 or int16 where
+  // the query has such a profile: the end-cell kernels then stage fewer
+  // bytes), built before the forward pass: they depend on the queries only.
+  // A view's passes read at most its rows rounded up to a pass (<= 1024
+  // rows) from its window start m - 1 - e: m + 1024 rows cover every view.
+  std::vector<uint64_t> mir_off(nq + 1, 0);  // bytes
+  std::vector<uint32_t> mir_ld(nq, 0), mir_es(nq, 4);
+  std::vector<MirrorJob> mj;
+  std::vector<uint32_t> chunk_job, chunk_first;
+  for (uint64_t q = 0; q < nq; q++) {
+    if (hit_off[q + 1] > hit_off[q] && queries[q]) {
+      mir_ld[q] = (queries[q]->m + kMaxPassRows + 63) & ~63u;
+      if (queries[q]->fits8 && queries[q]->prof8.p)
+        mir_es[q] = 1;
+      else if (queries[q]->fits16 && queries[q]->prof16.p)
+        mir_es[q] = 2;
+    }
+    mir_off[q + 1] = mir_off[q] + (((uint64_t)kCodes * mir_ld[q] * mir_es[q] + 255) & ~255ull);
+    if (mir_ld[q]) {
+      chunk_first.push_back((uint32_t)chunk_job.size());
+      const void *src = mir_es[q] == 1   ? (const void *)queries[q]->prof8.p
+                        : mir_es[q] == 2 ? (const void *)queries[q]->prof16.p
+                                         : (const void *)queries[q]->prof32.p;
+      mj.push_back(MirrorJob{src, mir_off[q], queries[q]->ld, queries[q]->m, mir_ld[q], mir_es[q]});
+      for (uint32_t c = 0; c < (mir_ld[q] + 255) / 256; c++) chunk_job.push_back((uint32_t)(mj.size() - 1));
+    }
+  }
+  {
+    std::lock_guard<std::mutex> lk(ctx->mu);
+    ST_HIP(hipSetDevice(ctx->device));
+    // grow-only buffers, uploads in stream order: the host tables are locals
+    // of this call, which waits for the stream before they go
+    hipStream_t st = ctx->stream;
+    if (w.mirror.reserve(std::max<uint64_t>(mir_off[nq], 1)) != hipSuccess ||
+        w.mj.put(mj.data(), mj.size(), st) != hipSuccess || w.cj.put(chunk_job.data(), chunk_job.size(), st) != hipSuccess ||
+        w.cf.put(chunk_first.data(), chunk_first.size(), st) != hipSuccess)
+      return fail(ST_ERR_NOMEM, "st_align_spans_batch: device allocation failed");
+    ST_HIP(launch_mirror_prof(w.mj.p, w.cj.p, w.cf.p, (uint32_t)chunk_job.size(), w.mirror.p, st));
+  }
+  // every return from here waits for the stream: copies from this call's
+  // host tables may still be queued on it
+  struct StreamWait {
+    hipStream_t s;
+    ~StreamWait() { (void)hipStreamSynchronize(s); }
+  } stream_wait{ctx->stream};
   // 1. forward pass: score and the column-major-first end cell
   std::vector<st_score_end> fw(nh);
-  if (!ctx->spans) ctx->spans = new SpanWork();
-  ctx->spans->launches.clear();
-  ctx->spans->n_forward = 0;
   int rc = score_end_multi(ctx, db, queries, nq, hit_off, hit_tids, gap_open, gap_extend, fw.data(),
-                           "st_align_spans_batch", nullptr, 0, &ctx->spans->launches);
+                           "st_align_spans_batch", nullptr, 0, &w.launches, w.phase_ms);
   if (rc) return rc;
-  ctx->spans->n_forward = ctx->spans->launches.size();
+  w.n_forward = w.launches.size();
+  const double t_rev = wall_ms();
   for (uint64_t k = 0; k < nh; k++) {
     out[k].score = fw[k].score;
     out[k].reserved = 0;
@@ -1441,27 +1592,23 @@ int st_align_spans_batch(st_ctx *ctx, const st_db *db, const st_qprof *const *qu
           return fail(ST_ERR_DEVICE, "st_align_spans_batch: end cell outside its problem");
         rv.push_back({(uint32_t)q, (uint32_t)fw[k].query_end, k});
       }
-    std::stable_sort(rv.begin() + r0, rv.end(), [](const Rev &a, const Rev &b) { return a.e < b.e; });
+    if (rv.size() - r0 <= 64) {
+      // a short run: stable insertion sort, no allocation
+      for (size_t i = r0 + 1; i < rv.size(); i++) {
+        const Rev x = rv[i];
+        size_t j = i;
+        for (; j > r0 && rv[j - 1].e > x.e; j--) rv[j] = rv[j - 1];
+        rv[j] = x;
+      }
+    } else {
+      std::stable_sort(rv.begin() + r0, rv.end(), [](const Rev &a, const Rev &b) { return a.e < b.e; });
+    }
   }
   if (rv.empty()) return ST_OK;
   const uint64_t nr = rv.size();
   if (nr > 0xffffffffull) return fail(ST_ERR_INVALID, "st_align_spans_batch: more than 2^32 hits");
-  // row-reversed profiles of the queries with hits (int16 where the query
-  // has an int16 profile: the end-cell kernels then stage half the bytes),
-  // and the reversed target prefixes (database layout: 16-byte aligned
-  // starts, code 21 padding)
-  std::vector<uint64_t> mir_off(nq + 1, 0);  // bytes
-  std::vector<uint32_t> mir_ld(nq, 0), mir_es(nq, 4);
-  // a view's passes read at most its rows rounded up to a pass (<= 1024
-  // rows) from its window start m - 1 - e: m + 1024 rows cover every view
-  for (const Rev &r : rv) mir_ld[r.q] = (queries[r.q]->m + kMaxPassRows + 63) & ~63u;
-  for (uint64_t q = 0; q < nq; q++) {
-    if (queries[q]->fits8 && queries[q]->prof8.p)
-      mir_es[q] = 1;
-    else if (queries[q]->fits16 && queries[q]->prof16.p)
-      mir_es[q] = 2;
-    mir_off[q + 1] = mir_off[q] + (((uint64_t)kCodes * mir_ld[q] * mir_es[q] + 255) & ~255ull);
-  }
+  // the reversed target prefixes (database layout: 16-byte aligned starts,
+  // code 21 padding)
   std::vector<RevPrefix> pre(nr);
   std::vector<uint64_t> doff(nr + 1, 0);
   std::vector<uint32_t> rlen(nr);
@@ -1478,35 +1625,16 @@ int st_align_spans_batch(st_ctx *ctx, const st_db *db, const st_qprof *const *qu
   rdb->hlen = rlen;
   rdb->max_len = 0;
   for (uint32_t l : rlen) rdb->max_len = std::max(rdb->max_len, l);
-  // one mirror job per query with hits; its 256-row chunks in one launch
-  std::vector<MirrorJob> mj;
-  std::vector<uint32_t> chunk_job, chunk_first;
-  for (uint64_t q = 0; q < nq; q++)
-    if (mir_ld[q]) {
-      chunk_first.push_back((uint32_t)chunk_job.size());
-      const void *src = mir_es[q] == 1   ? (const void *)queries[q]->prof8.p
-                        : mir_es[q] == 2 ? (const void *)queries[q]->prof16.p
-                                         : (const void *)queries[q]->prof32.p;
-      mj.push_back(MirrorJob{src, mir_off[q], queries[q]->ld, queries[q]->m, mir_ld[q], mir_es[q]});
-      for (uint32_t c = 0; c < (mir_ld[q] + 255) / 256; c++) chunk_job.push_back((uint32_t)(mj.size() - 1));
-    }
-  SpanWork &w = *ctx->spans;
   {
     std::lock_guard<std::mutex> lk(ctx->mu);
     ST_HIP(hipSetDevice(ctx->device));
-    // grow-only buffers; the host tables are locals of this call, which
-    // waits for the stream below before they go
+    // in stream order, before the reverse pass's launches; the host tables
+    // stay alive until this call has waited for the stream
     hipStream_t st = ctx->stream;
-    if (w.mirror.reserve(std::max<uint64_t>(mir_off[nq], 1)) != hipSuccess ||
-        w.pre.put(pre.data(), nr, st) != hipSuccess || w.mj.put(mj.data(), mj.size(), st) != hipSuccess ||
-        w.cj.put(chunk_job.data(), chunk_job.size(), st) != hipSuccess ||
-        w.cf.put(chunk_first.data(), chunk_first.size(), st) != hipSuccess ||
-        rdb->off.put(doff.data(), nr + 1, st) != hipSuccess || rdb->len.put(rlen.data(), nr, st) != hipSuccess ||
-        rdb->arena.reserve(doff[nr] + 64) != hipSuccess)
+    if (w.pre.put(pre.data(), nr, st) != hipSuccess || rdb->off.put(doff.data(), nr + 1, st) != hipSuccess ||
+        rdb->len.put(rlen.data(), nr, st) != hipSuccess || rdb->arena.reserve(doff[nr] + 64) != hipSuccess)
       return fail(ST_ERR_NOMEM, "st_align_spans_batch: device allocation failed");
-    ST_HIP(launch_mirror_prof(w.mj.p, w.cj.p, w.cf.p, (uint32_t)chunk_job.size(), w.mirror.p, st));
     ST_HIP(launch_rev_prefix(db->arena.p, db->off.p, w.pre.p, (uint32_t)nr, rdb->off.p, rdb->arena.p, st));
-    ST_HIP(hipStreamSynchronize(st));
   }
   DevBuf<uint8_t> &mirror = w.mirror;
   // one profile view per (query, end row): stride = the mirror's rows, rows
@@ -1520,7 +1648,10 @@ int st_align_spans_batch(st_ctx *ctx, const st_db *db, const st_qprof *const *qu
   std::vector<const st_qprof *> vq;
   vq.reserve(nviews);
   std::vector<uint64_t> voff{0};
+  voff.reserve(nviews + 1);
   std::vector<int32_t> vopen, vext;
+  vopen.reserve(nviews);
+  vext.reserve(nviews);
   std::vector<uint32_t> vt(nr);
   for (uint64_t k = 0; k < nr; k++) {
     if (k == 0 || rv[k].q != rv[k - 1].q || rv[k].e != rv[k - 1].e) {
@@ -1562,8 +1693,10 @@ int st_align_spans_batch(st_ctx *ctx, const st_db *db, const st_qprof *const *qu
     win_off.push_back(v);
   }
   std::vector<st_score_end> rs(nr);
+  w.phase_ms[3] = wall_ms() - t_rev;
   rc = score_end_multi(ctx, rdb, vq.data(), vq.size(), voff.data(), vt.data(), vopen.data(), vext.data(),
-                       rs.data(), "st_align_spans_batch", win_off.data(), win_off.size() - 1, &w.launches);
+                       rs.data(), "st_align_spans_batch", win_off.data(), win_off.size() - 1, &w.launches,
+                       w.phase_ms + 4);
   if (rc) return rc;
   for (uint64_t k = 0; k < nr; k++) {
     st_span &s = out[rv[k].hit];
@@ -1575,6 +1708,14 @@ int st_align_spans_batch(st_ctx *ctx, const st_db *db, const st_qprof *const *qu
     s.query_start = s.query_end - rs[k].query_end;
     s.target_start = s.target_end - rs[k].target_end;
   }
+  w.phase_ms[7] = wall_ms() - t_call;
+  return ST_OK;
+}
+
+int st_align_spans_phases(st_ctx *ctx, double ms[ST_SPAN_PHASES]) {
+  if (!ctx || !ms) return fail(ST_ERR_INVALID, "st_align_spans_phases: null argument");
+  std::lock_guard<std::mutex> sl(ctx->spans_mu);
+  for (int k = 0; k < ST_SPAN_PHASES; k++) ms[k] = ctx->spans ? ctx->spans->phase_ms[k] : 0.0;
   return ST_OK;
 }
 

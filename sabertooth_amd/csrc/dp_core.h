@@ -264,7 +264,7 @@ __device__ __forceinline__ void run_pass(const GroupCtx<P> &c, const typename P:
                                          bool first, bool final_pass, const uint4 *rd, uint4 *wr,
                                          typename P::T e0, typename P::T vopen, typename P::T vext,
                                          typename P::T &best, typename P::K (&top)[P::kPer][4],
-                                         typename P::T *edge, int lane, int &bj, int &bi,
+                                         typename P::T *edge, int lane, int (&bj)[2], int (&bi)[2],
                                          const AgeClock *ac = nullptr) {
   using T = typename P::T;
   using K = typename P::K;
@@ -285,9 +285,9 @@ __device__ __forceinline__ void run_pass(const GroupCtx<P> &c, const typename P:
   T out_h = 0, out_f = e0, out_u = 0, out_b = 0;
   // END: this lane's first optimum of the pass (its cells in column-major
   // order: columns ascending, rows ascending in a column), merged into the
-  // unit's after the pass
+  // unit's after the pass; Pk16: one per int16 half (pair slot)
   T pbest = 0;
-  int pbj = 0x7fffffff, pbi = 0;
+  int pbj[2] = {0x7fffffff, 0x7fffffff}, pbi[2] = {0, 0};
   // the group's first lane streams residues: cw = columns [4u, 4u + 4),
   // nw = the next four; the boundary row of the previous pass one ahead
   uint32_t cw[2] = {kSent4, kSent4}, nw[2] = {kSent4, kSent4};
@@ -406,18 +406,39 @@ __device__ __forceinline__ void run_pass(const GroupCtx<P> &c, const typename P:
           E[k] = P::max(P::subf(E[k], vext), ho);
           F = P::max(P::subf(F, vext), ho);
         }
-        if (END && __builtin_expect(cm > pbest, 0)) {
-          // a strictly larger value in this column: the first optimum of
-          // the pass so far is this column's first row holding it (a lane
-          // sees its columns in increasing j, a column's rows in increasing
-          // i); a rare, divergent branch
-          int kk = R - 1;
+        if constexpr (END && kPer == 1) {
+          if (__builtin_expect(cm > pbest, 0)) {
+            // a strictly larger value in this column: the first optimum of
+            // the pass so far is this column's first row holding it (a lane
+            // sees its columns in increasing j, a column's rows in
+            // increasing i); a rare, divergent branch
+            int kk = R - 1;
 #pragma unroll
-          for (int k = R - 1; k >= 0; k--)
-            if (H[k] == cm && (P::kFloor || base + l * R + k < m)) kk = k;
-          pbest = cm;
-          pbj = j;
-          pbi = base + l * R + kk;
+            for (int k = R - 1; k >= 0; k--)
+              if (H[k] == cm && (P::kFloor || base + l * R + k < m)) kk = k;
+            pbest = cm;
+            pbj[0] = j;
+            pbi[0] = base + l * R + kk;
+          }
+        } else if constexpr (END) {
+          // the same per int16 half: a half of the column maximum above the
+          // half's pass optimum (floored kernel: padded rows and columns
+          // hold zero scores, never a strict rise past the real cells)
+          const T up = P::max(cm, pbest);
+          if (__builtin_expect(up != pbest, 0)) {
+#pragma unroll
+            for (int x = 0; x < kPer; x++) {
+              const int cx = P::half(cm, x);
+              int kk = R - 1;
+#pragma unroll
+              for (int k = R - 1; k >= 0; k--)
+                if (P::half(H[k], x) == cx) kk = k;
+              const bool rise = cx > P::half(pbest, x);
+              pbj[x] = rise ? j : pbj[x];
+              pbi[x] = rise ? base + l * R + kk : pbi[x];
+            }
+            pbest = up;
+          }
         }
         out_h = H[R - 1];
         out_f = F;
@@ -488,11 +509,16 @@ __device__ __forceinline__ void run_pass(const GroupCtx<P> &c, const typename P:
     // the pass's first optimum against the earlier passes' (score desc,
     // column asc, row asc): a later pass holds lower rows, so it wins a tie
     // only at a smaller column
-    if ((int)pbest > (int)best || ((int)pbest == (int)best && (int)pbest > 0 && (pbj < bj || (pbj == bj && pbi < bi)))) {
-      best = pbest;
-      bj = pbj;
-      bi = pbi;
+    // (selects, not conditional stores into the arrays: the compiler
+    // dropped such stores of the packed kernel's merge)
+#pragma unroll
+    for (int x = 0; x < kPer; x++) {
+      const int pb = P::half(pbest, x), b = P::half(best, x);
+      const bool take = pb > b || (pb == b && pb > 0 && (pbj[x] < bj[x] || (pbj[x] == bj[x] && pbi[x] < bi[x])));
+      bj[x] = take ? pbj[x] : bj[x];
+      bi[x] = take ? pbi[x] : bi[x];
     }
+    best = P::max(best, pbest);
   }
 }
 
@@ -503,7 +529,7 @@ __device__ __forceinline__ void run_pass_sel(bool last, const GroupCtx<P> &c, co
                                              int base, bool first, bool final_pass, const uint4 *rd, uint4 *wr,
                                              typename P::T e0, typename P::T vopen, typename P::T vext,
                                              typename P::T &best, typename P::K (&top)[P::kPer][4],
-                                             typename P::T *edge, int lane, int &bj, int &bi,
+                                             typename P::T *edge, int lane, int (&bj)[2], int (&bi)[2],
                                              const AgeClock *ac = nullptr) {
   if (c.G > 16) {
     if (last)
@@ -519,6 +545,27 @@ __device__ __forceinline__ void run_pass_sel(bool last, const GroupCtx<P> &c, co
     else
       run_pass<P, R, UNG, false, END, false, AGE>(c, lds, stride, base, first, final_pass, rd, wr, e0, vopen, vext,
                                                   best, top, edge, lane, bj, bi, ac);
+  }
+}
+
+// One butterfly step of the group's end-cell merge: the partner lane's
+// (score, column, row) per pair slot against this lane's, under (score
+// desc, column asc, row asc); the caller then takes the max of the scores.
+template <class P>
+__device__ __forceinline__ void end_merge(typename P::T o, typename P::T best, int (&bj)[2], int (&bi)[2], int off) {
+  int nj[2], ni[2];
+#pragma unroll
+  for (int x = 0; x < P::kPer; x++) {
+    const int oj = __shfl_xor(bj[x], off), oi = __shfl_xor(bi[x], off);
+    const int ov = P::half(o, x), bv = P::half(best, x);
+    const bool take = ov > bv || (ov == bv && (oj < bj[x] || (oj == bj[x] && oi < bi[x])));
+    nj[x] = take ? oj : bj[x];
+    ni[x] = take ? oi : bi[x];
+  }
+#pragma unroll
+  for (int x = 0; x < P::kPer; x++) {
+    bj[x] = nj[x];
+    bi[x] = ni[x];
   }
 }
 
@@ -588,7 +635,7 @@ __device__ __forceinline__ void score_unit(const QueryDev q, const PairDev *upai
 #pragma unroll
     for (int k = 0; k < 4; k++) r.top[x][k] = 0;
   T best = 0;
-  int bj = 0x7fffffff, bi = 0;
+  int bj[2] = {0x7fffffff, 0x7fffffff}, bi[2] = {0, 0};
   for (int p = 0; p < (int)passes; p++) {
     const int base = p * GR;
     // stage the profile rows [base, base + GR) of all codes into LDS
@@ -616,14 +663,7 @@ __device__ __forceinline__ void score_unit(const QueryDev q, const PairDev *upai
   // best and top-4 over the group's lanes
   for (int off = 1; off < c.G; off <<= 1) {
     const T o = (T)__shfl_xor((int)best, off);
-    if (END) {
-      // (score desc, column asc, row asc)
-      const int oj = __shfl_xor(bj, off), oi = __shfl_xor(bi, off);
-      if ((int)o > (int)best || ((int)o == (int)best && (oj < bj || (oj == bj && oi < bi)))) {
-        bj = oj;
-        bi = oi;
-      }
-    }
+    if (END) end_merge<P>(o, best, bj, bi, off);
     best = P::max(best, o);
     if (UNG) {
 #pragma unroll
@@ -638,8 +678,8 @@ __device__ __forceinline__ void score_unit(const QueryDev q, const PairDev *upai
   }
   r.best = best;
   if (END) {
-    r.bj = bj;
-    r.bi = bi;
+    r.bj = bj[0];
+    r.bi = bi[0];
   }
 }
 

@@ -57,7 +57,7 @@ __device__ __forceinline__ void run_last_lower(int RL, const GroupCtx<P> &c, con
                                                int base, bool first, const uint4 *rd, uint4 *wr, typename P::T e0,
                                                typename P::T vopen, typename P::T vext, typename P::T &best,
                                                typename P::K (&top)[P::kPer][4], typename P::T *edge, int lane,
-                                               int &bj, int &bi) {
+                                               int (&bj)[2], int (&bi)[2]) {
   if constexpr (R > 4) {
     if (RL == 4) {
       run_pass_sel<P, 4, UNG, END>(UNG, c, lds, stride, base, first, true, rd, wr, e0, vopen, vext, best, top, edge,
@@ -135,7 +135,7 @@ __device__ __forceinline__ void dp_unit(const Unit &u, const QueryDev *__restric
     for (int k = 0; k < 4; k++) top[x][k] = 0;
 
   T best = 0;
-  int bj = 0x7fffffff, bi = 0;
+  int bj[2] = {0x7fffffff, 0x7fffffff}, bi[2] = {0, 0};
   for (int p = 0; p < (int)u.passes; p++) {
     const int base = p * GR;
     const bool final_pass = p == (int)u.passes - 1;
@@ -176,14 +176,7 @@ __device__ __forceinline__ void dp_unit(const Unit &u, const QueryDev *__restric
   // best and top-4 over the group's lanes
   for (int off = 1; off < c.G; off <<= 1) {
     const T o = (T)__shfl_xor((int)best, off);
-    if (END) {
-      // (score desc, column asc, row asc)
-      const int oj = __shfl_xor(bj, off), oi = __shfl_xor(bi, off);
-      if ((int)o > (int)best || ((int)o == (int)best && (oj < bj || (oj == bj && oi < bi)))) {
-        bj = oj;
-        bi = oi;
-      }
-    }
+    if (END) end_merge<P>(o, best, bj, bi, off);
     best = P::max(best, o);
     if (UNG) {
 #pragma unroll
@@ -245,8 +238,8 @@ __device__ __forceinline__ void dp_unit(const Unit &u, const QueryDev *__restric
           if (k < o.sat_cap) o.sat[1 + k] = outi[x];
         }
         if (END) {
-          o.qend[outi[x]] = (int)best > 0 ? bi : -1;
-          o.tend[outi[x]] = (int)best > 0 ? bj : -1;
+          o.qend[outi[x]] = po.swg > 0 ? bi[x] : -1;
+          o.tend[outi[x]] = po.swg > 0 ? bj[x] : -1;
         }
       }
   }
@@ -297,7 +290,7 @@ __device__ __forceinline__ void dp_unit_win(const Unit &u, const QueryDev *__res
   c.has[1] = false;
   K top[1][4] = {{0, 0, 0, 0}};
   T best = 0;
-  int bj = 0x7fffffff, bi = 0;
+  int bj[2] = {0x7fffffff, 0x7fffffff}, bi[2] = {0, 0};
   L *const mine = lds + c.g * GR;
   for (int p = 0; p < (int)u.passes; p++) {
     const int base = p * GR;
@@ -317,11 +310,7 @@ __device__ __forceinline__ void dp_unit_win(const Unit &u, const QueryDev *__res
   // the group's optimum: (score desc, column asc, row asc)
   for (int off = 1; off < c.G; off <<= 1) {
     const T ob = (T)__shfl_xor((int)best, off);
-    const int oj = __shfl_xor(bj, off), oi = __shfl_xor(bi, off);
-    if ((int)ob > (int)best || ((int)ob == (int)best && (oj < bj || (oj == bj && oi < bi)))) {
-      bj = oj;
-      bi = oi;
-    }
+    end_merge<P>(ob, best, bj, bi, off);
     best = P::max(best, ob);
   }
   if (c.l == 0 && valid) {
@@ -329,8 +318,8 @@ __device__ __forceinline__ void dp_unit_win(const Unit &u, const QueryDev *__res
     po.swg = (int)best;
     po.flag = 2;  // exact int32
     o.pout[outi] = po;
-    o.qend[outi] = (int)best > 0 ? bi : -1;
-    o.tend[outi] = (int)best > 0 ? bj : -1;
+    o.qend[outi] = (int)best > 0 ? bi[0] : -1;
+    o.tend[outi] = (int)best > 0 ? bj[0] : -1;
   }
 }
 
@@ -340,7 +329,7 @@ __device__ __forceinline__ void dp_unit_win(const Unit &u, const QueryDev *__res
 // END (int32 kernels without the evidence): also the end cell of the
 // optimum, first in column-major order, into qend / tend.
 template <class P, int R, bool UNG, bool END>
-__global__ __launch_bounds__(64, (R == 8 && !END) ? 4 : (R == 0 ? 3 : 2)) void dp_kernel(
+__global__ __launch_bounds__(64, (R == 8 && (!END || P::kPer == 2)) ? 4 : (R == 0 ? 3 : 2)) void dp_kernel(
     const Unit *__restrict__ units, uint32_t nunits, uint32_t *__restrict__ counter,
     const PairDev *__restrict__ pairs, const QueryDev *__restrict__ queries,
     const uint32_t *__restrict__ pair_q, const uint8_t *__restrict__ arena, const uint64_t *__restrict__ toff,
@@ -406,7 +395,8 @@ int dp_occupancy(int kind, int R, bool ung, size_t lds_bytes, bool end) {
 #define ST_OCC(PP, RR, UU, EE) \
   (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, dp_kernel<PP, RR, UU, EE>, 64, lds_bytes)
   if (end) {
-    if (kind == 2) ST_OCC(I32N, 8, false, true);
+    if (kind == 0) ST_OCC(Pk16, 0, false, true);
+    else if (kind == 2) ST_OCC(I32N, 8, false, true);
     else if (kind == 6) ST_OCC(I32N8, 8, false, true);
     else if (kind == 5) {
       if (R == 8) ST_OCC(I32F8, 8, false, true); else ST_OCC(I32F8, 16, false, true);
@@ -446,10 +436,10 @@ hipError_t clear_u32(uint32_t *p, uint32_t n, hipStream_t s) {
 
 // kind: 0 = Pk16, 1 = I32F, 2 = I32N, 3 = I32F16, 4 = I32N16, 5 = I32F8,
 // 6 = I32N8 (the last four: end-cell instances only); R: 8 or 16 (the
-// unfloored kinds: 8); end: the score + end-cell instances (int32, no
-// evidence)
+// unfloored kinds: 8; Pk16: 0, each unit's own height); end: the score +
+// end-cell instances (no evidence)
 hipError_t launch_dp(const DpLaunch &a, int kind, int R, bool ung, hipStream_t s, bool end) {
-  if (kind >= 3 && (!end || ung)) return hipErrorInvalidValue;
+  if ((kind >= 3 && (!end || ung)) || (end && kind == 0 && R != 0)) return hipErrorInvalidValue;
   hipLaunchKernelGGL(zero_u32, dim3(1), dim3(64), 0, s, a.counter, 1u);
 #define ST_DP(PP, RR, UU, EE)                                                                       \
   hipLaunchKernelGGL((dp_kernel<PP, RR, UU, EE>), dim3(a.grid), dim3(64),                            \
@@ -457,7 +447,8 @@ hipError_t launch_dp(const DpLaunch &a, int kind, int R, bool ung, hipStream_t s
                      a.counter, a.pairs, a.queries, a.pair_q, a.arena, a.toff, a.tlen, a.open, a.ext, a.passbuf, \
                      a.passbuf_half, a.out, a.lds_stride)
   if (end) {
-    if (kind == 2) ST_DP(I32N, 8, false, true);
+    if (kind == 0) ST_DP(Pk16, 0, false, true);
+    else if (kind == 2) ST_DP(I32N, 8, false, true);
     else if (kind == 6) ST_DP(I32N8, 8, false, true);
     else if (kind == 5) {
       if (R == 8) ST_DP(I32F8, 8, false, true); else ST_DP(I32F8, 16, false, true);
