@@ -240,24 +240,38 @@ def score_batch(qp: QueryProfile, db: TargetDb, jobs) -> np.ndarray:
     return out
 
 
+class SpansCall:
+    """The arguments of st_align_spans_batch packed once (query handles, hit
+    offsets and target ids, gap costs) with an output buffer, for callers
+    that trace back the same hit lists repeatedly: run() is then the C-ABI
+    call alone, as a caller in the reference's language makes it, and
+    returns that buffer (each run overwrites it)."""
+
+    def __init__(self, db: TargetDb, queries: Sequence[QueryProfile], hit_lists, gaps: Sequence[GapCosts]):
+        nq = len(queries)
+        self.db, self.queries, self.nq = db, list(queries), nq
+        self.off = np.zeros(nq + 1, dtype=np.uint64)
+        if nq:
+            self.off[1:] = np.cumsum(np.fromiter((len(h) for h in hit_lists), dtype=np.uint64, count=nq))
+        self.tids = np.ascontiguousarray(np.concatenate(hit_lists) if nq else np.zeros(0), dtype=np.uint32)
+        self.op = np.fromiter((g.open for g in gaps), dtype=np.int32, count=nq)
+        self.ex = np.fromiter((g.extend for g in gaps), dtype=np.int32, count=nq)
+        self.qh = np.fromiter((q.handle.value for q in queries), dtype=np.uint64, count=nq)
+        # every entry is written by each call
+        self.out = np.empty(max(len(self.tids), 1), dtype=_lib.SPAN_DT)
+
+    def run(self, ctx: Context) -> np.ndarray:
+        check(lib().st_align_spans_batch(ctx.handle, self.db.handle, ptr(self.qh), self.nq, ptr(self.off),
+                                         ptr(self.tids), ptr(self.op), ptr(self.ex), ptr(self.out)))
+        return self.out[: len(self.tids)]
+
+
 def align_spans(ctx: Context, db: TargetDb, queries: Sequence[QueryProfile], hit_lists,
                 gaps: Sequence[GapCosts]) -> np.ndarray:
     """Score, end and start of each hit (SPAN_DT, hits in list order): the
     forward and reversed-prefix score passes of align_canonical
     (src/align.rs:847-866) on the device."""
-    nq = len(queries)
-    off = np.zeros(nq + 1, dtype=np.uint64)
-    off[1:] = np.cumsum([len(h) for h in hit_lists]) if nq else []
-    tids = (np.concatenate([np.asarray(h, dtype=np.uint32) for h in hit_lists]) if nq
-            else np.zeros(0, np.uint32))
-    tids = np.ascontiguousarray(tids, dtype=np.uint32)
-    op = np.array([g.open for g in gaps], dtype=np.int32)
-    ex = np.array([g.extend for g in gaps], dtype=np.int32)
-    qh = (C.c_void_p * max(nq, 1))(*[q.handle.value for q in queries])
-    out = np.zeros(max(len(tids), 1), dtype=_lib.SPAN_DT)
-    check(lib().st_align_spans_batch(ctx.handle, db.handle, qh, nq, ptr(off), ptr(tids), ptr(op), ptr(ex),
-                                     ptr(out)))
-    return out[: len(tids)]
+    return SpansCall(db, queries, hit_lists, gaps).run(ctx)
 
 
 def align_spans_launches(ctx: Context):

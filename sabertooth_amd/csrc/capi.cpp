@@ -3,6 +3,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <atomic>
 #include <cctype>
 #include <chrono>
 #include <cmath>
@@ -594,6 +595,10 @@ bool plan_windows(Plan &pl, std::vector<Unit> &pending, uint32_t rep_query, cons
     ord[k] = {(((m + 63) / 64) << 32) | n, (uint32_t)k};
   }
   std::sort(ord.begin(), ord.end());
+  // each element's (m, n) in unit order: the shape search reads them often
+  std::vector<std::pair<uint32_t, uint32_t>> mn(count);
+  for (size_t x = 0; x < count; x++)
+    mn[x] = {qps[views[ord[x].second]]->m, db->hlen[tids[ord[x].second]]};
   const PlanEnv own = env ? PlanEnv{} : plan_env();
   const PlanEnv &E = env ? *env : own;
   for (size_t s = 0; s < count;) {
@@ -606,7 +611,7 @@ bool plan_windows(Plan &pl, std::vector<Unit> &pending, uint32_t rep_query, cons
       uint64_t mmax = 0, np = 0;
       double useful = 0;
       for (size_t x = s; x < e; x++) {
-        const uint64_t m = qps[views[ord[x].second]]->m, n = db->hlen[tids[ord[x].second]];
+        const uint64_t m = mn[x].first, n = mn[x].second;
         mmax = std::max(mmax, m);
         np = std::max(np, n);
         useful += (double)m * (double)n;
@@ -632,8 +637,8 @@ bool plan_windows(Plan &pl, std::vector<Unit> &pending, uint32_t rep_query, cons
     uint64_t mmax = 0, np = 0;
     for (size_t x = s; x < e; x++) {
       const uint32_t k = ord[x].second;
-      mmax = std::max<uint64_t>(mmax, qps[views[k]]->m);
-      np = std::max<uint64_t>(np, db->hlen[tids[k]]);
+      mmax = std::max<uint64_t>(mmax, mn[x].first);
+      np = std::max<uint64_t>(np, mn[x].second);
       PairDev pd;
       pd.tid = tids[k];
       pd.out = (uint32_t)outs[k];
@@ -924,6 +929,13 @@ struct st_batch {
   DevBuf<PairOut> d_pout;       // score-only batches
   DevBuf<uint8_t> d_res;        // evidence batches: st_pair_result rows
   DevBuf<uint32_t> d_sat;       // int16 pairs at the clamp: count, then rows
+  // end-cell read-back staging, kept between refills of the batch object
+  std::vector<PairOut> h_pout;
+  std::vector<int32_t> h_qend, h_tend;
+  // window units' early stop: each pair's known score, per result row
+  bool has_stop = false;
+  std::vector<int32_t> stop;
+  DevBuf<int32_t> d_stop;
   DevBuf<uint4> d_passbuf;
   // exact int32 rescoring of saturated int16 pairs (built at run time)
   Plan resc;
@@ -995,6 +1007,7 @@ static int launch_plan(st_batch *b, const Plan &pl, const Unit *d_units, const P
     a.out.sat_cap = (uint32_t)b->npairs;
     a.out.qend = b->d_qend.p;
     a.out.tend = b->d_tend.p;
+    a.out.stop = &pl == &b->plan && b->has_stop ? b->d_stop.p : nullptr;
     a.lds_stride = kind_lds_stride(L.gr_max, L.kind);
     const bool timed = timing && &pl == &b->plan && b->lev.size() == 2 * pl.launches.size();
     const size_t li = (size_t)(&L - pl.launches.data());
@@ -1023,7 +1036,8 @@ static int batch_create_impl(st_ctx *ctx, const st_db *db, const st_qprof *const
                              const uint64_t *cand_off, const uint32_t *cand_tids, const int32_t *gap_open,
                              const int32_t *gap_extend, const int32_t *min_diag_score, bool ung,
                              st_batch **out, bool end = false, st_batch *reuse = nullptr,
-                             const uint64_t *win_off = nullptr, uint64_t nwin = 0) {
+                             const uint64_t *win_off = nullptr, uint64_t nwin = 0,
+                             const int32_t *stop_scores = nullptr) {
   if (!ctx || !db || !out || (nq && (!queries || !cand_off || !gap_open || !gap_extend)))
     return fail(ST_ERR_INVALID, "st_batch_create: null argument");
   *out = nullptr;
@@ -1052,8 +1066,8 @@ static int batch_create_impl(st_ctx *ctx, const st_db *db, const st_qprof *const
   b->ung = ung;
   b->end = end;
   b->npairs = np;
-  b->plan = Plan();
-  b->resc = Plan();
+  b->plan.clear();
+  b->resc.clear();
   b->rescored = 0;
   b->qps.assign(queries, queries + nq);
   b->open.assign(gap_open, gap_open + nq);
@@ -1064,35 +1078,44 @@ static int batch_create_impl(st_ctx *ctx, const st_db *db, const st_qprof *const
     b->gate.assign(nq, 0);
   b->cand_tids.assign(cand_tids, cand_tids + np);
   b->cand_q.resize(np);
-  uint64_t cells = 0;
-  for (uint64_t q = 0; q < nq; q++)
-    for (uint64_t k = cand_off[q]; k < cand_off[q + 1]; k++) {
-      b->cand_q[k] = (uint32_t)q;
-      cells += (uint64_t)b->qps[q]->m * db->hlen[cand_tids[k]];
-    }
-  b->cells = cells;
+  b->has_stop = stop_scores != nullptr;
+  if (stop_scores) b->stop.assign(stop_scores, stop_scores + np);
   b->qkind.resize(nq);
+  b->qv.resize(nq);
   // ST_END_PK16=0 keeps every end-cell batch on the int32 kernels
   const char *e16 = std::getenv("ST_END_PK16");
   const bool end_pk16 = !(e16 && e16[0] == '0');
+  // one pass over the queries (each handle is read once): cells, the
+  // candidates' query, the precision class and the device view
+  uint64_t cells = 0;
   for (uint64_t q = 0; q < nq; q++) {
-    b->qkind[q] = kind_for(b->qps[q], b->open[q], b->ext[q]);
-    // the packed kernel's diagonal keys hold m + n - 1 <= kMaxDiagsPk16
+    const st_qprof *qp = b->qps[q];
+    b->qv[q] = qp->view;
     uint32_t nmax = 0;
-    for (uint64_t k = cand_off[q]; k < cand_off[q + 1]; k++) nmax = std::max(nmax, db->hlen[cand_tids[k]]);
-    if (b->qkind[q] == kPk16 && (uint64_t)b->qps[q]->m + nmax - 1 > kMaxDiagsPk16) b->qkind[q] = kI32F;
+    uint64_t tl = 0;
+    for (uint64_t k = cand_off[q]; k < cand_off[q + 1]; k++) {
+      const uint32_t n = db->hlen[cand_tids[k]];
+      b->cand_q[k] = (uint32_t)q;
+      tl += n;
+      nmax = std::max(nmax, n);
+    }
+    cells += (uint64_t)qp->m * tl;
+    b->qkind[q] = kind_for(qp, b->open[q], b->ext[q]);
+    // the packed kernel's diagonal keys hold m + n - 1 <= kMaxDiagsPk16
+    if (b->qkind[q] == kPk16 && (uint64_t)qp->m + nmax - 1 > kMaxDiagsPk16) b->qkind[q] = kI32F;
     // the end cell: the packed int16 kernel (two pairs per group, an int32
     // rescoring at the clamp) where the query has its own int16 profile and
     // the gaps allow it; the window units of profile views, and the rest,
     // on the int32 kernels over the int8 or int16 profile where there is one
     if (end) {
-      const bool p8 = b->qps[q]->fits8 && b->qps[q]->view.prof8;
-      const bool p16 = b->qps[q]->fits16 && b->qps[q]->view.prof16;
+      const bool p8 = qp->fits8 && qp->view.prof8;
+      const bool p16 = qp->fits16 && qp->view.prof16;
       if (!(b->qkind[q] == kPk16 && p16 && !win_off && end_pk16))
         b->qkind[q] = kind_unfloored(b->qkind[q]) ? (p8 ? kI32N8 : p16 ? kI32N16 : kI32N)
                                                   : (p8 ? kI32F8 : p16 ? kI32F16 : kI32F);
     }
   }
+  b->cells = cells;
   std::lock_guard<std::mutex> lk(ctx->mu);
   if (hipSetDevice(ctx->device) != hipSuccess) {
     drop();
@@ -1143,13 +1166,11 @@ static int batch_create_impl(st_ctx *ctx, const st_db *db, const st_qprof *const
       const Unit &U = b->plan.units[u];
       for (uint32_t k = 0; k < U.npairs; k++) {
         const uint32_t v = (U.query >> 31) ? b->plan.pair_q[U.pair0 + k] : U.query;
-        L.useful_cells += (uint64_t)b->qps[v]->m * db->hlen[b->plan.pairs[U.pair0 + k].tid];
+        L.useful_cells += (uint64_t)b->qv[v].m * db->hlen[b->plan.pairs[U.pair0 + k].tid];
       }
       L.exec_cells += (uint64_t)64 * (L.kind == kPk16 ? 2 : 1) * ((U.passes - 1) * U.R + U.r_last) * (U.n_pad + U.G - 1);
     }
   }
-  b->qv.resize(nq);
-  for (uint64_t q = 0; q < nq; q++) b->qv[q] = b->qps[q]->view;
   // the tables are copied in stream order from the batch's own vectors
   // (unchanged until the batch is planned again or destroyed, after its runs
   // have synchronised); outputs and the pass buffer are grow-only
@@ -1166,7 +1187,8 @@ static int batch_create_impl(st_ctx *ctx, const st_db *db, const st_qprof *const
             b->d_sat.reserve(np + 1) == hipSuccess &&
             b->d_passbuf.reserve(std::max<uint64_t>(b->plan.buf_elems, 1)) == hipSuccess &&
             (!end || (b->d_qend.reserve(std::max<uint64_t>(np, 1)) == hipSuccess &&
-                      b->d_tend.reserve(std::max<uint64_t>(np, 1)) == hipSuccess));
+                      b->d_tend.reserve(std::max<uint64_t>(np, 1)) == hipSuccess)) &&
+            (!stop_scores || b->d_stop.put(b->stop.data(), np, st) == hipSuccess);
   if (!ok) {
     drop();
     return fail(ST_ERR_NOMEM, "st_batch_create: device allocation failed");
@@ -1232,7 +1254,7 @@ static int batch_run_locked(st_batch *b, bool graph = false) {
   }
   ST_HIP(hipEventRecord(b->ev[1], ctx->stream));
   // int16 pairs whose best reached 32767 are rescored in exact int32
-  b->resc = Plan();
+  b->resc.clear();
   b->rescored = 0;
   ST_HIP(hipEventRecord(b->ev[2], ctx->stream));
   ST_HIP(hipEventRecord(b->ev[3], ctx->stream));
@@ -1407,7 +1429,7 @@ static int score_end_multi(st_ctx *ctx, const st_db *db, const st_qprof *const *
                            const uint64_t *off, const uint32_t *tids, const int32_t *open, const int32_t *ext,
                            st_score_end *out, const char *who, const uint64_t *win_off = nullptr,
                            uint64_t nwin = 0, std::vector<st_launch_info> *launches = nullptr,
-                           double *phase_ms = nullptr);
+                           double *phase_ms = nullptr, const int32_t *stop_scores = nullptr);
 
 // host wall clock in milliseconds
 static double wall_ms() {
@@ -1429,7 +1451,7 @@ int st_sw_score_end_batch(st_ctx *ctx, const st_qprof *qp, const st_db *db, cons
 static int score_end_multi(st_ctx *ctx, const st_db *db, const st_qprof *const *qps, uint64_t nq,
                            const uint64_t *off, const uint32_t *tids, const int32_t *open, const int32_t *ext,
                            st_score_end *out, const char *who, const uint64_t *win_off, uint64_t nwin,
-                           std::vector<st_launch_info> *launches, double *phase_ms) {
+                           std::vector<st_launch_info> *launches, double *phase_ms, const int32_t *stop_scores) {
   const uint64_t count = nq ? off[nq] : 0;
   if (count == 0) return ST_OK;
   // the context's end-cell batch object, refilled: its device buffers and
@@ -1438,7 +1460,7 @@ static int score_end_multi(st_ctx *ctx, const st_db *db, const st_qprof *const *
   const double t0 = wall_ms();
   st_batch *b = nullptr;
   int rc = batch_create_impl(ctx, db, qps, nq, off, tids, open, ext, nullptr, false, &b, true, ctx->end_batch,
-                             win_off, nwin);
+                             win_off, nwin, stop_scores);
   if (rc) return rc;
   ctx->end_batch = b;
   const double t1 = wall_ms();
@@ -1454,8 +1476,11 @@ static int score_end_multi(st_ctx *ctx, const st_db *db, const st_qprof *const *
   }
   if (!rc) {
     std::lock_guard<std::mutex> lk(ctx->mu);
-    std::vector<PairOut> po(count);
-    std::vector<int32_t> qe(count), te(count);
+    std::vector<PairOut> &po = b->h_pout;
+    std::vector<int32_t> &qe = b->h_qend, &te = b->h_tend;
+    po.resize(count);
+    qe.resize(count);
+    te.resize(count);
     hipError_t e = hipMemcpyAsync(po.data(), b->d_pout.p, count * sizeof(PairOut), hipMemcpyDeviceToHost, ctx->stream);
     if (e == hipSuccess) e = hipMemcpyAsync(qe.data(), b->d_qend.p, count * 4, hipMemcpyDeviceToHost, ctx->stream);
     if (e == hipSuccess) e = hipMemcpyAsync(te.data(), b->d_tend.p, count * 4, hipMemcpyDeviceToHost, ctx->stream);
@@ -1483,8 +1508,24 @@ static int score_end_multi(st_ctx *ctx, const st_db *db, const st_qprof *const *
 // calls need (one call at a time, under the context's spans_mu): the
 // row-reversed profiles, the reversed target prefixes as a database, and
 // the tables of the kernels that write them.
+// A reverse problem: query, end row, hit.
+struct SpanRev {
+  uint32_t q, e;
+  uint64_t hit;
+};
+
 struct SpanWork {
   std::vector<st_qprof> views;  // the reversed prefixes' profile views, reused
+  // host tables of a call, kept with their storage for the next call (tens
+  // of thousands of entries each: fresh allocations cost page faults)
+  std::vector<uint64_t> mir_off, doff, voff, win_off, rq_off, vq_off;
+  std::vector<uint32_t> mir_ld, mir_es, chunk_job, chunk_first, rlen, vt;
+  std::vector<MirrorJob> h_mj;
+  std::vector<st_score_end> fw, rs;
+  std::vector<SpanRev> rv;
+  std::vector<RevPrefix> h_pre;
+  std::vector<const st_qprof *> vq;
+  std::vector<int32_t> vopen, vext, stop;
   std::vector<st_launch_info> launches;  // the last call's: forward, then reverse
   uint64_t n_forward = 0;
   double phase_ms[ST_SPAN_PHASES] = {};  // the last call's host phases
@@ -1496,6 +1537,12 @@ struct SpanWork {
 };
 
 static void span_work_free(SpanWork *w) { delete w; }
+
+// ST_SPANS_STOP=0 runs every reverse problem over its whole rectangle
+static bool spans_stop_enabled() {
+  const char *e = std::getenv("ST_SPANS_STOP");
+  return !(e && e[0] == '0');
+}
 
 int st_align_spans_batch(st_ctx *ctx, const st_db *db, const st_qprof *const *queries, uint64_t nq,
                          const uint64_t *hit_off, const uint32_t *hit_tids, const int32_t *gap_open,
@@ -1517,10 +1564,16 @@ int st_align_spans_batch(st_ctx *ctx, const st_db *db, const st_qprof *const *qu
   // bytes), built before the forward pass: they depend on the queries only.
   // A view's passes read at most its rows rounded up to a pass (<= 1024
   // rows) from its window start m - 1 - e: m + 1024 rows cover every view.
-  std::vector<uint64_t> mir_off(nq + 1, 0);  // bytes
-  std::vector<uint32_t> mir_ld(nq, 0), mir_es(nq, 4);
-  std::vector<MirrorJob> mj;
-  std::vector<uint32_t> chunk_job, chunk_first;
+  std::vector<uint64_t> &mir_off = w.mir_off;  // bytes
+  std::vector<uint32_t> &mir_ld = w.mir_ld, &mir_es = w.mir_es;
+  std::vector<MirrorJob> &mj = w.h_mj;
+  std::vector<uint32_t> &chunk_job = w.chunk_job, &chunk_first = w.chunk_first;
+  mir_off.assign(nq + 1, 0);
+  mir_ld.assign(nq, 0);
+  mir_es.assign(nq, 4);
+  mj.clear();
+  chunk_job.clear();
+  chunk_first.clear();
   for (uint64_t q = 0; q < nq; q++) {
     if (hit_off[q + 1] > hit_off[q] && queries[q]) {
       mir_ld[q] = (queries[q]->m + kMaxPassRows + 63) & ~63u;
@@ -1558,7 +1611,8 @@ int st_align_spans_batch(st_ctx *ctx, const st_db *db, const st_qprof *const *qu
     ~StreamWait() { (void)hipStreamSynchronize(s); }
   } stream_wait{ctx->stream};
   // 1. forward pass: score and the column-major-first end cell
-  std::vector<st_score_end> fw(nh);
+  std::vector<st_score_end> &fw = w.fw;
+  fw.resize(nh);
   int rc = score_end_multi(ctx, db, queries, nq, hit_off, hit_tids, gap_open, gap_extend, fw.data(),
                            "st_align_spans_batch", nullptr, 0, &w.launches, w.phase_ms);
   if (rc) return rc;
@@ -1574,57 +1628,136 @@ int st_align_spans_batch(st_ctx *ctx, const st_db *db, const st_qprof *const *qu
   // 2. reverse pass over the hits with a positive score, one problem per
   // (query, end row): the reversed prefix rows[..=e] is the window of the
   // query's row-reversed profile that starts at row m - 1 - e
-  struct Rev {
-    uint32_t q, e;
-    uint64_t hit;
-  };
   // (query, end row) order: the hits come query by query, so each query's
-  // run is sorted by end row on its own (stable: hit order within a row)
-  std::vector<Rev> rv;
-  rv.reserve(nh);
+  // run is sorted by end row on its own (stable: hit order within a row).
+  // The tables are built on the planning threads, a run of queries each:
+  // first the runs and their view counts, then the views, the reversed
+  // target prefixes and the stop scores in place.
+  std::vector<uint64_t> &rq_off = w.rq_off, &vq_off = w.vq_off;
+  rq_off.assign(nq + 1, 0);
   for (uint64_t q = 0; q < nq; q++) {
-    const size_t r0 = rv.size();
-    for (uint64_t k = hit_off[q]; k < hit_off[q + 1]; k++)
-      if (fw[k].score > 0) {
-        // the gathers below read rows [0, query_end] and residues [0, target_end]
-        if (fw[k].query_end < 0 || (uint64_t)fw[k].query_end >= queries[q]->m || fw[k].target_end < 0 ||
-            (uint64_t)fw[k].target_end >= db->hlen[hit_tids[k]])
-          return fail(ST_ERR_DEVICE, "st_align_spans_batch: end cell outside its problem");
-        rv.push_back({(uint32_t)q, (uint32_t)fw[k].query_end, k});
-      }
-    if (rv.size() - r0 <= 64) {
-      // a short run: stable insertion sort, no allocation
-      for (size_t i = r0 + 1; i < rv.size(); i++) {
-        const Rev x = rv[i];
-        size_t j = i;
-        for (; j > r0 && rv[j - 1].e > x.e; j--) rv[j] = rv[j - 1];
-        rv[j] = x;
-      }
-    } else {
-      std::stable_sort(rv.begin() + r0, rv.end(), [](const Rev &a, const Rev &b) { return a.e < b.e; });
-    }
+    uint64_t cnt = 0;
+    for (uint64_t k = hit_off[q]; k < hit_off[q + 1]; k++) cnt += fw[k].score > 0;
+    rq_off[q + 1] = rq_off[q] + cnt;
   }
-  if (rv.empty()) return ST_OK;
-  const uint64_t nr = rv.size();
+  const uint64_t nr = rq_off[nq];
+  if (nr == 0) return ST_OK;
   if (nr > 0xffffffffull) return fail(ST_ERR_INVALID, "st_align_spans_batch: more than 2^32 hits");
+  std::vector<SpanRev> &rv = w.rv;
+  rv.resize(nr);
+  vq_off.assign(nq + 1, 0);
+  const unsigned T = (unsigned)std::min<uint64_t>({(uint64_t)WorkPool::get().width(), nq, nr / 2048 + 1});
+  std::atomic<int> bad_end{0};
+  WorkPool::get().run(T, [&](unsigned t) {
+    for (uint64_t q = nq * t / T; q < nq * (t + 1) / T; q++) {
+      uint64_t i = rq_off[q];
+      const uint64_t r0 = i;
+      for (uint64_t k = hit_off[q]; k < hit_off[q + 1]; k++)
+        if (fw[k].score > 0) {
+          // the gathers below read rows [0, query_end] and residues [0, target_end]
+          if (fw[k].query_end < 0 || (uint64_t)fw[k].query_end >= queries[q]->m || fw[k].target_end < 0 ||
+              (uint64_t)fw[k].target_end >= db->hlen[hit_tids[k]])
+            bad_end.store(1, std::memory_order_relaxed);
+          rv[i++] = SpanRev{(uint32_t)q, (uint32_t)std::max<int64_t>(fw[k].query_end, 0), k};
+        }
+      if (i - r0 <= 64) {
+        // a short run: stable insertion sort, no allocation
+        for (uint64_t a = r0 + 1; a < i; a++) {
+          const SpanRev x = rv[a];
+          uint64_t b = a;
+          for (; b > r0 && rv[b - 1].e > x.e; b--) rv[b] = rv[b - 1];
+          rv[b] = x;
+        }
+      } else {
+        std::stable_sort(rv.begin() + r0, rv.begin() + i, [](const SpanRev &x, const SpanRev &y) { return x.e < y.e; });
+      }
+      uint64_t nv = 0;
+      for (uint64_t a = r0; a < i; a++) nv += a == r0 || rv[a].e != rv[a - 1].e;
+      vq_off[q + 1] = nv;
+    }
+  });
+  if (bad_end.load()) return fail(ST_ERR_DEVICE, "st_align_spans_batch: end cell outside its problem");
+  // one profile view per (query, end row): stride = the mirror's rows, rows
+  // readable from the window's start = its extent (the planner keeps every
+  // pass inside it); each query's views are consecutive and planned
+  // together as one source, several views to a unit
+  std::vector<uint64_t> &win_off = w.win_off;
+  win_off.assign(1, 0);
+  for (uint64_t q = 0; q < nq; q++) {
+    vq_off[q + 1] += vq_off[q];
+    if (vq_off[q + 1] > vq_off[q]) win_off.push_back(vq_off[q + 1]);
+  }
+  const uint64_t nviews = vq_off[nq];
+  // st_qprof objects hold no allocations of their own here (views only)
+  if (w.views.size() < nviews) w.views = std::vector<st_qprof>(nviews + nviews / 4);
+  st_qprof *views = w.views.data();
+  std::vector<const st_qprof *> &vq = w.vq;
+  std::vector<uint64_t> &voff = w.voff;
+  std::vector<int32_t> &vopen = w.vopen, &vext = w.vext, &stop = w.stop;
+  std::vector<uint32_t> &vt = w.vt, &rlen = w.rlen;
+  std::vector<RevPrefix> &pre = w.h_pre;
+  vq.resize(nviews);
+  voff.resize(nviews + 1);
+  vopen.resize(nviews);
+  vext.resize(nviews);
+  vt.resize(nr);
+  rlen.resize(nr);
+  pre.resize(nr);
+  stop.resize(nr);
+  const uint8_t *mirror = w.mirror.p;
+  WorkPool::get().run(T, [&](unsigned t) {
+    for (uint64_t q = nq * t / T; q < nq * (t + 1) / T; q++) {
+      uint64_t v = vq_off[q];
+      const uint32_t o0 = queries[q]->m - 1, ld = mir_ld[q], es = mir_es[q];
+      const uint8_t *mb = mirror + mir_off[q];
+      for (uint64_t k = rq_off[q]; k < rq_off[q + 1]; k++) {
+        const uint32_t e = rv[k].e;
+        if (k == rq_off[q] || e != rv[k - 1].e) {
+          const uint32_t o = o0 - e;
+          st_qprof *vp = &views[v];
+          vp->ctx = ctx;
+          vp->m = e + 1;
+          vp->ld = ld - o;
+          vp->fits8 = es == 1;
+          vp->fits16 = es <= 2;
+          vp->view.prof8 = es == 1 ? (const int8_t *)mb + o : nullptr;
+          vp->view.prof16 = es == 2 ? (const int16_t *)mb + o : nullptr;
+          vp->view.prof32 = es == 4 ? (const int32_t *)mb + o : nullptr;
+          vp->view.m = e + 1;
+          vp->view.ld = ld;
+          vq[v] = vp;
+          vopen[v] = gap_open[q];
+          vext[v] = gap_extend[q];
+          voff[v] = k;
+          v++;
+        }
+        const uint64_t h = rv[k].hit;
+        vt[k] = (uint32_t)k;
+        rlen[k] = (uint32_t)(fw[h].target_end + 1);
+        pre[k] = RevPrefix{hit_tids[h], rlen[k]};
+        // each reversed prefix is known to reach its hit's score: the
+        // reverse pass stops once the first column holding it is complete
+        stop[k] = fw[h].score;
+      }
+    }
+  });
+  voff[nviews] = nr;
   // the reversed target prefixes (database layout: 16-byte aligned starts,
   // code 21 padding)
-  std::vector<RevPrefix> pre(nr);
-  std::vector<uint64_t> doff(nr + 1, 0);
-  std::vector<uint32_t> rlen(nr);
+  std::vector<uint64_t> &doff = w.doff;
+  doff.resize(nr + 1);
+  doff[0] = 0;
+  uint32_t max_len = 0;
   for (uint64_t k = 0; k < nr; k++) {
-    const uint64_t h = rv[k].hit;
-    rlen[k] = (uint32_t)(fw[h].target_end + 1);
-    pre[k] = RevPrefix{hit_tids[h], rlen[k]};
     doff[k + 1] = doff[k] + ((rlen[k] + 15ull) & ~15ull);
+    max_len = std::max(max_len, rlen[k]);
   }
   st_db *rdb = &ctx->spans->rdb;
   rdb->ctx = ctx;
   rdb->n_targets = nr;
   rdb->n_res = doff[nr];
   rdb->hlen = rlen;
-  rdb->max_len = 0;
-  for (uint32_t l : rlen) rdb->max_len = std::max(rdb->max_len, l);
+  rdb->max_len = max_len;
   {
     std::lock_guard<std::mutex> lk(ctx->mu);
     ST_HIP(hipSetDevice(ctx->device));
@@ -1636,67 +1769,12 @@ int st_align_spans_batch(st_ctx *ctx, const st_db *db, const st_qprof *const *qu
       return fail(ST_ERR_NOMEM, "st_align_spans_batch: device allocation failed");
     ST_HIP(launch_rev_prefix(db->arena.p, db->off.p, w.pre.p, (uint32_t)nr, rdb->off.p, rdb->arena.p, st));
   }
-  DevBuf<uint8_t> &mirror = w.mirror;
-  // one profile view per (query, end row): stride = the mirror's rows, rows
-  // readable from the window's start = its extent (the planner keeps every
-  // pass inside it)
-  uint64_t nviews = 0;
-  for (uint64_t k = 0; k < nr; k++) nviews += k == 0 || rv[k].q != rv[k - 1].q || rv[k].e != rv[k - 1].e;
-  // st_qprof objects hold no allocations of their own here (views only)
-  if (w.views.size() < nviews) w.views = std::vector<st_qprof>(nviews + nviews / 4);
-  st_qprof *views = w.views.data();
-  std::vector<const st_qprof *> vq;
-  vq.reserve(nviews);
-  std::vector<uint64_t> voff{0};
-  voff.reserve(nviews + 1);
-  std::vector<int32_t> vopen, vext;
-  vopen.reserve(nviews);
-  vext.reserve(nviews);
-  std::vector<uint32_t> vt(nr);
-  for (uint64_t k = 0; k < nr; k++) {
-    if (k == 0 || rv[k].q != rv[k - 1].q || rv[k].e != rv[k - 1].e) {
-      const st_qprof *src = queries[rv[k].q];
-      const uint32_t o = src->m - 1 - rv[k].e;
-      st_qprof *v = &views[vq.size()];
-      v->ctx = ctx;
-      v->m = rv[k].e + 1;
-      v->ld = mir_ld[rv[k].q] - o;
-      uint8_t *mb = mirror.p + mir_off[rv[k].q];
-      const uint32_t es = mir_es[rv[k].q];
-      v->fits8 = es == 1;
-      v->fits16 = es <= 2;
-      v->view.prof8 = es == 1 ? (int8_t *)mb + o : nullptr;
-      v->view.prof16 = es == 2 ? (int16_t *)mb + o : nullptr;
-      v->view.prof32 = es == 4 ? (int32_t *)mb + o : nullptr;
-      v->view.m = v->m;
-      v->view.ld = mir_ld[rv[k].q];
-      vq.push_back(v);
-      vopen.push_back(gap_open[rv[k].q]);
-      vext.push_back(gap_extend[rv[k].q]);
-      voff.push_back(voff.back());
-    }
-    vt[k] = (uint32_t)k;
-    voff.back()++;
-  }
-  // the views of one query are consecutive (rv is sorted by query): each
-  // query's views are planned together, several views to a unit
-  std::vector<uint64_t> win_off{0};
-  {
-    uint64_t v = 0;
-    for (uint64_t k = 0; k < nr; k++) {
-      const bool nv = k == 0 || rv[k].q != rv[k - 1].q || rv[k].e != rv[k - 1].e;
-      if (nv) {
-        if (k > 0 && rv[k].q != rv[k - 1].q) win_off.push_back(v);
-        v++;
-      }
-    }
-    win_off.push_back(v);
-  }
-  std::vector<st_score_end> rs(nr);
+  std::vector<st_score_end> &rs = w.rs;
+  rs.resize(nr);
   w.phase_ms[3] = wall_ms() - t_rev;
   rc = score_end_multi(ctx, rdb, vq.data(), vq.size(), voff.data(), vt.data(), vopen.data(), vext.data(),
                        rs.data(), "st_align_spans_batch", win_off.data(), win_off.size() - 1, &w.launches,
-                       w.phase_ms + 4);
+                       w.phase_ms + 4, spans_stop_enabled() ? stop.data() : nullptr);
   if (rc) return rc;
   for (uint64_t k = 0; k < nr; k++) {
     st_span &s = out[rv[k].hit];

@@ -255,17 +255,36 @@ struct GroupCtx {
   bool has[2];
 };
 
+// The first column where a group's pair reaches its known score `target`:
+// each lane's first column holding it (its running pass maximum rises
+// strictly, so pbj is where pbest was first reached), the minimum over the
+// group's G lanes, and the limit carried over from earlier passes.
+template <class P>
+__device__ __forceinline__ int group_stop_column(typename P::T pbest, int pbj, int target, int carried, int G) {
+  int col = (int)pbest >= target ? pbj : 0x7fffffff;
+  for (int off = 1; off < G; off <<= 1) col = min(col, __shfl_xor(col, off));
+  return min(col, carried);
+}
+
 // One pass of G*R rows over all n_pad columns.  LAST: the pass holding the
 // matrix's last padded row (bottom-row diagonal ends are folded here).
 // WIDE: groups of 32 or 64 lanes (boundaries cross 16-lane DPP rows).
 // AGE: the serving kernel's wave priority by request age (ac).
-template <class P, int R, bool UNG, bool LAST, bool END, bool WIDE, bool AGE = false>
+// STOP (end-cell kernels, one pair per group): the group's pair is known to
+// reach *stop_at; the pass ends once every group has completed the column
+// where its pair first reached it, or the column *climit carried over from
+// the earlier passes (the first such column so far), whichever comes first:
+// a later column cannot hold an earlier optimum in column-major order, and
+// the next pass needs the boundary row only up to that column.  *climit is
+// updated for the next pass.
+template <class P, int R, bool UNG, bool LAST, bool END, bool WIDE, bool AGE = false, bool STOP = false>
 __device__ __forceinline__ void run_pass(const GroupCtx<P> &c, const typename P::L *lds, int stride, int base,
                                          bool first, bool final_pass, const uint4 *rd, uint4 *wr,
                                          typename P::T e0, typename P::T vopen, typename P::T vext,
                                          typename P::T &best, typename P::K (&top)[P::kPer][4],
                                          typename P::T *edge, int lane, int (&bj)[2], int (&bi)[2],
-                                         const AgeClock *ac = nullptr) {
+                                         const AgeClock *ac = nullptr, const int *stop_at = nullptr,
+                                         int *climit = nullptr) {
   using T = typename P::T;
   using K = typename P::K;
   constexpr int kPer = P::kPer;
@@ -310,11 +329,22 @@ __device__ __forceinline__ void run_pass(const GroupCtx<P> &c, const typename P:
   // bottom row (LAST): the value the group's last lane produced at step t is
   // kept by lane t mod G and folded every G steps
   T pending = kInvalid;
+  // STOP: the first column holding the known score in the earlier passes
+  const int carried = STOP ? *climit : 0;
 
   const int steps = n_pad + G - 1;
   for (int t0 = 0; t0 < steps; t0 += kStepUnroll) {
     if constexpr (AGE) {
       if ((t0 & 63) == 0) set_age_priority(*ac);
+    }
+    if constexpr (STOP) {
+      // every 16 steps: this group's first column holding the known score
+      // (this pass or an earlier one); the wave leaves once every group
+      // has run the steps that complete it (column + G - 1)
+      if ((t0 & 15) == 0 && t0 > 0) {
+        const int lim = group_stop_column<P>(pbest, pbj[0], *stop_at, *climit, G);
+        if (__all(t0 >= lim + G)) break;
+      }
     }
 #pragma unroll
     for (int s = 0; s < kStepUnroll; s++) {
@@ -407,7 +437,10 @@ __device__ __forceinline__ void run_pass(const GroupCtx<P> &c, const typename P:
           F = P::max(P::subf(F, vext), ho);
         }
         if constexpr (END && kPer == 1) {
-          if (__builtin_expect(cm > pbest, 0)) {
+          // STOP: columns past the limit carried from the earlier passes
+          // read boundary rows those passes did not write, and cannot hold
+          // the optimum: they are not tracked
+          if (__builtin_expect(cm > pbest, 0) && (!STOP || j <= carried)) {
             // a strictly larger value in this column: the first optimum of
             // the pass so far is this column's first row holding it (a lane
             // sees its columns in increasing j, a column's rows in
@@ -505,6 +538,7 @@ __device__ __forceinline__ void run_pass(const GroupCtx<P> &c, const typename P:
           insert4<K>(top[x], P::key(P::half(v, x), (uint32_t)(d + m - 1)));
     }
   }
+  if constexpr (STOP) *climit = group_stop_column<P>(pbest, pbj[0], *stop_at, *climit, G);
   if (END) {
     // the pass's first optimum against the earlier passes' (score desc,
     // column asc, row asc): a later pass holds lower rows, so it wins a tie
@@ -524,27 +558,29 @@ __device__ __forceinline__ void run_pass(const GroupCtx<P> &c, const typename P:
 
 // One pass with the instance for the group width and the pass position
 // chosen once (uniform per unit), not per step.
-template <class P, int R, bool UNG, bool END, bool AGE = false>
+template <class P, int R, bool UNG, bool END, bool AGE = false, bool STOP = false>
 __device__ __forceinline__ void run_pass_sel(bool last, const GroupCtx<P> &c, const typename P::L *lds, int stride,
                                              int base, bool first, bool final_pass, const uint4 *rd, uint4 *wr,
                                              typename P::T e0, typename P::T vopen, typename P::T vext,
                                              typename P::T &best, typename P::K (&top)[P::kPer][4],
                                              typename P::T *edge, int lane, int (&bj)[2], int (&bi)[2],
-                                             const AgeClock *ac = nullptr) {
+                                             const AgeClock *ac = nullptr, const int *stop_at = nullptr,
+                                             int *climit = nullptr) {
   if (c.G > 16) {
     if (last)
-      run_pass<P, R, UNG, true, END, true, AGE>(c, lds, stride, base, first, true, rd, wr, e0, vopen, vext, best,
-                                                top, edge, lane, bj, bi, ac);
+      run_pass<P, R, UNG, true, END, true, AGE, STOP>(c, lds, stride, base, first, true, rd, wr, e0, vopen, vext,
+                                                      best, top, edge, lane, bj, bi, ac, stop_at, climit);
     else
-      run_pass<P, R, UNG, false, END, true, AGE>(c, lds, stride, base, first, final_pass, rd, wr, e0, vopen, vext,
-                                                 best, top, edge, lane, bj, bi, ac);
+      run_pass<P, R, UNG, false, END, true, AGE, STOP>(c, lds, stride, base, first, final_pass, rd, wr, e0, vopen,
+                                                       vext, best, top, edge, lane, bj, bi, ac, stop_at, climit);
   } else {
     if (last)
-      run_pass<P, R, UNG, true, END, false, AGE>(c, lds, stride, base, first, true, rd, wr, e0, vopen, vext, best,
-                                                 top, edge, lane, bj, bi, ac);
+      run_pass<P, R, UNG, true, END, false, AGE, STOP>(c, lds, stride, base, first, true, rd, wr, e0, vopen, vext,
+                                                       best, top, edge, lane, bj, bi, ac, stop_at, climit);
     else
-      run_pass<P, R, UNG, false, END, false, AGE>(c, lds, stride, base, first, final_pass, rd, wr, e0, vopen, vext,
-                                                  best, top, edge, lane, bj, bi, ac);
+      run_pass<P, R, UNG, false, END, false, AGE, STOP>(c, lds, stride, base, first, final_pass, rd, wr, e0,
+                                                        vopen, vext, best, top, edge, lane, bj, bi, ac, stop_at,
+                                                        climit);
   }
 }
 

@@ -292,6 +292,12 @@ __device__ __forceinline__ void dp_unit_win(const Unit &u, const QueryDev *__res
   T best = 0;
   int bj[2] = {0x7fffffff, 0x7fffffff}, bi[2] = {0, 0};
   L *const mine = lds + c.g * GR;
+  // early stop (floored kernels, when the launch gives each pair's known
+  // score): a group without a pair is done at once; a pair's limit starts
+  // unbounded
+  const bool stop = P::kFloor && o.stop != nullptr;
+  const int stop_at = valid && stop ? o.stop[outi] : (1 << 30);
+  int climit = valid ? (1 << 30) : -c.G;
   for (int p = 0; p < (int)u.passes; p++) {
     const int base = p * GR;
     // this group's rows [base, base + GR) of its view, zeros past its m
@@ -304,8 +310,13 @@ __device__ __forceinline__ void dp_unit_win(const Unit &u, const QueryDev *__res
     const uint4 *rd = myb + (uint64_t)((p + 1) & 1) * passbuf_half + (uint64_t)c.g * c.n_pad;
     uint4 *wr = myb + (uint64_t)(p & 1) * passbuf_half + (uint64_t)c.g * c.n_pad;
     const bool final_pass = p == (int)u.passes - 1;
-    run_pass_sel<P, R, false, true>(false, c, mine, lds_stride, base, p == 0, final_pass, rd, wr, e0, vopen, vext,
-                                    best, top, edge, lane, bj, bi);
+    if (stop)
+      run_pass_sel<P, R, false, true, false, P::kFloor>(false, c, mine, lds_stride, base, p == 0, final_pass, rd, wr,
+                                                        e0, vopen, vext, best, top, edge, lane, bj, bi, nullptr,
+                                                        &stop_at, &climit);
+    else
+      run_pass_sel<P, R, false, true>(false, c, mine, lds_stride, base, p == 0, final_pass, rd, wr, e0, vopen, vext,
+                                      best, top, edge, lane, bj, bi);
   }
   // the group's optimum: (score desc, column asc, row asc)
   for (int off = 1; off < c.G; off <<= 1) {

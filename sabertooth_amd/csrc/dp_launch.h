@@ -17,6 +17,10 @@ struct DpOut {
   uint32_t *sat;         // packed int16 pairs at the clamp: sat[0] = count, sat[1 + k] = their result rows
   uint32_t sat_cap;      // entries after sat[0]
   int32_t *qend, *tend;  // score + end launches: end cell per result row
+  // window units (end-cell launches of floored kernels): the score each
+  // pair is known to reach, per result row; its pass stops once the first
+  // column holding it is complete (null: no early stop)
+  const int32_t *stop;
 };
 
 struct DpLaunch {

@@ -165,16 +165,19 @@ struct st_postings {
 };
 
 struct st_qprof {
+  // what planning reads, first and together (profile views are built by
+  // the ten thousand per call): the context, sizes, precision flags and the
+  // device view
   st_ctx *ctx = nullptr;
-  std::vector<int32_t> rows;
   uint32_t m = 0, ld = 0;
   bool fits16 = false;
   bool fits8 = false;  // every value in [-128, 127]: prof8 holds them
+  st::QueryDev view{};
+  std::vector<int32_t> rows;
   st::DevBuf<int8_t> prof8;
   st::DevBuf<int16_t> prof16;
   st::DevBuf<int32_t> prof32;
   st::DevBuf<st::QueryDev> qd;  // a one-entry table for single-query calls
-  st::QueryDev view{};
 };
 
 namespace st {
@@ -243,6 +246,14 @@ struct Plan {
   std::vector<Launch> launches;
   uint64_t buf_elems = 0;
   uint64_t exec_cells = 0;  // cells the launches execute, padding included
+  // empty, keeping the vectors' storage for the next plan
+  void clear() {
+    units.clear();
+    pairs.clear();
+    pair_q.clear();
+    launches.clear();
+    buf_elems = exec_cells = 0;
+  }
 };
 
 // Add the pairs of query qi (result rows outs[], target ids tids[]) to the
@@ -271,7 +282,7 @@ bool plan_query(Plan &pl, std::vector<Unit> &pending, uint32_t qi, const st_qpro
                 const PlanLimits *lim = nullptr, const PlanEnv *env = nullptr);
 
 // Unit step cap for a batch of `batch_cells` DP cells: short units when the
-// batch is small (latency), up to 12000 steps for large batches.
+// batch is small (latency), growing with the batch without an upper bound.
 double unit_steps_cap(uint64_t batch_cells);
 
 // Group pending units into launches (kind, R, gaps, LDS class), longest

@@ -181,3 +181,32 @@ def test_no_hits_and_zero_scores(ctx):
     assert len(st.align_spans(ctx, db, [qp], [np.zeros(0, np.uint32)], [GapCosts(11, 1)])) == 0
     got = st.align_spans(ctx, db, [qp], [np.array([0, 0], np.uint32)], [GapCosts(11, 1)])
     assert [tuple(int(r[f]) for f in FIELDS) for r in got] == [(0, -1, -1, -1, -1)] * 2
+
+
+@pytest.mark.parametrize("G", ["1", "8", "64"])
+def test_reverse_early_stop_equals_full_rectangles(ctx, monkeypatch, G):
+    # short local matches inside long targets: the reverse pass stops at the
+    # first column holding the forward score, and with ST_SPANS_STOP=0 it
+    # runs every reversed prefix to its end; both equal the oracle (several
+    # passes per problem at the narrow group widths)
+    monkeypatch.setenv("ST_FORCE_G", G)
+    rng = np.random.default_rng(77)
+    queries = [rng.integers(0, 20, size=L, dtype=np.uint8) for L in (45, 300, 610)]
+    targets = []
+    for q in queries:
+        for _ in range(5):
+            a = int(rng.integers(0, len(q) // 2))
+            seg = _mutate(rng, q[a:a + int(rng.integers(20, 60))], 0.8, 0.02)
+            targets.append(np.concatenate([rng.integers(0, 20, size=int(rng.integers(100, 700)), dtype=np.uint8),
+                                           seg, rng.integers(0, 20, size=int(rng.integers(0, 300)),
+                                                             dtype=np.uint8)]))
+    targets += [rng.integers(0, 20, size=int(rng.integers(1, 800)), dtype=np.uint8) for _ in range(20)]
+    hits = [np.arange(len(targets), dtype=np.uint32)] * len(queries)
+    rows = [O.seq_rows(q) for q in queries]
+    gaps = [GapCosts(11, 1), GapCosts(6, 2), GapCosts(11, 1)]
+    stopped, db = _check(ctx, rows, targets, hits, gaps)
+    monkeypatch.setenv("ST_SPANS_STOP", "0")
+    qps = [st.QueryProfile.from_rows(ctx, r) for r in rows]
+    full = st.align_spans(ctx, db, qps, hits, gaps)
+    for f in FIELDS:
+        assert (stopped[f] == full[f]).all(), f
