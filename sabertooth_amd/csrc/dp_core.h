@@ -28,15 +28,18 @@ __device__ __forceinline__ uint4 ldg_u4(const uint4 *p) {
 }
 
 constexpr uint32_t kSent4 = 0x15151515u;  // four zero-score codes
-#ifndef ST_STEP_UNROLL
-#define ST_STEP_UNROLL 4
-#endif
 // 1: a step's profile words are gathered one step ahead
 #ifndef ST_SCORE_PREFETCH
 #define ST_SCORE_PREFETCH 0
 #endif
-// DP steps per loop iteration
-constexpr int kStepUnroll = ST_STEP_UNROLL;
+// DP steps per loop iteration: one in the batch evidence kernels, four in
+// the end-cell kernels and the serving kernel (measured on MI355X: the
+// batched, PSSM and long-pair workloads run 1.0-1.7% faster with one step
+// per iteration, the alignment spans 3% and saturated serving 4-8% slower)
+template <bool UNG, bool END>
+constexpr int step_unroll() {
+  return UNG && !END ? 1 : 4;
+}
 
 // ---- arithmetic policies -----------------------------------------------------
 
@@ -277,7 +280,7 @@ __device__ __forceinline__ int group_stop_column(typename P::T pbest, int pbj, i
 // a later column cannot hold an earlier optimum in column-major order, and
 // the next pass needs the boundary row only up to that column.  *climit is
 // updated for the next pass.
-template <class P, int R, bool UNG, bool LAST, bool END, bool WIDE, bool AGE = false, bool STOP = false>
+template <class P, int R, bool UNG, bool LAST, bool END, bool WIDE, bool AGE = false, bool STOP = false, int UNR = 0>
 __device__ __forceinline__ void run_pass(const GroupCtx<P> &c, const typename P::L *lds, int stride, int base,
                                          bool first, bool final_pass, const uint4 *rd, uint4 *wr,
                                          typename P::T e0, typename P::T vopen, typename P::T vext,
@@ -333,6 +336,7 @@ __device__ __forceinline__ void run_pass(const GroupCtx<P> &c, const typename P:
   const int carried = STOP ? *climit : 0;
 
   const int steps = n_pad + G - 1;
+  constexpr int kStepUnroll = UNR ? UNR : step_unroll<UNG, END>();
   for (int t0 = 0; t0 < steps; t0 += kStepUnroll) {
     if constexpr (AGE) {
       if ((t0 & 63) == 0) set_age_priority(*ac);
@@ -558,7 +562,7 @@ __device__ __forceinline__ void run_pass(const GroupCtx<P> &c, const typename P:
 
 // One pass with the instance for the group width and the pass position
 // chosen once (uniform per unit), not per step.
-template <class P, int R, bool UNG, bool END, bool AGE = false, bool STOP = false>
+template <class P, int R, bool UNG, bool END, bool AGE = false, bool STOP = false, int UNR = 0>
 __device__ __forceinline__ void run_pass_sel(bool last, const GroupCtx<P> &c, const typename P::L *lds, int stride,
                                              int base, bool first, bool final_pass, const uint4 *rd, uint4 *wr,
                                              typename P::T e0, typename P::T vopen, typename P::T vext,
@@ -568,17 +572,17 @@ __device__ __forceinline__ void run_pass_sel(bool last, const GroupCtx<P> &c, co
                                              int *climit = nullptr) {
   if (c.G > 16) {
     if (last)
-      run_pass<P, R, UNG, true, END, true, AGE, STOP>(c, lds, stride, base, first, true, rd, wr, e0, vopen, vext,
+      run_pass<P, R, UNG, true, END, true, AGE, STOP, UNR>(c, lds, stride, base, first, true, rd, wr, e0, vopen, vext,
                                                       best, top, edge, lane, bj, bi, ac, stop_at, climit);
     else
-      run_pass<P, R, UNG, false, END, true, AGE, STOP>(c, lds, stride, base, first, final_pass, rd, wr, e0, vopen,
+      run_pass<P, R, UNG, false, END, true, AGE, STOP, UNR>(c, lds, stride, base, first, final_pass, rd, wr, e0, vopen,
                                                        vext, best, top, edge, lane, bj, bi, ac, stop_at, climit);
   } else {
     if (last)
-      run_pass<P, R, UNG, true, END, false, AGE, STOP>(c, lds, stride, base, first, true, rd, wr, e0, vopen, vext,
+      run_pass<P, R, UNG, true, END, false, AGE, STOP, UNR>(c, lds, stride, base, first, true, rd, wr, e0, vopen, vext,
                                                        best, top, edge, lane, bj, bi, ac, stop_at, climit);
     else
-      run_pass<P, R, UNG, false, END, false, AGE, STOP>(c, lds, stride, base, first, final_pass, rd, wr, e0,
+      run_pass<P, R, UNG, false, END, false, AGE, STOP, UNR>(c, lds, stride, base, first, final_pass, rd, wr, e0,
                                                         vopen, vext, best, top, edge, lane, bj, bi, ac, stop_at,
                                                         climit);
   }
@@ -693,8 +697,9 @@ __device__ __forceinline__ void score_unit(const QueryDev q, const PairDev *upai
     const uint4 *rd = myb + (uint64_t)((p + 1) & 1) * passbuf_half + (uint64_t)c.g * c.n_pad;
     uint4 *wr = myb + (uint64_t)(p & 1) * passbuf_half + (uint64_t)c.g * c.n_pad;
     const bool final_pass = p == (int)passes - 1;
-    run_pass_sel<P, R, UNG, END, AGE>(UNG && final_pass, c, lds, lds_stride, base, p == 0, final_pass, rd, wr, e0,
-                                      vopen, vext, best, r.top, edge, lane, bj, bi, ac);
+    // the serving kernel keeps four steps per loop iteration
+    run_pass_sel<P, R, UNG, END, AGE, false, 4>(UNG && final_pass, c, lds, lds_stride, base, p == 0, final_pass, rd,
+                                                wr, e0, vopen, vext, best, r.top, edge, lane, bj, bi, ac);
   }
   // best and top-4 over the group's lanes
   for (int off = 1; off < c.G; off <<= 1) {
