@@ -332,9 +332,9 @@ static int forced_groups() {
 // 64-lane groups for cross-row shuffles.
 static const int kShapeRows[3] = {8, 12, 16};
 static const double kShapePs[3][7] = {
-    {1.98, 0.80, 0.292, 0.268, 0.289, 0.279, 0.41},   // R = 8
-    {1.27, 0.48, 0.267, 0.239, 0.250, 0.296, 0.531},  // R = 12
-    {0.89, 0.40, 0.235, 0.230, 0.240, 0.330, 0.670},  // R = 16
+    {1.98, 0.80, 0.286, 0.265, 0.283, 0.272, 0.40},   // R = 8
+    {1.27, 0.48, 0.252, 0.233, 0.245, 0.289, 0.534},  // R = 12
+    {0.88, 0.405, 0.236, 0.230, 0.237, 0.325, 0.665}, // R = 16
 };
 
 // The cost table in use: the compiled one, or the 21 values (rows R = 8,
