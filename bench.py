@@ -191,14 +191,20 @@ def _serve_mode(st, ctx, db, reqs, cells, ref_of, n_iso, label, **cfg):
     check(range(n_iso), outs)
     log(f"[{label}] isolated latency over {n_iso} requests: p50 {np.percentile(iso, 50):.2f} ms, "
         f"p99 {np.percentile(iso, 99):.2f} ms")
-    # saturated: every request queued at once
+    # saturated: every request queued at once, three bursts (a 2,000-request
+    # burst lasts about 0.14 s and varies by about 10% between bursts); the
+    # median burst is reported, every burst's results are checked
     # the server's wall time: first submission to last completion (the
     # Python request records and output arrays are built before it)
-    outs, lat_sat = srv.run_trace(reqs, np.zeros(n))
-    t_sat = srv.last_trace_ms / 1e3
-    check(range(n), outs)
+    bursts = []
+    for _ in range(3):
+        outs, lat_b = srv.run_trace(reqs, np.zeros(n))
+        bursts.append((srv.last_trace_ms / 1e3, lat_b))
+        check(range(n), outs)
+    t_sat, lat_sat = sorted(bursts, key=lambda b: b[0])[1]
     sat_qps = n / t_sat
-    log(f"[{label}] saturated: {n} requests in {1e3 * t_sat:.1f} ms, {sat_qps:.0f} q/s, "
+    log(f"[{label}] saturated: {n} requests in {1e3 * t_sat:.1f} ms (median of "
+        f"{', '.join(f'{1e3 * b[0]:.1f}' for b in bursts)} ms), {sat_qps:.0f} q/s, "
         f"{cells.sum() / t_sat / 1e9:.0f} GCUPS")
     rng = np.random.default_rng(3)
     rates = [50.0, 200.0, 500.0, 1000.0, 2000.0, 0.5 * sat_qps, 0.8 * sat_qps]
@@ -227,7 +233,8 @@ def _serve_mode(st, ctx, db, reqs, cells, ref_of, n_iso, label, **cfg):
         "config": cfg,
         "saturated": {"ms": round(1e3 * t_sat, 3), "queries_per_s": round(sat_qps, 1),
                       "gcups": round(cells.sum() / t_sat / 1e9, 1), "p50_ms": _pct(lat_sat, 50),
-                      "p99_ms": _pct(lat_sat, 99)},
+                      "p99_ms": _pct(lat_sat, 99), "bursts_ms": [round(1e3 * b[0], 3) for b in bursts],
+                      "reported": "median of the bursts"},
         "isolated": {"requests": n_iso, "p50_ms": _pct(iso, 50), "p99_ms": _pct(iso, 99)},
         "poisson": sweep,
         "p99_at_80pct_over_isolated_p50": round(at80["p99_ms"] / float(np.percentile(iso, 50)), 3),
@@ -462,9 +469,12 @@ def serving_leg(a, O, st, synth):
     # plain batch of every request
     b = st.Batch(ctx, db, qps, w.cands, gaps, w.gates)
     b.run()
-    ts = time.perf_counter()
-    b.run()
-    t_batch = time.perf_counter() - ts
+    runs = []
+    for _ in range(3):
+        ts = time.perf_counter()
+        b.run()
+        runs.append(time.perf_counter() - ts)
+    t_batch = sorted(runs)[1]  # the median run, as for the served bursts
     ref = b.results()
     offs = b.offsets.astype(np.int64)
     ref_of = [ref[offs[k]:offs[k + 1]] for k in range(n)]
